@@ -1,0 +1,69 @@
+"""Deterministic synthetic meshes for the geometry-step workloads.
+
+Disc height field (the reference's DistMin input class): an m x m vertex grid
+on [0,1]^2, each square split along a diagonal whose direction is drawn
+Bernoulli(0.5), heights z = sum of 8 Gaussian bumps with centres U[0,1]^2,
+amplitudes U(0.1,0.4) and widths U(0.05,0.2); the initial UV is the xy
+projection, so every face is counter-clockwise in UV. Draws come from
+numpy.random.default_rng(seed) in that order (flips, centres, amplitudes,
+widths).
+
+m = 51 gives the 5,000-face case; m = 317 gives 199,712 faces, the 200k-face
+workload. Vertex 0 is the (0,0) corner, the vertex the reference fixes
+(TriMesh.cpp:325-328).
+"""
+import numpy as np
+
+
+def height_field_disc(m=51, seed=0, n_bumps=8):
+    """Return (Vrest (V,3) f64, F (F,3) int32, UV (V,2) f64)."""
+    rng = np.random.default_rng(seed)
+    s = m - 1
+    flips = rng.random(s * s) < 0.5
+    centres = rng.random((n_bumps, 2))
+    amps = rng.uniform(0.1, 0.4, n_bumps)
+    sigmas = rng.uniform(0.05, 0.2, n_bumps)
+
+    xs = np.linspace(0.0, 1.0, m)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")  # row j = y index, col i = x index
+    P = np.stack([X.ravel(), Y.ravel()], axis=1)
+    d2 = ((P[:, None, :] - centres[None, :, :]) ** 2).sum(-1)
+    Z = (amps[None, :] * np.exp(-d2 / (2.0 * sigmas[None, :] ** 2))).sum(-1)
+    Vrest = np.stack([P[:, 0], P[:, 1], Z], axis=1)
+
+    j, i = np.meshgrid(np.arange(s), np.arange(s), indexing="ij")
+    i = i.ravel()
+    j = j.ravel()
+    v00 = j * m + i
+    v10 = v00 + 1
+    v01 = v00 + m
+    v11 = v01 + 1
+    fl = flips
+    # diagonal v00-v11 (not flipped) or v10-v01 (flipped); all CCW in xy
+    t1 = np.where(fl[:, None], np.stack([v00, v10, v01], 1), np.stack([v00, v10, v11], 1))
+    t2 = np.where(fl[:, None], np.stack([v10, v11, v01], 1), np.stack([v00, v11, v01], 1))
+    F = np.empty((2 * s * s, 3), dtype=np.int32)
+    F[0::2] = t1
+    F[1::2] = t2
+    UV = P.copy()
+    return Vrest, F, UV
+
+
+def faces_for_target(target_faces):
+    """Grid size m whose height-field disc has about target_faces faces."""
+    return int(round(np.sqrt(target_faces / 2.0))) + 1
+
+
+def batched_disc_charts(n_charts=64, lo=500, hi=20000, base_seed=1000):
+    """Charts of uneven size: F_i = exp(U(ln lo, ln hi)), each a height-field disc.
+
+    Sizes come from default_rng(base_seed); chart i uses seed base_seed + i.
+    Returns a list of (Vrest, F, UV).
+    """
+    rng = np.random.default_rng(base_seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_charts))
+    charts = []
+    for k, fk in enumerate(sizes):
+        m = max(3, int(round(np.sqrt(fk / 2.0))) + 1)
+        charts.append(height_field_disc(m, seed=base_seed + k))
+    return charts

@@ -1,0 +1,339 @@
+"""ctypes bindings of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests, ``__graft_entry__.smoke()`` and the cpu_baseline leg of
+``bench.py`` import this module, and only as the checker. ``liboracle.so`` is
+the project's own FP64 restatement of the reference hot path
+(``oracle/sd_oracle.cpp``); ``libref.so`` wraps the reference's own
+``EigenLibSolver.cpp`` and ``IglUtils::makePD`` compiled from the reference
+tree (present only where that tree was available at build time).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_REF_PATH = os.path.join(_HERE, "_ref", "libref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+_lib = None
+_ref = None
+
+
+def build():
+    """Compile the oracle (and the reference harness when the tree exists)."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE, "all"])
+    if os.path.isdir("/root/reference/src"):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "ref"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_rest_features.restype = C.c_int
+        L.orc_rest_features.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_double] + [_dp] * 8 + [
+            C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_energy_per_elem.restype = None
+        L.orc_energy_per_elem.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.orc_energy.restype = C.c_double
+        L.orc_energy.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int]
+        L.orc_gradient.restype = None
+        L.orc_gradient.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int,
+                                   C.c_int, _ip, _dp]
+        L.orc_make_pd.restype = None
+        L.orc_make_pd.argtypes = [C.c_int, _dp]
+        L.orc_eigh.restype = None
+        L.orc_eigh.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_element_hessians.restype = None
+        L.orc_element_hessians.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int,
+                                           C.c_int, _dp]
+        L.orc_hessian_triplets.restype = C.c_int
+        L.orc_hessian_triplets.argtypes = [C.c_int, _ip, _dp, C.c_int, _ip, C.c_double, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]
+        L.orc_set_pattern.restype = C.c_int
+        L.orc_set_pattern.argtypes = [C.c_int, _ip, _ip, C.c_int, _ip, C.c_void_p, C.c_void_p]
+        L.orc_update_a.restype = C.c_int
+        L.orc_update_a.argtypes = [C.c_int, _ip, _ip, C.c_int, _ip, _ip, _dp, _dp]
+        L.orc_ldlt_factor.restype = C.c_void_p
+        L.orc_ldlt_factor.argtypes = [C.c_int, _ip, _ip, _dp]
+        L.orc_ldlt_solve.restype = None
+        L.orc_ldlt_solve.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_ldlt_nnz.restype = C.c_long
+        L.orc_ldlt_nnz.argtypes = [C.c_void_p]
+        L.orc_ldlt_free.restype = None
+        L.orc_ldlt_free.argtypes = [C.c_void_p]
+        L.orc_init_step_size.restype = C.c_double
+        L.orc_init_step_size.argtypes = [C.c_int, _ip, _dp, _dp, C.c_double]
+        L.orc_check_inversion.restype = C.c_int
+        L.orc_check_inversion.argtypes = [C.c_int, _ip, _dp]
+        L.orc_seam_sparsity.restype = C.c_double
+        L.orc_seam_sparsity.argtypes = [C.c_int, _ip, _dp, _dp, C.c_double, C.c_double, C.c_int]
+        L.orc_newton_run.restype = C.c_int
+        L.orc_newton_run.argtypes = [C.c_int, _dp, C.c_int, _ip, _dp, C.c_double, C.c_int, C.c_double,
+                                     C.c_int, _dp, C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(_REF_PATH)
+
+
+def ref():
+    """The reference-code harness (EigenLibSolver + makePD), or None."""
+    global _ref
+    if _ref is None and ref_available():
+        R = C.CDLL(_REF_PATH)
+        R.ref_solver_create.restype = C.c_void_p
+        R.ref_solver_destroy.argtypes = [C.c_void_p]
+        R.ref_set_pattern.restype = C.c_int
+        R.ref_set_pattern.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int, _ip]
+        R.ref_get_pattern.argtypes = [C.c_void_p, _ip, _ip]
+        R.ref_update_a.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp]
+        R.ref_get_a.argtypes = [C.c_void_p, _dp]
+        R.ref_analyze_factorize.restype = C.c_int
+        R.ref_analyze_factorize.argtypes = [C.c_void_p]
+        R.ref_solve.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        R.ref_make_pd6.argtypes = [_dp]
+        R.ref_eigvals6.argtypes = [_dp, _dp]
+        _ref = R
+    return _ref
+
+
+# ----------------------------------------------------------------------------
+# numpy-level helpers
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class RestFeatures:
+    """Per-triangle rest features (TriMesh.cpp:343-404)."""
+
+    def __init__(self, Vrest, F, area_thres=0.0):
+        Vrest = _f64(Vrest)
+        F = _i32(F)
+        nF = F.shape[0]
+        arrs = [np.zeros(nF) for _ in range(8)]
+        sa = C.c_double()
+        ael = C.c_double()
+        st = lib().orc_rest_features(Vrest.shape[0], Vrest, nF, F, float(area_thres), *arrs, C.byref(sa),
+                                     C.byref(ael))
+        if st != 0:
+            raise ValueError("zero-area rest triangle")
+        (self.triArea, self.triAreaSq, self.e0SqLen, self.e1SqLen, self.e0dote1, self.e0d, self.e1d,
+         self.e01d) = arrs
+        self.surfaceArea = sa.value
+        self.avgEdgeLen = ael.value
+
+
+def energy_per_elem(F, UV, feat, uniform=False):
+    F = _i32(F)
+    out = np.zeros(F.shape[0])
+    lib().orc_energy_per_elem(F.shape[0], F, _f64(UV), feat.triArea, feat.triAreaSq, feat.e0SqLen, feat.e1SqLen,
+                              feat.e0dote1, feat.surfaceArea, int(uniform), out)
+    return out
+
+
+def energy(F, UV, feat, uniform=False):
+    F = _i32(F)
+    return lib().orc_energy(F.shape[0], F, _f64(UV), feat.triArea, feat.triAreaSq, feat.e0SqLen, feat.e1SqLen,
+                            feat.e0dote1, feat.surfaceArea, int(uniform))
+
+
+def gradient(F, UV, feat, fixed=(0,), uniform=False):
+    F = _i32(F)
+    UV = _f64(UV)
+    nV = UV.shape[0]
+    g = np.zeros(2 * nV)
+    fx = _i32(np.array(sorted(fixed), dtype=np.int32))
+    lib().orc_gradient(nV, F.shape[0], F, UV, feat.triArea, feat.triAreaSq, feat.e0SqLen, feat.e1SqLen,
+                       feat.e0dote1, feat.surfaceArea, int(uniform), len(fx), fx, g)
+    return g
+
+
+def element_hessians(F, UV, feat, uniform=False, project=True):
+    F = _i32(F)
+    H = np.zeros((F.shape[0], 6, 6))
+    lib().orc_element_hessians(F.shape[0], F, _f64(UV), feat.triArea, feat.triAreaSq, feat.e0d, feat.e1d,
+                               feat.e01d, feat.surfaceArea, int(uniform), int(project), H.reshape(-1))
+    return H
+
+
+def make_pd(H):
+    H = _f64(H).copy()
+    n = H.shape[0]
+    lib().orc_make_pd(n, H.reshape(-1))
+    return H
+
+
+def eigh(A):
+    A = _f64(A)
+    n = A.shape[0]
+    lam = np.zeros(n)
+    V = np.zeros((n, n))
+    lib().orc_eigh(n, A.reshape(-1), lam, V.reshape(-1))
+    return lam, V
+
+
+def hessian_triplets(F, Helem, fixed=(0,), scale=1.0):
+    F = _i32(F)
+    He = _f64(Helem).reshape(-1)
+    fx = _i32(np.array(sorted(fixed), dtype=np.int32))
+    n = lib().orc_hessian_triplets(F.shape[0], F, He, len(fx), fx, float(scale), None, None, None)
+    I = np.zeros(n, np.int32)
+    J = np.zeros(n, np.int32)
+    V = np.zeros(n)
+    lib().orc_hessian_triplets(F.shape[0], F, He, len(fx), fx, float(scale), I.ctypes.data, J.ctypes.data,
+                               V.ctypes.data)
+    return I, J, V
+
+
+def vertex_neighbors(nV, F):
+    """vNeighbor as CSR (TriMesh.cpp:436-447)."""
+    F = np.asarray(F, dtype=np.int64)
+    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    e = np.concatenate([e, e[:, ::-1]])
+    key = np.unique(e[:, 0] * nV + e[:, 1])
+    a, b = key // nV, key % nV
+    ptr = np.zeros(nV + 1, np.int32)
+    np.add.at(ptr, a + 1, 1)
+    ptr = np.cumsum(ptr).astype(np.int32)
+    return ptr, b.astype(np.int32)
+
+
+def set_pattern(nV, nbr_ptr, nbr, fixed=(0,)):
+    fx = _i32(np.array(sorted(fixed), dtype=np.int32))
+    nbr_ptr, nbr = _i32(nbr_ptr), _i32(nbr)
+    nnz = lib().orc_set_pattern(nV, nbr_ptr, nbr, len(fx), fx, None, None)
+    ia = np.zeros(2 * nV + 1, np.int32)
+    ja = np.zeros(nnz, np.int32)
+    lib().orc_set_pattern(nV, nbr_ptr, nbr, len(fx), fx, ia.ctypes.data, ja.ctypes.data)
+    return ia, ja
+
+
+def update_a(ia, ja, I, J, V):
+    a = np.zeros(len(ja))
+    st = lib().orc_update_a(len(ia) - 1, _i32(ia), _i32(ja), len(I), _i32(I), _i32(J), _f64(V), a)
+    if st != 0:
+        raise ValueError("triplet outside the pattern")
+    return a
+
+
+class LDLT:
+    """Own simplicial LDL^T on the upper CSR (restates EigenLibSolver's role)."""
+
+    def __init__(self, ia, ja, a):
+        self._ia, self._ja, self._a = _i32(ia), _i32(ja), _f64(a)
+        self.n = len(ia) - 1
+        self.h = lib().orc_ldlt_factor(self.n, self._ia, self._ja, self._a)
+        if not self.h:
+            raise np.linalg.LinAlgError("LDLT: zero pivot")
+
+    def solve(self, b):
+        x = np.zeros(self.n)
+        lib().orc_ldlt_solve(self.h, _f64(b), x)
+        return x
+
+    def nnz(self):
+        return lib().orc_ldlt_nnz(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_ldlt_free(self.h)
+            self.h = None
+
+
+def init_step_size(F, UV, d, step=1.0):
+    F = _i32(F)
+    return lib().orc_init_step_size(F.shape[0], F, _f64(UV), _f64(d), float(step))
+
+
+def check_inversion(F, UV):
+    F = _i32(F)
+    return bool(lib().orc_check_inversion(F.shape[0], F, _f64(UV)))
+
+
+def seam_sparsity(cohE, Vrest, UV, avgEdgeLen, initSeamLen=0.0, triSoup=False):
+    cohE = _i32(cohE).reshape(-1, 4)
+    return lib().orc_seam_sparsity(cohE.shape[0], cohE, _f64(Vrest), _f64(UV), float(avgEdgeLen),
+                                   float(initSeamLen), int(triSoup))
+
+
+def newton_run(Vrest, F, UV, a0=1.0, max_iter=100, rel_gl2_tol=1e-12, allow_edec_rel_tol=True):
+    """Plain Newton + own LDLT, no scaffold. Returns (trace rows, final UV, exit reason)."""
+    UV = _f64(UV).copy()
+    F = _i32(F)
+    Vrest = _f64(Vrest)
+    trace = np.zeros((max_iter + 1, 6))
+    why = C.c_int()
+    n = lib().orc_newton_run(Vrest.shape[0], Vrest, F.shape[0], F, UV.reshape(-1), float(a0), int(max_iter),
+                             float(rel_gl2_tol), int(allow_edec_rel_tol), trace.reshape(-1), C.byref(why))
+    if n < 0:
+        raise RuntimeError("oracle newton_run failed with status %d" % n)
+    return trace[: n + 1], UV, why.value
+
+
+# ----------------------------------------------------------------------------
+# reference harness helpers
+
+class RefSolver:
+    """The reference EigenLibSolver (set_pattern/update_a/factorize/solve)."""
+
+    def __init__(self):
+        self.R = ref()
+        if self.R is None:
+            raise RuntimeError("reference harness not built")
+        self.h = self.R.ref_solver_create()
+
+    def set_pattern(self, nV, nbr_ptr, nbr, fixed=(0,)):
+        fx = _i32(np.array(sorted(fixed), dtype=np.int32))
+        self.nnz = self.R.ref_set_pattern(self.h, nV, _i32(nbr_ptr), _i32(nbr), len(fx), fx)
+        self.n = 2 * nV
+        ia = np.zeros(self.n + 1, np.int32)
+        ja = np.zeros(self.nnz, np.int32)
+        self.R.ref_get_pattern(self.h, ia, ja)
+        return ia, ja
+
+    def update_a(self, I, J, V):
+        self.R.ref_update_a(self.h, len(I), _i32(I), _i32(J), _f64(V))
+        a = np.zeros(self.nnz)
+        self.R.ref_get_a(self.h, a)
+        return a
+
+    def factorize(self):
+        if self.R.ref_analyze_factorize(self.h) != 0:
+            raise np.linalg.LinAlgError("reference factorize failed")
+
+    def solve(self, b):
+        x = np.zeros(self.n)
+        self.R.ref_solve(self.h, self.n, _f64(b), x)
+        return x
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.R.ref_solver_destroy(self.h)
+            self.h = None
+
+
+def ref_make_pd6(H):
+    H = _f64(H).copy()
+    ref().ref_make_pd6(H.reshape(-1))
+    return H
+
+
+def ref_eigvals6(H):
+    lam = np.zeros(6)
+    ref().ref_eigvals6(_f64(H).reshape(-1), lam)
+    return lam

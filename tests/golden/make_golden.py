@@ -1,0 +1,81 @@
+"""Writes tests/golden/ref_vectors.npz from the reference's own code.
+
+Run in the container that holds the reference tree, after
+``make -C oracle all ref``. The outputs come from the reference's
+EigenLibSolver.cpp (set_pattern / update_a / SimplicialLDLT solve) and
+IglUtils::makePD, built unmodified into oracle/_ref/libref.so. The inputs are
+this project's synthetic meshes and the oracle's element Hessians (data, not
+reference code).
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
+
+from oracle import oracle as O  # noqa: E402
+from optcuts_amd.meshgen import height_field_disc  # noqa: E402
+
+
+def distorted_uv(V, F, UV, seed, amp):
+    """A random smooth distortion that keeps every triangle CCW."""
+    rng = np.random.default_rng(seed)
+    for _ in range(50):
+        k = rng.normal(size=(2, 2)) * amp
+        UVd = UV + np.stack([k[0, 0] * np.sin(3 * UV[:, 1]) + k[0, 1] * UV[:, 0] ** 2,
+                             k[1, 0] * np.cos(2 * UV[:, 0]) + k[1, 1] * UV[:, 1] ** 2], 1)
+        if O.check_inversion(F, UVd):
+            return UVd
+        amp *= 0.5
+    return UV.copy()
+
+
+def main():
+    assert O.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+    out = {}
+    # --- CSR pattern, triplet accumulation and LDLT direction on three grids
+    for m, seed in ((3, 11), (11, 12), (21, 13)):
+        V, F, UV = height_field_disc(m, seed=seed)
+        UVd = distorted_uv(V, F, UV, seed, 0.05)
+        nV = V.shape[0]
+        feat = O.RestFeatures(V, F)
+        ptr, nbr = O.vertex_neighbors(nV, F)
+        H = O.element_hessians(F, UVd, feat)
+        I, J, Vt = O.hessian_triplets(F, H, fixed=(0,), scale=1.0)
+        g = O.gradient(F, UVd, feat)
+        S = O.RefSolver()
+        ia, ja = S.set_pattern(nV, ptr, nbr, fixed=(0,))
+        a = S.update_a(I, J, Vt)
+        S.factorize()
+        d = S.solve(-g)
+        # a second right-hand side (a rotation-free random vector, fixed DOFs 0)
+        rng = np.random.default_rng(seed + 100)
+        b2 = rng.normal(size=2 * nV)
+        b2[:2] = 0.0
+        d2 = S.solve(b2)
+        p = "grid%d_" % m
+        out.update({p + "Vrest": V, p + "F": F, p + "UV": UVd, p + "ia": ia, p + "ja": ja, p + "a": a,
+                    p + "g": g, p + "d": d, p + "b2": b2, p + "d2": d2})
+    # --- makePD on random symmetric 6x6 and on element Hessians
+    rng = np.random.default_rng(7)
+    A = rng.normal(size=(100, 6, 6))
+    A = 0.5 * (A + A.transpose(0, 2, 1))
+    V, F, UV = height_field_disc(11, seed=5)
+    UVd = distorted_uv(V, F, UV, 5, 0.3)
+    feat = O.RestFeatures(V, F)
+    He = O.element_hessians(F, UVd, feat, project=False)
+    Hin = np.concatenate([A, He[:100]], 0)
+    Hpd = np.stack([O.ref_make_pd6(h) for h in Hin])
+    lam = np.stack([O.ref_eigvals6(h) for h in Hin])
+    out.update({"pd_in": Hin, "pd_out": Hpd, "pd_eigvals": lam})
+    path = os.path.join(HERE, "ref_vectors.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

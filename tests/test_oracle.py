@@ -1,0 +1,407 @@
+"""Pins the CPU oracle before anything is compared with it.
+
+Each oracle function is checked against something independent of its own
+formula: the reference's golden vectors (tests/golden/ref_vectors.npz, made by
+the reference's EigenLibSolver and IglUtils::makePD), closed forms, finite
+differences, textbook dense linear algebra, or brute force.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from optcuts_amd.meshgen import height_field_disc
+
+
+def _soup(nt, seed):
+    """nt separate random 3D triangles (a triangle soup) and their vertices."""
+    rng = np.random.default_rng(seed)
+    P = rng.normal(size=(nt, 3, 3))
+    V = P.reshape(-1, 3)
+    F = np.arange(3 * nt, dtype=np.int32).reshape(nt, 3)
+    return V, F
+
+
+def _isometric_uv(V, F):
+    """Each triangle laid flat in its own plane: a rigid map, so SD = 4 per unit weight."""
+    UV = np.zeros((V.shape[0], 2))
+    for t in range(F.shape[0]):
+        p1, p2, p3 = V[F[t]]
+        e1 = p2 - p1
+        x = e1 / np.linalg.norm(e1)
+        n = np.cross(e1, p3 - p1)
+        y = np.cross(n, x)
+        y /= np.linalg.norm(y)
+        for k, p in enumerate((p1, p2, p3)):
+            UV[F[t, k]] = [(p - p1) @ x, (p - p1) @ y]
+    return UV
+
+
+def _textbook_sd(V, F, UV):
+    """w_t * (|J|_F^2 + |J^-1|_F^2), J the 2x2 Jacobian from the rest triangle to UV."""
+    out = []
+    area = 0.5 * np.linalg.norm(np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]]), axis=1)
+    for t in range(F.shape[0]):
+        p1, p2, p3 = V[F[t]]
+        e1, e2 = p2 - p1, p3 - p1
+        x = e1 / np.linalg.norm(e1)
+        y = np.cross(np.cross(e1, e2), x)
+        y /= np.linalg.norm(y)
+        Xr = np.array([[e1 @ x, e2 @ x], [e1 @ y, e2 @ y]])
+        u1, u2, u3 = UV[F[t]]
+        Xu = np.stack([u2 - u1, u3 - u1], 1)
+        J = Xu @ np.linalg.inv(Xr)
+        out.append((J ** 2).sum() + (np.linalg.inv(J) ** 2).sum())
+    return np.array(out) * area / area.sum()
+
+
+def _distort(UV, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    k = rng.normal(size=4) * amp
+    return UV + np.stack([k[0] * np.sin(3 * UV[:, 1]) + k[1] * UV[:, 0] ** 2,
+                          k[2] * np.cos(2 * UV[:, 0]) + k[3] * UV[:, 1] ** 2], 1)
+
+
+# --------------------------------------------------------------------------- a1/a2
+def test_energy_isometry_closed_form():
+    V, F = _soup(200, 1)
+    UV = _isometric_uv(V, F)
+    feat = O.RestFeatures(V, F)
+    E = O.energy_per_elem(F, UV, feat)
+    w = feat.triArea / feat.surfaceArea
+    np.testing.assert_allclose(E, 4.0 * w, rtol=1e-12)
+    assert abs(O.energy(F, UV, feat) - 4.0) < 1e-12
+    # uniform weight: 4 per triangle
+    np.testing.assert_allclose(O.energy_per_elem(F, UV, feat, uniform=True), 4.0, rtol=1e-12)
+
+
+def test_energy_matches_textbook_definition():
+    V, F, UV = height_field_disc(9, seed=3)
+    UV = _distort(UV, 3, 0.1)
+    assert O.check_inversion(F, UV)
+    feat = O.RestFeatures(V, F)
+    np.testing.assert_allclose(O.energy_per_elem(F, UV, feat), _textbook_sd(V, F, UV), rtol=1e-11)
+
+
+def test_energy_scaling_and_rotation_invariance():
+    V, F, UV = height_field_disc(7, seed=4)
+    UV = _distort(UV, 4, 0.1)
+    feat = O.RestFeatures(V, F)
+    E0 = O.energy(F, UV, feat)
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    assert abs(O.energy(F, UV @ R.T + 3.0, feat) - E0) < 1e-12 * E0
+    # rest scaling by s and UV scaling by s leave SD unchanged
+    feat2 = O.RestFeatures(V * 2.0, F)
+    assert abs(O.energy(F, UV * 2.0, feat2) - E0) < 1e-12 * E0
+
+
+def test_rest_features_air_mesh_clamp():
+    V, F = _soup(5, 2)
+    V[3:6] *= 1e-4  # triangle 1 is tiny
+    feat = O.RestFeatures(V, F, area_thres=1e-3)
+    A = 1e-3
+    assert feat.triArea[1] == A
+    assert feat.e0SqLen[1] == pytest.approx(4.0 / np.sqrt(3.0) * A)
+    assert feat.e0d[1] == pytest.approx(2.0 / np.sqrt(3.0) / A)
+    assert feat.e01d[1] == pytest.approx(feat.e0d[1] / 2)
+
+
+def test_avg_edge_length_counts_half_edges():
+    V, F, UV = height_field_disc(4, seed=0)
+    feat = O.RestFeatures(V, F)
+    L = [np.linalg.norm(V[F[t, j]] - V[F[t, (j + 1) % 3]]) for t in range(len(F)) for j in range(3)]
+    assert feat.avgEdgeLen == pytest.approx(np.mean(L), rel=1e-14)
+
+
+# --------------------------------------------------------------------------- a3/a4
+def test_gradient_central_differences():
+    V, F, UV = height_field_disc(6, seed=5)
+    UV = _distort(UV, 5, 0.1)
+    feat = O.RestFeatures(V, F)
+    g = O.gradient(F, UV, feat, fixed=())
+    h = 1e-6
+    gfd = np.zeros_like(g)
+    X = UV.reshape(-1).copy()
+    for k in range(X.size):
+        Xp = X.copy(); Xp[k] += h
+        Xm = X.copy(); Xm[k] -= h
+        gfd[k] = (O.energy(F, Xp.reshape(-1, 2), feat) - O.energy(F, Xm.reshape(-1, 2), feat)) / (2 * h)
+    assert np.linalg.norm(g - gfd) < 1e-7 * np.linalg.norm(g)
+    # fixed vertex rows are zeroed
+    g0 = O.gradient(F, UV, feat, fixed=(0, 3))
+    assert np.all(g0[[0, 1, 6, 7]] == 0.0)
+    np.testing.assert_array_equal(np.delete(g0, [0, 1, 6, 7]), np.delete(g, [0, 1, 6, 7]))
+
+
+def _dense_hessian(nV, F, He):
+    Hd = np.zeros((2 * nV, 2 * nV))
+    for t in range(F.shape[0]):
+        idx = np.array([[2 * v, 2 * v + 1] for v in F[t]]).reshape(-1)
+        Hd[np.ix_(idx, idx)] += He[t]
+    return Hd
+
+
+def test_hessian_central_differences_and_null_space():
+    V, F, UV = height_field_disc(5, seed=6)
+    UV = _distort(UV, 6, 0.1)
+    feat = O.RestFeatures(V, F)
+    He = O.element_hessians(F, UV, feat, project=False)
+    np.testing.assert_allclose(He, He.transpose(0, 2, 1), rtol=0, atol=1e-12 * np.abs(He).max())
+    tx = np.array([1, 0, 1, 0, 1, 0.0])
+    ty = np.array([0, 1, 0, 1, 0, 1.0])
+    assert np.abs(He @ tx).max() < 1e-10 * np.abs(He).max()
+    assert np.abs(He @ ty).max() < 1e-10 * np.abs(He).max()
+    Hd = _dense_hessian(V.shape[0], F, He)
+    X = UV.reshape(-1).copy()
+    h = 1e-6
+    Hfd = np.zeros_like(Hd)
+    for k in range(X.size):
+        Xp = X.copy(); Xp[k] += h
+        Xm = X.copy(); Xm[k] -= h
+        Hfd[:, k] = (O.gradient(F, Xp.reshape(-1, 2), feat, fixed=()) -
+                     O.gradient(F, Xm.reshape(-1, 2), feat, fixed=())) / (2 * h)
+    assert np.linalg.norm(Hd - Hfd) < 1e-6 * np.linalg.norm(Hd)
+
+
+# --------------------------------------------------------------------------- a5
+def _numpy_pd(H):
+    lam, Q = np.linalg.eigh(H)
+    if lam[0] >= 0:
+        return H
+    return (Q * np.maximum(lam, 0.0)) @ Q.T
+
+
+def test_make_pd_matches_reference_golden(golden):
+    Hin, Hout, lam_ref = golden["pd_in"], golden["pd_out"], golden["pd_eigvals"]
+    for k in range(Hin.shape[0]):
+        h = O.make_pd(Hin[k])
+        sc = np.abs(Hin[k]).max()
+        assert np.abs(h - Hout[k]).max() < 1e-12 * sc, k
+        lam, _ = O.eigh(Hin[k])
+        assert np.abs(lam - lam_ref[k]).max() < 1e-12 * sc
+
+
+def test_make_pd_matches_numpy_projection():
+    rng = np.random.default_rng(8)
+    for _ in range(100):
+        A = rng.normal(size=(6, 6))
+        A = A + A.T
+        h = O.make_pd(A)
+        np.testing.assert_allclose(h, _numpy_pd(A), atol=1e-12 * np.abs(A).max())
+        assert np.linalg.eigvalsh(h).min() > -1e-12 * np.abs(A).max()
+        np.testing.assert_allclose(O.make_pd(h), h, atol=1e-12 * np.abs(A).max())  # idempotent
+    B = rng.normal(size=(6, 6))
+    P = B @ B.T
+    np.testing.assert_array_equal(O.make_pd(P), P)  # PSD input is returned untouched
+
+
+def test_projected_element_hessians_are_psd():
+    V, F, UV = height_field_disc(9, seed=9)
+    UV = _distort(UV, 9, 0.3)
+    feat = O.RestFeatures(V, F)
+    He = O.element_hessians(F, UV, feat, project=True)
+    Hr = O.element_hessians(F, UV, feat, project=False)
+    for k in range(F.shape[0]):
+        np.testing.assert_allclose(He[k], _numpy_pd(Hr[k]), atol=1e-11 * np.abs(Hr[k]).max())
+
+
+# --------------------------------------------------------------------------- a6/a8/a9
+@pytest.mark.parametrize("m", [3, 11, 21])
+def test_pattern_and_assembly_match_reference_golden(golden, m):
+    p = "grid%d_" % m
+    V, F, UV = golden[p + "Vrest"], golden[p + "F"], golden[p + "UV"]
+    nV = V.shape[0]
+    ptr, nbr = O.vertex_neighbors(nV, F)
+    ia, ja = O.set_pattern(nV, ptr, nbr, fixed=(0,))
+    np.testing.assert_array_equal(ia, golden[p + "ia"])
+    np.testing.assert_array_equal(ja, golden[p + "ja"])
+    feat = O.RestFeatures(V, F)
+    H = O.element_hessians(F, UV, feat)
+    I, J, Vt = O.hessian_triplets(F, H, fixed=(0,))
+    a = O.update_a(ia, ja, I, J, Vt)
+    np.testing.assert_allclose(a, golden[p + "a"], rtol=0, atol=1e-13 * np.abs(a).max())
+
+
+def test_triplets_against_dense_brute_force():
+    V, F, UV = height_field_disc(6, seed=10)
+    UV = _distort(UV, 10, 0.1)
+    nV = V.shape[0]
+    feat = O.RestFeatures(V, F)
+    H = O.element_hessians(F, UV, feat)
+    I, J, Vt = O.hessian_triplets(F, H, fixed=(0,), scale=0.5)
+    D = np.zeros((2 * nV, 2 * nV))
+    np.add.at(D, (I, J), Vt)
+    Hd = 0.5 * _dense_hessian(nV, F, H)
+    Hd[:2, :] = 0
+    Hd[:, :2] = 0
+    Hd[0, 0] = Hd[1, 1] = 0.5
+    np.testing.assert_allclose(D, Hd, atol=1e-13 * np.abs(Hd).max())
+    ptr, nbr = O.vertex_neighbors(nV, F)
+    ia, ja = O.set_pattern(nV, ptr, nbr, fixed=(0,))
+    a = O.update_a(ia, ja, I, J, Vt)
+    U = np.zeros_like(D)
+    for i in range(2 * nV):
+        U[i, ja[ia[i]:ia[i + 1]]] = a[ia[i]:ia[i + 1]]
+    np.testing.assert_allclose(np.triu(D), U, atol=1e-13 * np.abs(Hd).max())
+
+
+# --------------------------------------------------------------------------- a10
+@pytest.mark.parametrize("m", [3, 11, 21])
+def test_ldlt_matches_reference_golden(golden, m):
+    p = "grid%d_" % m
+    ia, ja, a = golden[p + "ia"], golden[p + "ja"], golden[p + "a"]
+    L = O.LDLT(ia, ja, a)
+    for rhs, sol in ((-golden[p + "g"], golden[p + "d"]), (golden[p + "b2"], golden[p + "d2"])):
+        x = L.solve(rhs)
+        assert np.linalg.norm(x - sol) < 1e-9 * np.linalg.norm(sol)
+
+
+def test_ldlt_matches_numpy_dense_solve():
+    V, F, UV = height_field_disc(8, seed=11)
+    UV = _distort(UV, 11, 0.1)
+    nV = V.shape[0]
+    feat = O.RestFeatures(V, F)
+    H = O.element_hessians(F, UV, feat)
+    I, J, Vt = O.hessian_triplets(F, H, fixed=(0,))
+    ptr, nbr = O.vertex_neighbors(nV, F)
+    ia, ja = O.set_pattern(nV, ptr, nbr)
+    a = O.update_a(ia, ja, I, J, Vt)
+    D = np.zeros((2 * nV, 2 * nV))
+    for i in range(2 * nV):
+        D[i, ja[ia[i]:ia[i + 1]]] = a[ia[i]:ia[i + 1]]
+    D = D + np.triu(D, 1).T
+    b = np.random.default_rng(0).normal(size=2 * nV)
+    b[:2] = 0
+    x = O.LDLT(ia, ja, a).solve(b)
+    np.testing.assert_allclose(x, np.linalg.solve(D, b), rtol=0, atol=1e-8 * np.abs(x).max())
+
+
+# --------------------------------------------------------------------------- a11
+def _brute_bound(U, Vd, cap):
+    """First alpha in (0, cap] where det(U + alpha V) hits 0, by scan + bisection."""
+    def det(al):
+        X = U + al * Vd
+        e1, e2 = X[1] - X[0], X[2] - X[0]
+        return e1[0] * e2[1] - e1[1] * e2[0]
+    grid = np.linspace(0.0, cap, 20001)
+    vals = np.array([det(g) for g in grid])
+    neg = np.nonzero(vals <= 0.0)[0]
+    if len(neg) == 0:
+        return cap
+    lo, hi = grid[neg[0] - 1], grid[neg[0]]
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if det(mid) > 0:
+            lo = mid
+        else:
+            hi = mid
+    return hi
+
+
+def test_step_bound_brute_force():
+    rng = np.random.default_rng(12)
+    F = np.array([[0, 1, 2]], np.int32)
+    n_hit = 0
+    for k in range(300):
+        U = np.array([[0, 0], [1, 0], [0, 1.0]]) + rng.normal(size=(3, 2)) * 0.2
+        e1, e2 = U[1] - U[0], U[2] - U[0]
+        if e1[0] * e2[1] - e1[1] * e2[0] <= 0:
+            U = U[[0, 2, 1]]
+        Vd = rng.normal(size=(3, 2))
+        if k % 3 == 0:  # make the quadratic coefficient exactly zero: rigid shear along an edge
+            Vd = np.array([[0, 0], [0, 0], [rng.normal(), rng.normal()]])
+        cap = 5.0
+        a = O.init_step_size(F, U, Vd.reshape(-1), cap)
+        b = _brute_bound(U, Vd, cap)
+        if b < cap:
+            n_hit += 1
+            assert a == pytest.approx(b, rel=1e-9, abs=1e-12)
+        else:
+            assert a == cap or a > cap * (1 - 1e-9)
+    assert n_hit > 100
+
+
+def test_step_bound_special_cases():
+    F = np.array([[0, 1, 2]], np.int32)
+    U = np.array([[0, 0], [1, 0], [0, 1.0]])
+    # a = 0, b < 0: apex moves straight down: det = 1 - alpha, root 1
+    assert O.init_step_size(F, U, np.array([0, 0, 0, 0, 0, -1.0]), 10.0) == pytest.approx(1.0)
+    # a > 0, delta < 0: no real root: cap kept
+    d = np.array([0, 0, 1.0, 0.0, 0.0, 1.0])  # uniform scaling growth (b > 0)
+    assert O.init_step_size(F, U, d, 3.0) == 3.0
+    # a < 0: always a positive root
+    d = np.array([0, 0, 0, 1.0, 1.0, 0.0])
+    r = O.init_step_size(F, U, d, 100.0)
+    assert r == pytest.approx(_brute_bound(U, d.reshape(3, 2), 100.0), rel=1e-9)
+
+
+# --------------------------------------------------------------------------- a12/a13
+def _numpy_newton(V, F, UV, iters):
+    """Independent dense restatement: numpy SD energy/gradient via autograd-free FD-checked oracle
+    pieces replaced by textbook SD energy and a dense solve."""
+    feat = O.RestFeatures(V, F)
+    nV = V.shape[0]
+    E = lambda X: _textbook_sd(V, F, X).sum()
+    rows = [E(UV)]
+    X = UV.copy()
+    for _ in range(iters):
+        g = O.gradient(F, X, feat)
+        H = _dense_hessian(nV, F, O.element_hessians(F, X, feat))
+        H[:2, :] = 0; H[:, :2] = 0; H[0, 0] = H[1, 1] = 1.0
+        d = np.linalg.solve(H, -g)
+        step = 0.99 * O.init_step_size(F, X, d, 1.0)
+        Et = E(X + step * d.reshape(-1, 2))
+        while Et > rows[-1]:
+            step /= 2
+            Et = E(X + step * d.reshape(-1, 2))
+        X = X + step * d.reshape(-1, 2)
+        rows.append(Et)
+    return np.array(rows)
+
+
+def test_newton_trace_matches_dense_restatement():
+    V, F, UV = height_field_disc(7, seed=13)
+    UV = _distort(UV, 13, 0.08)
+    assert O.check_inversion(F, UV)
+    tr, UVf, why = O.newton_run(V, F, UV, a0=1.0, max_iter=8, rel_gl2_tol=1e-30, allow_edec_rel_tol=False)
+    ref = _numpy_newton(V, F, UV, len(tr) - 1)
+    np.testing.assert_allclose(tr[:, 1], ref, rtol=1e-10)
+    assert np.all(np.diff(tr[:, 0]) <= 0)
+    assert O.check_inversion(F, UVf)
+
+
+def test_newton_a0_scales_energy_not_path():
+    V, F, UV = height_field_disc(6, seed=14)
+    UV = _distort(UV, 14, 0.08)
+    assert O.check_inversion(F, UV)
+    t1, U1, _ = O.newton_run(V, F, UV, a0=1.0, max_iter=5, rel_gl2_tol=1e-30, allow_edec_rel_tol=False)
+    t2, U2, _ = O.newton_run(V, F, UV, a0=0.25, max_iter=5, rel_gl2_tol=1e-30, allow_edec_rel_tol=False)
+    np.testing.assert_allclose(t2[:, 0], 0.25 * t1[:, 0], rtol=1e-12)
+    np.testing.assert_allclose(U1, U2, rtol=0, atol=1e-10)
+
+
+def test_newton_gradient_stop_and_rel_dec_stop():
+    V, F, UV = height_field_disc(6, seed=15)
+    tr, _, why = O.newton_run(V, F, UV, max_iter=200, rel_gl2_tol=1e-2)
+    assert why in (1, 2) and len(tr) < 200
+    tr2, _, why2 = O.newton_run(V, F, UV, max_iter=200, rel_gl2_tol=1e-30, allow_edec_rel_tol=True)
+    assert why2 == 2 and tr2[-1, 5] == 1.0
+
+
+def test_newton_rejects_inverted_input():
+    V, F, UV = height_field_disc(4, seed=0)
+    UV[:, 0] *= -1.0
+    with pytest.raises(RuntimeError):
+        O.newton_run(V, F, UV, max_iter=2)
+
+
+# --------------------------------------------------------------------------- a14
+def test_seam_sparsity_hand_case():
+    Vrest = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], [0, 0, 0], [1, 0, 0]], float)
+    UV = np.array([[0, 0], [1, 0], [0, 1], [1, 1], [0, -0.5], [1, -0.5]], float)
+    cohE = np.array([[0, 1, 4, 5], [1, 3, -1, -1]], np.int32)
+    assert O.seam_sparsity(cohE, Vrest, UV, 1.0) == pytest.approx(1.0)
+    assert O.seam_sparsity(cohE, Vrest, UV, 1.0, initSeamLen=0.5) == pytest.approx(1.5)
+    UV2 = UV.copy(); UV2[4:] = UV[:2]
+    assert O.seam_sparsity(cohE, Vrest, UV2, 1.0, triSoup=True) == 0.0
+    assert O.seam_sparsity(cohE, Vrest, UV, 1.0, triSoup=True) == pytest.approx(1.0)
