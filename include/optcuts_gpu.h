@@ -1,0 +1,157 @@
+/*
+ * optcuts_gpu.h — C ABI of the MI355X (gfx950) geometry-step library.
+ *
+ * One context (oc_ctx) owns a HIP stream and the device state of one
+ * parameterization problem: a mesh ("block 0", the TriMesh handed to
+ * SymDirichletEnergy), an optional bijectivity air mesh ("block 1", the
+ * Scaffold's airMesh), the assembled Hessian and the solver workspace.
+ * All pointers in this header are HOST pointers to plain arrays; the library
+ * copies to and from HBM itself. No call allocates after the set-up calls
+ * (oc_set_mesh, oc_set_air_mesh, oc_solver_set_pattern) have sized the
+ * context. Every function returns an OC_* status; nothing throws.
+ *
+ * Array conventions (row-major, C order):
+ *   Vrest  nV x 3 doubles       F      nF x 3 int32 (counter-clockwise in UV)
+ *   UV     nV x 2 doubles       DOF vectors (gradient, directions) 2*nV
+ *   doubles interleaved [u0, v0, u1, v1, ...] (the reference's order,
+ *   SymDirichletEnergy.cpp:287,301).
+ * Reference locations are relative to the OptCuts source tree.
+ */
+#ifndef OPTCUTS_GPU_H
+#define OPTCUTS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    OC_OK = 0,
+    OC_ERR_ARG = -1,            /* bad argument or size */
+    OC_ERR_HIP = -2,            /* HIP runtime error (message in oc_last_error) */
+    OC_ERR_STATE = -3,          /* call out of order (e.g. no mesh set) */
+    OC_ERR_PATTERN = -4,        /* triplet (i<=j) outside the sparsity pattern */
+    OC_ERR_INVERTED = -5,       /* a UV triangle has negative signed area */
+    OC_ERR_DEGENERATE = -6,     /* a rest triangle has zero area */
+    OC_ERR_NOT_CONVERGED = -7,  /* linear solve stopped before its tolerance */
+    OC_ERR_BREAKDOWN = -8,      /* PCG breakdown: matrix not positive definite */
+    OC_ERR_NO_DEVICE = -9       /* no usable gfx950 device */
+};
+
+enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1 };
+
+typedef struct oc_ctx oc_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+int oc_ctx_create(int device, oc_ctx** out);
+int oc_ctx_destroy(oc_ctx* ctx);
+/* Last error message of this context (static storage, valid until the next call). */
+const char* oc_last_error(const oc_ctx* ctx);
+/* Name of the device the context runs on, e.g. "gfx950". */
+int oc_device_arch(const oc_ctx* ctx, char* buf, int buflen);
+
+/* ---- mesh (TriMesh data of SymDirichletEnergy) -------------------------- */
+/* Replaces the TriMesh fields the energy reads: V_rest, F, fixedVert and the
+ * rest features computed by TriMesh::computeFeatures (TriMesh.cpp:323-404;
+ * areaThres is TriMesh::areaThres_AM, 0 for a surface mesh).
+ * fixedV: vertex ids, any order (the reference fixes vertex 0,
+ * TriMesh.cpp:325-328). Returns OC_ERR_DEGENERATE on a zero-area triangle
+ * (TriMesh.cpp:368-372,399-402). */
+int oc_set_mesh(oc_ctx* ctx, int nV, const double* Vrest, int nF, const int* F, int nFixed, const int* fixedV,
+                double areaThres);
+/* TriMesh::V (the UV coordinates), nV x 2. */
+int oc_set_uv(oc_ctx* ctx, const double* UV);
+int oc_get_uv(oc_ctx* ctx, double* UV);
+/* surfaceArea and avgEdgeLen as TriMesh::computeFeatures computes them (TriMesh.cpp:385,403). */
+int oc_mesh_stats(oc_ctx* ctx, double* surfaceArea, double* avgEdgeLen);
+
+/* ---- Energy interface (Energy.hpp:16-46, SymDirichletEnergy.hpp:15-42) ---- */
+/* Energy::computeEnergyVal (Energy.cpp:35-40): sum of per-element SD. */
+int oc_energy(oc_ctx* ctx, int uniformWeight, double* E);
+/* SymDirichletEnergy::getEnergyValPerElem (SymDirichletEnergy.cpp:24-46), nF values. */
+int oc_energy_per_elem(oc_ctx* ctx, int uniformWeight, double* Eelem);
+/* SymDirichletEnergy::computeGradient (SymDirichletEnergy.cpp:258-304), 2*nV values, fixed rows 0. */
+int oc_gradient(oc_ctx* ctx, int uniformWeight, double* g);
+/* SymDirichletEnergy::computeHessian(data, V, I, J) (SymDirichletEnergy.cpp:429-549): the PSD-projected
+ * element Hessians as triplets in the reference's order (IglUtils::addBlockToMatrix, IglUtils.cpp:396-451,
+ * then the identity on fixed DOFs, IglUtils.cpp:361-394). With I == NULL only *count is set. capacity is the
+ * length of I/J/V. */
+int oc_hessian_triplets(oc_ctx* ctx, int uniformWeight, long long capacity, int* I, int* J, double* V,
+                        long long* count);
+/* SymDirichletEnergy::initStepSize (SymDirichletEnergy.cpp:551-610): *stepSize = min(*stepSize, first
+ * positive root of det(U + a dir) over the triangles); dir has 2*nV values. */
+int oc_init_step_size(oc_ctx* ctx, const double* dir, double* stepSize);
+/* TriMesh::checkInversion (TriMesh.cpp:1736-1756): *ok = 1 if no triangle has negative signed UV area. */
+int oc_check_inversion(oc_ctx* ctx, int* ok);
+
+/* ---- bijectivity scaffold (Scaffold.cpp:27-313) ------------------------- */
+/* The air mesh built by the Scaffold constructor: nVair vertices with UV airUV (nVair x 2; its rest shape is
+ * this UV with z = 0, Scaffold.cpp:174-177), nFair triangles Fair in air-mesh indices, the first nBnd air
+ * vertices are the mesh vertices bnd[0..nBnd) (Scaffold::bnd, localVI2Global for l < nBnd) and the others are
+ * appended after the mesh vertices (Scaffold.cpp:179-184). areaThres is airMesh.areaThres_AM. Air vertices that
+ * are fixed mesh vertices are fixed (Scaffold.cpp:186-192). wScaf is the scaffold weight w_scaf
+ * (Optimizer.cpp:87); each air energy is scaled by wScaf / nFair (Optimizer.cpp:776,795,839). Pass nFair = 0 to
+ * drop the scaffold. */
+int oc_set_air_mesh(oc_ctx* ctx, int nVair, const double* airUV, int nFair, const int* Fair, int nBnd,
+                    const int* bnd, double areaThres, double wScaf);
+
+/* ---- LinSysSolver interface (LinSysSolver.hpp:22-256) -------------------- */
+/* LinSysSolver::set_pattern(vNeighbor, fixedVert) (LinSysSolver.hpp:37-135) + EigenLibSolver::set_pattern
+ * (EigenLibSolver.cpp:21-39): nbrPtr/nbr is vNeighbor as CSR over nV vertices. Builds the upper-triangle CSR
+ * over 2*nV DOFs (0-based, the reference's ia/ja) and the device block-CSR it maps to. *nnz = ja length. */
+int oc_solver_set_pattern(oc_ctx* ctx, int nV, const int* nbrPtr, const int* nbr, int nFixed, const int* fixedV,
+                          int* nnz);
+int oc_solver_get_pattern(oc_ctx* ctx, int* ia, int* ja);
+/* LinSysSolver::update_a (LinSysSolver.hpp:138-159): a = 0; a[slot(i,j)] += v for triplets with i <= j,
+ * in triplet order. OC_ERR_PATTERN if such a triplet has no slot. */
+int oc_solver_update_a(oc_ctx* ctx, long long nT, const int* I, const int* J, const double* V);
+int oc_solver_get_a(oc_ctx* ctx, double* a);
+/* analyze_pattern / factorize (EigenLibSolver.cpp:71-94): no symbolic phase; factorize builds the
+ * preconditioner of the selected kind from the current values. */
+int oc_solver_analyze_pattern(oc_ctx* ctx);
+int oc_solver_factorize(oc_ctx* ctx);
+/* EigenLibSolver::solve (EigenLibSolver.cpp:96-108): x = A^-1 rhs by PCG to relTol. OC_ERR_NOT_CONVERGED when
+ * maxIter is reached first (x holds the last iterate). */
+int oc_solver_solve(oc_ctx* ctx, const double* rhs, double* x);
+/* LinSysSolver::multiply (LinSysSolver.hpp:172-187): Ax = A x with the full symmetric matrix. */
+int oc_solver_multiply(oc_ctx* ctx, const double* x, double* Ax);
+int oc_solver_set_params(oc_ctx* ctx, int precond, double relTol, int maxIter);
+int oc_solver_last_stats(oc_ctx* ctx, int* iters, double* relResidual);
+
+/* ---- device-resident Newton driver (Optimizer.cpp:154-261,505-704) ------- */
+typedef struct {
+    double E_total;   /* lastEnergyVal after the step: a0*E_SD + scaffold term */
+    double E_SD;      /* mesh SD energy (energyVal_ET[0]) */
+    double E_scaf;    /* scaffold term (energyVal_scaffold) */
+    double grad_sq;   /* ||g||^2 of the combined gradient at the start of the iteration */
+    double step;      /* accepted step size */
+    int halvings;     /* step halvings taken in the line search */
+    int stopped;      /* 1 if the line search or the relative-decrease test stopped the solve */
+    int converged;    /* 1 if ||g||^2 < targetGRes (no step taken) */
+    int pcg_iters;    /* PCG iterations of this Newton step */
+    double pcg_rel_res;
+} oc_iter_stats;
+
+/* Optimizer::precompute (Optimizer.cpp:154-201) for energyParams = {a0}: sets targetGRes =
+ * a0 (nV-nFixed)/nV relGL2Tol (Optimizer.cpp:675-678), builds the combined pattern (mesh + air mesh) and
+ * evaluates the initial energy (*E0 = a0 E_SD + scaffold term). allowEDecRelTol: Optimizer::setAllowEDecRelTol.
+ * pcgRelTol/pcgMaxIter/precond select the Newton-system solve. */
+int oc_newton_init(oc_ctx* ctx, double a0, double relGL2Tol, int allowEDecRelTol, int precond, double pcgRelTol,
+                   int pcgMaxIter, double* E0);
+/* One Optimizer::solve(1) iteration (Optimizer.cpp:206-231 with solve_oneStep :505-573 and lineSearch
+ * :575-652): gradient and convergence test, Hessian assembly, PCG direction, injectivity bound, line search,
+ * step. *done = 1 when the solve returned 1 (gradient tolerance reached or line search stopped). */
+int oc_newton_step(oc_ctx* ctx, oc_iter_stats* stats, int* done);
+
+/* ---- timing of the dominant kernels (HIP events on the context stream) ---- */
+/* Average device time in ms (HIP events around each launch) of `reps` launches of: 0 element pass (E, g,
+ * projected H), 1 block assembly, 2 one PCG iteration (both kernels), 3 line-search candidates, 4 step bound,
+ * 5 PCG SpMV kernel, 6 PCG update kernel. Needs oc_newton_init; the current search direction is kept. */
+int oc_time_kernel(oc_ctx* ctx, int which, int reps, double* ms);
+/* Sizes used for algorithmic-byte accounting: triangles, vertices (combined), stored blocks, upper blocks. */
+int oc_system_sizes(oc_ctx* ctx, long long* nF, long long* nV, long long* nnzBlocks, long long* nUpperBlocks);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPTCUTS_GPU_H */

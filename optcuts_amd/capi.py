@@ -1,0 +1,271 @@
+"""ctypes binding of liboptcuts_gpu.so (the C ABI in include/optcuts_gpu.h).
+
+The library is the product path: every call below runs HIP kernels on the
+GPU. There is no CPU fallback; loading fails loudly when the library is not
+built or no gfx950 device is present.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "liboptcuts_gpu.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "optcuts_gpu.h")
+
+OC_OK = 0
+STATUS = {
+    0: "OC_OK", -1: "OC_ERR_ARG", -2: "OC_ERR_HIP", -3: "OC_ERR_STATE", -4: "OC_ERR_PATTERN",
+    -5: "OC_ERR_INVERTED", -6: "OC_ERR_DEGENERATE", -7: "OC_ERR_NOT_CONVERGED", -8: "OC_ERR_BREAKDOWN",
+    -9: "OC_ERR_NO_DEVICE",
+}
+PRECOND_JACOBI = 0
+PRECOND_BLOCK_JACOBI = 1
+
+
+class OCError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS.get(status, status), msg))
+        self.status = status
+
+
+class IterStats(C.Structure):
+    _fields_ = [("E_total", C.c_double), ("E_SD", C.c_double), ("E_scaf", C.c_double), ("grad_sq", C.c_double),
+                ("step", C.c_double), ("halvings", C.c_int), ("stopped", C.c_int), ("converged", C.c_int),
+                ("pcg_iters", C.c_int), ("pcg_rel_res", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_ctx = C.c_void_p
+
+_SIGS = {
+    "oc_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "oc_ctx_destroy": (C.c_int, [_ctx]),
+    "oc_last_error": (C.c_char_p, [_ctx]),
+    "oc_device_arch": (C.c_int, [_ctx, C.c_char_p, C.c_int]),
+    "oc_set_mesh": (C.c_int, [_ctx, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, C.c_double]),
+    "oc_set_uv": (C.c_int, [_ctx, _dp]),
+    "oc_get_uv": (C.c_int, [_ctx, _dp]),
+    "oc_mesh_stats": (C.c_int, [_ctx, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "oc_energy": (C.c_int, [_ctx, C.c_int, C.POINTER(C.c_double)]),
+    "oc_energy_per_elem": (C.c_int, [_ctx, C.c_int, _dp]),
+    "oc_gradient": (C.c_int, [_ctx, C.c_int, _dp]),
+    "oc_hessian_triplets": (C.c_int, [_ctx, C.c_int, C.c_longlong, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
+    "oc_init_step_size": (C.c_int, [_ctx, _dp, C.POINTER(C.c_double)]),
+    "oc_check_inversion": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
+    "oc_set_air_mesh": (C.c_int, [_ctx, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_double]),
+    "oc_solver_set_pattern": (C.c_int, [_ctx, C.c_int, _ip, _ip, C.c_int, _ip, C.POINTER(C.c_int)]),
+    "oc_solver_get_pattern": (C.c_int, [_ctx, _ip, _ip]),
+    "oc_solver_update_a": (C.c_int, [_ctx, C.c_longlong, _ip, _ip, _dp]),
+    "oc_solver_get_a": (C.c_int, [_ctx, _dp]),
+    "oc_solver_analyze_pattern": (C.c_int, [_ctx]),
+    "oc_solver_factorize": (C.c_int, [_ctx]),
+    "oc_solver_solve": (C.c_int, [_ctx, _dp, _dp]),
+    "oc_solver_multiply": (C.c_int, [_ctx, _dp, _dp]),
+    "oc_solver_set_params": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int]),
+    "oc_solver_last_stats": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "oc_newton_init": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
+                                 C.POINTER(C.c_double)]),
+    "oc_newton_step": (C.c_int, [_ctx, C.POINTER(IterStats), C.POINTER(C.c_int)]),
+    "oc_time_kernel": (C.c_int, [_ctx, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "oc_system_sizes": (C.c_int, [_ctx] + [C.POINTER(C.c_longlong)] * 4),
+}
+
+_lib = None
+
+
+def header_functions():
+    """Names of every function the public header declares."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(oc_\w+)\s*\(", text, flags=re.M)))
+
+
+def load():
+    """Load the library (no GPU call is made here)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("liboptcuts_gpu.so is not built (run optcuts_amd.build): " + LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device context (one parameterization problem) on GPU `device`."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = C.c_void_p()
+        st = self.L.oc_ctx_create(device, C.byref(h))
+        if st != OC_OK:
+            raise OCError(st, "no usable GPU device %d" % device)
+        self.h = h
+        self.nV = self.nF = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st, ok=(OC_OK,)):
+        if st not in ok:
+            raise OCError(st, self.L.oc_last_error(self.h).decode())
+        return st
+
+    def arch(self):
+        buf = C.create_string_buffer(64)
+        self._chk(self.L.oc_device_arch(self.h, buf, 64))
+        return buf.value.decode()
+
+    # ---- mesh / Energy ------------------------------------------------------
+    def set_mesh(self, Vrest, F, fixed=(0,), area_thres=0.0):
+        Vrest, F = _f64(Vrest), _i32(F)
+        fx = _i32(np.array(fixed, dtype=np.int32).reshape(-1))
+        self._chk(self.L.oc_set_mesh(self.h, Vrest.shape[0], Vrest, F.shape[0], F, len(fx), fx, float(area_thres)))
+        self.nV, self.nF = Vrest.shape[0], F.shape[0]
+
+    def set_uv(self, UV):
+        UV = _f64(UV)
+        assert UV.shape == (self.nV, 2)
+        self._chk(self.L.oc_set_uv(self.h, UV))
+
+    def get_uv(self):
+        out = np.zeros((self.nV, 2))
+        self._chk(self.L.oc_get_uv(self.h, out))
+        return out
+
+    def mesh_stats(self):
+        a, e = C.c_double(), C.c_double()
+        self._chk(self.L.oc_mesh_stats(self.h, C.byref(a), C.byref(e)))
+        return a.value, e.value
+
+    def energy(self, uniform=False):
+        e = C.c_double()
+        self._chk(self.L.oc_energy(self.h, int(uniform), C.byref(e)))
+        return e.value
+
+    def energy_per_elem(self, uniform=False):
+        out = np.zeros(self.nF)
+        self._chk(self.L.oc_energy_per_elem(self.h, int(uniform), out))
+        return out
+
+    def gradient(self, uniform=False):
+        out = np.zeros(2 * self.nV)
+        self._chk(self.L.oc_gradient(self.h, int(uniform), out))
+        return out
+
+    def hessian_triplets(self, uniform=False):
+        n = C.c_longlong()
+        self._chk(self.L.oc_hessian_triplets(self.h, int(uniform), 0, None, None, None, C.byref(n)))
+        I = np.zeros(n.value, np.int32)
+        J = np.zeros(n.value, np.int32)
+        V = np.zeros(n.value)
+        self._chk(self.L.oc_hessian_triplets(self.h, int(uniform), n.value, I.ctypes.data, J.ctypes.data,
+                                             V.ctypes.data, C.byref(n)))
+        return I, J, V
+
+    def init_step_size(self, d, step=1.0):
+        s = C.c_double(step)
+        self._chk(self.L.oc_init_step_size(self.h, _f64(d), C.byref(s)))
+        return s.value
+
+    def check_inversion(self):
+        ok = C.c_int()
+        self._chk(self.L.oc_check_inversion(self.h, C.byref(ok)))
+        return bool(ok.value)
+
+    def set_air_mesh(self, airUV, Fair, bnd, area_thres, w_scaf):
+        airUV, Fair, bnd = _f64(airUV), _i32(Fair), _i32(bnd)
+        self._chk(self.L.oc_set_air_mesh(self.h, airUV.shape[0], airUV, Fair.shape[0], Fair, len(bnd), bnd,
+                                         float(area_thres), float(w_scaf)))
+
+    # ---- LinSysSolver --------------------------------------------------------
+    def solver_set_pattern(self, nV, nbr_ptr, nbr, fixed=(0,)):
+        fx = _i32(np.array(fixed, dtype=np.int32).reshape(-1))
+        nnz = C.c_int()
+        self._chk(self.L.oc_solver_set_pattern(self.h, nV, _i32(nbr_ptr), _i32(nbr), len(fx), fx, C.byref(nnz)))
+        self._sol_n, self._sol_nnz = 2 * nV, nnz.value
+        ia = np.zeros(2 * nV + 1, np.int32)
+        ja = np.zeros(nnz.value, np.int32)
+        self._chk(self.L.oc_solver_get_pattern(self.h, ia, ja))
+        return ia, ja
+
+    def solver_update_a(self, I, J, V):
+        self._chk(self.L.oc_solver_update_a(self.h, len(I), _i32(I), _i32(J), _f64(V)))
+
+    def solver_get_a(self):
+        a = np.zeros(self._sol_nnz)
+        self._chk(self.L.oc_solver_get_a(self.h, a))
+        return a
+
+    def solver_set_params(self, precond=PRECOND_JACOBI, rel_tol=1e-10, max_iter=100000):
+        self._chk(self.L.oc_solver_set_params(self.h, int(precond), float(rel_tol), int(max_iter)))
+
+    def solver_analyze_pattern(self):
+        self._chk(self.L.oc_solver_analyze_pattern(self.h))
+
+    def solver_factorize(self):
+        self._chk(self.L.oc_solver_factorize(self.h))
+
+    def solver_solve(self, rhs, allow_not_converged=False):
+        x = np.zeros(self._sol_n)
+        ok = (OC_OK, -7) if allow_not_converged else (OC_OK,)
+        self._chk(self.L.oc_solver_solve(self.h, _f64(rhs), x), ok)
+        return x
+
+    def solver_multiply(self, x):
+        y = np.zeros(self._sol_n)
+        self._chk(self.L.oc_solver_multiply(self.h, _f64(x), y))
+        return y
+
+    def solver_last_stats(self):
+        it, rr = C.c_int(), C.c_double()
+        self._chk(self.L.oc_solver_last_stats(self.h, C.byref(it), C.byref(rr)))
+        return it.value, rr.value
+
+    # ---- Newton driver -------------------------------------------------------
+    def newton_init(self, a0=1.0, rel_gl2_tol=1e-12, allow_edec_rel_tol=True, precond=PRECOND_JACOBI,
+                    pcg_rel_tol=1e-10, pcg_max_iter=20000):
+        e = C.c_double()
+        self._chk(self.L.oc_newton_init(self.h, float(a0), float(rel_gl2_tol), int(allow_edec_rel_tol),
+                                        int(precond), float(pcg_rel_tol), int(pcg_max_iter), C.byref(e)))
+        return e.value
+
+    def newton_step(self):
+        s = IterStats()
+        done = C.c_int()
+        self._chk(self.L.oc_newton_step(self.h, C.byref(s), C.byref(done)))
+        return s.as_dict(), bool(done.value)
+
+    def time_kernel(self, which, reps=20):
+        ms = C.c_double()
+        self._chk(self.L.oc_time_kernel(self.h, int(which), int(reps), C.byref(ms)))
+        return ms.value
+
+    def system_sizes(self):
+        v = [C.c_longlong() for _ in range(4)]
+        self._chk(self.L.oc_system_sizes(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)

@@ -1,0 +1,1352 @@
+// Host side of the C ABI declared in include/optcuts_gpu.h: context, HBM
+// layout, sparsity-pattern and slot-map construction, and the
+// device-resident Newton step. Kernels live in oc_kernels.hip.
+#include "../../include/optcuts_gpu.h"
+#include "oc_common.hpp"
+#include "oc_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace {
+
+using oc::PcgCtrl;
+using oc::TriSet;
+
+// Device allocation owned by the context. Grows, never shrinks.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes && p) return hipSuccess;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        const size_t nb = b < 256 ? 256 : b;
+        hipError_t e = hipMalloc(&p, nb);
+        if (e == hipSuccess) bytes = nb;
+        return e;
+    }
+    template <typename T>
+    T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+};
+
+// Full symmetric 2x2-block CSR over nV vertices.
+struct BlockPattern {
+    int nV = 0;
+    std::vector<int> rowPtr, colIdx, diagPos;
+    DevBuf dRowPtr, dColIdx, dDiagPos, dVals, dMinv;
+    long nnzb() const { return colIdx.size(); }
+    int find(int u, int w) const
+    {
+        const int* b = colIdx.data() + rowPtr[u];
+        const int* e = colIdx.data() + rowPtr[u + 1];
+        const int* p = std::lower_bound(b, e, w);
+        return (p != e && *p == w) ? int(p - colIdx.data()) : -1;
+    }
+};
+
+}  // namespace
+
+struct oc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    char arch[64] = {0};
+
+    // ---- mesh block (block 0) -----------------------------------------------
+    int nV = 0, nF = 0;
+    std::vector<int> Fh;            // nF x 3
+    std::vector<int> fixedSorted;   // mesh fixed vertices, ascending
+    std::vector<unsigned char> fixedMaskMesh;
+    double surfaceArea = 0.0, avgEdgeLen = 0.0;
+    // mesh-only gather lists (Energy interface)
+    std::vector<int> mvcPtr, mvcSrc;
+    DevBuf dMvcPtr, dMvcSrc;
+    std::vector<long long> tripOff;  // per mesh triangle
+    std::vector<unsigned char> vFree;
+    long long nTripElem = 0;
+    DevBuf dTripOff, dVFree;
+
+    // ---- air block (block 1) -------------------------------------------------
+    int nVair = 0, nFair = 0, nBnd = 0;
+    double wScaf = 0.0;
+    std::vector<int> FairGlobal;        // nFair x 3 in combined numbering
+    std::vector<unsigned char> airFixedGlobal;  // per combined vertex: fixed in the air mesh
+
+    // ---- combined triangle storage (mesh then air) ----------------------------
+    DevBuf dF0, dF1, dF2, dTriArea, dA2, dE0, dE1, dE01;
+    DevBuf dVrest, dTmpVrest, dTmpF;
+    DevBuf dMeshSums, dAirSums;  // 3 doubles each
+    DevBuf dUV;                  // combined vertices x 2
+    DevBuf dFixedMaskMesh;
+
+    // scratch
+    DevBuf dPartials, dCounters, dScalars, dOrdMin, dLsOut, dEelem, dElemG, dElemH, dG, dTmpVec, dDir;
+    DevBuf dTrI, dTrJ, dTrV;
+
+    // ---- Newton system (combined) ---------------------------------------------
+    bool sysReady = false;
+    int nVt = 0, nFt = 0;
+    BlockPattern sys;
+    std::vector<int> ubRow, ubCol, ubPosU, ubPosL, ubCPtr, ubCSrc, vcPtr, vcSrc;
+    std::vector<unsigned char> fixedMaskSys;
+    std::vector<double> fixedDiag;
+    DevBuf dUbRow, dUbCol, dUbPosU, dUbPosL, dUbCPtr, dUbCSrc, dVcPtr, dVcSrc, dFixedMaskSys, dFixedDiag;
+    double a0 = 1.0, relGL2Tol = 1e-12, targetGRes = 0.0;
+    int allowEDecRelTol = 1;
+    double lastE = 0.0, lastESD = 0.0, lastEScaf = 0.0;
+    int nPrecond = OC_PRECOND_JACOBI;
+    double pcgTol = 1e-10;
+    int pcgMaxIter = 10000;
+
+    // ---- PCG workspace ---------------------------------------------------------
+    DevBuf dX, dR, dZ, dP0, dP1, dQ, dB, dCtrl, dPcgPart;
+    int lastIters = 0;
+    double lastRelRes = 0.0;
+
+    // ---- LinSysSolver drop-in ---------------------------------------------------
+    BlockPattern sol;
+    int solN = 0;
+    std::vector<int> solIa, solJa, solDst, solMir;
+    std::vector<double> solA;
+    DevBuf dSolA, dSolDst, dSolMir, dSolB, dSolX;
+    int solPrecond = OC_PRECOND_JACOBI;
+    double solTol = 1e-10;
+    int solMaxIter = 100000;
+    bool solFactorized = false;
+
+    const char* fail(const std::string& m)
+    {
+        err = m;
+        return err.c_str();
+    }
+};
+
+#define OC_HIP(ctx, call)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                     \
+            return OC_ERR_HIP;                                                                  \
+        }                                                                                       \
+    } while (0)
+
+#define OC_ARG(ctx, cond, msg)           \
+    do {                                 \
+        if (!(cond)) {                   \
+            (ctx)->err = (msg);          \
+            return OC_ERR_ARG;           \
+        }                                \
+    } while (0)
+
+namespace {
+
+constexpr int kMaxPartialBlocks = 1 << 16;
+
+int ensure_scratch(oc_ctx* c, long nItems)
+{
+    const long blocks = oc::grid_for(nItems);
+    // 3*kLSCand compensated pairs per block is the widest publisher
+    OC_HIP(c, c->dPartials.ensure(sizeof(double) * (size_t)blocks * 2 * 3 * oc::kLSCand + 4096));
+    if (!c->dCounters.p) {
+        OC_HIP(c, c->dCounters.ensure(sizeof(unsigned int) * 64));
+        OC_HIP(c, hipMemsetAsync(c->dCounters.p, 0, c->dCounters.bytes, c->stream));
+    }
+    if (!c->dScalars.p) {
+        OC_HIP(c, c->dScalars.ensure(sizeof(double) * 64));
+        OC_HIP(c, hipMemsetAsync(c->dScalars.p, 0, c->dScalars.bytes, c->stream));
+    }
+    OC_HIP(c, c->dOrdMin.ensure(sizeof(unsigned long long) * 8));
+    OC_HIP(c, c->dLsOut.ensure(sizeof(double) * 4 * oc::kLSCand));
+    (void)kMaxPartialBlocks;
+    return OC_OK;
+}
+
+// counters: [0] features mesh, [1] features air, [2] element esum, [3] gather grad,
+//           [4] line search, [5] inversion count (plain counter)
+unsigned int* counter(oc_ctx* c, int k) { return c->dCounters.as<unsigned int>() + k; }
+// scalars: [0..2] scratch sums, [4..5] esum, [6] gsq
+double* scalar(oc_ctx* c, int k) { return c->dScalars.as<double>() + k; }
+
+TriSet mesh_set(oc_ctx* c, int uniform)
+{
+    TriSet T;
+    T.nF = c->nF;
+    T.nFmesh = c->nF;
+    T.f0 = c->dF0.as<int>();
+    T.f1 = c->dF1.as<int>();
+    T.f2 = c->dF2.as<int>();
+    T.triArea = c->dTriArea.as<double>();
+    T.A2 = c->dA2.as<double>();
+    T.E0 = c->dE0.as<double>();
+    T.E1 = c->dE1.as<double>();
+    T.E01 = c->dE01.as<double>();
+    T.surfaceArea = c->dMeshSums.as<double>();
+    T.uniformMesh = uniform;
+    T.sMesh = 1.0;
+    T.sAir = 1.0;
+    return T;
+}
+
+TriSet system_set(oc_ctx* c)
+{
+    TriSet T = mesh_set(c, 0);
+    T.nF = c->nF + c->nFair;
+    T.sMesh = c->a0;
+    T.sAir = c->nFair ? c->wScaf / (double)c->nFair : 0.0;
+    return T;
+}
+
+// vertex -> (triangle*3 + corner) lists, triangles ascending
+void build_vertex_lists(int nV, int nF, const int* F, std::vector<int>& ptr, std::vector<int>& src)
+{
+    ptr.assign(nV + 1, 0);
+    for (int t = 0; t < nF; ++t)
+        for (int k = 0; k < 3; ++k) ptr[F[3 * t + k] + 1]++;
+    for (int v = 0; v < nV; ++v) ptr[v + 1] += ptr[v];
+    src.resize(ptr[nV]);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int t = 0; t < nF; ++t)
+        for (int k = 0; k < 3; ++k) src[fill[F[3 * t + k]]++] = 3 * t + k;
+}
+
+template <typename T>
+int upload(oc_ctx* c, DevBuf& d, const std::vector<T>& h)
+{
+    OC_HIP(c, d.ensure(sizeof(T) * (h.size() ? h.size() : 1)));
+    if (!h.empty()) OC_HIP(c, hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
+    return OC_OK;
+}
+
+// Build the full block pattern from a per-vertex neighbour relation (each
+// row sorted, unique, including the diagonal) restricted by the fixed mask:
+// fixed rows keep only the diagonal, free rows drop fixed columns.
+void build_block_pattern(BlockPattern& P, int nV, const std::vector<std::vector<int>>& rows,
+                         const std::vector<unsigned char>& fixedMask)
+{
+    P.nV = nV;
+    P.rowPtr.assign(nV + 1, 0);
+    P.colIdx.clear();
+    P.diagPos.assign(nV, -1);
+    for (int u = 0; u < nV; ++u) {
+        if (fixedMask[u]) {
+            P.diagPos[u] = (int)P.colIdx.size();
+            P.colIdx.push_back(u);
+        } else {
+            for (int w : rows[u]) {
+                if (w != u && fixedMask[w]) continue;
+                if (w == u) P.diagPos[u] = (int)P.colIdx.size();
+                P.colIdx.push_back(w);
+            }
+        }
+        P.rowPtr[u + 1] = (int)P.colIdx.size();
+    }
+}
+
+int upload_pattern(oc_ctx* c, BlockPattern& P)
+{
+    if (upload(c, P.dRowPtr, P.rowPtr)) return OC_ERR_HIP;
+    if (upload(c, P.dColIdx, P.colIdx)) return OC_ERR_HIP;
+    if (upload(c, P.dDiagPos, P.diagPos)) return OC_ERR_HIP;
+    OC_HIP(c, P.dVals.ensure(sizeof(double) * 4 * (size_t)std::max<long>(1, P.nnzb())));
+    OC_HIP(c, P.dMinv.ensure(sizeof(double) * 4 * (size_t)std::max(1, P.nV)));
+    return OC_OK;
+}
+
+int ensure_pcg(oc_ctx* c, int nV)
+{
+    const size_t b = sizeof(double) * 2 * (size_t)std::max(1, nV);
+    OC_HIP(c, c->dX.ensure(b));
+    OC_HIP(c, c->dR.ensure(b));
+    OC_HIP(c, c->dZ.ensure(b));
+    OC_HIP(c, c->dP0.ensure(b));
+    OC_HIP(c, c->dP1.ensure(b));
+    OC_HIP(c, c->dQ.ensure(b));
+    OC_HIP(c, c->dB.ensure(b));
+    if (!c->dCtrl.p) {
+        OC_HIP(c, c->dCtrl.ensure(sizeof(PcgCtrl)));
+        OC_HIP(c, hipMemsetAsync(c->dCtrl.p, 0, sizeof(PcgCtrl), c->stream));
+    }
+    OC_HIP(c, c->dPcgPart.ensure(sizeof(double) * (4 * (size_t)oc::pcg_grid_update(nV) +
+                                                   (size_t)oc::pcg_grid_spmv(nV))));
+    return ensure_scratch(c, 2L * nV);
+}
+
+// PCG on pattern P for right-hand side b (device), solution in c->dX.
+// Iterations are queued in batches; ctrl is read back between batches.
+int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxIter, int& iters, double& relRes)
+{
+    const int nV = P.nV;
+    PcgCtrl* ctrl = c->dCtrl.as<PcgCtrl>();
+    PcgCtrl h{};
+    h.tol2 = relTol * relTol;
+    h.maxit = maxIter;
+    OC_HIP(c, hipMemcpyAsync(&ctrl->tol2, &h.tol2, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    OC_HIP(c, hipMemcpyAsync(&ctrl->maxit, &h.maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const int GU = oc::pcg_grid_update(nV);
+    double* partU[2] = {c->dPcgPart.as<double>(), c->dPcgPart.as<double>() + 2 * (size_t)GU};
+    double* partS = c->dPcgPart.as<double>() + 4 * (size_t)GU;
+    oc::launch_pcg_init(c->stream, nV, b, P.dMinv.as<double>(), c->dX.as<double>(), c->dR.as<double>(),
+                        c->dZ.as<double>(), partU[0], ctrl);
+    double* pbuf[2] = {c->dP0.as<double>(), c->dP1.as<double>()};
+    int it = 0;
+    int batch = 32;
+    for (;;) {
+        for (int k = 0; k < batch; ++k, ++it) {
+            double* pOld = pbuf[it & 1];
+            double* pNew = pbuf[(it + 1) & 1];
+            oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.as<double>(),
+                                c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), partU[it & 1], partS, ctrl);
+            oc::launch_pcg_update(c->stream, it, nV, pNew, c->dQ.as<double>(), P.dMinv.as<double>(),
+                                  c->dX.as<double>(), c->dR.as<double>(), c->dZ.as<double>(), partS,
+                                  partU[(it + 1) & 1], ctrl);
+        }
+        OC_HIP(c, hipGetLastError());
+        OC_HIP(c, hipMemcpyAsync(&h, ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, c->stream));
+        OC_HIP(c, hipStreamSynchronize(c->stream));
+        if (h.done) break;
+        if (batch < 512) batch *= 2;
+    }
+    iters = h.iter;
+    relRes = h.bb > 0.0 ? std::sqrt(h.rr / h.bb) : 0.0;
+    if (h.done == 3) {
+        c->err = "PCG breakdown (p^T A p <= 0)";
+        return OC_ERR_BREAKDOWN;
+    }
+    if (h.done == 2) {
+        c->err = "PCG reached maxIter before the tolerance";
+        return OC_ERR_NOT_CONVERGED;
+    }
+    return OC_OK;
+}
+
+int ensure_tri_storage(oc_ctx* c, int nFtot)
+{
+    const size_t i4 = sizeof(int) * (size_t)std::max(1, nFtot), d8 = sizeof(double) * (size_t)std::max(1, nFtot);
+    // grow preserving the mesh part
+    auto grow = [&](DevBuf& b, size_t bytes, size_t keep) -> hipError_t {
+        if (b.p && b.bytes >= bytes) return hipSuccess;
+        DevBuf nb;
+        hipError_t e = nb.ensure(bytes);
+        if (e != hipSuccess) return e;
+        if (b.p && keep) {
+            e = hipMemcpyAsync(nb.p, b.p, std::min(keep, b.bytes), hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) return e;
+            e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return e;
+        }
+        std::swap(b.p, nb.p);
+        std::swap(b.bytes, nb.bytes);
+        return hipSuccess;
+    };
+    const size_t ki = sizeof(int) * (size_t)c->nF, kd = sizeof(double) * (size_t)c->nF;
+    OC_HIP(c, grow(c->dF0, i4, ki));
+    OC_HIP(c, grow(c->dF1, i4, ki));
+    OC_HIP(c, grow(c->dF2, i4, ki));
+    OC_HIP(c, grow(c->dTriArea, d8, kd));
+    OC_HIP(c, grow(c->dA2, d8, kd));
+    OC_HIP(c, grow(c->dE0, d8, kd));
+    OC_HIP(c, grow(c->dE1, d8, kd));
+    OC_HIP(c, grow(c->dE01, d8, kd));
+    return OC_OK;
+}
+
+int upload_tri_ids(oc_ctx* c, int offset, int n, const int* F)
+{
+    std::vector<int> a(n), b(n), d(n);
+    for (int t = 0; t < n; ++t) {
+        a[t] = F[3 * t];
+        b[t] = F[3 * t + 1];
+        d[t] = F[3 * t + 2];
+    }
+    if (n == 0) return OC_OK;
+    OC_HIP(c, hipMemcpyAsync(c->dF0.as<int>() + offset, a.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    OC_HIP(c, hipMemcpyAsync(c->dF1.as<int>() + offset, b.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    OC_HIP(c, hipMemcpyAsync(c->dF2.as<int>() + offset, d.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));  // host vectors go out of scope
+    return OC_OK;
+}
+
+int energy_sums(oc_ctx* c, const TriSet& T, double* Emesh, double* Eair)
+{
+    if (ensure_scratch(c, T.nF)) return OC_ERR_HIP;
+    oc::launch_element(c->stream, T, c->dUV.as<double>(), nullptr, nullptr, nullptr, 0, c->dPartials.as<double>(),
+                       scalar(c, 4));
+    OC_HIP(c, hipGetLastError());
+    double e[2];
+    OC_HIP(c, hipMemcpyAsync(e, scalar(c, 4), sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    *Emesh = e[0];
+    *Eair = e[1];
+    return OC_OK;
+}
+
+int count_inverted(oc_ctx* c, const TriSet& T, unsigned int* n)
+{
+    OC_HIP(c, hipMemsetAsync(counter(c, 5), 0, sizeof(unsigned int), c->stream));
+    oc::launch_count_inverted(c->stream, T, c->dUV.as<double>(), counter(c, 5));
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipMemcpyAsync(n, counter(c, 5), sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int step_bound(oc_ctx* c, const TriSet& T, const double* dDir, double cap, double* out)
+{
+    unsigned long long* om = c->dOrdMin.as<unsigned long long>();
+    oc::launch_set_u64(c->stream, om, oc::dbl_to_ordered_host(cap));
+    oc::launch_step_bound(c->stream, T, c->dUV.as<double>(), dDir, om, cap);
+    OC_HIP(c, hipGetLastError());
+    unsigned long long r;
+    OC_HIP(c, hipMemcpyAsync(&r, om, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    *out = oc::ordered_to_dbl_host(r);
+    return OC_OK;
+}
+
+// ---- Newton system pattern ------------------------------------------------------
+int build_system(oc_ctx* c)
+{
+    const int nVt = c->nV + (c->nFair ? c->nVair - c->nBnd : 0);
+    const int nFt = c->nF + c->nFair;
+    c->nVt = nVt;
+    c->nFt = nFt;
+    std::vector<int> Ft(3 * (size_t)nFt);
+    std::copy(c->Fh.begin(), c->Fh.end(), Ft.begin());
+    std::copy(c->FairGlobal.begin(), c->FairGlobal.end(), Ft.begin() + 3 * (size_t)c->nF);
+
+    c->fixedMaskSys.assign(nVt, 0);
+    for (int v : c->fixedSorted) c->fixedMaskSys[v] = 1;
+    const double sAir = c->nFair ? c->wScaf / (double)c->nFair : 0.0;
+    c->fixedDiag.assign(nVt, 0.0);
+    for (int v : c->fixedSorted) {
+        c->fixedDiag[v] = c->a0;
+        if (c->nFair && !c->airFixedGlobal.empty() && c->airFixedGlobal[v]) c->fixedDiag[v] += sAir;
+    }
+    // neighbour rows (with diagonal)
+    std::vector<std::vector<int>> rows(nVt);
+    for (int t = 0; t < nFt; ++t)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) rows[Ft[3 * t + a]].push_back(Ft[3 * t + b]);
+    for (int v = 0; v < nVt; ++v) {
+        auto& r = rows[v];
+        r.push_back(v);
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+    }
+    build_block_pattern(c->sys, nVt, rows, c->fixedMaskSys);
+    rows.clear();
+    rows.shrink_to_fit();
+    // upper blocks
+    c->ubRow.clear();
+    c->ubCol.clear();
+    c->ubPosU.clear();
+    c->ubPosL.clear();
+    std::vector<int> posToUB(c->sys.nnzb(), -1);
+    for (int u = 0; u < nVt; ++u)
+        for (int k = c->sys.rowPtr[u]; k < c->sys.rowPtr[u + 1]; ++k) {
+            const int w = c->sys.colIdx[k];
+            if (w < u) continue;
+            posToUB[k] = (int)c->ubRow.size();
+            c->ubRow.push_back(u);
+            c->ubCol.push_back(w);
+            c->ubPosU.push_back(k);
+            c->ubPosL.push_back(w == u ? -1 : c->sys.find(w, u));
+        }
+    const int nUB = (int)c->ubRow.size();
+    // contributions per upper block, triangle order
+    c->ubCPtr.assign(nUB + 1, 0);
+    auto for_each_contrib = [&](auto&& fn) {
+        for (int t = 0; t < nFt; ++t) {
+            const int* f = &Ft[3 * (size_t)t];
+            for (int a = 0; a < 3; ++a) {
+                if (c->fixedMaskSys[f[a]]) continue;
+                for (int b = 0; b < 3; ++b) {
+                    if (c->fixedMaskSys[f[b]]) continue;
+                    if (f[a] > f[b]) continue;
+                    if (f[a] == f[b] && a != b) continue;
+                    // element block (a,b) oriented rows f[a], cols f[b]
+                    const int lo = a < b ? a : b, hi = a < b ? b : a;
+                    const int id = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+                    const unsigned int code = (unsigned int)(6 * t + id) | (a > b ? 0x80000000u : 0u);
+                    const int ub = posToUB[c->sys.find(f[a], f[b])];
+                    fn(ub, code);
+                }
+            }
+        }
+    };
+    for_each_contrib([&](int ub, unsigned int) { c->ubCPtr[ub + 1]++; });
+    for (int b = 0; b < nUB; ++b) c->ubCPtr[b + 1] += c->ubCPtr[b];
+    c->ubCSrc.assign(c->ubCPtr[nUB], 0);
+    {
+        std::vector<int> fill(c->ubCPtr.begin(), c->ubCPtr.end() - 1);
+        for_each_contrib([&](int ub, unsigned int code) { c->ubCSrc[fill[ub]++] = (int)code; });
+    }
+    build_vertex_lists(nVt, nFt, Ft.data(), c->vcPtr, c->vcSrc);
+
+    if (upload_pattern(c, c->sys)) return OC_ERR_HIP;
+    if (upload(c, c->dUbRow, c->ubRow) || upload(c, c->dUbCol, c->ubCol) || upload(c, c->dUbPosU, c->ubPosU) ||
+        upload(c, c->dUbPosL, c->ubPosL) || upload(c, c->dUbCPtr, c->ubCPtr) || upload(c, c->dUbCSrc, c->ubCSrc) ||
+        upload(c, c->dVcPtr, c->vcPtr) || upload(c, c->dVcSrc, c->vcSrc) ||
+        upload(c, c->dFixedMaskSys, c->fixedMaskSys) || upload(c, c->dFixedDiag, c->fixedDiag))
+        return OC_ERR_HIP;
+    OC_HIP(c, c->dElemG.ensure(sizeof(double) * 6 * (size_t)std::max(1, nFt)));
+    OC_HIP(c, c->dElemH.ensure(sizeof(double) * 24 * (size_t)std::max(1, nFt)));
+    OC_HIP(c, c->dG.ensure(sizeof(double) * 2 * (size_t)std::max(1, nVt)));
+    if (ensure_pcg(c, nVt)) return OC_ERR_HIP;
+    if (ensure_scratch(c, std::max(nFt, nUB))) return OC_ERR_HIP;
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->sysReady = true;
+    return OC_OK;
+}
+
+int system_energy(oc_ctx* c, double* E, double* Emesh, double* Eair)
+{
+    TriSet T = system_set(c);
+    double em, ea;
+    if (int st = energy_sums(c, T, &em, &ea)) return st;
+    *Emesh = em;
+    *Eair = c->nFair ? ea * (c->wScaf / (double)c->nFair) : 0.0;
+    *E = c->a0 * em + *Eair;
+    return OC_OK;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+int oc_ctx_create(int device, oc_ctx** out)
+{
+    if (!out) return OC_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return OC_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return OC_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return OC_ERR_NO_DEVICE;
+    oc_ctx* c = new oc_ctx();
+    c->device = device;
+    std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return OC_ERR_HIP;
+    }
+    *out = c;
+    return OC_OK;
+}
+
+int oc_ctx_destroy(oc_ctx* c)
+{
+    if (!c) return OC_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return OC_OK;
+}
+
+const char* oc_last_error(const oc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int oc_device_arch(const oc_ctx* c, char* buf, int buflen)
+{
+    if (!c || !buf || buflen <= 0) return OC_ERR_ARG;
+    std::snprintf(buf, buflen, "%s", c->arch);
+    return OC_OK;
+}
+
+int oc_set_mesh(oc_ctx* c, int nV, const double* Vrest, int nF, const int* F, int nFixed, const int* fixedV,
+                double areaThres)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, nV > 0 && nF > 0 && Vrest && F && nFixed >= 0 && (nFixed == 0 || fixedV), "oc_set_mesh: bad sizes");
+    for (long k = 0; k < 3L * nF; ++k) OC_ARG(c, F[k] >= 0 && F[k] < nV, "oc_set_mesh: vertex index out of range");
+    for (int k = 0; k < nFixed; ++k) OC_ARG(c, fixedV[k] >= 0 && fixedV[k] < nV, "oc_set_mesh: fixed id out of range");
+    (void)hipSetDevice(c->device);
+    c->sysReady = false;
+    c->nFair = 0;
+    c->nVair = 0;
+    c->nBnd = 0;
+    c->FairGlobal.clear();
+    c->airFixedGlobal.clear();
+    c->nV = nV;
+    c->nF = nF;
+    c->Fh.assign(F, F + 3 * (size_t)nF);
+    c->fixedSorted.assign(fixedV, fixedV + nFixed);
+    std::sort(c->fixedSorted.begin(), c->fixedSorted.end());
+    c->fixedSorted.erase(std::unique(c->fixedSorted.begin(), c->fixedSorted.end()), c->fixedSorted.end());
+    c->fixedMaskMesh.assign(nV, 0);
+    for (int v : c->fixedSorted) c->fixedMaskMesh[v] = 1;
+
+    if (int st = ensure_tri_storage(c, nF)) return st;
+    if (int st = upload_tri_ids(c, 0, nF, F)) return st;
+    // features (areaThres applies to every triangle of this block)
+    OC_HIP(c, c->dVrest.ensure(sizeof(double) * 3 * (size_t)nV));
+    OC_HIP(c, hipMemcpyAsync(c->dVrest.p, Vrest, sizeof(double) * 3 * (size_t)nV, hipMemcpyHostToDevice, c->stream));
+    if (ensure_scratch(c, nF)) return OC_ERR_HIP;
+    OC_HIP(c, c->dMeshSums.ensure(sizeof(double) * 4));
+    oc::launch_rest_features(c->stream, nF, c->dF0.as<int>(), c->dF1.as<int>(), c->dF2.as<int>(),
+                             c->dVrest.as<double>(), areaThres, c->dTriArea.as<double>(), c->dA2.as<double>(),
+                             c->dE0.as<double>(), c->dE1.as<double>(), c->dE01.as<double>(),
+                             c->dPartials.as<double>(), c->dMeshSums.as<double>());
+    OC_HIP(c, hipGetLastError());
+    double sums[3];
+    OC_HIP(c, hipMemcpyAsync(sums, c->dMeshSums.p, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->surfaceArea = sums[0];
+    c->avgEdgeLen = sums[1] / (3.0 * nF);
+    if (sums[2] > 0.0) {
+        c->err = "a rest triangle has zero area";
+        return OC_ERR_DEGENERATE;
+    }
+    // mesh-only gather lists and reference triplet offsets
+    build_vertex_lists(nV, nF, F, c->mvcPtr, c->mvcSrc);
+    c->tripOff.resize(nF);
+    c->vFree.resize(nF);
+    long long off = 0;
+    for (int t = 0; t < nF; ++t) {
+        unsigned char m = 0;
+        int nfree = 0;
+        for (int k = 0; k < 3; ++k)
+            if (!c->fixedMaskMesh[F[3 * t + k]]) {
+                m |= (unsigned char)(1u << k);
+                ++nfree;
+            }
+        c->vFree[t] = m;
+        c->tripOff[t] = off;
+        off += 4LL * nfree * nfree;
+    }
+    c->nTripElem = off;
+    if (upload(c, c->dMvcPtr, c->mvcPtr) || upload(c, c->dMvcSrc, c->mvcSrc) || upload(c, c->dTripOff, c->tripOff) ||
+        upload(c, c->dVFree, c->vFree) || upload(c, c->dFixedMaskMesh, c->fixedMaskMesh))
+        return OC_ERR_HIP;
+    OC_HIP(c, c->dUV.ensure(sizeof(double) * 2 * (size_t)nV));
+    OC_HIP(c, hipMemsetAsync(c->dUV.p, 0, sizeof(double) * 2 * (size_t)nV, c->stream));
+    OC_HIP(c, c->dElemG.ensure(sizeof(double) * 6 * (size_t)nF));
+    OC_HIP(c, c->dG.ensure(sizeof(double) * 2 * (size_t)nV));
+    OC_HIP(c, c->dEelem.ensure(sizeof(double) * (size_t)nF));
+    OC_HIP(c, c->dDir.ensure(sizeof(double) * 2 * (size_t)nV));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int oc_set_uv(oc_ctx* c, const double* UV)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0, "oc_set_uv: no mesh");
+    OC_ARG(c, UV != nullptr, "oc_set_uv: null UV");
+    (void)hipSetDevice(c->device);
+    OC_HIP(c, hipMemcpyAsync(c->dUV.p, UV, sizeof(double) * 2 * (size_t)c->nV, hipMemcpyHostToDevice, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int oc_get_uv(oc_ctx* c, double* UV)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && UV, "oc_get_uv: no mesh");
+    (void)hipSetDevice(c->device);
+    OC_HIP(c, hipMemcpyAsync(UV, c->dUV.p, sizeof(double) * 2 * (size_t)c->nV, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int oc_mesh_stats(oc_ctx* c, double* surfaceArea, double* avgEdgeLen)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0, "oc_mesh_stats: no mesh");
+    if (surfaceArea) *surfaceArea = c->surfaceArea;
+    if (avgEdgeLen) *avgEdgeLen = c->avgEdgeLen;
+    return OC_OK;
+}
+
+int oc_energy(oc_ctx* c, int uniformWeight, double* E)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && E, "oc_energy: no mesh");
+    (void)hipSetDevice(c->device);
+    double em, ea;
+    if (int st = energy_sums(c, mesh_set(c, uniformWeight), &em, &ea)) return st;
+    *E = em;
+    return OC_OK;
+}
+
+int oc_energy_per_elem(oc_ctx* c, int uniformWeight, double* Eelem)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && Eelem, "oc_energy_per_elem: no mesh");
+    (void)hipSetDevice(c->device);
+    oc::launch_element(c->stream, mesh_set(c, uniformWeight), c->dUV.as<double>(), c->dEelem.as<double>(), nullptr,
+                       nullptr, 0, nullptr, nullptr);
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipMemcpyAsync(Eelem, c->dEelem.p, sizeof(double) * (size_t)c->nF, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int oc_gradient(oc_ctx* c, int uniformWeight, double* g)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && g, "oc_gradient: no mesh");
+    (void)hipSetDevice(c->device);
+    if (ensure_scratch(c, c->nV)) return OC_ERR_HIP;
+    oc::launch_element(c->stream, mesh_set(c, uniformWeight), c->dUV.as<double>(), nullptr, c->dElemG.as<double>(),
+                       nullptr, 0, nullptr, nullptr);
+    oc::launch_gather_grad(c->stream, c->nV, c->dMvcPtr.as<int>(), c->dMvcSrc.as<int>(),
+                           c->dFixedMaskMesh.as<unsigned char>(), c->dElemG.as<double>(), c->dG.as<double>(),
+                           c->dPartials.as<double>(), nullptr);
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipMemcpyAsync(g, c->dG.p, sizeof(double) * 2 * (size_t)c->nV, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int oc_hessian_triplets(oc_ctx* c, int uniformWeight, long long capacity, int* I, int* J, double* V,
+                        long long* count)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && count, "oc_hessian_triplets: no mesh");
+    const long long total = c->nTripElem + 2LL * (long long)c->fixedSorted.size();
+    *count = total;
+    if (!I) return OC_OK;
+    OC_ARG(c, J && V && capacity >= total, "oc_hessian_triplets: capacity too small");
+    (void)hipSetDevice(c->device);
+    const size_t ne = (size_t)std::max<long long>(1, c->nTripElem);
+    OC_HIP(c, c->dTrI.ensure(sizeof(int) * ne));
+    OC_HIP(c, c->dTrJ.ensure(sizeof(int) * ne));
+    OC_HIP(c, c->dTrV.ensure(sizeof(double) * ne));
+    oc::launch_triplets(c->stream, mesh_set(c, uniformWeight), c->dUV.as<double>(), c->dTripOff.as<long long>(),
+                        c->dVFree.as<unsigned char>(), c->dTrI.as<int>(), c->dTrJ.as<int>(), c->dTrV.as<double>());
+    OC_HIP(c, hipGetLastError());
+    if (c->nTripElem) {
+        OC_HIP(c, hipMemcpyAsync(I, c->dTrI.p, sizeof(int) * c->nTripElem, hipMemcpyDeviceToHost, c->stream));
+        OC_HIP(c, hipMemcpyAsync(J, c->dTrJ.p, sizeof(int) * c->nTripElem, hipMemcpyDeviceToHost, c->stream));
+        OC_HIP(c, hipMemcpyAsync(V, c->dTrV.p, sizeof(double) * c->nTripElem, hipMemcpyDeviceToHost, c->stream));
+    }
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    long long o = c->nTripElem;
+    for (int v : c->fixedSorted)
+        for (int d = 0; d < 2; ++d) {
+            I[o] = J[o] = 2 * v + d;
+            V[o] = 1.0;
+            ++o;
+        }
+    return OC_OK;
+}
+
+int oc_init_step_size(oc_ctx* c, const double* dir, double* stepSize)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && dir && stepSize, "oc_init_step_size: no mesh");
+    (void)hipSetDevice(c->device);
+    if (ensure_scratch(c, c->nF)) return OC_ERR_HIP;
+    OC_HIP(c, hipMemcpyAsync(c->dDir.p, dir, sizeof(double) * 2 * (size_t)c->nV, hipMemcpyHostToDevice, c->stream));
+    double r;
+    if (int st = step_bound(c, mesh_set(c, 0), c->dDir.as<double>(), *stepSize, &r)) return st;
+    *stepSize = r;
+    return OC_OK;
+}
+
+int oc_check_inversion(oc_ctx* c, int* ok)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && ok, "oc_check_inversion: no mesh");
+    (void)hipSetDevice(c->device);
+    if (ensure_scratch(c, c->nF)) return OC_ERR_HIP;
+    unsigned int n = 0;
+    if (int st = count_inverted(c, mesh_set(c, 0), &n)) return st;
+    *ok = (n == 0) ? 1 : 0;
+    return OC_OK;
+}
+
+int oc_set_air_mesh(oc_ctx* c, int nVair, const double* airUV, int nFair, const int* Fair, int nBnd, const int* bnd,
+                    double areaThres, double wScaf)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0, "oc_set_air_mesh: no mesh");
+    (void)hipSetDevice(c->device);
+    c->sysReady = false;
+    if (nFair == 0) {
+        c->nFair = c->nVair = c->nBnd = 0;
+        c->FairGlobal.clear();
+        c->airFixedGlobal.clear();
+        return OC_OK;
+    }
+    OC_ARG(c, nVair > 0 && airUV && Fair && nBnd >= 0 && nBnd <= nVair && (nBnd == 0 || bnd) && wScaf >= 0.0,
+           "oc_set_air_mesh: bad sizes");
+    for (int k = 0; k < nBnd; ++k) OC_ARG(c, bnd[k] >= 0 && bnd[k] < c->nV, "oc_set_air_mesh: bnd out of range");
+    for (long k = 0; k < 3L * nFair; ++k) OC_ARG(c, Fair[k] >= 0 && Fair[k] < nVair, "oc_set_air_mesh: bad Fair");
+    const int nVt = c->nV + nVair - nBnd;
+    c->nVair = nVair;
+    c->nFair = nFair;
+    c->nBnd = nBnd;
+    c->wScaf = wScaf;
+    std::vector<int> l2g(nVair);
+    for (int l = 0; l < nVair; ++l) l2g[l] = l < nBnd ? bnd[l] : c->nV + l - nBnd;
+    c->FairGlobal.resize(3 * (size_t)nFair);
+    for (long k = 0; k < 3L * nFair; ++k) c->FairGlobal[k] = l2g[Fair[k]];
+    c->airFixedGlobal.assign(nVt, 0);
+    for (int l = 0; l < nBnd; ++l)
+        if (c->fixedMaskMesh[bnd[l]]) c->airFixedGlobal[bnd[l]] = 1;
+
+    if (int st = ensure_tri_storage(c, c->nF + nFair)) return st;
+    // air rest features from its own rest shape (UV, z = 0) and local indices
+    std::vector<double> vr(3 * (size_t)nVair);
+    for (int l = 0; l < nVair; ++l) {
+        vr[3 * l] = airUV[2 * l];
+        vr[3 * l + 1] = airUV[2 * l + 1];
+        vr[3 * l + 2] = 0.0;
+    }
+    if (upload(c, c->dTmpVrest, vr)) return OC_ERR_HIP;
+    if (int st = upload_tri_ids(c, c->nF, nFair, Fair)) return st;  // local ids first, for the features
+    OC_HIP(c, c->dAirSums.ensure(sizeof(double) * 4));
+    if (ensure_scratch(c, nFair)) return OC_ERR_HIP;
+    const int o = c->nF;
+    oc::launch_rest_features(c->stream, nFair, c->dF0.as<int>() + o, c->dF1.as<int>() + o, c->dF2.as<int>() + o,
+                             c->dTmpVrest.as<double>(), areaThres, c->dTriArea.as<double>() + o,
+                             c->dA2.as<double>() + o, c->dE0.as<double>() + o, c->dE1.as<double>() + o,
+                             c->dE01.as<double>() + o, c->dPartials.as<double>(), c->dAirSums.as<double>());
+    OC_HIP(c, hipGetLastError());
+    double sums[3];
+    OC_HIP(c, hipMemcpyAsync(sums, c->dAirSums.p, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    if (sums[2] > 0.0) {
+        c->err = "an air-mesh triangle has zero area";
+        return OC_ERR_DEGENERATE;
+    }
+    if (int st = upload_tri_ids(c, c->nF, nFair, c->FairGlobal.data())) return st;
+    // combined UV: keep the mesh part, append the interior air vertices
+    DevBuf nu;
+    OC_HIP(c, nu.ensure(sizeof(double) * 2 * (size_t)nVt));
+    OC_HIP(c, hipMemcpyAsync(nu.p, c->dUV.p, sizeof(double) * 2 * (size_t)c->nV, hipMemcpyDeviceToDevice, c->stream));
+    if (nVair > nBnd)
+        OC_HIP(c, hipMemcpyAsync(nu.as<double>() + 2 * (size_t)c->nV, airUV + 2 * (size_t)nBnd,
+                                 sizeof(double) * 2 * (size_t)(nVair - nBnd), hipMemcpyHostToDevice, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    std::swap(c->dUV.p, nu.p);
+    std::swap(c->dUV.bytes, nu.bytes);
+    return OC_OK;
+}
+
+// ---- LinSysSolver drop-in ------------------------------------------------------------
+int oc_solver_set_pattern(oc_ctx* c, int nV, const int* nbrPtr, const int* nbr, int nFixed, const int* fixedV,
+                          int* nnz)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, nV > 0 && nbrPtr && nbr && nFixed >= 0 && (nFixed == 0 || fixedV), "oc_solver_set_pattern: bad args");
+    (void)hipSetDevice(c->device);
+    std::vector<unsigned char> isFixed(nV, 0);
+    for (int k = 0; k < nFixed; ++k) {
+        OC_ARG(c, fixedV[k] >= 0 && fixedV[k] < nV, "oc_solver_set_pattern: fixed id out of range");
+        isFixed[fixedV[k]] = 1;
+    }
+    // reference upper CSR (LinSysSolver.hpp:37-135), 0-based
+    std::vector<int>& ia = c->solIa;
+    std::vector<int>& ja = c->solJa;
+    ia.assign(2 * (size_t)nV + 1, 0);
+    ja.clear();
+    std::vector<std::vector<int>> rows(nV);
+    for (int v = 0; v < nV; ++v) {
+        std::vector<int> nb(nbr + nbrPtr[v], nbr + nbrPtr[v + 1]);
+        for (int w : nb) OC_ARG(c, w >= 0 && w < nV, "oc_solver_set_pattern: neighbour out of range");
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+        if (!isFixed[v]) {
+            std::vector<int> rowx = {2 * v, 2 * v + 1};
+            for (int w : nb)
+                if (!isFixed[w] && w > v) {
+                    rowx.push_back(2 * w);
+                    rowx.push_back(2 * w + 1);
+                }
+            for (int x : rowx) ja.push_back(x);
+            ia[2 * v + 1] = ia[2 * v] + (int)rowx.size();
+            for (size_t k = 1; k < rowx.size(); ++k) ja.push_back(rowx[k]);
+            ia[2 * v + 2] = ia[2 * v + 1] + (int)rowx.size() - 1;
+        } else {
+            ja.push_back(2 * v);
+            ja.push_back(2 * v + 1);
+            ia[2 * v + 1] = ia[2 * v] + 1;
+            ia[2 * v + 2] = ia[2 * v + 1] + 1;
+        }
+    }
+    // block rows of the full symmetric matrix
+    for (int i = 0; i < 2 * nV; ++i)
+        for (int k = ia[i]; k < ia[i + 1]; ++k) {
+            const int u = i / 2, w = ja[k] / 2;
+            rows[u].push_back(w);
+            rows[w].push_back(u);
+        }
+    for (int v = 0; v < nV; ++v) {
+        auto& r = rows[v];
+        r.push_back(v);
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+    }
+    std::vector<unsigned char> noFixed(nV, 0);  // the rows above already exclude fixed couplings
+    build_block_pattern(c->sol, nV, rows, noFixed);
+    const int nnzU = (int)ja.size();
+    c->solDst.resize(nnzU);
+    c->solMir.resize(nnzU);
+    for (int i = 0; i < 2 * nV; ++i)
+        for (int k = ia[i]; k < ia[i + 1]; ++k) {
+            const int j = ja[k];
+            const int u = i / 2, w = j / 2, ri = i % 2, cj = j % 2;
+            const int P = c->sol.find(u, w);
+            c->solDst[k] = 4 * P + 2 * ri + cj;
+            if (u != w) c->solMir[k] = 4 * c->sol.find(w, u) + 2 * cj + ri;
+            else c->solMir[k] = (ri != cj) ? 4 * P + 2 * cj + ri : -1;
+        }
+    c->solN = 2 * nV;
+    c->solA.assign(nnzU, 0.0);
+    if (upload_pattern(c, c->sol)) return OC_ERR_HIP;
+    if (upload(c, c->dSolDst, c->solDst) || upload(c, c->dSolMir, c->solMir)) return OC_ERR_HIP;
+    OC_HIP(c, c->dSolA.ensure(sizeof(double) * (size_t)std::max(1, nnzU)));
+    OC_HIP(c, c->dSolB.ensure(sizeof(double) * 2 * (size_t)nV));
+    OC_HIP(c, c->dSolX.ensure(sizeof(double) * 2 * (size_t)nV));
+    if (ensure_pcg(c, nV)) return OC_ERR_HIP;
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->solFactorized = false;
+    if (nnz) *nnz = nnzU;
+    return OC_OK;
+}
+
+int oc_solver_get_pattern(oc_ctx* c, int* ia, int* ja)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0 && ia && ja, "oc_solver_get_pattern: no pattern");
+    std::copy(c->solIa.begin(), c->solIa.end(), ia);
+    std::copy(c->solJa.begin(), c->solJa.end(), ja);
+    return OC_OK;
+}
+
+int oc_solver_update_a(oc_ctx* c, long long nT, const int* I, const int* J, const double* V)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0, "oc_solver_update_a: no pattern");
+    OC_ARG(c, nT >= 0 && (nT == 0 || (I && J && V)), "oc_solver_update_a: bad triplets");
+    (void)hipSetDevice(c->device);
+    std::fill(c->solA.begin(), c->solA.end(), 0.0);
+    const int* ia = c->solIa.data();
+    const int* ja = c->solJa.data();
+    for (long long k = 0; k < nT; ++k) {
+        const int i = I[k], j = J[k];
+        if (i > j) continue;
+        if (i < 0 || j >= c->solN) {
+            c->err = "oc_solver_update_a: triplet index out of range";
+            return OC_ERR_PATTERN;
+        }
+        const int* b = ja + ia[i];
+        const int* e = ja + ia[i + 1];
+        const int* p = std::lower_bound(b, e, j);
+        if (p == e || *p != j) {
+            c->err = "oc_solver_update_a: triplet outside the pattern";
+            return OC_ERR_PATTERN;
+        }
+        c->solA[p - ja] += V[k];
+    }
+    const int nnz = (int)c->solA.size();
+    OC_HIP(c, hipMemcpyAsync(c->dSolA.p, c->solA.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, c->stream));
+    OC_HIP(c, hipMemsetAsync(c->sol.dVals.p, 0, sizeof(double) * 4 * (size_t)c->sol.nnzb(), c->stream));
+    oc::launch_expand_upper(c->stream, nnz, c->dSolA.as<double>(), c->dSolDst.as<int>(), c->dSolMir.as<int>(),
+                            c->sol.dVals.as<double>());
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->solFactorized = false;
+    return OC_OK;
+}
+
+int oc_solver_get_a(oc_ctx* c, double* a)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0 && a, "oc_solver_get_a: no pattern");
+    std::copy(c->solA.begin(), c->solA.end(), a);
+    return OC_OK;
+}
+
+int oc_solver_analyze_pattern(oc_ctx* c)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0, "oc_solver_analyze_pattern: no pattern");
+    return OC_OK;
+}
+
+int oc_solver_factorize(oc_ctx* c)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0, "oc_solver_factorize: no pattern");
+    (void)hipSetDevice(c->device);
+    oc::launch_jacobi_from_bsr(c->stream, c->sol.nV, c->sol.dDiagPos.as<int>(), c->sol.dVals.as<double>(),
+                               c->sol.dMinv.as<double>(), c->solPrecond == OC_PRECOND_BLOCK_JACOBI);
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->solFactorized = true;
+    return OC_OK;
+}
+
+int oc_solver_solve(oc_ctx* c, const double* rhs, double* x)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0 && rhs && x, "oc_solver_solve: no pattern");
+    if (!c->solFactorized) {
+        c->err = "oc_solver_solve: factorize first";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    OC_HIP(c, hipMemcpyAsync(c->dSolB.p, rhs, sizeof(double) * c->solN, hipMemcpyHostToDevice, c->stream));
+    int it;
+    double rr;
+    const int st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it, rr);
+    c->lastIters = it;
+    c->lastRelRes = rr;
+    if (st != OC_OK && st != OC_ERR_NOT_CONVERGED) return st;
+    OC_HIP(c, hipMemcpyAsync(x, c->dX.p, sizeof(double) * c->solN, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return st;
+}
+
+int oc_solver_multiply(oc_ctx* c, const double* x, double* Ax)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0 && x && Ax, "oc_solver_multiply: no pattern");
+    (void)hipSetDevice(c->device);
+    OC_HIP(c, hipMemcpyAsync(c->dSolB.p, x, sizeof(double) * c->solN, hipMemcpyHostToDevice, c->stream));
+    oc::launch_spmv(c->stream, c->sol.nV, c->sol.dRowPtr.as<int>(), c->sol.dColIdx.as<int>(),
+                    c->sol.dVals.as<double>(), c->dSolB.as<double>(), c->dSolX.as<double>());
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipMemcpyAsync(Ax, c->dSolX.p, sizeof(double) * c->solN, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int oc_solver_set_params(oc_ctx* c, int precond, double relTol, int maxIter)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI) && relTol > 0.0 && maxIter > 0,
+           "oc_solver_set_params: bad parameters");
+    c->solPrecond = precond;
+    c->solTol = relTol;
+    c->solMaxIter = maxIter;
+    c->solFactorized = false;
+    return OC_OK;
+}
+
+int oc_solver_last_stats(oc_ctx* c, int* iters, double* relResidual)
+{
+    if (!c) return OC_ERR_ARG;
+    if (iters) *iters = c->lastIters;
+    if (relResidual) *relResidual = c->lastRelRes;
+    return OC_OK;
+}
+
+// ---- Newton driver ------------------------------------------------------------------
+int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, int precond, double pcgRelTol,
+                   int pcgMaxIter, double* E0)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0, "oc_newton_init: no mesh");
+    OC_ARG(c, a0 > 0.0 && relGL2Tol > 0.0 && pcgRelTol > 0.0 && pcgMaxIter > 0 &&
+                  (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI),
+           "oc_newton_init: bad parameters");
+    (void)hipSetDevice(c->device);
+    c->a0 = a0;
+    c->relGL2Tol = relGL2Tol;
+    c->allowEDecRelTol = allowEDecRelTol;
+    c->nPrecond = precond;
+    c->pcgTol = pcgRelTol;
+    c->pcgMaxIter = pcgMaxIter;
+    // Optimizer constructor: exit(-1) on an inverted initial map (Optimizer.cpp:65-67)
+    if (ensure_scratch(c, c->nF + c->nFair)) return OC_ERR_HIP;
+    unsigned int ninv = 0;
+    if (int st = count_inverted(c, mesh_set(c, 0), &ninv)) return st;
+    if (ninv) {
+        c->err = "inverted triangle in the initial UV map";
+        return OC_ERR_INVERTED;
+    }
+    if (int st = build_system(c)) return st;
+    c->targetGRes = a0 * (double)(c->nV - (int)c->fixedSorted.size()) / (double)c->nV * relGL2Tol;
+    if (int st = system_energy(c, &c->lastE, &c->lastESD, &c->lastEScaf)) return st;
+    if (E0) *E0 = c->lastE;
+    return OC_OK;
+}
+
+int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, S && done, "oc_newton_step: null output");
+    if (!c->sysReady) {
+        c->err = "oc_newton_step: call oc_newton_init first";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    std::memset(S, 0, sizeof(*S));
+    *done = 0;
+    const TriSet T = system_set(c);
+    const int nVt = c->nVt;
+    hipStream_t s = c->stream;
+    // gradient (and element Hessians in the same pass)
+    oc::launch_element(s, T, c->dUV.as<double>(), nullptr, c->dElemG.as<double>(), c->dElemH.as<double>(), 1,
+                       nullptr, nullptr);
+    oc::launch_gather_grad(s, nVt, c->dVcPtr.as<int>(), c->dVcSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
+                           c->dElemG.as<double>(), c->dG.as<double>(), c->dPartials.as<double>(), scalar(c, 6));
+    OC_HIP(c, hipGetLastError());
+    double gsq;
+    OC_HIP(c, hipMemcpyAsync(&gsq, scalar(c, 6), sizeof(double), hipMemcpyDeviceToHost, s));
+    OC_HIP(c, hipStreamSynchronize(s));
+    S->grad_sq = gsq;
+    S->E_total = c->lastE;
+    S->E_SD = c->lastESD;
+    S->E_scaf = c->lastEScaf;
+    if (gsq < c->targetGRes) {  // Optimizer.cpp:215-221
+        S->converged = 1;
+        *done = 1;
+        return OC_OK;
+    }
+    // Hessian assembly + direction: H d = -g
+    oc::launch_assemble<double>(s, (int)c->ubRow.size(), c->dUbRow.as<int>(), c->dUbCol.as<int>(),
+                                c->dUbPosU.as<int>(), c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(),
+                                c->dUbCSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
+                                c->dFixedDiag.as<double>(), c->dElemH.as<double>(), c->sys.dVals.as<double>(),
+                                c->sys.dMinv.as<double>(), c->nPrecond == OC_PRECOND_BLOCK_JACOBI);
+    oc::launch_scale_copy(s, 2L * nVt, -1.0, c->dG.as<double>(), c->dB.as<double>());
+    int it = 0;
+    double rr = 0.0;
+    int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr);
+    S->pcg_iters = it;
+    S->pcg_rel_res = rr;
+    if (st != OC_OK && st != OC_ERR_NOT_CONVERGED) return st;
+    const double* dir = c->dX.as<double>();
+    // line search (Optimizer.cpp:575-652)
+    double bound;
+    if (int e = step_bound(c, T, dir, 1.0, &bound)) return e;
+    double stepSize = bound * 0.99;
+    double lastE = c->lastE, lastEScafStart = c->lastEScaf;
+    if (c->nFair) {  // the scaffold changed: recompute the energy (Optimizer.cpp:588-592)
+        if (int e = system_energy(c, &lastE, &c->lastESD, &lastEScafStart)) return e;
+        c->lastE = lastE;
+        c->lastEScaf = lastEScafStart;
+    }
+    constexpr int K = oc::kLSCand;
+    double cand[4 * K];
+    int k0 = -1;  // first index of the candidates held in cand
+    auto fetch = [&](int kstart) -> int {
+        oc::launch_linesearch(s, T, c->dUV.as<double>(), dir, stepSize, kstart, c->dPartials.as<double>(),
+                              c->dLsOut.as<double>());
+        OC_HIP(c, hipGetLastError());
+        OC_HIP(c, hipMemcpyAsync(cand, c->dLsOut.p, sizeof(cand), hipMemcpyDeviceToHost, s));
+        OC_HIP(c, hipStreamSynchronize(s));
+        k0 = kstart;
+        return OC_OK;
+    };
+    auto get = [&](int k, double* E, double* Esd, double* Es, bool* inv) -> int {
+        if (k0 < 0 || k < k0 || k >= k0 + K)
+            if (int e = fetch(k)) return e;
+        const int j = k - k0;
+        *E = cand[j];
+        *Esd = cand[K + j];
+        *Es = cand[2 * K + j] * T.sAir;
+        *inv = cand[3 * K + j] > 0.0;
+        return OC_OK;
+    };
+    bool stopped = false;
+    int k = 0;
+    double alpha = stepSize;     // current trial step
+    double alphaCommit = alpha;  // step of the UV left in place
+    double testE, testESD, testES;
+    bool inv;
+    if (int e = get(0, &testE, &testESD, &testES, &inv)) return e;
+    int halvings = 0;
+    while (testE > lastE) {
+        const double na = alpha / 2.0;
+        ++halvings;
+        if (na == 0.0) {
+            stopped = true;
+            alpha = 0.0;
+            break;
+        }
+        alpha = na;
+        alphaCommit = alpha;
+        ++k;
+        if (int e = get(k, &testE, &testESD, &testES, &inv)) return e;
+    }
+    // post-check of inversions (Optimizer.cpp:615-629)
+    while (!stopped && inv) {
+        const double na = alpha / 2.0;
+        ++halvings;
+        if (na == 0.0) {
+            stopped = true;
+            alpha = 0.0;
+            break;
+        }
+        alpha = na;
+        alphaCommit = alpha;
+        ++k;
+        if (int e = get(k, &testE, &testESD, &testES, &inv)) return e;
+    }
+    oc::launch_axpy_commit(s, 2L * nVt, alphaCommit, dir, c->dUV.as<double>());
+    OC_HIP(c, hipGetLastError());
+    double lastEDec = lastE - testE;
+    if (c->nFair) lastEDec += (-lastEScafStart + testES);
+    if (c->allowEDecRelTol && (lastEDec / lastE < 1.0e-6 * alpha) && (alpha > 1.0e-3)) stopped = true;
+    c->lastE = testE;
+    c->lastESD = testESD;
+    c->lastEScaf = testES;
+    OC_HIP(c, hipStreamSynchronize(s));
+    S->E_total = testE;
+    S->E_SD = testESD;
+    S->E_scaf = testES;
+    S->step = alpha;
+    S->halvings = halvings;
+    S->stopped = stopped ? 1 : 0;
+    *done = stopped ? 1 : 0;
+    return OC_OK;
+}
+
+int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, ms && reps > 0 && which >= 0 && which <= 6, "oc_time_kernel: bad args");
+    if (!c->sysReady) {
+        c->err = "oc_time_kernel: call oc_newton_init first";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    hipStream_t s = c->stream;
+    const TriSet T = system_set(c);
+    PcgCtrl* ctrl = c->dCtrl.as<PcgCtrl>();
+    // Timing copies of the partial records: the update reads a live p.q record
+    // and the SpMV a live (r.z, r.r) record; the kernels write into scratch
+    // records so the values they read stay fixed across repetitions.
+    const int GU = oc::pcg_grid_update(c->nVt), GS = oc::pcg_grid_spmv(c->nVt);
+    double* tp = c->dPartials.as<double>();
+    double* tU = tp;                         // 2*GU read by spmv
+    double* tS = tp + 2 * (size_t)GU;        // GS   read by update
+    double* tUo = tS + GS;                   // 2*GU written by update
+    double* tSo = tUo + 2 * (size_t)GU;      // GS   written by spmv
+    auto spmv = [&]() {
+        oc::launch_pcg_spmv(s, 1, c->nVt, c->sys.dRowPtr.as<int>(), c->sys.dColIdx.as<int>(),
+                            c->sys.dVals.as<double>(), c->dZ.as<double>(), c->dP0.as<double>(), c->dP1.as<double>(),
+                            c->dQ.as<double>(), tU, tSo, ctrl);
+    };
+    auto update = [&]() {
+        oc::launch_pcg_update(s, 1, c->nVt, c->dP1.as<double>(), c->dQ.as<double>(), c->sys.dMinv.as<double>(),
+                              c->dTmpVec.as<double>(), c->dR.as<double>(), c->dZ.as<double>(), tS, tUo, ctrl);
+    };
+    auto launch = [&]() {
+        switch (which) {
+        case 0:
+            oc::launch_element(s, T, c->dUV.as<double>(), nullptr, c->dElemG.as<double>(), c->dElemH.as<double>(), 1,
+                               nullptr, nullptr);
+            break;
+        case 1:
+            oc::launch_assemble<double>(s, (int)c->ubRow.size(), c->dUbRow.as<int>(), c->dUbCol.as<int>(),
+                                        c->dUbPosU.as<int>(), c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(),
+                                        c->dUbCSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
+                                        c->dFixedDiag.as<double>(), c->dElemH.as<double>(),
+                                        c->sys.dVals.as<double>(), c->sys.dMinv.as<double>(),
+                                        c->nPrecond == OC_PRECOND_BLOCK_JACOBI);
+            break;
+        case 2:
+            spmv();
+            update();
+            break;
+        case 3:
+            oc::launch_linesearch(s, T, c->dUV.as<double>(), c->dX.as<double>(), 0.5, 0, c->dPartials.as<double>(),
+                                  c->dLsOut.as<double>());
+            break;
+        case 4:
+            oc::launch_step_bound(s, T, c->dUV.as<double>(), c->dX.as<double>(), c->dOrdMin.as<unsigned long long>(),
+                                  1.0);
+            break;
+        case 5:
+            spmv();
+            break;
+        case 6:
+            update();
+            break;
+        }
+    };
+    if (which == 2 || which == 5 || which == 6) {
+        // Keep the PCG kernels doing full work on the current system: a
+        // private copy of x (dTmpVec) so the last direction is not
+        // disturbed, done cleared, iteration cap out of reach.
+        OC_HIP(c, c->dTmpVec.ensure(sizeof(double) * 2 * (size_t)c->nVt));
+        OC_HIP(c, hipMemcpyAsync(c->dTmpVec.p, c->dX.p, sizeof(double) * 2 * (size_t)c->nVt,
+                                 hipMemcpyDeviceToDevice, s));
+        PcgCtrl h{};
+        OC_HIP(c, hipMemcpyAsync(&h, ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, s));
+        OC_HIP(c, hipStreamSynchronize(s));
+        h.done = 0;
+        h.iter = 1;
+        h.maxit = 1 << 30;
+        h.tol2 = 0.0;
+        h.bb = 1.0;
+        h.rzSlot[0] = h.rzSlot[1] = 1.0;
+        OC_HIP(c, hipMemcpyAsync(ctrl, &h, sizeof(PcgCtrl), hipMemcpyHostToDevice, s));
+        // records: (r.z, r.r) = (1, 1) so the SpMV neither stops nor breaks down; p.q = 1
+        std::vector<double> rec(2 * (size_t)GU + GS, 0.0);
+        rec[0] = 1.0;
+        rec[1] = 1.0;
+        rec[2 * (size_t)GU] = 1.0;
+        if (ensure_scratch(c, std::max<long>(4L * GU + 2L * GS, 2L * c->nVt))) return OC_ERR_HIP;
+        OC_HIP(c, hipMemcpyAsync(tU, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice, s));
+        OC_HIP(c, hipStreamSynchronize(s));
+    }
+    hipEvent_t e0, e1;
+    OC_HIP(c, hipEventCreate(&e0));
+    OC_HIP(c, hipEventCreate(&e1));
+    launch();  // warm
+    double total = 0.0;
+    for (int r = 0; r < reps; ++r) {
+        OC_HIP(c, hipEventRecord(e0, s));
+        launch();
+        OC_HIP(c, hipEventRecord(e1, s));
+        OC_HIP(c, hipEventSynchronize(e1));
+        float t = 0.f;
+        OC_HIP(c, hipEventElapsedTime(&t, e0, e1));
+        total += t;
+    }
+    *ms = total / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    OC_HIP(c, hipGetLastError());
+    if (which == 2 || which == 5 || which == 6) {
+        // leave the solver state re-armed for the next Newton step
+        PcgCtrl h{};
+        OC_HIP(c, hipMemcpyAsync(&h, ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, s));
+        OC_HIP(c, hipStreamSynchronize(s));
+        h.done = 1;
+        OC_HIP(c, hipMemcpyAsync(ctrl, &h, sizeof(PcgCtrl), hipMemcpyHostToDevice, s));
+        OC_HIP(c, hipStreamSynchronize(s));
+    }
+    return OC_OK;
+}
+
+int oc_system_sizes(oc_ctx* c, long long* nF, long long* nV, long long* nnzBlocks, long long* nUpperBlocks)
+{
+    if (!c) return OC_ERR_ARG;
+    if (nF) *nF = c->sysReady ? c->nFt : c->nF;
+    if (nV) *nV = c->sysReady ? c->nVt : c->nV;
+    if (nnzBlocks) *nnzBlocks = c->sysReady ? c->sys.nnzb() : 0;
+    if (nUpperBlocks) *nUpperBlocks = c->sysReady ? (long long)c->ubRow.size() : 0;
+    return OC_OK;
+}
+
+}  // extern "C"

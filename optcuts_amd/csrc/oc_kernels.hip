@@ -1,0 +1,782 @@
+// Geometry-step kernels for gfx950 (CDNA4).
+//
+// Layout in HBM:
+//   UV, gradient, search direction: interleaved per vertex (double2), the
+//     reference's DOF order [u0, v0, u1, v1, ...].
+//   Triangles: SoA vertex ids f0/f1/f2 and SoA rest features (coalesced).
+//   Hessian: full symmetric 2x2-block CSR (one block per vertex pair,
+//     row-major [xx, xy, yx, yy]).
+//   Element buffers: elemG[t*3 + k] (double2), elemH[t*6 + blk] (4 values).
+// Every sum is deterministic: element contributions are gathered per vertex
+// or per block in triangle order, and grid reductions finish in the last
+// block in block-index order.
+#include "oc_common.hpp"
+#include "oc_kernels.hpp"
+#include "oc_sd_element.hpp"
+
+namespace oc {
+
+// ----------------------------------------------------------------------------
+// Grid-wide sums. Each block publishes its compensated partials with plain
+// stores; a one-block finisher kernel launched next on the same stream
+// reduces them in block-index order (deterministic). The kernel boundary
+// provides the cross-XCD visibility, so no kernel issues a device-scope
+// fence (which on a multi-XCD part writes back and invalidates the L2).
+template <int NV>
+__device__ __forceinline__ void finish_csum(const double* partials, int nblk, double* out)
+{
+    CSum acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = CSum{0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            acc[k] = csum_add(acc[k], CSum{partials[(size_t)b * 2 * NV + 2 * k], partials[(size_t)b * 2 * NV + 2 * k + 1]});
+    block_csum<NV>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out[k] = acc[k].s + acc[k].c;
+}
+
+template <int NV>
+__global__ void __launch_bounds__(kBlock) k_finish_csum(const double* __restrict__ partials, int nblk, double* out)
+{
+    finish_csum<NV>(partials, nblk, out);
+}
+
+template <int NV>
+__device__ __forceinline__ void publish_csum(CSum (&acc)[NV], double* partials)
+{
+    block_csum<NV>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            partials[(size_t)blockIdx.x * 2 * NV + 2 * k] = acc[k].s;
+            partials[(size_t)blockIdx.x * 2 * NV + 2 * k + 1] = acc[k].c;
+        }
+}
+
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_rest_features(int nF, const int* __restrict__ f0,
+                                                          const int* __restrict__ f1, const int* __restrict__ f2,
+                                                          const double* __restrict__ Vrest, double areaThres,
+                                                          double* triArea, double* A2, double* E0, double* E1,
+                                                          double* E01, double* partials)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    CSum acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    if (t < nF) {
+        const int i[3] = {f0[t], f1[t], f2[t]};
+        double P[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) P[k][d] = Vrest[3 * (size_t)i[k] + d];
+        const double a[3] = {P[1][0] - P[0][0], P[1][1] - P[0][1], P[1][2] - P[0][2]};
+        const double b[3] = {P[2][0] - P[0][0], P[2][1] - P[0][1], P[2][2] - P[0][2]};
+        const double n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        double area = 0.5 * sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        acc[2].s = (area == 0.0) ? 1.0 : 0.0;
+        if (area < areaThres) {
+            area = areaThres;
+            A2[t] = areaThres * areaThres;
+            const double e = 2.0 / sqrt(3.0) / areaThres;
+            E0[t] = e;
+            E1[t] = e;
+            E01[t] = e / 2.0;
+        } else {
+            const double a2 = area * area;
+            A2[t] = a2;
+            E0[t] = (a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) / 2. / a2;
+            E1[t] = (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]) / 2. / a2;
+            E01[t] = (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / 2. / a2;
+        }
+        triArea[t] = area;
+        acc[0].s = area;
+        double el = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int q = (k + 1) % 3;
+            const double dx = P[k][0] - P[q][0], dy = P[k][1] - P[q][1], dz = P[k][2] - P[q][2];
+            el += sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        acc[1].s = el;
+    }
+    publish_csum<3>(acc, partials);
+}
+
+void launch_rest_features(hipStream_t s, int nF, const int* f0, const int* f1, const int* f2, const double* Vrest,
+                          double areaThres, double* triArea, double* A2, double* E0, double* E1, double* E01,
+                          double* partials, double* sums)
+{
+    const int g = grid_for(nF);
+    hipLaunchKernelGGL(k_rest_features, dim3(g), dim3(kBlock), 0, s, nF, f0, f1, f2, Vrest, areaThres, triArea, A2,
+                       E0, E1, E01, partials);
+    hipLaunchKernelGGL(k_finish_csum<3>, dim3(1), dim3(kBlock), 0, s, partials, g, sums);
+}
+
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ RestT<double> load_rest(const TriSet& T, int t, bool isMesh)
+{
+    RestT<double> r;
+    r.w = (isMesh && !T.uniformMesh) ? T.triArea[t] / *T.surfaceArea : 1.0;
+    r.A2 = T.A2[t];
+    r.E0 = T.E0[t];
+    r.E1 = T.E1[t];
+    r.E01 = T.E01[t];
+    return r;
+}
+
+template <bool WANT_G, bool WANT_H>
+__global__ void __launch_bounds__(kBlock) k_element(TriSet T, const double2* __restrict__ UV, double* Eelem,
+                                                    double2* elemG, double4* elemH, int project, double* partials)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    CSum acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    if (t < T.nF) {
+        const bool isMesh = t < T.nFmesh;
+        const double2 a = UV[T.f0[t]], b = UV[T.f1[t]], c = UV[T.f2[t]];
+        const RestT<double> r = load_rest(T, t, isMesh);
+        const double U[3][2] = {{a.x, a.y}, {b.x, b.y}, {c.x, c.y}};
+        double g[6], h[6][4];
+        const double E = sd_element<double, WANT_H>(U, r, g, h);
+        if (Eelem) Eelem[t] = E;
+        acc[0].s = isMesh ? E : 0.0;
+        acc[1].s = isMesh ? 0.0 : E;
+        const double sc = isMesh ? T.sMesh : T.sAir;
+        if (WANT_G) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) elemG[3 * (size_t)t + k] = make_double2(sc * g[2 * k], sc * g[2 * k + 1]);
+        }
+        if (WANT_H) {
+            if (project) sd_project_psd<double>(h);
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                elemH[6 * (size_t)t + k] = make_double4(sc * h[k][0], sc * h[k][1], sc * h[k][2], sc * h[k][3]);
+        }
+    }
+    if (partials) publish_csum<2>(acc, partials);
+}
+
+void launch_element(hipStream_t s, const TriSet& T, const double* UV, double* Eelem, double* elemG, double* elemH,
+                    int project, double* partials, double* esum)
+{
+    const dim3 grid(grid_for(T.nF)), blk(kBlock);
+    const double2* uv = reinterpret_cast<const double2*>(UV);
+    double* part = esum ? partials : nullptr;
+    if (elemH)
+        hipLaunchKernelGGL((k_element<true, true>), grid, blk, 0, s, T, uv, Eelem, reinterpret_cast<double2*>(elemG),
+                           reinterpret_cast<double4*>(elemH), project, part);
+    else if (elemG)
+        hipLaunchKernelGGL((k_element<true, false>), grid, blk, 0, s, T, uv, Eelem, reinterpret_cast<double2*>(elemG),
+                           nullptr, project, part);
+    else
+        hipLaunchKernelGGL((k_element<false, false>), grid, blk, 0, s, T, uv, Eelem, nullptr, nullptr, project, part);
+    if (esum) hipLaunchKernelGGL(k_finish_csum<2>, dim3(1), dim3(kBlock), 0, s, partials, (int)grid.x, esum);
+}
+
+// ----------------------------------------------------------------------------
+// Reference-order triplets for one triangle block (mesh only; unscaled).
+__global__ void __launch_bounds__(kBlock) k_triplets(TriSet T, const double2* __restrict__ UV,
+                                                     const long long* __restrict__ tripOff,
+                                                     const unsigned char* __restrict__ vFree, int* I, int* J,
+                                                     double* V)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T.nF) return;
+    const unsigned char fm = vFree[t];
+    if (!fm) return;
+    const bool isMesh = t < T.nFmesh;
+    const int vi[3] = {T.f0[t], T.f1[t], T.f2[t]};
+    const double2 a = UV[vi[0]], b = UV[vi[1]], c = UV[vi[2]];
+    const RestT<double> r = load_rest(T, t, isMesh);
+    const double U[3][2] = {{a.x, a.y}, {b.x, b.y}, {c.x, c.y}};
+    double g[6], h[6][4];
+    sd_element<double, true>(U, r, g, h);
+    sd_project_psd<double>(h);
+    long long o = tripOff[t];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        if (!((fm >> p) & 1)) continue;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (!((fm >> q) & 1)) continue;
+            const int lo = p < q ? p : q, hi = p < q ? q : p;
+            const int id = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+#pragma unroll
+            for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    V[o] = p <= q ? h[id][2 * rr + cc] : h[id][2 * cc + rr];
+                    I[o] = 2 * vi[p] + rr;
+                    J[o] = 2 * vi[q] + cc;
+                    ++o;
+                }
+        }
+    }
+}
+
+void launch_triplets(hipStream_t s, const TriSet& T, const double* UV, const long long* tripOff,
+                     const unsigned char* vFree, int* I, int* J, double* V)
+{
+    hipLaunchKernelGGL(k_triplets, dim3(grid_for(T.nF)), dim3(kBlock), 0, s, T,
+                       reinterpret_cast<const double2*>(UV), tripOff, vFree, I, J, V);
+}
+
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_gather_grad(int nV, const int* __restrict__ vcPtr,
+                                                        const int* __restrict__ vcSrc,
+                                                        const unsigned char* __restrict__ fixedMask,
+                                                        const double2* __restrict__ elemG, double2* g,
+                                                        double* partials)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double sq[1] = {0.0};
+    if (v < nV) {
+        double2 s = make_double2(0.0, 0.0);
+        if (!fixedMask[v]) {
+            for (int k = vcPtr[v]; k < vcPtr[v + 1]; ++k) {
+                const double2 e = elemG[vcSrc[k]];
+                s.x += e.x;
+                s.y += e.y;
+            }
+        }
+        g[v] = s;
+        sq[0] = s.x * s.x + s.y * s.y;
+    }
+    if (partials) {
+        CSum c[1] = {CSum{sq[0], 0.0}};
+        publish_csum<1>(c, partials);
+    }
+}
+
+void launch_gather_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSrc, const unsigned char* fixedMask,
+                        const double* elemG, double* g, double* partials, double* gsq)
+{
+    const int grid = grid_for(nV);
+    hipLaunchKernelGGL(k_gather_grad, dim3(grid), dim3(kBlock), 0, s, nV, vcPtr, vcSrc, fixedMask,
+                       reinterpret_cast<const double2*>(elemG), reinterpret_cast<double2*>(g),
+                       gsq ? partials : nullptr);
+    if (gsq) hipLaunchKernelGGL(k_finish_csum<1>, dim3(1), dim3(kBlock), 0, s, partials, grid, gsq);
+}
+
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double4 load_block(const double4* H, int src) { return H[src]; }
+__device__ __forceinline__ double4 load_block(const float4* H, int src)
+{
+    const float4 f = H[src];
+    return make_double4(f.x, f.y, f.z, f.w);
+}
+
+__device__ __forceinline__ void jacobi_block(const double4& d, bool blockJacobi, double* M)
+{
+    if (blockJacobi) {
+        const double det = d.x * d.w - d.y * d.z;
+        M[0] = d.w / det;
+        M[1] = -d.y / det;
+        M[2] = -d.z / det;
+        M[3] = d.x / det;
+    } else {
+        M[0] = 1.0 / d.x;
+        M[1] = 0.0;
+        M[2] = 0.0;
+        M[3] = 1.0 / d.w;
+    }
+}
+
+template <typename HV>
+__global__ void __launch_bounds__(kBlock) k_assemble(int nUB, const int* __restrict__ ubRow,
+                                                     const int* __restrict__ ubCol, const int* __restrict__ ubPosU,
+                                                     const int* __restrict__ ubPosL, const int* __restrict__ ubCPtr,
+                                                     const int* __restrict__ ubCSrc,
+                                                     const unsigned char* __restrict__ fixedMask,
+                                                     const double* __restrict__ fixedDiag, const HV* __restrict__ elemH,
+                                                     double4* vals, double* Minv, int blockJacobi)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nUB) return;
+    const int u = ubRow[b], w = ubCol[b];
+    double4 s = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int k = ubCPtr[b]; k < ubCPtr[b + 1]; ++k) {
+        const unsigned int code = (unsigned int)ubCSrc[k];
+        double4 e = load_block(elemH, (int)(code & 0x7fffffffu));
+        if (code >> 31) {
+            const double tmp = e.y;
+            e.y = e.z;
+            e.z = tmp;
+        }
+        s.x += e.x;
+        s.y += e.y;
+        s.z += e.z;
+        s.w += e.w;
+    }
+    if (u == w) {
+        if (fixedMask[u]) {
+            const double fd = fixedDiag[u];
+            s = make_double4(fd, 0.0, 0.0, fd);
+        } else {
+            s.z = s.y;  // the lower in-block entry mirrors the accumulated upper one
+        }
+        vals[ubPosU[b]] = s;
+        jacobi_block(s, blockJacobi != 0, Minv + 4 * (size_t)u);
+    } else {
+        vals[ubPosU[b]] = s;
+        vals[ubPosL[b]] = make_double4(s.x, s.z, s.y, s.w);
+    }
+}
+
+template <typename HT>
+void launch_assemble(hipStream_t s, int nUB, const int* ubRow, const int* ubCol, const int* ubPosU,
+                     const int* ubPosL, const int* ubCPtr, const int* ubCSrc, const unsigned char* fixedMask,
+                     const double* fixedDiag, const HT* elemH, double* vals, double* Minv, int blockJacobi)
+{
+    using HV = typename std::conditional<std::is_same<HT, float>::value, float4, double4>::type;
+    hipLaunchKernelGGL(k_assemble<HV>, dim3(grid_for(nUB)), dim3(kBlock), 0, s, nUB, ubRow, ubCol, ubPosU, ubPosL,
+                       ubCPtr, ubCSrc, fixedMask, fixedDiag, reinterpret_cast<const HV*>(elemH),
+                       reinterpret_cast<double4*>(vals), Minv, blockJacobi);
+}
+template void launch_assemble<double>(hipStream_t, int, const int*, const int*, const int*, const int*, const int*,
+                                      const int*, const unsigned char*, const double*, const double*, double*,
+                                      double*, int);
+template void launch_assemble<float>(hipStream_t, int, const int*, const int*, const int*, const int*, const int*,
+                                     const int*, const unsigned char*, const double*, const float*, double*,
+                                     double*, int);
+
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_step_bound(TriSet T, const double2* __restrict__ UV,
+                                                       const double2* __restrict__ dir, unsigned long long* ordMin,
+                                                       double cap)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double bound = cap;
+    if (t < T.nF) {
+        const int vi[3] = {T.f0[t], T.f1[t], T.f2[t]};
+        double U[3][2], Vd[3][2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double2 u = UV[vi[k]], d = dir[vi[k]];
+            U[k][0] = u.x;
+            U[k][1] = u.y;
+            Vd[k][0] = d.x;
+            Vd[k][1] = d.y;
+        }
+        bound = sd_step_bound(U, Vd, cap);
+    }
+    // NaN bounds never lower the step (the reference's 'bound < stepSize' is false)
+    bound = wave_min(bound != bound ? cap : bound);
+    if ((threadIdx.x % kWave) == 0 && bound < cap) atomicMin(ordMin, dbl_to_ordered(bound));
+}
+
+void launch_step_bound(hipStream_t s, const TriSet& T, const double* UV, const double* dir,
+                       unsigned long long* ordMin, double cap)
+{
+    hipLaunchKernelGGL(k_step_bound, dim3(grid_for(T.nF)), dim3(kBlock), 0, s, T,
+                       reinterpret_cast<const double2*>(UV), reinterpret_cast<const double2*>(dir), ordMin, cap);
+}
+
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_linesearch(TriSet T, const double2* __restrict__ UV,
+                                                       const double2* __restrict__ dir, double alphaK0,
+                                                       double* partials)
+{
+    constexpr int K = kLSCand;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    CSum acc[3 * K];
+#pragma unroll
+    for (int k = 0; k < 3 * K; ++k) acc[k] = CSum{0.0, 0.0};
+    if (t < T.nF) {
+        const bool isMesh = t < T.nFmesh;
+        const int vi[3] = {T.f0[t], T.f1[t], T.f2[t]};
+        double2 u0[3], d[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            u0[k] = UV[vi[k]];
+            d[k] = dir[vi[k]];
+        }
+        const RestT<double> r = load_rest(T, t, isMesh);
+        double alpha = alphaK0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double p[3][2];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                p[q][0] = add_rn(u0[q].x, mul_rn(alpha, d[q].x));
+                p[q][1] = add_rn(u0[q].y, mul_rn(alpha, d[q].y));
+            }
+            double dbA;
+            const double E = sd_energy<double>(p[0][0], p[0][1], p[1][0], p[1][1], p[2][0], p[2][1], r, dbA);
+            acc[k].s = isMesh ? E : 0.0;
+            acc[K + k].s = isMesh ? 0.0 : E;
+            acc[2 * K + k].s = (dbA < 0.0) ? 1.0 : 0.0;  // inversion count (exact small integers)
+            alpha = alpha / 2.0;
+        }
+    }
+    publish_csum<3 * K>(acc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) k_finish_linesearch(const double* __restrict__ partials, int nblk,
+                                                              double sM, double sA, double* out)
+{
+    constexpr int K = kLSCand;
+    __shared__ double res[3 * K];
+    finish_csum<3 * K>(partials, nblk, res);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            out[k] = sM * res[k] + sA * res[K + k];
+            out[K + k] = res[k];
+            out[2 * K + k] = res[K + k];
+            out[3 * K + k] = res[2 * K + k];
+        }
+    }
+}
+
+void launch_linesearch(hipStream_t s, const TriSet& T, const double* UV, const double* dir, double alpha0, int k0,
+                       double* partials, double* out)
+{
+    double a = alpha0;
+    for (int k = 0; k < k0; ++k) a = a / 2.0;
+    const int grid = grid_for(T.nF);
+    hipLaunchKernelGGL(k_linesearch, dim3(grid), dim3(kBlock), 0, s, T, reinterpret_cast<const double2*>(UV),
+                       reinterpret_cast<const double2*>(dir), a, partials);
+    hipLaunchKernelGGL(k_finish_linesearch, dim3(1), dim3(kBlock), 0, s, partials, grid, T.sMesh, T.sAir, out);
+}
+
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_axpy_commit(long n, double alpha, const double* __restrict__ d,
+                                                        double* UV)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) UV[i] = add_rn(UV[i], mul_rn(alpha, d[i]));
+}
+
+void launch_axpy_commit(hipStream_t s, long n, double alpha, const double* d, double* UV)
+{
+    hipLaunchKernelGGL(k_axpy_commit, dim3(grid_for(n)), dim3(kBlock), 0, s, n, alpha, d, UV);
+}
+
+// ----------------------------------------------------------------------------
+// PCG (preconditioned conjugate gradients, Hestenes & Stiefel 1952) with a
+// 2x2-block Jacobi preconditioner, two kernels per iteration i:
+//   spmv(i)  : rz_i, rr_i = sums of update(i-1)'s per-block partials (every
+//              block reduces them itself, in the same order, so all blocks
+//              agree); convergence test rr_i <= tol^2 bb; beta = rz_i/rz_{i-1};
+//              p_new = z + beta p_old (own rows); q = A p_new, the neighbours'
+//              p_new formed on the fly from z and p_old, so no grid-wide wait
+//              sits between the update of p and A p; partials of p.q.
+//   update(i): pq = sum of spmv(i)'s partials; alpha = rz_i/pq; x += alpha p;
+//              r -= alpha q; z = M r; partials of r.z and r.r.
+// Scalars shared across kernels (rz_i, bb, done) are written by block 0 only,
+// into slots that no other block of the same kernel reads. No atomics and no
+// device-scope fences: kernel boundaries order everything.
+// Rows are split into one contiguous chunk per workgroup, so the neighbour
+// gathers of a chunk stay in the L2 of the XCD that runs it.
+__device__ __forceinline__ double2 apply_M(const double* Minv, int v, double2 r)
+{
+    const double4 m = reinterpret_cast<const double4*>(Minv)[v];
+    return make_double2(m.x * r.x + m.y * r.y, m.z * r.x + m.w * r.y);
+}
+
+// Sum of NV-value partial records (record b at part[b*NV + k]) over nblk
+// records, identical in every block.
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nblk, double (&out)[NV])
+{
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += part[(size_t)b * NV + k];
+    block_sum<NV>(acc);
+    __shared__ double sh[NV];
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k] = acc[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = sh[k];
+}
+
+__device__ __forceinline__ void row_chunk(int nV, int& r0, int& r1)
+{
+    const int chunk = (nV + (int)gridDim.x - 1) / (int)gridDim.x;
+    r0 = (int)blockIdx.x * chunk;
+    r1 = min(nV, r0 + chunk);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pcg_init(int nV, const double2* __restrict__ b,
+                                                     const double* __restrict__ Minv, double2* x, double2* r,
+                                                     double2* z, double* partU, PcgCtrl* ctrl)
+{
+    int r0, r1;
+    row_chunk(nV, r0, r1);
+    double acc[2] = {0.0, 0.0};
+    for (int v = r0 + threadIdx.x; v < r1; v += blockDim.x) {
+        const double2 bv = b[v];
+        x[v] = make_double2(0.0, 0.0);
+        r[v] = bv;
+        const double2 zv = apply_M(Minv, v, bv);
+        z[v] = zv;
+        acc[0] += bv.x * zv.x + bv.y * zv.y;
+        acc[1] += bv.x * bv.x + bv.y * bv.y;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        partU[2 * blockIdx.x] = acc[0];
+        partU[2 * blockIdx.x + 1] = acc[1];
+        if (blockIdx.x == 0) {
+            ctrl->done = 0;
+            ctrl->iter = 0;
+        }
+    }
+}
+
+int pcg_grid_update(int nV) { return std::max(1, std::min((nV + kBlock - 1) / kBlock, kPcgMaxGrid)); }
+int pcg_grid_spmv(int nV)
+{
+    constexpr int rowsPerPass = kBlock / kSpmvLanes;
+    return std::max(1, std::min((nV + rowsPerPass - 1) / rowsPerPass, kPcgMaxGrid));
+}
+
+void launch_pcg_init(hipStream_t s, int nV, const double* b, const double* Minv, double* x, double* r, double* z,
+                     double* partU, PcgCtrl* ctrl)
+{
+    hipLaunchKernelGGL(k_pcg_init, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, nV,
+                       reinterpret_cast<const double2*>(b), Minv, reinterpret_cast<double2*>(x),
+                       reinterpret_cast<double2*>(r), reinterpret_cast<double2*>(z), partU, ctrl);
+}
+
+// q_v = sum_k A_vk (z_w + beta p_w) over the row's blocks, kSpmvLanes lanes per row.
+__device__ __forceinline__ double2 row_apply(const int* __restrict__ rowPtr, const int* __restrict__ colIdx,
+                                             const double4* __restrict__ vals, const double2* __restrict__ z,
+                                             const double2* __restrict__ pOld, double beta, int v, int lane)
+{
+    double yx = 0.0, yy = 0.0;
+    const int k1 = rowPtr[v + 1];
+    for (int k = rowPtr[v] + lane; k < k1; k += kSpmvLanes) {
+        const int w = colIdx[k];
+        const double4 a = vals[k];
+        const double2 zw = z[w];
+        double px = zw.x, py = zw.y;
+        if (pOld) {
+            const double2 pw = pOld[w];
+            px += beta * pw.x;
+            py += beta * pw.y;
+        }
+        yx += a.x * px + a.y * py;
+        yy += a.z * px + a.w * py;
+    }
+#pragma unroll
+    for (int off = kSpmvLanes / 2; off > 0; off >>= 1) {
+        yx += __shfl_xor(yx, off, kSpmvLanes);
+        yy += __shfl_xor(yy, off, kSpmvLanes);
+    }
+    return make_double2(yx, yy);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* __restrict__ rowPtr,
+                                                     const int* __restrict__ colIdx,
+                                                     const double4* __restrict__ vals,
+                                                     const double2* __restrict__ z,
+                                                     const double2* __restrict__ pOld, double2* pNew, double2* q,
+                                                     const double* __restrict__ partUin, int nPartU, double* partS,
+                                                     PcgCtrl* ctrl)
+{
+    if (ctrl->done) return;
+    double ru[2];
+    reduce_partials<2>(partUin, nPartU, ru);
+    const double rz = ru[0], rr = ru[1];
+    const double bb = (it == 0) ? rr : ctrl->bb;
+    const bool conv = (bb == 0.0) || (rr <= ctrl->tol2 * bb);
+    const bool capped = it >= ctrl->maxit;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (it == 0) ctrl->bb = bb;
+        ctrl->rzSlot[it & 1] = rz;
+        ctrl->rr = rr;
+        ctrl->iter = it;
+        if (conv) ctrl->done = 1;
+        else if (capped) ctrl->done = 2;
+    }
+    if (conv || capped) return;
+    const double beta = (it == 0) ? 0.0 : rz / ctrl->rzSlot[(it + 1) & 1];
+    int r0, r1;
+    row_chunk(nV, r0, r1);
+    const int sub = threadIdx.x / kSpmvLanes, lane = threadIdx.x % kSpmvLanes;
+    constexpr int rowsPerPass = kBlock / kSpmvLanes;
+    double acc[1] = {0.0};
+    for (int v = r0 + sub; v < r1; v += rowsPerPass) {
+        const double2 y = row_apply(rowPtr, colIdx, vals, z, it == 0 ? nullptr : pOld, beta, v, lane);
+        if (lane == 0) {
+            const double2 zv = z[v];
+            double2 pn = zv;
+            if (it != 0) {
+                const double2 pv = pOld[v];
+                pn = make_double2(zv.x + beta * pv.x, zv.y + beta * pv.y);
+            }
+            pNew[v] = pn;
+            q[v] = y;
+            acc[0] += pn.x * y.x + pn.y * y.y;
+        }
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) partS[blockIdx.x] = acc[0];
+}
+
+void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const double* vals,
+                     const double* z, const double* pOld, double* pNew, double* q, const double* partUin,
+                     double* partS, PcgCtrl* ctrl)
+{
+    hipLaunchKernelGGL(k_pcg_spmv, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, it, nV, rowPtr, colIdx,
+                       reinterpret_cast<const double4*>(vals), reinterpret_cast<const double2*>(z),
+                       reinterpret_cast<const double2*>(pOld), reinterpret_cast<double2*>(pNew),
+                       reinterpret_cast<double2*>(q), partUin, pcg_grid_update(nV), partS, ctrl);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pcg_update(int it, int nV, const double2* __restrict__ p,
+                                                       const double2* __restrict__ q,
+                                                       const double* __restrict__ Minv, double2* x, double2* r,
+                                                       double2* z, const double* __restrict__ partS, int nPartS,
+                                                       double* partUout, PcgCtrl* ctrl)
+{
+    if (ctrl->done) return;
+    double pq[1];
+    reduce_partials<1>(partS, nPartS, pq);
+    if (!(pq[0] > 0.0)) {  // breakdown: A not positive definite along p
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->done = 3;
+        return;
+    }
+    const double alpha = ctrl->rzSlot[it & 1] / pq[0];
+    int r0, r1;
+    row_chunk(nV, r0, r1);
+    double acc[2] = {0.0, 0.0};
+    for (int v = r0 + threadIdx.x; v < r1; v += blockDim.x) {
+        const double2 pv = p[v], qv = q[v];
+        double2 xv = x[v], rv = r[v];
+        xv.x += alpha * pv.x;
+        xv.y += alpha * pv.y;
+        rv.x -= alpha * qv.x;
+        rv.y -= alpha * qv.y;
+        x[v] = xv;
+        r[v] = rv;
+        const double2 zv = apply_M(Minv, v, rv);
+        z[v] = zv;
+        acc[0] += rv.x * zv.x + rv.y * zv.y;
+        acc[1] += rv.x * rv.x + rv.y * rv.y;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        partUout[2 * blockIdx.x] = acc[0];
+        partUout[2 * blockIdx.x + 1] = acc[1];
+    }
+}
+
+void launch_pcg_update(hipStream_t s, int it, int nV, const double* pNew, const double* q, const double* Minv,
+                       double* x, double* r, double* z, const double* partS, double* partUout, PcgCtrl* ctrl)
+{
+    hipLaunchKernelGGL(k_pcg_update, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, it, nV,
+                       reinterpret_cast<const double2*>(pNew), reinterpret_cast<const double2*>(q), Minv,
+                       reinterpret_cast<double2*>(x), reinterpret_cast<double2*>(r), reinterpret_cast<double2*>(z),
+                       partS, pcg_grid_spmv(nV), partUout, ctrl);
+}
+
+// ----------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_spmv(int nV, const int* __restrict__ rowPtr,
+                                                 const int* __restrict__ colIdx, const double4* __restrict__ vals,
+                                                 const double2* __restrict__ x, double2* y)
+{
+    int r0, r1;
+    row_chunk(nV, r0, r1);
+    const int sub = threadIdx.x / kSpmvLanes, lane = threadIdx.x % kSpmvLanes;
+    constexpr int rowsPerPass = kBlock / kSpmvLanes;
+    for (int v = r0 + sub; v < r1; v += rowsPerPass) {
+        const double2 s = row_apply(rowPtr, colIdx, vals, x, nullptr, 0.0, v, lane);
+        if (lane == 0) y[v] = s;
+    }
+}
+
+void launch_spmv(hipStream_t s, int nV, const int* rowPtr, const int* colIdx, const double* vals, const double* x,
+                 double* y)
+{
+    hipLaunchKernelGGL(k_spmv, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, nV, rowPtr, colIdx,
+                       reinterpret_cast<const double4*>(vals), reinterpret_cast<const double2*>(x),
+                       reinterpret_cast<double2*>(y));
+}
+
+__global__ void __launch_bounds__(kBlock) k_expand_upper(int nnz, const double* __restrict__ a,
+                                                         const int* __restrict__ dst, const int* __restrict__ mir,
+                                                         double* vals)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const double v = a[k];
+    vals[dst[k]] = v;
+    if (mir[k] >= 0) vals[mir[k]] = v;
+}
+
+void launch_expand_upper(hipStream_t s, int nnz, const double* a, const int* dst, const int* mir, double* vals)
+{
+    hipLaunchKernelGGL(k_expand_upper, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, a, dst, mir, vals);
+}
+
+__global__ void __launch_bounds__(kBlock) k_jacobi_from_bsr(int nV, const int* __restrict__ diagPos,
+                                                            const double4* __restrict__ vals, double* Minv,
+                                                            int blockJacobi)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    jacobi_block(vals[diagPos[v]], blockJacobi != 0, Minv + 4 * (size_t)v);
+}
+
+void launch_jacobi_from_bsr(hipStream_t s, int nV, const int* diagPos, const double* vals, double* Minv,
+                            int blockJacobi)
+{
+    hipLaunchKernelGGL(k_jacobi_from_bsr, dim3(grid_for(nV)), dim3(kBlock), 0, s, nV, diagPos,
+                       reinterpret_cast<const double4*>(vals), Minv, blockJacobi);
+}
+
+}  // namespace oc
+
+namespace oc {
+
+__global__ void __launch_bounds__(kBlock) k_count_inverted(TriSet T, const double2* __restrict__ UV,
+                                                           unsigned int* count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int inv = 0;
+    if (t < T.nF) {
+        const double2 a = UV[T.f0[t]], b = UV[T.f1[t]], c = UV[T.f2[t]];
+        const double e0x = b.x - a.x, e0y = b.y - a.y, e1x = c.x - a.x, e1y = c.y - a.y;
+        inv = (__dsub_rn(mul_rn(e0x, e1y), mul_rn(e0y, e1x)) < 0.0) ? 1 : 0;
+    }
+    const unsigned long long m = __ballot(inv);
+    if ((threadIdx.x % kWave) == 0 && m) atomicAdd(count, (unsigned int)__popcll(m));
+}
+
+void launch_count_inverted(hipStream_t s, const TriSet& T, const double* UV, unsigned int* count)
+{
+    hipLaunchKernelGGL(k_count_inverted, dim3(grid_for(T.nF)), dim3(kBlock), 0, s, T,
+                       reinterpret_cast<const double2*>(UV), count);
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v)
+{
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, p, v);
+}
+
+__global__ void __launch_bounds__(kBlock) k_scale_copy(long n, double sign, const double* __restrict__ g,
+                                                       double* b)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = sign * g[i];
+}
+
+void launch_scale_copy(hipStream_t s, long n, double sign, const double* g, double* b)
+{
+    hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(n)), dim3(kBlock), 0, s, n, sign, g, b);
+}
+
+}  // namespace oc

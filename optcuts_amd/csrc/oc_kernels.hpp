@@ -1,0 +1,123 @@
+// Launch wrappers of the geometry-step kernels (host-callable).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace oc {
+
+// Triangle set of one system, SoA in HBM. Triangles [0, nFmesh) are mesh
+// triangles (weight triArea/surfaceArea, scale sMesh), [nFmesh, nF) are air
+// triangles (uniform weight 1, scale sAir).
+struct TriSet {
+    int nF = 0, nFmesh = 0;
+    const int* f0 = nullptr;
+    const int* f1 = nullptr;
+    const int* f2 = nullptr;
+    const double* triArea = nullptr;  // mesh: rest area (w = triArea / *surfaceArea)
+    const double* A2 = nullptr;
+    const double* E0 = nullptr;
+    const double* E1 = nullptr;
+    const double* E01 = nullptr;
+    const double* surfaceArea = nullptr;  // device scalar (mesh block)
+    int uniformMesh = 0;                  // mesh triangles use w = 1
+    double sMesh = 1.0, sAir = 1.0;
+};
+
+// Per-triangle rest features (TriMesh.cpp:343-404). Vrest is nV x 3
+// row-major. Triangles with rest area < areaThres get the equilateral
+// clamp (:373-383). sums[0] += sum of areas, sums[1] += sum of half-edge
+// lengths, sums[2] += count of zero-area triangles (compensated; reduced by
+// a one-block finisher kernel).
+void launch_rest_features(hipStream_t s, int nF, const int* f0, const int* f1, const int* f2, const double* Vrest,
+                          double areaThres, double* triArea, double* A2, double* E0, double* E1, double* E01,
+                          double* partials, double* sums);
+
+// Element pass. Writes (when non-null): per-triangle energy Eelem[t] (w
+// applied, block scale not applied), gradient contributions elemG[t*6..]
+// and projected Hessian upper blocks elemH[t*24..] (both multiplied by w
+// and the block scale). When esum is non-null, esum[0] = sum of mesh
+// energies, esum[1] = sum of air energies (compensated).
+void launch_element(hipStream_t s, const TriSet& T, const double* UV, double* Eelem, double* elemG, double* elemH,
+                    int project, double* partials, double* esum);
+
+// Triplets in the reference order (SymDirichletEnergy.cpp:527-548 with
+// IglUtils::addBlockToMatrix), unscaled: triangle t writes at tripOff[t];
+// freeMask bit k of vFree[t] tells whether local vertex k is free.
+void launch_triplets(hipStream_t s, const TriSet& T, const double* UV, const long long* tripOff,
+                     const unsigned char* vFree, int* I, int* J, double* V);
+
+// g[v] = sum of contributions (in triangle order) unless vertex v is fixed;
+// gsq = ||g||^2 (last block).
+void launch_gather_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSrc, const unsigned char* fixedMask,
+                        const double* elemG, double* g, double* partials, double* gsq);
+
+// Upper-block assembly into the full 2x2-block CSR; diagonal blocks also
+// produce the Jacobi preconditioner (blockJacobi: inverse 2x2 block,
+// else inverse diagonal entries).
+template <typename HT>
+void launch_assemble(hipStream_t s, int nUB, const int* ubRow, const int* ubCol, const int* ubPosU,
+                     const int* ubPosL, const int* ubCPtr, const int* ubCSrc, const unsigned char* fixedMask,
+                     const double* fixedDiag, const HT* elemH, double* vals, double* Minv, int blockJacobi);
+
+// Minimum over triangles of the injectivity step bound, folded into
+// *ordMin (order-preserving uint64 of a double; min is order-independent).
+void launch_step_bound(hipStream_t s, const TriSet& T, const double* UV, const double* dir,
+                       unsigned long long* ordMin, double cap);
+
+// Line-search energies at K candidate steps alpha_k = alpha0 / 2^(k0+k),
+// k = 0..K-1: out[k] = sMesh*E_mesh + sAir*E_air, out[K+k] = E_mesh,
+// out[2K+k] = E_air, out[3K+k] = number of triangles with negative
+// doubled area.
+constexpr int kLSCand = 8;
+void launch_linesearch(hipStream_t s, const TriSet& T, const double* UV, const double* dir, double alpha0, int k0,
+                       double* partials, double* out);
+
+// UV[i] = UV[i] + alpha * d[i] for i < n (two roundings, as the reference's
+// stepForward, Optimizer.cpp:666-669).
+void launch_axpy_commit(hipStream_t s, long n, double alpha, const double* d, double* UV);
+
+// ---- PCG on the 2x2-block CSR ------------------------------------------------
+constexpr int kSpmvLanes = 8;     // lanes per block-row in the SpMV
+constexpr int kPcgMaxGrid = 1024;  // workgroups of the PCG kernels (bounds the partial records)
+
+struct PcgCtrl {
+    double bb, tol2, rr, pad0;
+    double rzSlot[2];
+    int iter, maxit, done, pad1;
+};
+
+// Workgroups of the update/init kernels and of the SpMV kernel for nV rows.
+int pcg_grid_update(int nV);
+int pcg_grid_spmv(int nV);
+
+// partU: per-block (r.z, r.r) records written by init / update(i) and read by
+// spmv(i+1); partS: per-block p.q records of spmv(i) read by update(i).
+void launch_pcg_init(hipStream_t s, int nV, const double* b, const double* Minv, double* x, double* r, double* z,
+                     double* partU, PcgCtrl* ctrl);
+void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const double* vals,
+                     const double* z, const double* pOld, double* pNew, double* q, const double* partUin,
+                     double* partS, PcgCtrl* ctrl);
+void launch_pcg_update(hipStream_t s, int it, int nV, const double* pNew, const double* q, const double* Minv,
+                       double* x, double* r, double* z, const double* partS, double* partUout, PcgCtrl* ctrl);
+
+// y = A x (2x2-block CSR).
+void launch_spmv(hipStream_t s, int nV, const int* rowPtr, const int* colIdx, const double* vals, const double* x,
+                 double* y);
+
+// Expand the reference's upper-triangle CSR values (per DOF) into the full
+// block CSR: vals[dst[k]] = a[k] and, for off-diagonal-block or strictly
+// lower-in-block entries, vals[mir[k]] = a[k].
+void launch_expand_upper(hipStream_t s, int nnz, const double* a, const int* dst, const int* mir, double* vals);
+void launch_jacobi_from_bsr(hipStream_t s, int nV, const int* diagPos, const double* vals, double* Minv,
+                            int blockJacobi);
+
+}  // namespace oc
+
+namespace oc {
+// *count += number of triangles with negative doubled signed UV area.
+void launch_count_inverted(hipStream_t s, const TriSet& T, const double* UV, unsigned int* count);
+// *p = v (one thread; keeps device scalars out of hipMemset so streams stay capturable).
+void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v);
+// b = sign * g for n doubles.
+void launch_scale_copy(hipStream_t s, long n, double sign, const double* g, double* b);
+}  // namespace oc

@@ -1,0 +1,215 @@
+"""GPU parity of the HIP geometry-step path against the CPU oracle and the
+reference golden vectors (FP64 plain form).
+
+Tolerances: per-element FP64 values 1e-12 relative (the kernels contract
+multiply-adds into FMAs, so they differ from the reference's x86 rounding by
+a few ulps); integer pattern data bit-exact; PCG solutions against the
+reference SimplicialLDLT 1e-8 relative at PCG tolerance 1e-12; Newton
+energies per iteration 1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from optcuts_amd import capi
+from optcuts_amd.meshgen import height_field_disc
+
+from _helpers import ScaffoldOracle, distort, grid_air_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def _mesh(m, seed, amp):
+    V, F, UV = height_field_disc(m, seed=seed)
+    UVd = distort(UV, seed, amp)
+    assert O.check_inversion(F, UVd)
+    return V, F, UVd
+
+
+def test_device_is_gfx950(ctx):
+    print("arch", ctx.arch(), flush=True)
+    assert "gfx950" in ctx.arch()
+
+
+@pytest.mark.parametrize("m,seed,amp", [(11, 1, 0.05), (51, 0, 0.08), (30, 7, 0.15)])
+def test_rest_energy_gradient_match_oracle(ctx, m, seed, amp):
+    V, F, UV = _mesh(m, seed, amp)
+    feat = O.RestFeatures(V, F)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    sa, ael = ctx.mesh_stats()
+    assert sa == pytest.approx(feat.surfaceArea, rel=1e-14)
+    assert ael == pytest.approx(feat.avgEdgeLen, rel=1e-14)
+    for uni in (False, True):
+        Ee = ctx.energy_per_elem(uni)
+        np.testing.assert_allclose(Ee, O.energy_per_elem(F, UV, feat, uniform=uni), rtol=1e-12)
+        assert ctx.energy(uni) == pytest.approx(O.energy(F, UV, feat, uniform=uni), rel=1e-13)
+        g = ctx.gradient(uni)
+        go = O.gradient(F, UV, feat, fixed=(0,), uniform=uni)
+        assert np.abs(g - go).max() <= 1e-12 * np.abs(go).max()
+        assert g[0] == 0.0 and g[1] == 0.0
+
+
+def test_hessian_triplets_match_oracle(ctx):
+    V, F, UV = _mesh(21, 3, 0.15)
+    feat = O.RestFeatures(V, F)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    I, J, Vv = ctx.hessian_triplets()
+    Io, Jo, Vo = O.hessian_triplets(F, O.element_hessians(F, UV, feat), fixed=(0,))
+    np.testing.assert_array_equal(I, Io)
+    np.testing.assert_array_equal(J, Jo)
+    assert np.abs(Vv - Vo).max() <= 1e-12 * np.abs(Vo).max()
+    # indefinite elements exist, so the projection is exercised
+    Hr = O.element_hessians(F, UV, feat, project=False)
+    assert (np.linalg.eigvalsh(Hr)[:, 0] < -1e-8 * np.abs(Hr).max()).sum() > 10
+
+
+def test_step_bound_matches_oracle(ctx):
+    V, F, UV = _mesh(31, 4, 0.05)
+    ctx.set_mesh(V, F)
+    ctx.set_uv(UV)
+    rng = np.random.default_rng(0)
+    for k in range(8):
+        d = rng.normal(size=2 * V.shape[0]) * (0.01 * 2 ** k)
+        ref = O.init_step_size(F, UV, d, 1.0)
+        got = ctx.init_step_size(d, 1.0)
+        assert got == pytest.approx(ref, rel=1e-14, abs=0), (k, got, ref)
+    assert ctx.init_step_size(np.zeros(2 * V.shape[0]), 0.7) == 0.7
+
+
+def test_check_inversion(ctx):
+    V, F, UV = _mesh(11, 2, 0.05)
+    ctx.set_mesh(V, F)
+    ctx.set_uv(UV)
+    assert ctx.check_inversion()
+    UV2 = UV.copy()
+    UV2[60] += [0.3, 0.3]
+    ctx.set_uv(UV2)
+    assert ctx.check_inversion() == O.check_inversion(F, UV2) is False
+
+
+@pytest.mark.parametrize("m", [3, 11, 21])
+def test_solver_matches_reference_golden(ctx, golden, m):
+    p = "grid%d_" % m
+    V, F, UV = golden[p + "Vrest"], golden[p + "F"], golden[p + "UV"]
+    nV = V.shape[0]
+    ptr, nbr = O.vertex_neighbors(nV, F)
+    ia, ja = ctx.solver_set_pattern(nV, ptr, nbr, fixed=(0,))
+    np.testing.assert_array_equal(ia, golden[p + "ia"])
+    np.testing.assert_array_equal(ja, golden[p + "ja"])
+    feat = O.RestFeatures(V, F)
+    I, J, Vt = O.hessian_triplets(F, O.element_hessians(F, UV, feat), fixed=(0,))
+    ctx.solver_update_a(I, J, Vt)
+    np.testing.assert_array_equal(ctx.solver_get_a(), O.update_a(ia, ja, I, J, Vt))
+    np.testing.assert_allclose(ctx.solver_get_a(), golden[p + "a"], rtol=0,
+                               atol=1e-13 * np.abs(golden[p + "a"]).max())
+    for prec in (capi.PRECOND_JACOBI, capi.PRECOND_BLOCK_JACOBI):
+        ctx.solver_set_params(prec, rel_tol=1e-13, max_iter=200000)
+        ctx.solver_analyze_pattern()
+        ctx.solver_factorize()
+        for rhs, sol in ((-golden[p + "g"], golden[p + "d"]), (golden[p + "b2"], golden[p + "d2"])):
+            x = ctx.solver_solve(rhs)
+            it, rr = ctx.solver_last_stats()
+            assert rr <= 1e-13
+            assert np.linalg.norm(x - sol) <= 1e-8 * np.linalg.norm(sol), (m, prec, it)
+
+
+def test_solver_multiply_and_bad_triplet(ctx):
+    V, F, UV = _mesh(9, 5, 0.1)
+    nV = V.shape[0]
+    ptr, nbr = O.vertex_neighbors(nV, F)
+    ia, ja = ctx.solver_set_pattern(nV, ptr, nbr, fixed=(0,))
+    feat = O.RestFeatures(V, F)
+    I, J, Vt = O.hessian_triplets(F, O.element_hessians(F, UV, feat), fixed=(0,))
+    ctx.solver_update_a(I, J, Vt)
+    D = np.zeros((2 * nV, 2 * nV))
+    np.add.at(D, (I, J), Vt)
+    U = np.triu(D)
+    A = U + np.triu(U, 1).T
+    x = np.random.default_rng(1).normal(size=2 * nV)
+    np.testing.assert_allclose(ctx.solver_multiply(x), A @ x, rtol=0, atol=1e-12 * np.abs(A @ x).max())
+    with pytest.raises(capi.OCError) as ei:
+        ctx.solver_update_a(np.array([0, 2], np.int32), np.array([0, 2 * nV - 1], np.int32), np.ones(2))
+    assert ei.value.status == -4
+
+
+def test_newton_trace_matches_oracle_5k_faces(ctx):
+    """Config-0 disc (51x51 grid, 5,000 faces, seed 0), DistMin, no scaffold."""
+    V, F, UV = height_field_disc(51, seed=0)
+    nV = V.shape[0]
+    n_it = 40
+    rel = 1e-16 * nV / (nV - 1)
+    tr, UVo, _ = O.newton_run(V, F, UV, a0=1.0, max_iter=n_it, rel_gl2_tol=rel, allow_edec_rel_tol=False)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    E0 = ctx.newton_init(a0=1.0, rel_gl2_tol=rel, allow_edec_rel_tol=False, pcg_rel_tol=1e-12,
+                         pcg_max_iter=100000)
+    assert E0 == pytest.approx(tr[0, 0], rel=1e-13)
+    mism = 0
+    for k in range(1, len(tr)):
+        s, done = ctx.newton_step()
+        print(k, s["E_total"], tr[k, 0], s["pcg_iters"], s["halvings"], int(tr[k, 4]), flush=True)
+        assert s["E_total"] == pytest.approx(tr[k, 0], rel=1e-9), k
+        assert s["grad_sq"] == pytest.approx(tr[k, 2], rel=1e-5, abs=1e-14), k
+        mism += int(s["halvings"] != int(tr[k, 4]))
+        if done:
+            break
+    # the halving count may differ only in the rounding-noise tail
+    assert mism <= 5
+    UVg = ctx.get_uv()
+    assert O.check_inversion(F, UVg)
+
+
+def test_newton_step_with_scaffold_matches_oracle(ctx):
+    """Mesh + bijectivity air mesh: one Newton step against the oracle's composition."""
+    m = 11
+    V, F, UV = height_field_disc(m, seed=3)
+    airUV, Fair, bnd = grid_air_mesh(m, rings=2)
+    assert O.check_inversion(Fair, airUV)
+    a0, w_scaf = 1.0, 0.01
+    area_thres = np.sqrt(3.0) / 4.0 * (0.5 * O.RestFeatures(V, F).avgEdgeLen) ** 2
+    so = ScaffoldOracle(V, F, airUV, Fair, bnd, area_thres, a0, w_scaf)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    ctx.set_air_mesh(airUV, Fair, bnd, area_thres, w_scaf)
+    E0 = ctx.newton_init(a0=a0, rel_gl2_tol=1e-30, allow_edec_rel_tol=False, pcg_rel_tol=1e-13,
+                         pcg_max_iter=100000)
+    X = np.concatenate([UV, airUV[len(bnd):]])
+    Eo = so.energy(X)
+    assert E0 == pytest.approx(Eo[0], rel=1e-13)
+    for k in range(3):
+        X, Eo_new, step, halv, gsq = so.newton_step(X, None)
+        s, done = ctx.newton_step()
+        assert s["grad_sq"] == pytest.approx(gsq, rel=1e-9)
+        assert s["E_total"] == pytest.approx(Eo_new, rel=1e-10), k
+        assert s["step"] == pytest.approx(step, rel=1e-7), k
+        assert s["halvings"] == halv
+    np.testing.assert_allclose(ctx.get_uv(), X[: V.shape[0]], rtol=0, atol=1e-9)
+
+
+def test_newton_200k_faces_properties(ctx):
+    """Full-size workload (317x317 grid, 199,712 faces): size-independent properties."""
+    V, F, UV = height_field_disc(317, seed=0)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    E_prev = ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, pcg_rel_tol=1e-8,
+                             pcg_max_iter=50000)
+    feat = O.RestFeatures(V, F)
+    assert E_prev == pytest.approx(O.energy(F, UV, feat), rel=1e-12)
+    for k in range(3):
+        s, done = ctx.newton_step()
+        print(k, s, flush=True)
+        assert s["pcg_rel_res"] <= 1e-8
+        assert s["E_total"] < E_prev
+        E_prev = s["E_total"]
+    UVg = ctx.get_uv()
+    assert O.check_inversion(F, UVg)
+    assert O.energy(F, UVg, feat) == pytest.approx(E_prev, rel=1e-12)
