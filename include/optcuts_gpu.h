@@ -105,6 +105,9 @@ int oc_solver_get_pattern(oc_ctx* ctx, int* ia, int* ja);
  * in triplet order. OC_ERR_PATTERN if such a triplet has no slot. */
 int oc_solver_update_a(oc_ctx* ctx, long long nT, const int* I, const int* J, const double* V);
 int oc_solver_get_a(oc_ctx* ctx, double* a);
+/* Set the upper-triangle CSR values directly (nnz = ja length), as EigenLibSolver::update_a copies LinSysSolver::a
+ * into its matrix (EigenLibSolver.cpp:62-67); for callers that accumulate a themselves. */
+int oc_solver_set_a(oc_ctx* ctx, const double* a);
 /* analyze_pattern / factorize (EigenLibSolver.cpp:71-94): no symbolic phase; factorize builds the
  * preconditioner of the selected kind from the current values. */
 int oc_solver_analyze_pattern(oc_ctx* ctx);

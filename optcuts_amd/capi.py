@@ -64,6 +64,7 @@ _SIGS = {
     "oc_solver_get_pattern": (C.c_int, [_ctx, _ip, _ip]),
     "oc_solver_update_a": (C.c_int, [_ctx, C.c_longlong, _ip, _ip, _dp]),
     "oc_solver_get_a": (C.c_int, [_ctx, _dp]),
+    "oc_solver_set_a": (C.c_int, [_ctx, _dp]),
     "oc_solver_analyze_pattern": (C.c_int, [_ctx]),
     "oc_solver_factorize": (C.c_int, [_ctx]),
     "oc_solver_solve": (C.c_int, [_ctx, _dp, _dp]),
@@ -215,6 +216,11 @@ class Context:
 
     def solver_update_a(self, I, J, V):
         self._chk(self.L.oc_solver_update_a(self.h, len(I), _i32(I), _i32(J), _f64(V)))
+
+    def solver_set_a(self, a):
+        a = _f64(a)
+        assert a.shape == (self._sol_nnz,)
+        self._chk(self.L.oc_solver_set_a(self.h, a))
 
     def solver_get_a(self):
         a = np.zeros(self._sol_nnz)

@@ -937,6 +937,9 @@ int oc_solver_get_pattern(oc_ctx* c, int* ia, int* ja)
     return OC_OK;
 }
 
+static int upload_sol_a_impl(oc_ctx* c);
+static int upload_sol_a(oc_ctx* c) { return upload_sol_a_impl(c); }
+
 int oc_solver_update_a(oc_ctx* c, long long nT, const int* I, const int* J, const double* V)
 {
     if (!c) return OC_ERR_ARG;
@@ -962,6 +965,11 @@ int oc_solver_update_a(oc_ctx* c, long long nT, const int* I, const int* J, cons
         }
         c->solA[p - ja] += V[k];
     }
+    return upload_sol_a(c);
+}
+
+static int upload_sol_a_impl(oc_ctx* c)
+{
     const int nnz = (int)c->solA.size();
     OC_HIP(c, hipMemcpyAsync(c->dSolA.p, c->solA.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, c->stream));
     OC_HIP(c, hipMemsetAsync(c->sol.dVals.p, 0, sizeof(double) * 4 * (size_t)c->sol.nnzb(), c->stream));
@@ -971,6 +979,15 @@ int oc_solver_update_a(oc_ctx* c, long long nT, const int* I, const int* J, cons
     OC_HIP(c, hipStreamSynchronize(c->stream));
     c->solFactorized = false;
     return OC_OK;
+}
+
+int oc_solver_set_a(oc_ctx* c, const double* a)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0 && a, "oc_solver_set_a: no pattern");
+    (void)hipSetDevice(c->device);
+    std::copy(a, a + c->solA.size(), c->solA.begin());
+    return upload_sol_a(c);
 }
 
 int oc_solver_get_a(oc_ctx* c, double* a)
