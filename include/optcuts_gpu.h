@@ -37,7 +37,10 @@ enum {
     OC_ERR_NO_DEVICE = -9       /* no usable gfx950 device */
 };
 
-enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1 };
+/* Newton-system preconditioners: point Jacobi (the plain form), 2x2 block Jacobi, and smoothed-aggregation
+ * algebraic multigrid in FP32 under FP64 PCG (the chosen form; needs the UV map for its rigid near-null space,
+ * so it is available to the device-resident Newton driver, not to the drop-in oc_solver_*). */
+enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1, OC_PRECOND_AMG = 2 };
 
 typedef struct oc_ctx oc_ctx;
 
@@ -132,6 +135,7 @@ typedef struct {
     int converged;    /* 1 if ||g||^2 < targetGRes (no step taken) */
     int pcg_iters;    /* PCG iterations of this Newton step */
     double pcg_rel_res;
+    int pcg_fallback; /* 1 if the AMG-preconditioned solve failed and Jacobi-PCG solved this step */
 } oc_iter_stats;
 
 /* Optimizer::precompute (Optimizer.cpp:154-201) for energyParams = {a0}: sets targetGRes =
@@ -148,7 +152,8 @@ int oc_newton_step(oc_ctx* ctx, oc_iter_stats* stats, int* done);
 /* ---- timing of the dominant kernels (HIP events on the context stream) ---- */
 /* Average device time in ms (HIP events around each launch) of `reps` launches of: 0 element pass (E, g,
  * projected H), 1 block assembly, 2 one PCG iteration (both kernels), 3 line-search candidates, 4 step bound,
- * 5 PCG SpMV kernel, 6 PCG update kernel. Needs oc_newton_init; the current search direction is kept. */
+ * 5 PCG SpMV kernel, 6 PCG update kernel, 7 one AMG V-cycle, 8 AMG numeric set-up. Needs oc_newton_init (7 and
+ * 8 with OC_PRECOND_AMG and one oc_newton_step); the current search direction is kept. */
 int oc_time_kernel(oc_ctx* ctx, int which, int reps, double* ms);
 /* Sizes used for algorithmic-byte accounting: triangles, vertices (combined), stored blocks, upper blocks. */
 int oc_system_sizes(oc_ctx* ctx, long long* nF, long long* nV, long long* nnzBlocks, long long* nUpperBlocks);

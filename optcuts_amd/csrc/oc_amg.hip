@@ -1,0 +1,1103 @@
+// Smoothed-aggregation AMG on the GPU (see oc_amg.hpp for the method).
+#include "oc_amg.hpp"
+#include "oc_amg_structure.hpp"
+#include "oc_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace oc {
+
+namespace {
+
+constexpr int NS = 3;              // near-null-space vectors: two translations and a rotation
+constexpr int kAmgMaxGrid = 1024;  // workgroups of a row kernel (bounds partial records)
+constexpr int kCoarseMaxDofs = 64;  // the coarsest inverse lives in LDS (64 x 64 doubles)
+constexpr int kLanczosSteps = 10;  // Lanczos steps of the lambda_max estimate
+constexpr int kMaxLevels = 10;
+
+template <int BS>
+struct Lanes {
+    static constexpr int value = BS == 2 ? 8 : 16;
+};
+
+__device__ __forceinline__ void chunk_rows(int n, int& r0, int& r1)
+{
+    const int c = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    r0 = (int)blockIdx.x * c;
+    r1 = min(n, r0 + c);
+}
+
+int rows_grid(int n, int rowsPerPass)
+{
+    return std::max(1, std::min((n + rowsPerPass - 1) / rowsPerPass, kAmgMaxGrid));
+}
+
+// y = sum over the row of A_uw x_w, LANES lanes per row (result in every lane of the group)
+template <typename T, int BS, int LANES, typename X>
+__device__ __forceinline__ void row_mv(const int* __restrict__ rowPtr, const int* __restrict__ colIdx,
+                                       const T* __restrict__ vals, const X* __restrict__ x, int u, int lane,
+                                       T (&y)[BS])
+{
+#pragma unroll
+    for (int i = 0; i < BS; ++i) y[i] = T(0);
+    const int k1 = rowPtr[u + 1];
+    for (int k = rowPtr[u] + lane; k < k1; k += LANES) {
+        const int w = colIdx[k];
+        const T* a = vals + (size_t)k * BS * BS;
+        T xw[BS];
+#pragma unroll
+        for (int j = 0; j < BS; ++j) xw[j] = (T)x[(size_t)w * BS + j];
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int j = 0; j < BS; ++j) y[i] += a[i * BS + j] * xw[j];
+    }
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < BS; ++i) y[i] += __shfl_xor(y[i], off, LANES);
+}
+
+template <typename T, int BS>
+__device__ __forceinline__ void blk_mv(const T* __restrict__ M, const T (&x)[BS], T (&y)[BS])
+{
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j < BS; ++j) s += M[i * BS + j] * x[j];
+        y[i] = s;
+    }
+}
+
+// Chebyshev parameters on [lmax/30, lmax], lmax = 1.1 * lambda estimate
+struct ChebP {
+    double theta, delta, sigma;
+};
+__device__ __forceinline__ ChebP cheb_params(double lam)
+{
+    const double lmax = 1.1 * lam, lmin = lmax / 30.0;
+    ChebP p;
+    p.theta = 0.5 * (lmax + lmin);
+    p.delta = 0.5 * (lmax - lmin);
+    p.sigma = p.theta / p.delta;
+    return p;
+}
+
+// ---------------------------------------------------------------------------- numeric set-up kernels
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cvt(long n, const double* __restrict__ in, T* out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (T)in[i];
+}
+
+// Inverse of each diagonal block; an exactly zero diagonal entry (a coarse
+// DOF whose prolongator column vanished) is set to 1 in A as well.
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_blockinv(int n, const int* __restrict__ diagPos, T* vals, T* Dinv)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    T* a = vals + (size_t)diagPos[u] * BS * BS;
+    double m[BS][BS];
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+#pragma unroll
+        for (int j = 0; j < BS; ++j) m[i][j] = (double)a[i * BS + j];
+        if (m[i][i] == 0.0) {
+            m[i][i] = 1.0;
+            a[i * BS + i] = T(1);
+        }
+    }
+    double inv[BS][BS];
+    if (BS == 2) {
+        const double det = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+        inv[0][0] = m[1][1] / det;
+        inv[0][1] = -m[0][1] / det;
+        inv[1][0] = -m[1][0] / det;
+        inv[1][1] = m[0][0] / det;
+    } else {
+        // Gauss-Jordan on the small block (SPD: no pivoting needed)
+        double w[BS][2 * BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int j = 0; j < 2 * BS; ++j) w[i][j] = j < BS ? m[i][j] : (j - BS == i ? 1.0 : 0.0);
+#pragma unroll
+        for (int p = 0; p < BS; ++p) {
+            const double piv = 1.0 / w[p][p];
+#pragma unroll
+            for (int j = 0; j < 2 * BS; ++j) w[p][j] *= piv;
+#pragma unroll
+            for (int i = 0; i < BS; ++i) {
+                if (i == p) continue;
+                const double f = w[i][p];
+#pragma unroll
+                for (int j = 0; j < 2 * BS; ++j) w[i][j] -= f * w[p][j];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int j = 0; j < BS; ++j) inv[i][j] = w[i][BS + j];
+    }
+    T* o = Dinv + (size_t)u * BS * BS;
+#pragma unroll
+    for (int i = 0; i < BS; ++i)
+#pragma unroll
+        for (int j = 0; j < BS; ++j) o[i * BS + j] = (T)inv[i][j];
+}
+
+__device__ __forceinline__ double hash_unit(unsigned int i)
+{
+    i ^= i >> 16;
+    i *= 0x7feb352du;
+    i ^= i >> 15;
+    i *= 0x846ca68bu;
+    i ^= i >> 16;
+    return (double)(i & 0xffffff) / (double)0x800000 - 1.0;
+}
+
+// lambda_max(D^-1 A) by Lanczos in the D inner product (D^-1 A is self-adjoint
+// there; D is the block diagonal with zero diagonal entries set to 1, as in
+// k_blockinv). Ritz values approach the extreme eigenvalues far faster than
+// power iteration does when the top of the spectrum is clustered (large
+// meshes), and they never exceed lambda_max.
+//
+// One kernel per step j with every grid reduction folded into the next step:
+// W_{j-1} = D^-1 A v_{j-1} (W_{-1} = D^-1 h for a fixed pseudo-random h),
+// alpha_{j-1} = v_{j-1}^T A v_{j-1}, gamma_{j-1} = |W_{j-1}|_D^2 and
+//   beta_j^2 = gamma_{j-1} - alpha_{j-1}^2 - beta_{j-1}^2,
+//   v_j = (W_{j-1} - alpha_{j-1} v_{j-1} - beta_{j-1} v_{j-2}) / beta_j,
+// where the v_j entries of neighbour rows are formed on the fly from the
+// stored W_{j-1}, v_{j-1}, v_{j-2}. Every workgroup reduces the previous
+// step's partial records itself (same order, same result); workgroup 0 keeps
+// the coefficients. coef: alpha[kLanczosSteps], beta[kLanczosSteps], then the
+// number of valid steps (fewer after an invariant subspace is found).
+struct LanczosBufs {
+    double* v[3];  // v_j in v[j % 3]
+    double* W[2];  // W_j in W[(j + 2) % 2] (W_{-1} in W[1])
+};
+
+__device__ __forceinline__ void lanczos_coeffs(int j, const double* __restrict__ partIn, int nPartIn,
+                                               const double* __restrict__ coef, double& aprev, double& bprev,
+                                               double& bj)
+{
+    double acc[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nPartIn; b += blockDim.x) {
+        acc[0] += partIn[2 * b];
+        acc[1] += partIn[2 * b + 1];
+    }
+    block_sum<2>(acc);
+    __shared__ double sh[2];
+    if (threadIdx.x == 0) {
+        sh[0] = acc[0];
+        sh[1] = acc[1];
+    }
+    __syncthreads();
+    aprev = j == 0 ? 0.0 : sh[0];
+    bprev = j <= 1 ? 0.0 : coef[kLanczosSteps + j - 1];
+    const double gamma = sh[1];
+    const double b2 = gamma - aprev * aprev - bprev * bprev;
+    bj = (gamma > 0.0 && b2 > 1e-10 * gamma) ? sqrt(b2) : 0.0;
+}
+
+template <typename T, int BS, int LANES>
+__global__ void __launch_bounds__(kBlock) k_lanczos(int j, int n, const int* __restrict__ rowPtr,
+                                                    const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                    const T* __restrict__ Dinv, LanczosBufs buf,
+                                                    const double* __restrict__ partIn, int nPartIn, double* partOut,
+                                                    double* coef)
+{
+    if (j > 0 && coef[2 * kLanczosSteps] < (double)kLanczosSteps) return;  // stopped earlier
+    double aprev, bprev, bj;
+    lanczos_coeffs(j, partIn, nPartIn, coef, aprev, bprev, bj);
+    if (bj == 0.0) {  // the Krylov space is invariant: T so far is exact
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (j > 0) coef[j - 1] = aprev;
+            coef[2 * kLanczosSteps] = (double)j;
+        }
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (j > 0) coef[j - 1] = aprev;
+        coef[kLanczosSteps + j] = bj;
+    }
+    const double* __restrict__ Wp = buf.W[(j + 1) & 1];
+    const double* __restrict__ v1 = buf.v[(j + 2) % 3];
+    const double* __restrict__ v2 = buf.v[(j + 1) % 3];
+    double* vj = buf.v[j % 3];
+    double* Wj = buf.W[j & 1];
+    const double inv = 1.0 / bj;
+    int r0, r1;
+    chunk_rows(n, r0, r1);
+    const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    constexpr int RPP = kBlock / LANES;
+    double acc[2] = {0.0, 0.0};
+    for (int u = r0 + sub; u < r1; u += RPP) {
+        double y[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) y[i] = 0.0;
+        const int k1 = rowPtr[u + 1];
+        for (int k = rowPtr[u] + lane; k < k1; k += LANES) {
+            const int w = colIdx[k];
+            const T* a = vals + (size_t)k * BS * BS;
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const size_t q = (size_t)w * BS + c;
+                double x = Wp[q];
+                if (j >= 1) x -= aprev * v1[q];
+                if (j >= 2) x -= bprev * v2[q];
+                x *= inv;
+#pragma unroll
+                for (int i = 0; i < BS; ++i) y[i] += (double)a[i * BS + c] * x;
+            }
+        }
+#pragma unroll
+        for (int off = LANES / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) y[i] += __shfl_xor(y[i], off, LANES);
+        if (lane == 0) {
+            const T* D = Dinv + (size_t)u * BS * BS;
+#pragma unroll
+            for (int i = 0; i < BS; ++i) {
+                const size_t q = (size_t)u * BS + i;
+                double x = Wp[q];
+                if (j >= 1) x -= aprev * v1[q];
+                if (j >= 2) x -= bprev * v2[q];
+                x *= inv;
+                vj[q] = x;
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < BS; ++c) s += (double)D[i * BS + c] * y[c];
+                Wj[q] = s;
+                acc[0] += x * y[i];
+                acc[1] += y[i] * s;
+            }
+        }
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        partOut[2 * blockIdx.x] = acc[0];
+        partOut[2 * blockIdx.x + 1] = acc[1];
+    }
+}
+
+// W_{-1} = D^-1 h and the partial records of h^T D^-1 h = |W_{-1}|_D^2.
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_lanczos_start(int n, const T* __restrict__ Dinv, double* W,
+                                                          double* partOut, double* coef)
+{
+    int r0, r1;
+    chunk_rows(n, r0, r1);
+    double acc[2] = {0.0, 0.0};
+    for (int u = r0 + (int)threadIdx.x; u < r1; u += kBlock) {
+        double h[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) h[i] = hash_unit((unsigned)(u * BS + i));
+        const T* D = Dinv + (size_t)u * BS * BS;
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < BS; ++c) s += (double)D[i * BS + c] * h[c];
+            W[(size_t)u * BS + i] = s;
+            acc[1] += h[i] * s;
+        }
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        partOut[2 * blockIdx.x] = acc[0];
+        partOut[2 * blockIdx.x + 1] = acc[1];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) coef[2 * kLanczosSteps] = (double)kLanczosSteps;
+}
+
+// Largest eigenvalue of the Lanczos tridiagonal (Sturm-count bisection).
+__global__ void __launch_bounds__(kBlock) k_lanczos_finish(const double* __restrict__ partIn, int nPartIn,
+                                                           double* coef, double* lam)
+{
+    int m = (int)coef[2 * kLanczosSteps];
+    if (m == kLanczosSteps) {
+        double acc[2] = {0.0, 0.0};
+        for (int b = threadIdx.x; b < nPartIn; b += blockDim.x) {
+            acc[0] += partIn[2 * b];
+            acc[1] += partIn[2 * b + 1];
+        }
+        block_sum<2>(acc);
+        if (threadIdx.x == 0) coef[kLanczosSteps - 1] = acc[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (m == 0) {  // start vector in the null space of A: no information
+        *lam = 1.0;
+        return;
+    }
+    const double* al = coef;
+    const double* be = coef + kLanczosSteps;  // be[i] couples rows i-1 and i (i >= 1)
+    double lo = al[0], hi = al[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i > 0 ? fabs(be[i]) : 0.0) + (i + 1 < m ? fabs(be[i + 1]) : 0.0);
+        lo = fmin(lo, al[i] - r);
+        hi = fmax(hi, al[i] + r);
+    }
+    // count(x) = number of eigenvalues of T below x; lambda_max = sup{x : count(x) < m}
+    for (int it = 0; it < 100 && hi - lo > 1e-13 * fmax(fabs(hi), 1e-300); ++it) {
+        const double x = 0.5 * (lo + hi);
+        int cnt = 0;
+        double d = 1.0;
+        for (int i = 0; i < m; ++i) {
+            const double b2 = i > 0 ? be[i] * be[i] : 0.0;
+            d = al[i] - x - (i > 0 ? b2 / d : 0.0);
+            if (d == 0.0) d = -1e-300;
+            if (d < 0.0) ++cnt;
+        }
+        if (cnt == m)
+            hi = x;
+        else
+            lo = x;
+    }
+    *lam = hi;
+}
+
+// Tentative prolongator: per aggregate, B (rows of its members) = Q R by
+// Cholesky of the 3x3 Gram matrix (columns whose remaining norm vanishes are
+// dropped: zero Q column, zero R row). Q rows go to Pt (per member node),
+// R to Bout (the coarse node's near-null-space rows).
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_tentative(int nc, const int* __restrict__ aggPtr,
+                                                      const int* __restrict__ aggMem,
+                                                      const double2* __restrict__ UV,
+                                                      const double* __restrict__ Bin, T* Pt, double* Bout)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int m0 = aggPtr[c], m1 = aggPtr[c + 1];
+    double cu = 0.0, cv = 0.0;
+    if (BS == 2) {
+        for (int m = m0; m < m1; ++m) {
+            const double2 p = UV[aggMem[m]];
+            cu += p.x;
+            cv += p.y;
+        }
+        cu /= (double)(m1 - m0);
+        cv /= (double)(m1 - m0);
+    }
+    auto brows = [&](int u, double (&B)[BS][NS]) {
+        if (BS == 2) {
+            const double2 p = UV[u];
+            B[0][0] = 1.0; B[0][1] = 0.0; B[0][2] = -(p.y - cv);
+            B[1][0] = 0.0; B[1][1] = 1.0; B[1][2] = (p.x - cu);
+        } else {
+#pragma unroll
+            for (int i = 0; i < BS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) B[i][j] = Bin[(size_t)u * BS * NS + i * NS + j];
+        }
+    };
+    double G[NS][NS] = {};
+    for (int m = m0; m < m1; ++m) {
+        double B[BS][NS];
+        brows(aggMem[m], B);
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int a = 0; a < NS; ++a)
+#pragma unroll
+                for (int b = 0; b < NS; ++b) G[a][b] += B[i][a] * B[i][b];
+    }
+    double R[NS][NS] = {};
+    bool keep[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double d = G[j][j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) d -= R[k][j] * R[k][j];
+        keep[j] = G[j][j] > 0.0 && d > 1e-12 * G[j][j];
+        if (keep[j]) {
+            R[j][j] = sqrt(d);
+#pragma unroll
+            for (int k = j + 1; k < NS; ++k) {
+                double s = G[j][k];
+#pragma unroll
+                for (int i = 0; i < j; ++i) s -= R[i][j] * R[i][k];
+                R[j][k] = s / R[j][j];
+            }
+        }
+    }
+    for (int m = m0; m < m1; ++m) {
+        const int u = aggMem[m];
+        double B[BS][NS];
+        brows(u, B);
+        T* o = Pt + (size_t)u * BS * NS;
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            double q[NS];
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                double s = B[i][j];
+#pragma unroll
+                for (int k = 0; k < j; ++k) s -= q[k] * R[k][j];
+                q[j] = keep[j] ? s / R[j][j] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < NS; ++j) o[i * NS + j] = (T)q[j];
+        }
+    }
+    // Coarse near-null space in one global basis: the rows above rotate about
+    // the aggregate centroid (well conditioned), while the next level needs the
+    // rotation about one common point (the UV origin), so R becomes R C with
+    // C = [[1, 0, -c_v], [0, 1, c_u], [0, 0, 1]] (rotation about the origin =
+    // rotation about c + translation (-c_v, c_u)).
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        double* o = Bout + (size_t)c * NS * NS + i * NS;
+        o[0] = R[i][0];
+        o[1] = R[i][1];
+        o[2] = BS == 2 ? (-cv * R[i][0] + cu * R[i][1] + R[i][2]) : R[i][2];
+    }
+}
+
+// P(u,c) = [c == agg(u)] Pt_u - omega Dinv_u sum_k A_k Pt_{col(k)}, omega = 4/(3 lambda)
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_smooth_p(int nP, const int* __restrict__ pRow,
+                                                     const unsigned char* __restrict__ pIsTent,
+                                                     const int* __restrict__ psPtr, const int* __restrict__ psA,
+                                                     const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                     const T* __restrict__ Dinv, const T* __restrict__ Pt,
+                                                     const double* __restrict__ lam, T* P)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nP) return;
+    const int u = pRow[j];
+    T S[BS][NS] = {};
+    for (int q = psPtr[j]; q < psPtr[j + 1]; ++q) {
+        const int k = psA[q];
+        const T* a = vals + (size_t)k * BS * BS;
+        const T* pt = Pt + (size_t)colIdx[k] * BS * NS;
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int l = 0; l < BS; ++l)
+#pragma unroll
+                for (int c = 0; c < NS; ++c) S[i][c] += a[i * BS + l] * pt[l * NS + c];
+    }
+    const T omega = (T)(4.0 / (3.0 * *lam));
+    const T* D = Dinv + (size_t)u * BS * BS;
+    const T* ptu = Pt + (size_t)u * BS * NS;
+    const bool tent = pIsTent[j] != 0;
+    T* o = P + (size_t)j * BS * NS;
+#pragma unroll
+    for (int i = 0; i < BS; ++i)
+#pragma unroll
+        for (int c = 0; c < NS; ++c) {
+            T s = T(0);
+#pragma unroll
+            for (int l = 0; l < BS; ++l) s += D[i * BS + l] * S[l][c];
+            o[i * NS + c] = (tent ? ptu[i * NS + c] : T(0)) - omega * s;
+        }
+}
+
+// T block = sum over (A block, P block) pairs of A_k P_j (BS x NS)
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_ap(int nT, const int* __restrict__ cPtr, const int* __restrict__ cA,
+                                               const int* __restrict__ cP, const T* __restrict__ vals,
+                                               const T* __restrict__ P, T* Tv)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    T S[BS][NS] = {};
+    for (int q = cPtr[t]; q < cPtr[t + 1]; ++q) {
+        const T* a = vals + (size_t)cA[q] * BS * BS;
+        const T* p = P + (size_t)cP[q] * BS * NS;
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int l = 0; l < BS; ++l)
+#pragma unroll
+                for (int c = 0; c < NS; ++c) S[i][c] += a[i * BS + l] * p[l * NS + c];
+    }
+    T* o = Tv + (size_t)t * BS * NS;
+#pragma unroll
+    for (int i = 0; i < BS; ++i)
+#pragma unroll
+        for (int c = 0; c < NS; ++c) o[i * NS + c] = S[i][c];
+}
+
+// coarse block = sum over (P block, T block) pairs of P_j^T T_t (NS x NS)
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_ptap(int nAc, const int* __restrict__ cPtr, const int* __restrict__ cP,
+                                                 const int* __restrict__ cT, const T* __restrict__ P,
+                                                 const T* __restrict__ Tv, T* Ac)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nAc) return;
+    T S[NS][NS] = {};
+    for (int q = cPtr[a]; q < cPtr[a + 1]; ++q) {
+        const T* p = P + (size_t)cP[q] * BS * NS;
+        const T* t = Tv + (size_t)cT[q] * BS * NS;
+#pragma unroll
+        for (int l = 0; l < BS; ++l)
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) S[i][j] += p[l * NS + i] * t[l * NS + j];
+    }
+    T* o = Ac + (size_t)a * NS * NS;
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) o[i * NS + j] = S[i][j];
+}
+
+// Dense coarsest matrix (double) and its inverse by in-place Gauss-Jordan in
+// LDS, one workgroup (SPD: pivots on the diagonal):
+//   a_pp' = 1/a_pp, a_pj' = a_pj/a_pp, a_ip' = -a_ip/a_pp, a_ij' = a_ij - a_ip a_pj/a_pp.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_inverse(int nn, int bs, const int* __restrict__ rowPtr,
+                                                       const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                       double* Inv)
+{
+    __shared__ double W[kCoarseMaxDofs * kCoarseMaxDofs];
+    const int N = nn * bs;
+    for (int i = threadIdx.x; i < N * N; i += blockDim.x) W[i] = 0.0;
+    __syncthreads();
+    for (int u = 0; u < nn; ++u)
+        for (int k = rowPtr[u] + threadIdx.x; k < rowPtr[u + 1]; k += blockDim.x) {
+            const int w = colIdx[k];
+            for (int i = 0; i < bs; ++i)
+                for (int j = 0; j < bs; ++j) W[(u * bs + i) * N + w * bs + j] = (double)vals[(size_t)k * bs * bs + i * bs + j];
+        }
+    __syncthreads();
+    for (int p = 0; p < N; ++p) {
+        const double app = W[p * N + p];
+        const double piv = app != 0.0 ? 1.0 / app : 0.0;
+        for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+            const int i = e / N, j = e % N;
+            if (i != p && j != p) W[e] -= W[i * N + p] * W[p * N + j] * piv;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < N; j += blockDim.x)
+            if (j != p) {
+                W[p * N + j] *= piv;
+                W[j * N + p] *= -piv;
+            }
+        if (threadIdx.x == 0) W[p * N + p] = piv;
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < N * N; i += blockDim.x) Inv[i] = W[i];
+}
+
+// ---------------------------------------------------------------------------- V-cycle kernels
+// pre-smoothing, first Chebyshev step from x = 0: r = D^-1 b, d = r/theta, x = d
+template <typename T, int BS, typename BIN>
+__global__ void __launch_bounds__(kBlock) k_cheb_pre1(const int* __restrict__ stop, int n, const BIN* __restrict__ bin, T* b,
+                                                      const T* __restrict__ Dinv, const double* __restrict__ lam,
+                                                      T* x, T* r, T* d)
+{
+    if (stop && *stop) return;
+    const ChebP cp = cheb_params(*lam);
+    int r0, r1;
+    chunk_rows(n, r0, r1);
+    for (int u = r0 + threadIdx.x; u < r1; u += blockDim.x) {
+        T bu[BS], ru[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) bu[i] = (T)bin[(size_t)u * BS + i];
+        if ((const void*)bin != (const void*)b)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) b[(size_t)u * BS + i] = bu[i];
+        blk_mv<T, BS>(Dinv + (size_t)u * BS * BS, bu, ru);
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            const T di = ru[i] / (T)cp.theta;
+            r[(size_t)u * BS + i] = ru[i];
+            d[(size_t)u * BS + i] = di;
+            x[(size_t)u * BS + i] = di;
+        }
+    }
+}
+
+// Chebyshev step k (k = 0 after pre1/post1): w = A d; r -= D^-1 w;
+// d' = rho1 rho0 d + 2 rho1/delta r; x += d'. d is read for the neighbours,
+// so the new d goes to dout.
+template <typename T, int BS, int LANES, bool FINAL>
+__global__ void __launch_bounds__(kBlock) k_cheb_step(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
+                                                      const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                      const T* __restrict__ Dinv, const double* __restrict__ lam,
+                                                      const T* __restrict__ din, T* dout, T* r, T* x,
+                                                      const double* __restrict__ rFinal, double* zFinal,
+                                                      double* rzPart)
+{
+    if (stop && *stop) return;
+    const ChebP cp = cheb_params(*lam);
+    const double rho0 = 1.0 / cp.sigma;
+    const double rho1 = 1.0 / (2.0 * cp.sigma - rho0);
+    const T c1 = (T)(rho1 * rho0), c2 = (T)(2.0 * rho1 / cp.delta);
+    int r0, r1;
+    chunk_rows(n, r0, r1);
+    const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    constexpr int RPP = kBlock / LANES;
+    double acc[1] = {0.0};
+    for (int u = r0 + sub; u < r1; u += RPP) {
+        T w[BS];
+        row_mv<T, BS, LANES>(rowPtr, colIdx, vals, din, u, lane, w);
+        if (lane == 0) {
+            T dw[BS];
+            blk_mv<T, BS>(Dinv + (size_t)u * BS * BS, w, dw);
+#pragma unroll
+            for (int i = 0; i < BS; ++i) {
+                const size_t o = (size_t)u * BS + i;
+                const T rn = r[o] - dw[i];
+                const T dn = c1 * din[o] + c2 * rn;
+                const T xn = x[o] + dn;
+                r[o] = rn;
+                dout[o] = dn;
+                x[o] = xn;
+                if (FINAL) {
+                    const double z = (double)xn;
+                    zFinal[o] = z;
+                    acc[0] += rFinal[o] * z;
+                }
+            }
+        }
+    }
+    if (FINAL) {
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) rzPart[blockIdx.x] = acc[0];
+    }
+}
+
+// residual res = b - A x
+template <typename T, int BS, int LANES>
+__global__ void __launch_bounds__(kBlock) k_residual(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
+                                                     const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                     const T* __restrict__ b, const T* __restrict__ x, T* res)
+{
+    if (stop && *stop) return;
+    int r0, r1;
+    chunk_rows(n, r0, r1);
+    const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    constexpr int RPP = kBlock / LANES;
+    for (int u = r0 + sub; u < r1; u += RPP) {
+        T y[BS];
+        row_mv<T, BS, LANES>(rowPtr, colIdx, vals, x, u, lane, y);
+        if (lane == 0)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) res[(size_t)u * BS + i] = b[(size_t)u * BS + i] - y[i];
+    }
+}
+
+// post-smoothing first step from the prolongated x: r = D^-1 (b - A x); d = r/theta; xout = x + d
+template <typename T, int BS, int LANES>
+__global__ void __launch_bounds__(kBlock) k_cheb_post1(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
+                                                       const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                       const T* __restrict__ Dinv, const double* __restrict__ lam,
+                                                       const T* __restrict__ b, const T* __restrict__ xin,
+                                                       T* xout, T* r, T* d)
+{
+    if (stop && *stop) return;
+    const ChebP cp = cheb_params(*lam);
+    int r0, r1;
+    chunk_rows(n, r0, r1);
+    const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    constexpr int RPP = kBlock / LANES;
+    for (int u = r0 + sub; u < r1; u += RPP) {
+        T y[BS];
+        row_mv<T, BS, LANES>(rowPtr, colIdx, vals, xin, u, lane, y);
+        if (lane == 0) {
+            T res[BS], ru[BS];
+#pragma unroll
+            for (int i = 0; i < BS; ++i) res[i] = b[(size_t)u * BS + i] - y[i];
+            blk_mv<T, BS>(Dinv + (size_t)u * BS * BS, res, ru);
+#pragma unroll
+            for (int i = 0; i < BS; ++i) {
+                const size_t o = (size_t)u * BS + i;
+                const T di = ru[i] / (T)cp.theta;
+                r[o] = ru[i];
+                d[o] = di;
+                xout[o] = xin[o] + di;
+            }
+        }
+    }
+}
+
+// bc_c = sum over P blocks (u,c) of P^T res_u, kRestrictLanes lanes per coarse node
+constexpr int kRestrictLanes = 16;
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ stop, int nc, const int* __restrict__ rPtr,
+                                                     const int* __restrict__ rP, const int* __restrict__ pRow,
+                                                     const T* __restrict__ P, const T* __restrict__ res, T* bc)
+{
+    if (stop && *stop) return;
+    int c0, c1;
+    chunk_rows(nc, c0, c1);
+    constexpr int LN = kRestrictLanes, RPP = kBlock / kRestrictLanes;
+    const int sub = threadIdx.x / LN, lane = threadIdx.x % LN;
+    for (int c = c0 + sub; c < c1; c += RPP) {
+        T s[NS] = {};
+        for (int q = rPtr[c] + lane; q < rPtr[c + 1]; q += LN) {
+            const int j = rP[q];
+            const T* p = P + (size_t)j * BS * NS;
+            const T* ru = res + (size_t)pRow[j] * BS;
+#pragma unroll
+            for (int l = 0; l < BS; ++l)
+#pragma unroll
+                for (int i = 0; i < NS; ++i) s[i] += p[l * NS + i] * ru[l];
+        }
+#pragma unroll
+        for (int off = LN / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int i = 0; i < NS; ++i) s[i] += __shfl_xor(s[i], off, LN);
+        if (lane == 0)
+#pragma unroll
+            for (int i = 0; i < NS; ++i) bc[(size_t)c * NS + i] = s[i];
+    }
+}
+
+// x_u += sum over P blocks (u,c) of P xc_c
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_prolong(const int* __restrict__ stop, int n, const int* __restrict__ pPtr, const int* __restrict__ pCol,
+                                                    const T* __restrict__ P, const T* __restrict__ xc, T* x)
+{
+    if (stop && *stop) return;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    T s[BS] = {};
+    for (int j = pPtr[u]; j < pPtr[u + 1]; ++j) {
+        const T* p = P + (size_t)j * BS * NS;
+        const T* xcv = xc + (size_t)pCol[j] * NS;
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int c = 0; c < NS; ++c) s[i] += p[i * NS + c] * xcv[c];
+    }
+#pragma unroll
+    for (int i = 0; i < BS; ++i) x[(size_t)u * BS + i] += s[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_apply(const int* __restrict__ stop, int N,
+                                                     const double* __restrict__ Inv, const T* __restrict__ b, T* x)
+{
+    if (stop && *stop) return;
+    __shared__ double bs[kCoarseMaxDofs];
+    for (int j = threadIdx.x; j < N; j += blockDim.x) bs[j] = (double)b[j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < N; ++j) s += Inv[(size_t)j * N + i] * bs[j];  // symmetric: column i = row i
+        x[i] = (T)s;
+    }
+}
+
+// z = x (FP64) and partial records of r.z over the same row chunks as the level-0 step kernel
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_finish_z(const int* __restrict__ stop, int n2, const T* __restrict__ x, const double* __restrict__ r,
+                                                     double* z, double* rzPart)
+{
+    if (stop && *stop) return;
+    int r0, r1;
+    chunk_rows(n2, r0, r1);
+    double acc[1] = {0.0};
+    for (int i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+        const double zi = (double)x[i];
+        z[i] = zi;
+        acc[0] += r[i] * zi;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) rzPart[blockIdx.x] = acc[0];
+}
+
+template <typename T>
+void dalloc(T*& p, size_t n)
+{
+    if (hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) p = nullptr;
+}
+
+template <typename V>
+void up(hipStream_t s, int*& d, const std::vector<V>& h)
+{
+    dalloc(d, h.size());
+    if (d && !h.empty()) (void)hipMemcpyAsync(d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice, s);
+}
+
+}  // namespace
+
+// ============================================================================ host structure
+template <typename T>
+struct Amg<T>::Level : AmgHostLevel {
+    // device
+    int *dRowPtr = nullptr, *dColIdx = nullptr, *dDiagPos = nullptr;
+    int *dAggPtr = nullptr, *dAggMem = nullptr, *dPPtr = nullptr, *dPCol = nullptr, *dPRow = nullptr,
+        *dPsPtr = nullptr, *dPsA = nullptr, *dRPtr = nullptr, *dRP = nullptr, *dTcPtr = nullptr, *dTcA = nullptr,
+        *dTcP = nullptr, *dAcPtr = nullptr, *dAcP = nullptr, *dAcT = nullptr;
+    unsigned char* dPIsTent = nullptr;
+    T *vals = nullptr, *Dinv = nullptr, *Pt = nullptr, *P = nullptr, *Tv = nullptr;
+    T *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr, *d = nullptr, *d2 = nullptr;
+    double *Bnext = nullptr, *lz = nullptr, *lam = nullptr;  // lz: 5 Lanczos vectors, then alpha/beta/count
+    ~Level()
+    {
+        for (void* p : {(void*)dRowPtr, (void*)dColIdx, (void*)dDiagPos, (void*)dAggPtr, (void*)dAggMem, (void*)dPPtr,
+                        (void*)dPCol, (void*)dPRow, (void*)dPsPtr, (void*)dPsA, (void*)dRPtr, (void*)dRP,
+                        (void*)dTcPtr, (void*)dTcA, (void*)dTcP, (void*)dAcPtr, (void*)dAcP, (void*)dAcT,
+                        (void*)dPIsTent, (void*)vals, (void*)Dinv, (void*)Pt, (void*)P, (void*)Tv, (void*)b,
+                        (void*)x, (void*)x2, (void*)r, (void*)d, (void*)d2, (void*)Bnext, (void*)lz, (void*)lam})
+            if (p) (void)hipFree(p);
+    }
+};
+
+template <typename T>
+Amg<T>::~Amg()
+{
+    release();
+}
+
+template <typename T>
+void Amg<T>::release()
+{
+    for (Level* l : L) delete l;
+    L.clear();
+    if (dCoarseInv) (void)hipFree(dCoarseInv);
+    if (dPartials) (void)hipFree(dPartials);
+    dCoarseInv = nullptr;
+    dPartials = nullptr;
+    ready_ = false;
+}
+
+template <typename T>
+int Amg<T>::rz_records() const
+{
+    return rows_grid(L.empty() ? 1 : L[0]->n, kBlock / Lanes<2>::value);
+}
+
+template <typename T>
+long Amg<T>::level_rows(int l) const
+{
+    return (long)L[l]->n * L[l]->bs;
+}
+
+template <typename T>
+long Amg<T>::level_blocks(int l) const
+{
+    return (long)L[l]->colIdx.size();
+}
+
+template <typename T>
+int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
+                  const std::vector<unsigned char>& fixedMask, std::string& err)
+{
+    release();
+    std::vector<AmgHostLevel> H;
+    if (amg_build_structure(nV, rowPtr, colIdx, fixedMask, kCoarseMaxDofs, kMaxLevels, H, err) != 0) return -1;
+    for (auto& h : H) {
+        Level* lv = new Level();
+        static_cast<AmgHostLevel&>(*lv) = std::move(h);
+        L.push_back(lv);
+    }
+    // ---- device storage
+    for (size_t l = 0; l < L.size(); ++l) {
+        Level& A = *L[l];
+        const int bs = A.bs;
+        up(s, A.dRowPtr, A.rowPtr);
+        up(s, A.dColIdx, A.colIdx);
+        up(s, A.dDiagPos, A.diagPos);
+        dalloc(A.vals, A.colIdx.size() * bs * bs);
+        dalloc(A.Dinv, (size_t)A.n * bs * bs);
+        dalloc(A.b, (size_t)A.n * bs);
+        dalloc(A.x, (size_t)A.n * bs);
+        dalloc(A.x2, (size_t)A.n * bs);
+        dalloc(A.r, (size_t)A.n * bs);
+        dalloc(A.d, (size_t)A.n * bs);
+        dalloc(A.d2, (size_t)A.n * bs);
+        dalloc(A.lam, 1);
+        dalloc(A.lz, (size_t)5 * A.n * bs + 2 * kLanczosSteps + 1);
+        if (l + 1 < L.size()) {
+            up(s, A.dAggPtr, A.aggPtr);
+            up(s, A.dAggMem, A.aggMem);
+            up(s, A.dPPtr, A.pPtr);
+            up(s, A.dPCol, A.pCol);
+            up(s, A.dPRow, A.pRow);
+            up(s, A.dPsPtr, A.psPtr);
+            up(s, A.dPsA, A.psA);
+            up(s, A.dRPtr, A.rPtr);
+            up(s, A.dRP, A.rP);
+            up(s, A.dTcPtr, A.tcPtr);
+            up(s, A.dTcA, A.tcA);
+            up(s, A.dTcP, A.tcP);
+            up(s, A.dAcPtr, A.acPtr);
+            up(s, A.dAcP, A.acP);
+            up(s, A.dAcT, A.acT);
+            dalloc(A.dPIsTent, A.pIsTent.size());
+            if (A.dPIsTent && !A.pIsTent.empty())
+                (void)hipMemcpyAsync(A.dPIsTent, A.pIsTent.data(), A.pIsTent.size(), hipMemcpyHostToDevice, s);
+            dalloc(A.Pt, (size_t)A.n * bs * NS);
+            (void)hipMemsetAsync(A.Pt, 0, sizeof(T) * (size_t)A.n * bs * NS, s);
+            dalloc(A.P, A.pCol.size() * bs * NS);
+            dalloc(A.Tv, A.tCol.size() * bs * NS);
+            dalloc(A.Bnext, (size_t)A.nc * NS * NS);
+        }
+    }
+    const Level& C = *L.back();
+    nCoarse = C.n * C.bs;
+    if (nCoarse > kCoarseMaxDofs) {
+        err = "AMG: coarsening stalled above the dense coarsest size";
+        release();
+        return -1;
+    }
+    dalloc(dCoarseInv, (size_t)nCoarse * nCoarse);
+    nPartials = 2 * kAmgMaxGrid;  // two doubles per workgroup record
+    dalloc(dPartials, (size_t)nPartials * 2);
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) {
+        err = "AMG: device allocation or upload failed";
+        release();
+        return -1;
+    }
+    ready_ = true;
+    return 0;
+}
+
+template <typename T>
+void Amg<T>::numeric(hipStream_t s, const double* A0vals, const double* UV)
+{
+    for (size_t l = 0; l < L.size(); ++l) {
+        Level& A = *L[l];
+        const int bs = A.bs;
+        const long nvals = (long)A.colIdx.size() * bs * bs;
+        if (l == 0) hipLaunchKernelGGL(k_cvt<T>, dim3(grid_for(nvals)), dim3(kBlock), 0, s, nvals, A0vals, A.vals);
+        if (bs == 2)
+            hipLaunchKernelGGL((k_blockinv<T, 2>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, A.n, A.dDiagPos, A.vals,
+                               A.Dinv);
+        else
+            hipLaunchKernelGGL((k_blockinv<T, 3>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, A.n, A.dDiagPos, A.vals,
+                               A.Dinv);
+        if (l + 1 == L.size()) break;
+        // lambda_max(D^-1 A) by Lanczos
+        const int lanes = bs == 2 ? Lanes<2>::value : Lanes<3>::value;
+        const int g = rows_grid(A.n, kBlock / lanes);
+        double* part[2] = {dPartials, dPartials + nPartials};
+        const size_t nv = (size_t)A.n * bs;
+        double* coef = A.lz + 5 * nv;
+        LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv}};
+        if (bs == 2)
+            hipLaunchKernelGGL((k_lanczos_start<T, 2>), dim3(g), dim3(kBlock), 0, s, A.n, A.Dinv, lb.W[1], part[1],
+                               coef);
+        else
+            hipLaunchKernelGGL((k_lanczos_start<T, 3>), dim3(g), dim3(kBlock), 0, s, A.n, A.Dinv, lb.W[1], part[1],
+                               coef);
+        for (int j = 0; j < kLanczosSteps; ++j) {
+            if (bs == 2)
+                hipLaunchKernelGGL((k_lanczos<T, 2, Lanes<2>::value>), dim3(g), dim3(kBlock), 0, s, j, A.n,
+                                   A.dRowPtr, A.dColIdx, A.vals, A.Dinv, lb, part[(j + 1) & 1], g, part[j & 1], coef);
+            else
+                hipLaunchKernelGGL((k_lanczos<T, 3, Lanes<3>::value>), dim3(g), dim3(kBlock), 0, s, j, A.n,
+                                   A.dRowPtr, A.dColIdx, A.vals, A.Dinv, lb, part[(j + 1) & 1], g, part[j & 1], coef);
+        }
+        hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(kBlock), 0, s, part[(kLanczosSteps + 1) & 1], g, coef,
+                           A.lam);
+        // tentative prolongator, smoothing, Galerkin product
+        Level& N = *L[l + 1];
+        const int nP = (int)A.pCol.size(), nT = (int)A.tCol.size(), nAc = (int)N.colIdx.size();
+        const double* Bin = l == 0 ? nullptr : L[l - 1]->Bnext;
+        if (bs == 2) {
+            hipLaunchKernelGGL((k_tentative<T, 2>), dim3(grid_for(A.nc)), dim3(kBlock), 0, s, A.nc, A.dAggPtr,
+                               A.dAggMem, reinterpret_cast<const double2*>(UV), Bin, A.Pt, A.Bnext);
+            hipLaunchKernelGGL((k_smooth_p<T, 2>), dim3(grid_for(nP)), dim3(kBlock), 0, s, nP, A.dPRow, A.dPIsTent,
+                               A.dPsPtr, A.dPsA, A.dColIdx, A.vals, A.Dinv, A.Pt, A.lam, A.P);
+            hipLaunchKernelGGL((k_ap<T, 2>), dim3(grid_for(nT)), dim3(kBlock), 0, s, nT, A.dTcPtr, A.dTcA, A.dTcP,
+                               A.vals, A.P, A.Tv);
+            hipLaunchKernelGGL((k_ptap<T, 2>), dim3(grid_for(nAc)), dim3(kBlock), 0, s, nAc, A.dAcPtr, A.dAcP,
+                               A.dAcT, A.P, A.Tv, N.vals);
+        } else {
+            hipLaunchKernelGGL((k_tentative<T, 3>), dim3(grid_for(A.nc)), dim3(kBlock), 0, s, A.nc, A.dAggPtr,
+                               A.dAggMem, reinterpret_cast<const double2*>(UV), Bin, A.Pt, A.Bnext);
+            hipLaunchKernelGGL((k_smooth_p<T, 3>), dim3(grid_for(nP)), dim3(kBlock), 0, s, nP, A.dPRow, A.dPIsTent,
+                               A.dPsPtr, A.dPsA, A.dColIdx, A.vals, A.Dinv, A.Pt, A.lam, A.P);
+            hipLaunchKernelGGL((k_ap<T, 3>), dim3(grid_for(nT)), dim3(kBlock), 0, s, nT, A.dTcPtr, A.dTcA, A.dTcP,
+                               A.vals, A.P, A.Tv);
+            hipLaunchKernelGGL((k_ptap<T, 3>), dim3(grid_for(nAc)), dim3(kBlock), 0, s, nAc, A.dAcPtr, A.dAcP,
+                               A.dAcT, A.P, A.Tv, N.vals);
+        }
+    }
+    const Level& C = *L.back();
+    hipLaunchKernelGGL(k_dense_inverse<T>, dim3(1), dim3(256), 0, s, C.n, C.bs, C.dRowPtr, C.dColIdx, C.vals,
+                       dCoarseInv);
+}
+
+template <typename T>
+void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
+{
+    Level& A = *L[l];
+    if (l + 1 == (int)L.size()) {
+        hipLaunchKernelGGL(k_dense_apply<T>, dim3(1), dim3(256), 0, s, stop, nCoarse, dCoarseInv, A.b, A.x);
+        return;
+    }
+    Level& N = *L[l + 1];
+    const int gUp = rows_grid(A.n, kBlock);
+    if (A.bs == 2) {
+        constexpr int LN = Lanes<2>::value;
+        const int g = rows_grid(A.n, kBlock / LN);
+        if (l > 0)
+            hipLaunchKernelGGL((k_cheb_pre1<T, 2, T>), dim3(gUp), dim3(kBlock), 0, s, stop, A.n, A.b, A.b, A.Dinv, A.lam,
+                               A.x, A.r, A.d);
+        hipLaunchKernelGGL((k_cheb_step<T, 2, LN, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
+                           A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL((k_residual<T, 2, LN>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+                           A.b, A.x, A.r);
+        hipLaunchKernelGGL((k_restrict<T, 2>), dim3(rows_grid(A.nc, kBlock / kRestrictLanes)), dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP,
+                           A.dPRow, A.P, A.r, N.b);
+        vcycle(s, l + 1, stop);
+        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
+                           N.x, A.x);
+        hipLaunchKernelGGL((k_cheb_post1<T, 2, LN>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+                           A.Dinv, A.lam, A.b, A.x, A.x2, A.r, A.d);
+    } else {
+        constexpr int LN = Lanes<3>::value;
+        const int g = rows_grid(A.n, kBlock / LN);
+        hipLaunchKernelGGL((k_cheb_pre1<T, 3, T>), dim3(gUp), dim3(kBlock), 0, s, stop, A.n, A.b, A.b, A.Dinv, A.lam, A.x,
+                           A.r, A.d);
+        hipLaunchKernelGGL((k_cheb_step<T, 3, LN, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
+                           A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL((k_residual<T, 3, LN>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+                           A.b, A.x, A.r);
+        hipLaunchKernelGGL((k_restrict<T, 3>), dim3(rows_grid(A.nc, kBlock / kRestrictLanes)), dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP,
+                           A.dPRow, A.P, A.r, N.b);
+        vcycle(s, l + 1, stop);
+        hipLaunchKernelGGL((k_prolong<T, 3>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
+                           N.x, A.x);
+        hipLaunchKernelGGL((k_cheb_post1<T, 3, LN>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+                           A.Dinv, A.lam, A.b, A.x, A.x2, A.r, A.d);
+        hipLaunchKernelGGL((k_cheb_step<T, 3, LN, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
+                           A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x2, nullptr, nullptr, nullptr);
+        std::swap(A.x, A.x2);
+    }
+}
+
+template <typename T>
+void Amg<T>::apply(hipStream_t s, const double* r, double* z, double* rzPart, const int* stop)
+{
+    Level& A = *L[0];
+    constexpr int LN = Lanes<2>::value;
+    const int g = rows_grid(A.n, kBlock / LN);
+    if (L.size() == 1) {
+        // single level: the dense inverse is the whole preconditioner
+        hipLaunchKernelGGL(k_cvt<T>, dim3(grid_for((long)A.n * 2)), dim3(kBlock), 0, s, (long)A.n * 2, r, A.b);
+        hipLaunchKernelGGL(k_dense_apply<T>, dim3(1), dim3(256), 0, s, stop, nCoarse, dCoarseInv, A.b, A.x);
+        hipLaunchKernelGGL(k_finish_z<T>, dim3(g), dim3(kBlock), 0, s, stop, A.n * 2, A.x, r, z, rzPart);
+        return;
+    }
+    // level 0 pre-smoothing reads the FP64 residual and keeps a T copy in A.b
+    hipLaunchKernelGGL((k_cheb_pre1<T, 2, double>), dim3(rows_grid(A.n, kBlock)), dim3(kBlock), 0, s, stop, A.n, r, A.b,
+                       A.Dinv, A.lam, A.x, A.r, A.d);
+    vcycle(s, 0, stop);
+    // final post-smoothing step writes z (FP64) and the r.z partial records
+    hipLaunchKernelGGL((k_cheb_step<T, 2, LN, true>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+                       A.Dinv, A.lam, A.d, A.d2, A.r, A.x2, r, z, rzPart);
+}
+
+template class Amg<float>;
+template class Amg<double>;
+
+}  // namespace oc

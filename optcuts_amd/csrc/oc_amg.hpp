@@ -1,0 +1,76 @@
+// Smoothed-aggregation algebraic multigrid preconditioner for the Newton
+// systems of the geometry step (the chosen solve form).
+//
+// Method (Vanek, Mandel & Brezina, "Algebraic multigrid by smoothed
+// aggregation for second and fourth order elliptic problems", 1996):
+//   - aggregates: greedy aggregation of the vertex graph, once per sparsity
+//     pattern (pattern only, so it is reused across Newton iterations);
+//   - near-null space: the rigid motions of the UV plane per vertex,
+//     [[1, 0, -(v - v_a)], [0, 1, (u - u_a)]] about the aggregate centroid,
+//     orthonormalised per aggregate (tentative prolongator) with the R
+//     factors as the coarse near-null space;
+//   - prolongator P = (I - 4/(3 lambda) D^-1 A) P_tent with lambda the
+//     largest eigenvalue of D^-1 A (Lanczos in the D inner product, D = block
+//     diagonal);
+//   - Galerkin coarse operators P^T A P;
+//   - V-cycle with Chebyshev (degree 2) smoothing on D^-1 A over
+//     [lambda/30, 1.1 lambda] and an explicit inverse on the coarsest level.
+// Every numeric step runs on the GPU from index maps built on the host once
+// per pattern, so a Newton iteration re-derives the whole hierarchy without a
+// host round trip. Values are stored in T (float for the chosen form); the
+// coarsest inverse and the QR factorisations are computed in double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace oc {
+
+template <typename T>
+class Amg {
+public:
+    Amg() = default;
+    ~Amg();
+    Amg(const Amg&) = delete;
+    Amg& operator=(const Amg&) = delete;
+
+    // Structure of the hierarchy for the 2x2-block pattern (rowPtr/colIdx
+    // over nV vertices, sorted rows with diagonals). Fixed vertices take no
+    // part in aggregation. Allocates all device storage. Returns 0 or -1
+    // (message in err).
+    int setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
+              const std::vector<unsigned char>& fixedMask, std::string& err);
+
+    // Numeric hierarchy from the fine-level FP64 block values (same layout as
+    // the pattern) and the current UV (for the rigid near-null space).
+    void numeric(hipStream_t s, const double* A0vals, const double* UV);
+
+    // z = M r (one V-cycle) for FP64 vectors of 2*nV; writes per-workgroup
+    // partial records of r.z into rzPart (rz_records() of them). Every
+    // kernel returns at once while *stop is nonzero (a converged PCG).
+    void apply(hipStream_t s, const double* r, double* z, double* rzPart, const int* stop);
+
+    int rz_records() const;
+    int levels() const { return (int)L.size(); }
+    long level_rows(int l) const;
+    long level_blocks(int l) const;
+    bool ready() const { return ready_; }
+
+private:
+    struct Level;
+    std::vector<Level*> L;
+    int nCoarse = 0;           // DOFs of the dense coarsest level
+    double* dCoarseInv = nullptr;  // nCoarse x nCoarse, double
+    double* dPartials = nullptr;
+    int nPartials = 0;
+    bool ready_ = false;
+    void release();
+    void vcycle(hipStream_t s, int l, const int* stop);
+};
+
+extern template class Amg<float>;
+extern template class Amg<double>;
+
+}  // namespace oc

@@ -2,6 +2,7 @@
 // layout, sparsity-pattern and slot-map construction, and the
 // device-resident Newton step. Kernels live in oc_kernels.hip.
 #include "../../include/optcuts_gpu.h"
+#include "oc_amg.hpp"
 #include "oc_common.hpp"
 #include "oc_kernels.hpp"
 
@@ -115,6 +116,9 @@ struct oc_ctx {
     double pcgTol = 1e-10;
     int pcgMaxIter = 10000;
 
+    oc::Amg<double> amg;  // chosen-form preconditioner
+    bool amgValid = false;
+
     // ---- PCG workspace ---------------------------------------------------------
     DevBuf dX, dR, dZ, dP0, dP1, dQ, dB, dCtrl, dPcgPart;
     int lastIters = 0;
@@ -158,6 +162,7 @@ struct oc_ctx {
 namespace {
 
 constexpr int kMaxPartialBlocks = 1 << 16;
+constexpr int kRzRecords = 2048;  // r.z partial records of an external preconditioner
 
 int ensure_scratch(oc_ctx* c, long nItems)
 {
@@ -284,13 +289,14 @@ int ensure_pcg(oc_ctx* c, int nV)
         OC_HIP(c, hipMemsetAsync(c->dCtrl.p, 0, sizeof(PcgCtrl), c->stream));
     }
     OC_HIP(c, c->dPcgPart.ensure(sizeof(double) * (4 * (size_t)oc::pcg_grid_update(nV) +
-                                                   (size_t)oc::pcg_grid_spmv(nV))));
+                                                   (size_t)oc::pcg_grid_spmv(nV) + kRzRecords)));
     return ensure_scratch(c, 2L * nV);
 }
 
 // PCG on pattern P for right-hand side b (device), solution in c->dX.
 // Iterations are queued in batches; ctrl is read back between batches.
-int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxIter, int& iters, double& relRes)
+int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxIter, int& iters, double& relRes,
+            oc::Amg<double>* amg = nullptr)
 {
     const int nV = P.nV;
     PcgCtrl* ctrl = c->dCtrl.as<PcgCtrl>();
@@ -299,29 +305,40 @@ int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxI
     h.maxit = maxIter;
     OC_HIP(c, hipMemcpyAsync(&ctrl->tol2, &h.tol2, sizeof(double), hipMemcpyHostToDevice, c->stream));
     OC_HIP(c, hipMemcpyAsync(&ctrl->maxit, &h.maxit, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    const int GU = oc::pcg_grid_update(nV);
+    const int GU = oc::pcg_grid_update(nV), GS = oc::pcg_grid_spmv(nV);
     double* partU[2] = {c->dPcgPart.as<double>(), c->dPcgPart.as<double>() + 2 * (size_t)GU};
     double* partS = c->dPcgPart.as<double>() + 4 * (size_t)GU;
-    oc::launch_pcg_init(c->stream, nV, b, P.dMinv.as<double>(), c->dX.as<double>(), c->dR.as<double>(),
-                        c->dZ.as<double>(), partU[0], ctrl);
+    double* partRZ = partS + GS;
+    const double* Minv = amg ? nullptr : P.dMinv.as<double>();
+    oc::launch_pcg_init(c->stream, nV, b, Minv, c->dX.as<double>(), c->dR.as<double>(), c->dZ.as<double>(), partU[0],
+                        ctrl);
+    if (amg) amg->apply(c->stream, c->dR.as<double>(), c->dZ.as<double>(), partRZ, &ctrl->done);
     double* pbuf[2] = {c->dP0.as<double>(), c->dP1.as<double>()};
     int it = 0;
-    int batch = 32;
+    int batch = amg ? 8 : 32;
+    const int maxBatch = amg ? 32 : 512;
     for (;;) {
         for (int k = 0; k < batch; ++k, ++it) {
             double* pOld = pbuf[it & 1];
             double* pNew = pbuf[(it + 1) & 1];
-            oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.as<double>(),
-                                c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), partU[it & 1], partS, ctrl);
-            oc::launch_pcg_update(c->stream, it, nV, pNew, c->dQ.as<double>(), P.dMinv.as<double>(),
-                                  c->dX.as<double>(), c->dR.as<double>(), c->dZ.as<double>(), partS,
-                                  partU[(it + 1) & 1], ctrl);
+            const double* pu = partU[it & 1];
+            if (amg)
+                oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.as<double>(),
+                                    c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), partRZ, amg->rz_records(), 1,
+                                    pu + 1, GU, 2, partS, ctrl);
+            else
+                oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.as<double>(),
+                                    c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), pu, GU, 2, pu + 1, GU, 2,
+                                    partS, ctrl);
+            oc::launch_pcg_update(c->stream, it, nV, pNew, c->dQ.as<double>(), Minv, c->dX.as<double>(),
+                                  c->dR.as<double>(), c->dZ.as<double>(), partS, partU[(it + 1) & 1], ctrl);
+            if (amg) amg->apply(c->stream, c->dR.as<double>(), c->dZ.as<double>(), partRZ, &ctrl->done);
         }
         OC_HIP(c, hipGetLastError());
         OC_HIP(c, hipMemcpyAsync(&h, ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, c->stream));
         OC_HIP(c, hipStreamSynchronize(c->stream));
         if (h.done) break;
-        if (batch < 512) batch *= 2;
+        if (batch < maxBatch) batch *= 2;
     }
     iters = h.iter;
     relRes = h.bb > 0.0 ? std::sqrt(h.rr / h.bb) : 0.0;
@@ -511,6 +528,19 @@ int build_system(oc_ctx* c)
     OC_HIP(c, c->dG.ensure(sizeof(double) * 2 * (size_t)std::max(1, nVt)));
     if (ensure_pcg(c, nVt)) return OC_ERR_HIP;
     if (ensure_scratch(c, std::max(nFt, nUB))) return OC_ERR_HIP;
+    c->amgValid = false;
+    if (c->nPrecond == OC_PRECOND_AMG) {
+        std::string e;
+        if (c->amg.setup(c->stream, nVt, c->sys.rowPtr, c->sys.colIdx, c->fixedMaskSys, e) != 0) {
+            c->err = e;
+            return OC_ERR_HIP;
+        }
+        if (c->amg.rz_records() > kRzRecords) {
+            c->err = "AMG: too many partial records";
+            return OC_ERR_STATE;
+        }
+        c->amgValid = true;
+    }
     OC_HIP(c, hipStreamSynchronize(c->stream));
     c->sysReady = true;
     return OC_OK;
@@ -1080,7 +1110,7 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     if (!c) return OC_ERR_ARG;
     OC_ARG(c, c->nV > 0, "oc_newton_init: no mesh");
     OC_ARG(c, a0 > 0.0 && relGL2Tol > 0.0 && pcgRelTol > 0.0 && pcgMaxIter > 0 &&
-                  (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI),
+                  (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI || precond == OC_PRECOND_AMG),
            "oc_newton_init: bad parameters");
     (void)hipSetDevice(c->device);
     c->a0 = a0;
@@ -1143,9 +1173,19 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
                                 c->dFixedDiag.as<double>(), c->dElemH.as<double>(), c->sys.dVals.as<double>(),
                                 c->sys.dMinv.as<double>(), c->nPrecond == OC_PRECOND_BLOCK_JACOBI);
     oc::launch_scale_copy(s, 2L * nVt, -1.0, c->dG.as<double>(), c->dB.as<double>());
+    if (c->amgValid) c->amg.numeric(s, c->sys.dVals.as<double>(), c->dUV.as<double>());
     int it = 0;
     double rr = 0.0;
-    int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr);
+    int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr,
+                     c->amgValid ? &c->amg : nullptr);
+    if (c->amgValid && st != OC_OK) {
+        // the multigrid-preconditioned solve broke down or hit its cap: this
+        // Newton step falls back to Jacobi-PCG (same tolerance)
+        S->pcg_fallback = 1;
+        int it2 = 0;
+        st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, std::max(c->pcgMaxIter, 200000), it2, rr, nullptr);
+        it += it2;
+    }
     S->pcg_iters = it;
     S->pcg_rel_res = rr;
     if (st != OC_OK && st != OC_ERR_NOT_CONVERGED) return st;
@@ -1239,7 +1279,11 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
 int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
 {
     if (!c) return OC_ERR_ARG;
-    OC_ARG(c, ms && reps > 0 && which >= 0 && which <= 6, "oc_time_kernel: bad args");
+    OC_ARG(c, ms && reps > 0 && which >= 0 && which <= 8, "oc_time_kernel: bad args");
+    if ((which == 7 || which == 8) && !c->amgValid) {
+        c->err = "oc_time_kernel: the AMG preconditioner is not active";
+        return OC_ERR_STATE;
+    }
     if (!c->sysReady) {
         c->err = "oc_time_kernel: call oc_newton_init first";
         return OC_ERR_STATE;
@@ -1260,7 +1304,7 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
     auto spmv = [&]() {
         oc::launch_pcg_spmv(s, 1, c->nVt, c->sys.dRowPtr.as<int>(), c->sys.dColIdx.as<int>(),
                             c->sys.dVals.as<double>(), c->dZ.as<double>(), c->dP0.as<double>(), c->dP1.as<double>(),
-                            c->dQ.as<double>(), tU, tSo, ctrl);
+                            c->dQ.as<double>(), tU, GU, 2, tU + 1, GU, 2, tSo, ctrl);
     };
     auto update = [&]() {
         oc::launch_pcg_update(s, 1, c->nVt, c->dP1.as<double>(), c->dQ.as<double>(), c->sys.dMinv.as<double>(),
@@ -1298,8 +1342,15 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
         case 6:
             update();
             break;
+        case 7:
+            c->amg.apply(s, c->dR.as<double>(), c->dTmpVec.as<double>(), c->dPartials.as<double>(), nullptr);
+            break;
+        case 8:
+            c->amg.numeric(s, c->sys.dVals.as<double>(), c->dUV.as<double>());
+            break;
         }
     };
+    if (which == 7) OC_HIP(c, c->dTmpVec.ensure(sizeof(double) * 2 * (size_t)c->nVt));
     if (which == 2 || which == 5 || which == 6) {
         // Keep the PCG kernels doing full work on the current system: a
         // private copy of x (dTmpVec) so the last direction is not

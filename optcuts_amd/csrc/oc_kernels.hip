@@ -467,6 +467,9 @@ void launch_axpy_commit(hipStream_t s, long n, double alpha, const double* d, do
 //              sits between the update of p and A p; partials of p.q.
 //   update(i): pq = sum of spmv(i)'s partials; alpha = rz_i/pq; x += alpha p;
 //              r -= alpha q; z = M r; partials of r.z and r.r.
+// With an external preconditioner (Minv == nullptr) init/update leave z
+// alone and the caller applies M between update(i) and spmv(i+1), handing
+// its own r.z partial records to spmv.
 // Scalars shared across kernels (rz_i, bb, done) are written by block 0 only,
 // into slots that no other block of the same kernel reads. No atomics and no
 // device-scope fences: kernel boundaries order everything.
@@ -499,6 +502,18 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
     for (int k = 0; k < NV; ++k) out[k] = sh[k];
 }
 
+// Sum of one value per record (record b at part[b*stride]), identical in every block.
+__device__ __forceinline__ double reduce_strided(const double* __restrict__ part, int nrec, int stride)
+{
+    double acc[1] = {0.0};
+    for (int b = threadIdx.x; b < nrec; b += blockDim.x) acc[0] += part[(size_t)b * stride];
+    block_sum<1>(acc);
+    __shared__ double sh;
+    if (threadIdx.x == 0) sh = acc[0];
+    __syncthreads();
+    return sh;
+}
+
 __device__ __forceinline__ void row_chunk(int nV, int& r0, int& r1)
 {
     const int chunk = (nV + (int)gridDim.x - 1) / (int)gridDim.x;
@@ -506,6 +521,7 @@ __device__ __forceinline__ void row_chunk(int nV, int& r0, int& r1)
     r1 = min(nV, r0 + chunk);
 }
 
+template <bool APPLY_M>
 __global__ void __launch_bounds__(kBlock) k_pcg_init(int nV, const double2* __restrict__ b,
                                                      const double* __restrict__ Minv, double2* x, double2* r,
                                                      double2* z, double* partU, PcgCtrl* ctrl)
@@ -517,9 +533,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init(int nV, const double2* __re
         const double2 bv = b[v];
         x[v] = make_double2(0.0, 0.0);
         r[v] = bv;
-        const double2 zv = apply_M(Minv, v, bv);
-        z[v] = zv;
-        acc[0] += bv.x * zv.x + bv.y * zv.y;
+        if (APPLY_M) {
+            const double2 zv = apply_M(Minv, v, bv);
+            z[v] = zv;
+            acc[0] += bv.x * zv.x + bv.y * zv.y;
+        }
         acc[1] += bv.x * bv.x + bv.y * bv.y;
     }
     block_sum<2>(acc);
@@ -543,9 +561,14 @@ int pcg_grid_spmv(int nV)
 void launch_pcg_init(hipStream_t s, int nV, const double* b, const double* Minv, double* x, double* r, double* z,
                      double* partU, PcgCtrl* ctrl)
 {
-    hipLaunchKernelGGL(k_pcg_init, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, nV,
-                       reinterpret_cast<const double2*>(b), Minv, reinterpret_cast<double2*>(x),
-                       reinterpret_cast<double2*>(r), reinterpret_cast<double2*>(z), partU, ctrl);
+    if (Minv)
+        hipLaunchKernelGGL(k_pcg_init<true>, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, nV,
+                           reinterpret_cast<const double2*>(b), Minv, reinterpret_cast<double2*>(x),
+                           reinterpret_cast<double2*>(r), reinterpret_cast<double2*>(z), partU, ctrl);
+    else
+        hipLaunchKernelGGL(k_pcg_init<false>, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, nV,
+                           reinterpret_cast<const double2*>(b), Minv, reinterpret_cast<double2*>(x),
+                           reinterpret_cast<double2*>(r), reinterpret_cast<double2*>(z), partU, ctrl);
 }
 
 // q_v = sum_k A_vk (z_w + beta p_w) over the row's blocks, kSpmvLanes lanes per row.
@@ -581,25 +604,27 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* 
                                                      const double4* __restrict__ vals,
                                                      const double2* __restrict__ z,
                                                      const double2* __restrict__ pOld, double2* pNew, double2* q,
-                                                     const double* __restrict__ partUin, int nPartU, double* partS,
-                                                     PcgCtrl* ctrl)
+                                                     const double* __restrict__ partRz, int nRz, int sRz,
+                                                     const double* __restrict__ partRr, int nRr, int sRr,
+                                                     double* partS, PcgCtrl* ctrl)
 {
     if (ctrl->done) return;
-    double ru[2];
-    reduce_partials<2>(partUin, nPartU, ru);
-    const double rz = ru[0], rr = ru[1];
+    const double rz = reduce_strided(partRz, nRz, sRz);
+    const double rr = reduce_strided(partRr, nRr, sRr);
     const double bb = (it == 0) ? rr : ctrl->bb;
     const bool conv = (bb == 0.0) || (rr <= ctrl->tol2 * bb);
     const bool capped = it >= ctrl->maxit;
+    const bool indef = !conv && !(rz > 0.0);  // preconditioner not positive definite on r
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (it == 0) ctrl->bb = bb;
         ctrl->rzSlot[it & 1] = rz;
         ctrl->rr = rr;
         ctrl->iter = it;
         if (conv) ctrl->done = 1;
+        else if (indef) ctrl->done = 3;
         else if (capped) ctrl->done = 2;
     }
-    if (conv || capped) return;
+    if (conv || capped || indef) return;
     const double beta = (it == 0) ? 0.0 : rz / ctrl->rzSlot[(it + 1) & 1];
     int r0, r1;
     row_chunk(nV, r0, r1);
@@ -625,15 +650,16 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* 
 }
 
 void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const double* vals,
-                     const double* z, const double* pOld, double* pNew, double* q, const double* partUin,
-                     double* partS, PcgCtrl* ctrl)
+                     const double* z, const double* pOld, double* pNew, double* q, const double* partRz, int nRz,
+                     int sRz, const double* partRr, int nRr, int sRr, double* partS, PcgCtrl* ctrl)
 {
     hipLaunchKernelGGL(k_pcg_spmv, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, it, nV, rowPtr, colIdx,
                        reinterpret_cast<const double4*>(vals), reinterpret_cast<const double2*>(z),
                        reinterpret_cast<const double2*>(pOld), reinterpret_cast<double2*>(pNew),
-                       reinterpret_cast<double2*>(q), partUin, pcg_grid_update(nV), partS, ctrl);
+                       reinterpret_cast<double2*>(q), partRz, nRz, sRz, partRr, nRr, sRr, partS, ctrl);
 }
 
+template <bool APPLY_M>
 __global__ void __launch_bounds__(kBlock) k_pcg_update(int it, int nV, const double2* __restrict__ p,
                                                        const double2* __restrict__ q,
                                                        const double* __restrict__ Minv, double2* x, double2* r,
@@ -660,9 +686,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int it, int nV, const dou
         rv.y -= alpha * qv.y;
         x[v] = xv;
         r[v] = rv;
-        const double2 zv = apply_M(Minv, v, rv);
-        z[v] = zv;
-        acc[0] += rv.x * zv.x + rv.y * zv.y;
+        if (APPLY_M) {
+            const double2 zv = apply_M(Minv, v, rv);
+            z[v] = zv;
+            acc[0] += rv.x * zv.x + rv.y * zv.y;
+        }
         acc[1] += rv.x * rv.x + rv.y * rv.y;
     }
     block_sum<2>(acc);
@@ -675,10 +703,16 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int it, int nV, const dou
 void launch_pcg_update(hipStream_t s, int it, int nV, const double* pNew, const double* q, const double* Minv,
                        double* x, double* r, double* z, const double* partS, double* partUout, PcgCtrl* ctrl)
 {
-    hipLaunchKernelGGL(k_pcg_update, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, it, nV,
-                       reinterpret_cast<const double2*>(pNew), reinterpret_cast<const double2*>(q), Minv,
-                       reinterpret_cast<double2*>(x), reinterpret_cast<double2*>(r), reinterpret_cast<double2*>(z),
-                       partS, pcg_grid_spmv(nV), partUout, ctrl);
+    if (Minv)
+        hipLaunchKernelGGL(k_pcg_update<true>, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, it, nV,
+                           reinterpret_cast<const double2*>(pNew), reinterpret_cast<const double2*>(q), Minv,
+                           reinterpret_cast<double2*>(x), reinterpret_cast<double2*>(r),
+                           reinterpret_cast<double2*>(z), partS, pcg_grid_spmv(nV), partUout, ctrl);
+    else
+        hipLaunchKernelGGL(k_pcg_update<false>, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, it, nV,
+                           reinterpret_cast<const double2*>(pNew), reinterpret_cast<const double2*>(q), Minv,
+                           reinterpret_cast<double2*>(x), reinterpret_cast<double2*>(r),
+                           reinterpret_cast<double2*>(z), partS, pcg_grid_spmv(nV), partUout, ctrl);
 }
 
 // ----------------------------------------------------------------------------

@@ -90,13 +90,15 @@ struct PcgCtrl {
 int pcg_grid_update(int nV);
 int pcg_grid_spmv(int nV);
 
-// partU: per-block (r.z, r.r) records written by init / update(i) and read by
-// spmv(i+1); partS: per-block p.q records of spmv(i) read by update(i).
+// partU: per-block (r.z, r.r) records written by init / update(i) (r.z only
+// when Minv is given: the block-Jacobi M is applied inline); partS: per-block
+// p.q records of spmv(i) read by update(i). spmv(i) sums r.z over nRz
+// records of stride sRz and r.r over nRr records of stride sRr.
 void launch_pcg_init(hipStream_t s, int nV, const double* b, const double* Minv, double* x, double* r, double* z,
                      double* partU, PcgCtrl* ctrl);
 void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const double* vals,
-                     const double* z, const double* pOld, double* pNew, double* q, const double* partUin,
-                     double* partS, PcgCtrl* ctrl);
+                     const double* z, const double* pOld, double* pNew, double* q, const double* partRz, int nRz,
+                     int sRz, const double* partRr, int nRr, int sRr, double* partS, PcgCtrl* ctrl);
 void launch_pcg_update(hipStream_t s, int it, int nV, const double* pNew, const double* q, const double* Minv,
                        double* x, double* r, double* z, const double* partS, double* partUout, PcgCtrl* ctrl);
 

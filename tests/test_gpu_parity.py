@@ -213,3 +213,44 @@ def test_newton_200k_faces_properties(ctx):
     UVg = ctx.get_uv()
     assert O.check_inversion(F, UVg)
     assert O.energy(F, UVg, feat) == pytest.approx(E_prev, rel=1e-12)
+
+
+def test_newton_trace_amg_matches_oracle_5k_faces(ctx):
+    """Chosen-form solve (FP32 smoothed-aggregation AMG under FP64 PCG) at a tight tolerance: the Newton
+    trajectory stays on the oracle's (LDL^T) trajectory to 1e-9 per iteration."""
+    V, F, UV = height_field_disc(51, seed=0)
+    nV = V.shape[0]
+    n_it = 25
+    rel = 1e-16 * nV / (nV - 1)
+    tr, UVo, _ = O.newton_run(V, F, UV, a0=1.0, max_iter=n_it, rel_gl2_tol=rel, allow_edec_rel_tol=False)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    ctx.newton_init(a0=1.0, rel_gl2_tol=rel, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                    pcg_rel_tol=1e-12, pcg_max_iter=2000)
+    for k in range(1, len(tr)):
+        s, done = ctx.newton_step()
+        print(k, s["E_total"], tr[k, 0], s["pcg_iters"], flush=True)
+        assert s["E_total"] == pytest.approx(tr[k, 0], rel=1e-9), k
+        assert s["pcg_iters"] < 80, s
+        if done:
+            break
+
+
+def test_amg_solve_200k_faces(ctx):
+    """200k-face disc: AMG-PCG reaches 1e-10 in a mesh-size independent number of iterations and gives the
+    same Newton step as Jacobi-PCG."""
+    V, F, UV = height_field_disc(317, seed=0)
+    out = {}
+    for prec in (capi.PRECOND_AMG, capi.PRECOND_JACOBI):
+        ctx.set_mesh(V, F, fixed=(0,))
+        ctx.set_uv(UV)
+        ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=prec, pcg_rel_tol=1e-10,
+                        pcg_max_iter=100000)
+        s, _ = ctx.newton_step()
+        print(prec, s, flush=True)
+        out[prec] = (s, ctx.get_uv())
+    s_amg, uv_amg = out[capi.PRECOND_AMG]
+    s_jac, uv_jac = out[capi.PRECOND_JACOBI]
+    assert s_amg["pcg_iters"] < 100
+    assert s_amg["E_total"] == pytest.approx(s_jac["E_total"], rel=1e-8)
+    assert np.abs(uv_amg - uv_jac).max() < 1e-6
