@@ -37,12 +37,24 @@ enum {
     OC_ERR_NO_DEVICE = -9       /* no usable gfx950 device */
 };
 
-/* Newton-system preconditioners: point Jacobi (the plain form), 2x2 block Jacobi, and smoothed-aggregation
- * algebraic multigrid in FP32 under FP64 PCG (the chosen form; needs the UV map for its rigid near-null space,
- * so it is available to the device-resident Newton driver, not to the drop-in oc_solver_*). */
-enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1, OC_PRECOND_AMG = 2 };
+/* Newton-system preconditioners: point Jacobi (the plain form), 2x2 block Jacobi, smoothed-aggregation
+ * algebraic multigrid under FP64 PCG (needs the UV map for its rigid near-null space), and the learned sparse
+ * factor of an oc_gnn network (oc_newton_set_learned). AMG and LEARNED are available to the device-resident
+ * Newton driver, not to the drop-in oc_solver_*. None has a reference counterpart: the reference factorizes
+ * every system with SimplicialLDLT (EigenLibSolver.cpp:71-108). */
+enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1, OC_PRECOND_AMG = 2, OC_PRECOND_LEARNED = 3 };
+
+/* Arithmetic of the device-resident Newton driver.
+ * FP64: every element term, the Hessian values and the PCG in FP64 (the plain form; matches the reference's
+ *   FP64 run).
+ * MIXED: element energy, gradient and projected Hessian in FP32 (on vertex positions relative to each
+ *   triangle's first vertex), energy and gradient sums in compensated FP64, the Hessian stored in FP32 and the
+ *   multigrid hierarchy in FP32; the injectivity step-bound roots, the inversion test, the PCG vectors,
+ *   residual recurrence and scalars stay FP64. */
+enum { OC_PRECISION_FP64 = 0, OC_PRECISION_MIXED = 1 };
 
 typedef struct oc_ctx oc_ctx;
+typedef struct oc_gnn oc_gnn;
 
 /* ---- context ------------------------------------------------------------ */
 int oc_ctx_create(int device, oc_ctx** out);
@@ -135,7 +147,9 @@ typedef struct {
     int converged;    /* 1 if ||g||^2 < targetGRes (no step taken) */
     int pcg_iters;    /* PCG iterations of this Newton step */
     double pcg_rel_res;
-    int pcg_fallback; /* 1 if the AMG-preconditioned solve failed and Jacobi-PCG solved this step */
+    int pcg_fallback; /* 0, or why the multigrid-preconditioned solve was abandoned for Jacobi-PCG on this
+                         step: 1 breakdown (r.z or p.Ap not positive), 2 iteration cap */
+    int pcg_iters_first; /* iterations of the first (multigrid) attempt when pcg_fallback != 0 */
 } oc_iter_stats;
 
 /* Optimizer::precompute (Optimizer.cpp:154-201) for energyParams = {a0}: sets targetGRes =
@@ -144,6 +158,13 @@ typedef struct {
  * pcgRelTol/pcgMaxIter/precond select the Newton-system solve. */
 int oc_newton_init(oc_ctx* ctx, double a0, double relGL2Tol, int allowEDecRelTol, int precond, double pcgRelTol,
                    int pcgMaxIter, double* E0);
+/* Selects OC_PRECISION_FP64 (default) or OC_PRECISION_MIXED for the next oc_newton_init. No reference
+ * counterpart (the reference is FP64 throughout). */
+int oc_newton_set_precision(oc_ctx* ctx, int precision);
+/* Binds the network used by OC_PRECOND_LEARNED from the next oc_newton_init on (null unbinds). The context
+ * does not own it; while bound, the network's pattern and activations belong to the context's Newton system,
+ * and it must not be used from another context or through oc_gnn_set_system. No reference counterpart. */
+int oc_newton_set_learned(oc_ctx* ctx, oc_gnn* net);
 /* One Optimizer::solve(1) iteration (Optimizer.cpp:206-231 with solve_oneStep :505-573 and lineSearch
  * :575-652): gradient and convergence test, Hessian assembly, PCG direction, injectivity bound, line search,
  * step. *done = 1 when the solve returned 1 (gradient tolerance reached or line search stopped). */
@@ -157,6 +178,43 @@ int oc_newton_step(oc_ctx* ctx, oc_iter_stats* stats, int* done);
 int oc_time_kernel(oc_ctx* ctx, int which, int reps, double* ms);
 /* Sizes used for algorithmic-byte accounting: triangles, vertices (combined), stored blocks, upper blocks. */
 int oc_system_sizes(oc_ctx* ctx, long long* nF, long long* nV, long long* nnzBlocks, long long* nUpperBlocks);
+/* Copies the Newton system of the last oc_newton_step to the host (any pointer may be null): the full
+ * symmetric 2x2-block CSR (rowPtr nV+1, colIdx nnzBlocks; nV and nnzBlocks from oc_system_sizes), its block
+ * values as doubles (4 per block, row-major [xx, xy, yx, yy]; widened from FP32 under OC_PRECISION_MIXED), the
+ * right-hand side -g (2 nV) and the fixed-vertex mask (nV bytes). No reference counterpart: it exports the
+ * method's own Hessians (training data of the learned preconditioner, solver diagnostics). */
+int oc_newton_get_system(oc_ctx* ctx, int* rowPtr, int* colIdx, double* vals, double* rhs, unsigned char* fixedMask);
+
+/* ---- learned preconditioner (no reference counterpart) --------------------------------------------------
+ * A 3-layer edge-message graph network (hidden width 32) over the vertex graph of a Newton system predicts a
+ * sparse lower-triangular block factor G on the Hessian pattern; PCG then uses M^-1 = C^-T G^T G C^-1 with
+ * C_u C_u^T the diagonal blocks (G = I is block Jacobi). Message and update products run on BF16 matrix cores
+ * with FP32 accumulation. Training minimises the probe estimate of |G A^ G^T - I|_F^2 / (2 nV),
+ * A^ = C^-1 A C^-T, with the library's own backward pass and Adam. Parameters are FP32, flat, in the order
+ * W_enc (32x4), b_enc, then per layer W_m (32x80), b_m, W_h (32x64), b_h, then W_o (4x80), b_o, W_d (3x32),
+ * b_d; oc_gnn_num_params() of them. */
+int oc_gnn_num_params(void);
+int oc_gnn_create(int device, oc_gnn** out); /* parameters and optimizer state start at zero */
+int oc_gnn_destroy(oc_gnn* net);
+const char* oc_gnn_last_error(const oc_gnn* net);
+int oc_gnn_set_params(oc_gnn* net, const float* theta, int n);
+int oc_gnn_get_params(oc_gnn* net, float* theta, int n);
+int oc_gnn_get_grads(oc_gnn* net, float* grad, int n);
+/* A system to train on or to precondition: full symmetric 2x2-block CSR (rows sorted with their diagonal
+ * block; nnzBlocks = rowPtr[nV]), block values as doubles (4 per block, row-major), fixed-vertex mask. The
+ * edge maps are rebuilt only when the pattern changes. */
+int oc_gnn_set_system(oc_gnn* net, int nV, const int* rowPtr, const int* colIdx, const double* vals,
+                      const unsigned char* fixedMask);
+/* Forward pass; when Gblocks is non-null it receives the factor per CSR block (4 floats: lower and diagonal
+ * blocks hold G, upper blocks zero). */
+int oc_gnn_forward(oc_gnn* net, float* Gblocks);
+int oc_gnn_zero_grad(oc_gnn* net);
+/* Forward + loss over nProbes Rademacher probes drawn from seed; adds the gradient to the accumulated one. */
+int oc_gnn_loss_grad(oc_gnn* net, int nProbes, unsigned int seed, double* loss);
+/* theta -= lr * m_hat / (sqrt(v_hat) + eps) with the accumulated gradient (Adam; t counts from 1). */
+int oc_gnn_adam_step(oc_gnn* net, float lr, float beta1, float beta2, float eps, int t);
+/* z = M^-1 r for the current system after oc_gnn_forward (2 nV doubles each). */
+int oc_gnn_precond_apply(oc_gnn* net, const double* r, double* z);
 
 #ifdef __cplusplus
 }

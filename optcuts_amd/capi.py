@@ -23,6 +23,9 @@ STATUS = {
 PRECOND_JACOBI = 0
 PRECOND_BLOCK_JACOBI = 1
 PRECOND_AMG = 2
+PRECOND_LEARNED = 3
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
 
 
 class OCError(RuntimeError):
@@ -34,7 +37,8 @@ class OCError(RuntimeError):
 class IterStats(C.Structure):
     _fields_ = [("E_total", C.c_double), ("E_SD", C.c_double), ("E_scaf", C.c_double), ("grad_sq", C.c_double),
                 ("step", C.c_double), ("halvings", C.c_int), ("stopped", C.c_int), ("converged", C.c_int),
-                ("pcg_iters", C.c_int), ("pcg_rel_res", C.c_double), ("pcg_fallback", C.c_int)]
+                ("pcg_iters", C.c_int), ("pcg_rel_res", C.c_double), ("pcg_fallback", C.c_int),
+                ("pcg_iters_first", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -43,6 +47,8 @@ class IterStats(C.Structure):
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _ctx = C.c_void_p
 
 _SIGS = {
@@ -72,6 +78,22 @@ _SIGS = {
     "oc_solver_multiply": (C.c_int, [_ctx, _dp, _dp]),
     "oc_solver_set_params": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int]),
     "oc_solver_last_stats": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "oc_newton_get_system": (C.c_int, [_ctx, _vp, _vp, _vp, _vp, _vp]),
+    "oc_newton_set_learned": (C.c_int, [_ctx, _vp]),
+    "oc_gnn_num_params": (C.c_int, []),
+    "oc_gnn_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "oc_gnn_destroy": (C.c_int, [_vp]),
+    "oc_gnn_last_error": (C.c_char_p, [_vp]),
+    "oc_gnn_set_params": (C.c_int, [_vp, _fp, C.c_int]),
+    "oc_gnn_get_params": (C.c_int, [_vp, _fp, C.c_int]),
+    "oc_gnn_get_grads": (C.c_int, [_vp, _fp, C.c_int]),
+    "oc_gnn_set_system": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp, _u8p]),
+    "oc_gnn_forward": (C.c_int, [_vp, _vp]),
+    "oc_gnn_zero_grad": (C.c_int, [_vp]),
+    "oc_gnn_loss_grad": (C.c_int, [_vp, C.c_int, C.c_uint, C.POINTER(C.c_double)]),
+    "oc_gnn_adam_step": (C.c_int, [_vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]),
+    "oc_gnn_precond_apply": (C.c_int, [_vp, _dp, _dp]),
+    "oc_newton_set_precision": (C.c_int, [_ctx, C.c_int]),
     "oc_newton_init": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.POINTER(C.c_double)]),
     "oc_newton_step": (C.c_int, [_ctx, C.POINTER(IterStats), C.POINTER(C.c_int)]),
@@ -254,8 +276,14 @@ class Context:
         return it.value, rr.value
 
     # ---- Newton driver -------------------------------------------------------
+    def set_learned(self, net):
+        """Bind a Gnn for PRECOND_LEARNED (None unbinds); the context keeps a reference."""
+        self._learned = net
+        self._chk(self.L.oc_newton_set_learned(self.h, net.h if net is not None else None))
+
     def newton_init(self, a0=1.0, rel_gl2_tol=1e-12, allow_edec_rel_tol=True, precond=PRECOND_JACOBI,
-                    pcg_rel_tol=1e-10, pcg_max_iter=20000):
+                    pcg_rel_tol=1e-10, pcg_max_iter=20000, precision=PRECISION_FP64):
+        self._chk(self.L.oc_newton_set_precision(self.h, int(precision)))
         e = C.c_double()
         self._chk(self.L.oc_newton_init(self.h, float(a0), float(rel_gl2_tol), int(allow_edec_rel_tol),
                                         int(precond), float(pcg_rel_tol), int(pcg_max_iter), C.byref(e)))
@@ -272,7 +300,90 @@ class Context:
         self._chk(self.L.oc_time_kernel(self.h, int(which), int(reps), C.byref(ms)))
         return ms.value
 
+    def get_system(self):
+        """The last Newton system: (rowPtr, colIdx, vals (nnzb x 2 x 2), rhs (2 nV), fixed mask)."""
+        _, nV, nnzb, _ = self.system_sizes()
+        rp = np.zeros(nV + 1, np.int32)
+        ci = np.zeros(max(nnzb, 1), np.int32)
+        va = np.zeros(4 * max(nnzb, 1))
+        b = np.zeros(2 * nV)
+        fm = np.zeros(nV, np.uint8)
+        self._chk(self.L.oc_newton_get_system(self.h, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, b.ctypes.data,
+                                              fm.ctypes.data))
+        return rp, ci[:nnzb], va[:4 * nnzb].reshape(-1, 2, 2), b, fm
+
     def system_sizes(self):
         v = [C.c_longlong() for _ in range(4)]
         self._chk(self.L.oc_system_sizes(self.h, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
+
+
+class Gnn:
+    """Learned preconditioner network on GPU `device` (see include/optcuts_gpu.h)."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = C.c_void_p()
+        st = self.L.oc_gnn_create(device, C.byref(h))
+        if st != OC_OK:
+            raise OCError(st, "no usable GPU device %d" % device)
+        self.h = h
+        self.n_params = self.L.oc_gnn_num_params()
+        self.nV = 0
+        self.rp = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oc_gnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st):
+        if st != OC_OK:
+            raise OCError(st, self.L.oc_gnn_last_error(self.h).decode())
+
+    def set_params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        self._chk(self.L.oc_gnn_set_params(self.h, t, t.size))
+
+    def get_params(self):
+        t = np.zeros(self.n_params, np.float32)
+        self._chk(self.L.oc_gnn_get_params(self.h, t, t.size))
+        return t
+
+    def get_grads(self):
+        t = np.zeros(self.n_params, np.float32)
+        self._chk(self.L.oc_gnn_get_grads(self.h, t, t.size))
+        return t
+
+    def set_system(self, rp, ci, vals, fixed):
+        rp, ci = _i32(rp), _i32(ci)
+        self._chk(self.L.oc_gnn_set_system(self.h, len(rp) - 1, rp, ci, _f64(np.asarray(vals).reshape(-1)),
+                                           np.ascontiguousarray(fixed, dtype=np.uint8)))
+        self.nV = len(rp) - 1
+        self.rp, self.ci = rp, ci
+
+    def forward(self):
+        """Factor per CSR block (nnzb x 2 x 2, float32)."""
+        out = np.zeros(4 * len(self.ci), np.float32)
+        self._chk(self.L.oc_gnn_forward(self.h, out.ctypes.data))
+        return out.reshape(-1, 2, 2)
+
+    def zero_grad(self):
+        self._chk(self.L.oc_gnn_zero_grad(self.h))
+
+    def loss_grad(self, n_probes, seed):
+        loss = C.c_double()
+        self._chk(self.L.oc_gnn_loss_grad(self.h, int(n_probes), int(seed) & 0xFFFFFFFF, C.byref(loss)))
+        return loss.value
+
+    def adam_step(self, lr, t, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._chk(self.L.oc_gnn_adam_step(self.h, float(lr), float(beta1), float(beta2), float(eps), int(t)))
+
+    def precond_apply(self, r):
+        r = _f64(r)
+        z = np.zeros_like(r)
+        self._chk(self.L.oc_gnn_precond_apply(self.h, r, z))
+        return z

@@ -87,8 +87,8 @@ __device__ __forceinline__ ChebP cheb_params(double lam)
 }
 
 // ---------------------------------------------------------------------------- numeric set-up kernels
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_cvt(long n, const double* __restrict__ in, T* out)
+template <typename TI, typename T>
+__global__ void __launch_bounds__(kBlock) k_cvt(long n, const TI* __restrict__ in, T* out)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = (T)in[i];
@@ -960,13 +960,20 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
 }
 
 template <typename T>
-void Amg<T>::numeric(hipStream_t s, const double* A0vals, const double* UV)
+void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double* UV)
 {
     for (size_t l = 0; l < L.size(); ++l) {
         Level& A = *L[l];
         const int bs = A.bs;
         const long nvals = (long)A.colIdx.size() * bs * bs;
-        if (l == 0) hipLaunchKernelGGL(k_cvt<T>, dim3(grid_for(nvals)), dim3(kBlock), 0, s, nvals, A0vals, A.vals);
+        if (l == 0) {
+            if (vals32)
+                hipLaunchKernelGGL((k_cvt<float, T>), dim3(grid_for(nvals)), dim3(kBlock), 0, s, nvals,
+                                   static_cast<const float*>(A0vals), A.vals);
+            else
+                hipLaunchKernelGGL((k_cvt<double, T>), dim3(grid_for(nvals)), dim3(kBlock), 0, s, nvals,
+                                   static_cast<const double*>(A0vals), A.vals);
+        }
         if (bs == 2)
             hipLaunchKernelGGL((k_blockinv<T, 2>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, A.n, A.dDiagPos, A.vals,
                                A.Dinv);
@@ -1083,7 +1090,7 @@ void Amg<T>::apply(hipStream_t s, const double* r, double* z, double* rzPart, co
     const int g = rows_grid(A.n, kBlock / LN);
     if (L.size() == 1) {
         // single level: the dense inverse is the whole preconditioner
-        hipLaunchKernelGGL(k_cvt<T>, dim3(grid_for((long)A.n * 2)), dim3(kBlock), 0, s, (long)A.n * 2, r, A.b);
+        hipLaunchKernelGGL((k_cvt<double, T>), dim3(grid_for((long)A.n * 2)), dim3(kBlock), 0, s, (long)A.n * 2, r, A.b);
         hipLaunchKernelGGL(k_dense_apply<T>, dim3(1), dim3(256), 0, s, stop, nCoarse, dCoarseInv, A.b, A.x);
         hipLaunchKernelGGL(k_finish_z<T>, dim3(g), dim3(kBlock), 0, s, stop, A.n * 2, A.x, r, z, rzPart);
         return;

@@ -23,16 +23,32 @@
 
 #include <hip/hip_runtime.h>
 
+#include "oc_precond.hpp"
+
 #include <string>
 #include <vector>
 
 namespace oc {
 
+// Precision-independent interface of the hierarchy (the Newton driver holds
+// one of the instantiations below).
+class AmgBase : public Preconditioner {
+public:
+    ~AmgBase() override = default;
+    virtual int setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
+                      const std::vector<unsigned char>& fixedMask, std::string& err) = 0;
+    virtual void numeric(hipStream_t s, const void* A0vals, int vals32, const double* UV) = 0;
+    virtual int levels() const = 0;
+    virtual long level_rows(int l) const = 0;
+    virtual long level_blocks(int l) const = 0;
+    virtual bool ready() const = 0;
+};
+
 template <typename T>
-class Amg {
+class Amg : public AmgBase {
 public:
     Amg() = default;
-    ~Amg();
+    ~Amg() override;
     Amg(const Amg&) = delete;
     Amg& operator=(const Amg&) = delete;
 
@@ -41,22 +57,23 @@ public:
     // part in aggregation. Allocates all device storage. Returns 0 or -1
     // (message in err).
     int setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
-              const std::vector<unsigned char>& fixedMask, std::string& err);
+              const std::vector<unsigned char>& fixedMask, std::string& err) override;
 
-    // Numeric hierarchy from the fine-level FP64 block values (same layout as
-    // the pattern) and the current UV (for the rigid near-null space).
-    void numeric(hipStream_t s, const double* A0vals, const double* UV);
+    // Numeric hierarchy from the fine-level block values (same layout as the
+    // pattern; double, or float when vals32) and the current UV (for the
+    // rigid near-null space).
+    void numeric(hipStream_t s, const void* A0vals, int vals32, const double* UV) override;
 
     // z = M r (one V-cycle) for FP64 vectors of 2*nV; writes per-workgroup
     // partial records of r.z into rzPart (rz_records() of them). Every
     // kernel returns at once while *stop is nonzero (a converged PCG).
-    void apply(hipStream_t s, const double* r, double* z, double* rzPart, const int* stop);
+    void apply(hipStream_t s, const double* r, double* z, double* rzPart, const int* stop) override;
 
-    int rz_records() const;
-    int levels() const { return (int)L.size(); }
-    long level_rows(int l) const;
-    long level_blocks(int l) const;
-    bool ready() const { return ready_; }
+    int rz_records() const override;
+    int levels() const override { return (int)L.size(); }
+    long level_rows(int l) const override;
+    long level_blocks(int l) const override;
+    bool ready() const override { return ready_; }
 
 private:
     struct Level;

@@ -4,6 +4,7 @@
 #include "../../include/optcuts_gpu.h"
 #include "oc_amg.hpp"
 #include "oc_common.hpp"
+#include "oc_gnn.hpp"
 #include "oc_kernels.hpp"
 
 #include <algorithm>
@@ -52,6 +53,7 @@ struct BlockPattern {
     int nV = 0;
     std::vector<int> rowPtr, colIdx, diagPos;
     DevBuf dRowPtr, dColIdx, dDiagPos, dVals, dMinv;
+    int vals32 = 0;  // dVals holds float blocks (room for double blocks is always allocated)
     long nnzb() const { return colIdx.size(); }
     int find(int u, int w) const
     {
@@ -116,8 +118,14 @@ struct oc_ctx {
     double pcgTol = 1e-10;
     int pcgMaxIter = 10000;
 
-    oc::Amg<double> amg;  // chosen-form preconditioner
+    int precision = OC_PRECISION_FP64;  // requested for the next oc_newton_init
+    int sysPrecision = OC_PRECISION_FP64;
+    oc::Amg<double> amg64;  // multigrid preconditioner, FP64 hierarchy
+    oc::Amg<float> amg32;   // FP32 hierarchy (mixed form)
+    oc::AmgBase* amg = nullptr;
     bool amgValid = false;
+    oc_gnn* learned = nullptr;             // bound by oc_newton_set_learned (not owned)
+    oc::Preconditioner* ext = nullptr;     // the external preconditioner of the Newton PCG, if any
 
     // ---- PCG workspace ---------------------------------------------------------
     DevBuf dX, dR, dZ, dP0, dP1, dQ, dB, dCtrl, dPcgPart;
@@ -296,7 +304,7 @@ int ensure_pcg(oc_ctx* c, int nV)
 // PCG on pattern P for right-hand side b (device), solution in c->dX.
 // Iterations are queued in batches; ctrl is read back between batches.
 int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxIter, int& iters, double& relRes,
-            oc::Amg<double>* amg = nullptr)
+            oc::Preconditioner* amg = nullptr)
 {
     const int nV = P.nV;
     PcgCtrl* ctrl = c->dCtrl.as<PcgCtrl>();
@@ -323,11 +331,11 @@ int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxI
             double* pNew = pbuf[(it + 1) & 1];
             const double* pu = partU[it & 1];
             if (amg)
-                oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.as<double>(),
+                oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.p, P.vals32,
                                     c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), partRZ, amg->rz_records(), 1,
                                     pu + 1, GU, 2, partS, ctrl);
             else
-                oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.as<double>(),
+                oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.p, P.vals32,
                                     c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), pu, GU, 2, pu + 1, GU, 2,
                                     partS, ctrl);
             oc::launch_pcg_update(c->stream, it, nV, pNew, c->dQ.as<double>(), Minv, c->dX.as<double>(),
@@ -400,11 +408,11 @@ int upload_tri_ids(oc_ctx* c, int offset, int n, const int* F)
     return OC_OK;
 }
 
-int energy_sums(oc_ctx* c, const TriSet& T, double* Emesh, double* Eair)
+int energy_sums(oc_ctx* c, const TriSet& T, double* Emesh, double* Eair, int f32 = 0)
 {
     if (ensure_scratch(c, T.nF)) return OC_ERR_HIP;
     oc::launch_element(c->stream, T, c->dUV.as<double>(), nullptr, nullptr, nullptr, 0, c->dPartials.as<double>(),
-                       scalar(c, 4));
+                       scalar(c, 4), f32);
     OC_HIP(c, hipGetLastError());
     double e[2];
     OC_HIP(c, hipMemcpyAsync(e, scalar(c, 4), sizeof(e), hipMemcpyDeviceToHost, c->stream));
@@ -518,6 +526,7 @@ int build_system(oc_ctx* c)
     build_vertex_lists(nVt, nFt, Ft.data(), c->vcPtr, c->vcSrc);
 
     if (upload_pattern(c, c->sys)) return OC_ERR_HIP;
+    c->sys.vals32 = c->sysPrecision == OC_PRECISION_MIXED;
     if (upload(c, c->dUbRow, c->ubRow) || upload(c, c->dUbCol, c->ubCol) || upload(c, c->dUbPosU, c->ubPosU) ||
         upload(c, c->dUbPosL, c->ubPosL) || upload(c, c->dUbCPtr, c->ubCPtr) || upload(c, c->dUbCSrc, c->ubCSrc) ||
         upload(c, c->dVcPtr, c->vcPtr) || upload(c, c->dVcSrc, c->vcSrc) ||
@@ -529,17 +538,39 @@ int build_system(oc_ctx* c)
     if (ensure_pcg(c, nVt)) return OC_ERR_HIP;
     if (ensure_scratch(c, std::max(nFt, nUB))) return OC_ERR_HIP;
     c->amgValid = false;
-    if (c->nPrecond == OC_PRECOND_AMG) {
+    c->amg = nullptr;
+    c->ext = nullptr;
+    if (c->nPrecond == OC_PRECOND_LEARNED) {
+        if (!c->learned) {
+            c->err = "OC_PRECOND_LEARNED needs a network (oc_newton_set_learned)";
+            return OC_ERR_STATE;
+        }
         std::string e;
-        if (c->amg.setup(c->stream, nVt, c->sys.rowPtr, c->sys.colIdx, c->fixedMaskSys, e) != 0) {
+        if (c->learned->net.set_pattern(c->stream, nVt, c->sys.rowPtr, c->sys.colIdx, c->fixedMaskSys, e) != 0) {
             c->err = e;
             return OC_ERR_HIP;
         }
-        if (c->amg.rz_records() > kRzRecords) {
+        c->learned->rowPtr.clear();  // the network's pattern now belongs to this context
+        if (c->learned->net.rz_records() > kRzRecords) {
+            c->err = "learned preconditioner: too many partial records";
+            return OC_ERR_STATE;
+        }
+        c->ext = &c->learned->net;
+    }
+    if (c->nPrecond == OC_PRECOND_AMG) {
+        std::string e;
+        c->amg = c->sysPrecision == OC_PRECISION_MIXED ? static_cast<oc::AmgBase*>(&c->amg32)
+                                                      : static_cast<oc::AmgBase*>(&c->amg64);
+        if (c->amg->setup(c->stream, nVt, c->sys.rowPtr, c->sys.colIdx, c->fixedMaskSys, e) != 0) {
+            c->err = e;
+            return OC_ERR_HIP;
+        }
+        if (c->amg->rz_records() > kRzRecords) {
             c->err = "AMG: too many partial records";
             return OC_ERR_STATE;
         }
         c->amgValid = true;
+        c->ext = c->amg;
     }
     OC_HIP(c, hipStreamSynchronize(c->stream));
     c->sysReady = true;
@@ -550,7 +581,7 @@ int system_energy(oc_ctx* c, double* E, double* Emesh, double* Eair)
 {
     TriSet T = system_set(c);
     double em, ea;
-    if (int st = energy_sums(c, T, &em, &ea)) return st;
+    if (int st = energy_sums(c, T, &em, &ea, c->sysPrecision == OC_PRECISION_MIXED)) return st;
     *Emesh = em;
     *Eair = c->nFair ? ea * (c->wScaf / (double)c->nFair) : 0.0;
     *E = c->a0 * em + *Eair;
@@ -1076,7 +1107,7 @@ int oc_solver_multiply(oc_ctx* c, const double* x, double* Ax)
     (void)hipSetDevice(c->device);
     OC_HIP(c, hipMemcpyAsync(c->dSolB.p, x, sizeof(double) * c->solN, hipMemcpyHostToDevice, c->stream));
     oc::launch_spmv(c->stream, c->sol.nV, c->sol.dRowPtr.as<int>(), c->sol.dColIdx.as<int>(),
-                    c->sol.dVals.as<double>(), c->dSolB.as<double>(), c->dSolX.as<double>());
+                    c->sol.dVals.p, 0, c->dSolB.as<double>(), c->dSolX.as<double>());
     OC_HIP(c, hipGetLastError());
     OC_HIP(c, hipMemcpyAsync(Ax, c->dSolX.p, sizeof(double) * c->solN, hipMemcpyDeviceToHost, c->stream));
     OC_HIP(c, hipStreamSynchronize(c->stream));
@@ -1110,7 +1141,8 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     if (!c) return OC_ERR_ARG;
     OC_ARG(c, c->nV > 0, "oc_newton_init: no mesh");
     OC_ARG(c, a0 > 0.0 && relGL2Tol > 0.0 && pcgRelTol > 0.0 && pcgMaxIter > 0 &&
-                  (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI || precond == OC_PRECOND_AMG),
+                  (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI || precond == OC_PRECOND_AMG ||
+                   precond == OC_PRECOND_LEARNED),
            "oc_newton_init: bad parameters");
     (void)hipSetDevice(c->device);
     c->a0 = a0;
@@ -1119,6 +1151,7 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     c->nPrecond = precond;
     c->pcgTol = pcgRelTol;
     c->pcgMaxIter = pcgMaxIter;
+    c->sysPrecision = c->precision;
     // Optimizer constructor: exit(-1) on an inverted initial map (Optimizer.cpp:65-67)
     if (ensure_scratch(c, c->nF + c->nFair)) return OC_ERR_HIP;
     unsigned int ninv = 0;
@@ -1134,6 +1167,51 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     return OC_OK;
 }
 
+int oc_newton_set_learned(oc_ctx* c, oc_gnn* net)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, !net || net->device == c->device, "oc_newton_set_learned: network on another device");
+    c->learned = net;
+    c->sysReady = false;  // the next oc_newton_init binds it
+    return OC_OK;
+}
+
+int oc_newton_set_precision(oc_ctx* c, int precision)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, precision == OC_PRECISION_FP64 || precision == OC_PRECISION_MIXED,
+           "oc_newton_set_precision: unknown precision");
+    c->precision = precision;
+    return OC_OK;
+}
+
+namespace {
+// Element pass + Hessian assembly of the Newton system in the driver's precision.
+void system_element_pass(oc_ctx* c, const TriSet& T)
+{
+    const int f32 = c->sysPrecision == OC_PRECISION_MIXED;
+    oc::launch_element(c->stream, T, c->dUV.as<double>(), nullptr, c->dElemG.as<double>(), c->dElemH.p, 1, nullptr,
+                       nullptr, f32);
+}
+
+void system_assemble(oc_ctx* c)
+{
+    const int nUB = (int)c->ubRow.size();
+    const int bj = c->nPrecond == OC_PRECOND_BLOCK_JACOBI;
+    if (c->sysPrecision == OC_PRECISION_MIXED)
+        oc::launch_assemble<float>(c->stream, nUB, c->dUbRow.as<int>(), c->dUbCol.as<int>(), c->dUbPosU.as<int>(),
+                                   c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(), c->dUbCSrc.as<int>(),
+                                   c->dFixedMaskSys.as<unsigned char>(), c->dFixedDiag.as<double>(),
+                                   c->dElemH.as<float>(), c->sys.dVals.p, c->sys.vals32, c->sys.dMinv.as<double>(), bj);
+    else
+        oc::launch_assemble<double>(c->stream, nUB, c->dUbRow.as<int>(), c->dUbCol.as<int>(), c->dUbPosU.as<int>(),
+                                    c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(), c->dUbCSrc.as<int>(),
+                                    c->dFixedMaskSys.as<unsigned char>(), c->dFixedDiag.as<double>(),
+                                    c->dElemH.as<double>(), c->sys.dVals.p, c->sys.vals32, c->sys.dMinv.as<double>(),
+                                    bj);
+}
+}  // namespace
+
 int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
 {
     if (!c) return OC_ERR_ARG;
@@ -1148,9 +1226,9 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     const TriSet T = system_set(c);
     const int nVt = c->nVt;
     hipStream_t s = c->stream;
+    const int f32 = c->sysPrecision == OC_PRECISION_MIXED;
     // gradient (and element Hessians in the same pass)
-    oc::launch_element(s, T, c->dUV.as<double>(), nullptr, c->dElemG.as<double>(), c->dElemH.as<double>(), 1,
-                       nullptr, nullptr);
+    system_element_pass(c, T);
     oc::launch_gather_grad(s, nVt, c->dVcPtr.as<int>(), c->dVcSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
                            c->dElemG.as<double>(), c->dG.as<double>(), c->dPartials.as<double>(), scalar(c, 6));
     OC_HIP(c, hipGetLastError());
@@ -1167,21 +1245,18 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
         return OC_OK;
     }
     // Hessian assembly + direction: H d = -g
-    oc::launch_assemble<double>(s, (int)c->ubRow.size(), c->dUbRow.as<int>(), c->dUbCol.as<int>(),
-                                c->dUbPosU.as<int>(), c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(),
-                                c->dUbCSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
-                                c->dFixedDiag.as<double>(), c->dElemH.as<double>(), c->sys.dVals.as<double>(),
-                                c->sys.dMinv.as<double>(), c->nPrecond == OC_PRECOND_BLOCK_JACOBI);
+    system_assemble(c);
     oc::launch_scale_copy(s, 2L * nVt, -1.0, c->dG.as<double>(), c->dB.as<double>());
-    if (c->amgValid) c->amg.numeric(s, c->sys.dVals.as<double>(), c->dUV.as<double>());
+    if (c->amgValid) c->amg->numeric(s, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>());
+    if (c->nPrecond == OC_PRECOND_LEARNED) c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
     int it = 0;
     double rr = 0.0;
-    int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr,
-                     c->amgValid ? &c->amg : nullptr);
-    if (c->amgValid && st != OC_OK) {
+    int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext);
+    if (c->ext && st != OC_OK) {
         // the multigrid-preconditioned solve broke down or hit its cap: this
         // Newton step falls back to Jacobi-PCG (same tolerance)
-        S->pcg_fallback = 1;
+        S->pcg_fallback = st == OC_ERR_BREAKDOWN ? 1 : 2;
+        S->pcg_iters_first = it;
         int it2 = 0;
         st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, std::max(c->pcgMaxIter, 200000), it2, rr, nullptr);
         it += it2;
@@ -1205,7 +1280,7 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     int k0 = -1;  // first index of the candidates held in cand
     auto fetch = [&](int kstart) -> int {
         oc::launch_linesearch(s, T, c->dUV.as<double>(), dir, stepSize, kstart, c->dPartials.as<double>(),
-                              c->dLsOut.as<double>());
+                              c->dLsOut.as<double>(), f32);
         OC_HIP(c, hipGetLastError());
         OC_HIP(c, hipMemcpyAsync(cand, c->dLsOut.p, sizeof(cand), hipMemcpyDeviceToHost, s));
         OC_HIP(c, hipStreamSynchronize(s));
@@ -1280,8 +1355,8 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
 {
     if (!c) return OC_ERR_ARG;
     OC_ARG(c, ms && reps > 0 && which >= 0 && which <= 8, "oc_time_kernel: bad args");
-    if ((which == 7 || which == 8) && !c->amgValid) {
-        c->err = "oc_time_kernel: the AMG preconditioner is not active";
+    if ((which == 7 || which == 8) && !c->ext) {
+        c->err = "oc_time_kernel: no multigrid or learned preconditioner is active";
         return OC_ERR_STATE;
     }
     if (!c->sysReady) {
@@ -1302,8 +1377,8 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
     double* tUo = tS + GS;                   // 2*GU written by update
     double* tSo = tUo + 2 * (size_t)GU;      // GS   written by spmv
     auto spmv = [&]() {
-        oc::launch_pcg_spmv(s, 1, c->nVt, c->sys.dRowPtr.as<int>(), c->sys.dColIdx.as<int>(),
-                            c->sys.dVals.as<double>(), c->dZ.as<double>(), c->dP0.as<double>(), c->dP1.as<double>(),
+        oc::launch_pcg_spmv(s, 1, c->nVt, c->sys.dRowPtr.as<int>(), c->sys.dColIdx.as<int>(), c->sys.dVals.p,
+                            c->sys.vals32, c->dZ.as<double>(), c->dP0.as<double>(), c->dP1.as<double>(),
                             c->dQ.as<double>(), tU, GU, 2, tU + 1, GU, 2, tSo, ctrl);
     };
     auto update = [&]() {
@@ -1313,16 +1388,10 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
     auto launch = [&]() {
         switch (which) {
         case 0:
-            oc::launch_element(s, T, c->dUV.as<double>(), nullptr, c->dElemG.as<double>(), c->dElemH.as<double>(), 1,
-                               nullptr, nullptr);
+            system_element_pass(c, T);
             break;
         case 1:
-            oc::launch_assemble<double>(s, (int)c->ubRow.size(), c->dUbRow.as<int>(), c->dUbCol.as<int>(),
-                                        c->dUbPosU.as<int>(), c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(),
-                                        c->dUbCSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
-                                        c->dFixedDiag.as<double>(), c->dElemH.as<double>(),
-                                        c->sys.dVals.as<double>(), c->sys.dMinv.as<double>(),
-                                        c->nPrecond == OC_PRECOND_BLOCK_JACOBI);
+            system_assemble(c);
             break;
         case 2:
             spmv();
@@ -1330,7 +1399,7 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
             break;
         case 3:
             oc::launch_linesearch(s, T, c->dUV.as<double>(), c->dX.as<double>(), 0.5, 0, c->dPartials.as<double>(),
-                                  c->dLsOut.as<double>());
+                                  c->dLsOut.as<double>(), c->sysPrecision == OC_PRECISION_MIXED);
             break;
         case 4:
             oc::launch_step_bound(s, T, c->dUV.as<double>(), c->dX.as<double>(), c->dOrdMin.as<unsigned long long>(),
@@ -1343,10 +1412,13 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
             update();
             break;
         case 7:
-            c->amg.apply(s, c->dR.as<double>(), c->dTmpVec.as<double>(), c->dPartials.as<double>(), nullptr);
+            c->ext->apply(s, c->dR.as<double>(), c->dTmpVec.as<double>(), c->dPartials.as<double>(), nullptr);
             break;
         case 8:
-            c->amg.numeric(s, c->sys.dVals.as<double>(), c->dUV.as<double>());
+            if (c->amgValid)
+                c->amg->numeric(s, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>());
+            else
+                c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
             break;
         }
     };
@@ -1404,6 +1476,35 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
         OC_HIP(c, hipMemcpyAsync(ctrl, &h, sizeof(PcgCtrl), hipMemcpyHostToDevice, s));
         OC_HIP(c, hipStreamSynchronize(s));
     }
+    return OC_OK;
+}
+
+int oc_newton_get_system(oc_ctx* c, int* rowPtr, int* colIdx, double* vals, double* rhs, unsigned char* fixedMask)
+{
+    if (!c) return OC_ERR_ARG;
+    if (!c->sysReady) {
+        c->err = "oc_newton_get_system: call oc_newton_init first";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    const BlockPattern& P = c->sys;
+    if (rowPtr) std::copy(P.rowPtr.begin(), P.rowPtr.end(), rowPtr);
+    if (colIdx) std::copy(P.colIdx.begin(), P.colIdx.end(), colIdx);
+    if (fixedMask) std::copy(c->fixedMaskSys.begin(), c->fixedMaskSys.end(), fixedMask);
+    const size_t nv = 4 * (size_t)P.nnzb();
+    if (vals) {
+        if (P.vals32) {
+            std::vector<float> f(nv);
+            OC_HIP(c, hipMemcpyAsync(f.data(), P.dVals.p, sizeof(float) * nv, hipMemcpyDeviceToHost, c->stream));
+            OC_HIP(c, hipStreamSynchronize(c->stream));
+            for (size_t i = 0; i < nv; ++i) vals[i] = f[i];
+        } else {
+            OC_HIP(c, hipMemcpyAsync(vals, P.dVals.p, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    if (rhs) OC_HIP(c, hipMemcpyAsync(rhs, c->dB.p, sizeof(double) * 2 * (size_t)c->nVt, hipMemcpyDeviceToHost,
+                                      c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
     return OC_OK;
 }
 

@@ -14,6 +14,8 @@
 #include "oc_kernels.hpp"
 #include "oc_sd_element.hpp"
 
+#include <type_traits>
+
 namespace oc {
 
 // ----------------------------------------------------------------------------
@@ -127,52 +129,106 @@ __device__ __forceinline__ RestT<double> load_rest(const TriSet& T, int t, bool 
     return r;
 }
 
-template <bool WANT_G, bool WANT_H>
+// Element precision EP = double follows the reference arithmetic on absolute
+// UV coordinates. EP = float (the mixed form) evaluates the element in FP32 on
+// vertex positions relative to the first vertex (differences taken in FP64,
+// so large UV offsets cost no accuracy) and accumulates energies in
+// compensated FP64.
+template <typename EP>
+__device__ __forceinline__ void element_coords(double2 a, double2 b, double2 c, EP (&U)[3][2])
+{
+    if constexpr (std::is_same<EP, double>::value) {
+        U[0][0] = a.x;
+        U[0][1] = a.y;
+        U[1][0] = b.x;
+        U[1][1] = b.y;
+        U[2][0] = c.x;
+        U[2][1] = c.y;
+    } else {
+        U[0][0] = EP(0);
+        U[0][1] = EP(0);
+        U[1][0] = EP(b.x - a.x);
+        U[1][1] = EP(b.y - a.y);
+        U[2][0] = EP(c.x - a.x);
+        U[2][1] = EP(c.y - a.y);
+    }
+}
+
+template <typename EP>
+__device__ __forceinline__ RestT<EP> rest_as(const RestT<double>& r)
+{
+    return RestT<EP>{EP(r.w), EP(r.A2), EP(r.E0), EP(r.E1), EP(r.E01)};
+}
+
+template <typename EP>
+using Vec4 = typename std::conditional<std::is_same<EP, float>::value, float4, double4>::type;
+
+template <typename EP, bool WANT_G, bool WANT_H>
 __global__ void __launch_bounds__(kBlock) k_element(TriSet T, const double2* __restrict__ UV, double* Eelem,
-                                                    double2* elemG, double4* elemH, int project, double* partials)
+                                                    double2* elemG, Vec4<EP>* elemH, int project, double* partials)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     CSum acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
     if (t < T.nF) {
         const bool isMesh = t < T.nFmesh;
         const double2 a = UV[T.f0[t]], b = UV[T.f1[t]], c = UV[T.f2[t]];
-        const RestT<double> r = load_rest(T, t, isMesh);
-        const double U[3][2] = {{a.x, a.y}, {b.x, b.y}, {c.x, c.y}};
-        double g[6], h[6][4];
-        const double E = sd_element<double, WANT_H>(U, r, g, h);
+        const RestT<EP> r = rest_as<EP>(load_rest(T, t, isMesh));
+        EP U[3][2];
+        element_coords<EP>(a, b, c, U);
+        EP g[6], h[6][4];
+        const double E = (double)sd_element<EP, WANT_H>(U, r, g, h);
         if (Eelem) Eelem[t] = E;
         acc[0].s = isMesh ? E : 0.0;
         acc[1].s = isMesh ? 0.0 : E;
         const double sc = isMesh ? T.sMesh : T.sAir;
         if (WANT_G) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) elemG[3 * (size_t)t + k] = make_double2(sc * g[2 * k], sc * g[2 * k + 1]);
+            for (int k = 0; k < 3; ++k)
+                elemG[3 * (size_t)t + k] = make_double2(sc * (double)g[2 * k], sc * (double)g[2 * k + 1]);
         }
         if (WANT_H) {
-            if (project) sd_project_psd<double>(h);
+            if (project) sd_project_psd<EP>(h);
+            const EP se = (EP)sc;
 #pragma unroll
-            for (int k = 0; k < 6; ++k)
-                elemH[6 * (size_t)t + k] = make_double4(sc * h[k][0], sc * h[k][1], sc * h[k][2], sc * h[k][3]);
+            for (int k = 0; k < 6; ++k) {
+                Vec4<EP> o;
+                o.x = se * h[k][0];
+                o.y = se * h[k][1];
+                o.z = se * h[k][2];
+                o.w = se * h[k][3];
+                elemH[6 * (size_t)t + k] = o;
+            }
         }
     }
     if (partials) publish_csum<2>(acc, partials);
 }
 
-void launch_element(hipStream_t s, const TriSet& T, const double* UV, double* Eelem, double* elemG, double* elemH,
-                    int project, double* partials, double* esum)
+template <typename EP>
+void launch_element_t(hipStream_t s, const TriSet& T, const double* UV, double* Eelem, double* elemG, void* elemH,
+                      int project, double* partials, double* esum)
 {
     const dim3 grid(grid_for(T.nF)), blk(kBlock);
     const double2* uv = reinterpret_cast<const double2*>(UV);
     double* part = esum ? partials : nullptr;
+    double2* eg = reinterpret_cast<double2*>(elemG);
+    Vec4<EP>* eh = reinterpret_cast<Vec4<EP>*>(elemH);
     if (elemH)
-        hipLaunchKernelGGL((k_element<true, true>), grid, blk, 0, s, T, uv, Eelem, reinterpret_cast<double2*>(elemG),
-                           reinterpret_cast<double4*>(elemH), project, part);
+        hipLaunchKernelGGL((k_element<EP, true, true>), grid, blk, 0, s, T, uv, Eelem, eg, eh, project, part);
     else if (elemG)
-        hipLaunchKernelGGL((k_element<true, false>), grid, blk, 0, s, T, uv, Eelem, reinterpret_cast<double2*>(elemG),
-                           nullptr, project, part);
+        hipLaunchKernelGGL((k_element<EP, true, false>), grid, blk, 0, s, T, uv, Eelem, eg, nullptr, project, part);
     else
-        hipLaunchKernelGGL((k_element<false, false>), grid, blk, 0, s, T, uv, Eelem, nullptr, nullptr, project, part);
+        hipLaunchKernelGGL((k_element<EP, false, false>), grid, blk, 0, s, T, uv, Eelem, nullptr, nullptr, project,
+                           part);
     if (esum) hipLaunchKernelGGL(k_finish_csum<2>, dim3(1), dim3(kBlock), 0, s, partials, (int)grid.x, esum);
+}
+
+void launch_element(hipStream_t s, const TriSet& T, const double* UV, double* Eelem, double* elemG, void* elemH,
+                    int project, double* partials, double* esum, int f32)
+{
+    if (f32)
+        launch_element_t<float>(s, T, UV, Eelem, elemG, elemH, project, partials, esum);
+    else
+        launch_element_t<double>(s, T, UV, Eelem, elemG, elemH, project, partials, esum);
 }
 
 // ----------------------------------------------------------------------------
@@ -261,6 +317,17 @@ void launch_gather_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSr
 }
 
 // ----------------------------------------------------------------------------
+template <typename OV>
+__device__ __forceinline__ OV cvt4(double x, double y, double z, double w)
+{
+    OV o;
+    o.x = (decltype(o.x))x;
+    o.y = (decltype(o.y))y;
+    o.z = (decltype(o.z))z;
+    o.w = (decltype(o.w))w;
+    return o;
+}
+
 __device__ __forceinline__ double4 load_block(const double4* H, int src) { return H[src]; }
 __device__ __forceinline__ double4 load_block(const float4* H, int src)
 {
@@ -284,14 +351,14 @@ __device__ __forceinline__ void jacobi_block(const double4& d, bool blockJacobi,
     }
 }
 
-template <typename HV>
+template <typename HV, typename OV>
 __global__ void __launch_bounds__(kBlock) k_assemble(int nUB, const int* __restrict__ ubRow,
                                                      const int* __restrict__ ubCol, const int* __restrict__ ubPosU,
                                                      const int* __restrict__ ubPosL, const int* __restrict__ ubCPtr,
                                                      const int* __restrict__ ubCSrc,
                                                      const unsigned char* __restrict__ fixedMask,
                                                      const double* __restrict__ fixedDiag, const HV* __restrict__ elemH,
-                                                     double4* vals, double* Minv, int blockJacobi)
+                                                     OV* vals, double* Minv, int blockJacobi)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nUB) return;
@@ -317,29 +384,35 @@ __global__ void __launch_bounds__(kBlock) k_assemble(int nUB, const int* __restr
         } else {
             s.z = s.y;  // the lower in-block entry mirrors the accumulated upper one
         }
-        vals[ubPosU[b]] = s;
+        vals[ubPosU[b]] = cvt4<OV>(s.x, s.y, s.z, s.w);
         jacobi_block(s, blockJacobi != 0, Minv + 4 * (size_t)u);
     } else {
-        vals[ubPosU[b]] = s;
-        vals[ubPosL[b]] = make_double4(s.x, s.z, s.y, s.w);
+        vals[ubPosU[b]] = cvt4<OV>(s.x, s.y, s.z, s.w);
+        vals[ubPosL[b]] = cvt4<OV>(s.x, s.z, s.y, s.w);
     }
 }
 
 template <typename HT>
 void launch_assemble(hipStream_t s, int nUB, const int* ubRow, const int* ubCol, const int* ubPosU,
                      const int* ubPosL, const int* ubCPtr, const int* ubCSrc, const unsigned char* fixedMask,
-                     const double* fixedDiag, const HT* elemH, double* vals, double* Minv, int blockJacobi)
+                     const double* fixedDiag, const HT* elemH, void* vals, int vals32, double* Minv, int blockJacobi)
 {
     using HV = typename std::conditional<std::is_same<HT, float>::value, float4, double4>::type;
-    hipLaunchKernelGGL(k_assemble<HV>, dim3(grid_for(nUB)), dim3(kBlock), 0, s, nUB, ubRow, ubCol, ubPosU, ubPosL,
-                       ubCPtr, ubCSrc, fixedMask, fixedDiag, reinterpret_cast<const HV*>(elemH),
-                       reinterpret_cast<double4*>(vals), Minv, blockJacobi);
+    const HV* eh = reinterpret_cast<const HV*>(elemH);
+    if (vals32)
+        hipLaunchKernelGGL((k_assemble<HV, float4>), dim3(grid_for(nUB)), dim3(kBlock), 0, s, nUB, ubRow, ubCol,
+                           ubPosU, ubPosL, ubCPtr, ubCSrc, fixedMask, fixedDiag, eh, reinterpret_cast<float4*>(vals),
+                           Minv, blockJacobi);
+    else
+        hipLaunchKernelGGL((k_assemble<HV, double4>), dim3(grid_for(nUB)), dim3(kBlock), 0, s, nUB, ubRow, ubCol,
+                           ubPosU, ubPosL, ubCPtr, ubCSrc, fixedMask, fixedDiag, eh, reinterpret_cast<double4*>(vals),
+                           Minv, blockJacobi);
 }
 template void launch_assemble<double>(hipStream_t, int, const int*, const int*, const int*, const int*, const int*,
-                                      const int*, const unsigned char*, const double*, const double*, double*,
+                                      const int*, const unsigned char*, const double*, const double*, void*, int,
                                       double*, int);
 template void launch_assemble<float>(hipStream_t, int, const int*, const int*, const int*, const int*, const int*,
-                                     const int*, const unsigned char*, const double*, const float*, double*,
+                                     const int*, const unsigned char*, const double*, const float*, void*, int,
                                      double*, int);
 
 // ----------------------------------------------------------------------------
@@ -375,6 +448,7 @@ void launch_step_bound(hipStream_t s, const TriSet& T, const double* UV, const d
 }
 
 // ----------------------------------------------------------------------------
+template <typename EP>
 __global__ void __launch_bounds__(kBlock) k_linesearch(TriSet T, const double2* __restrict__ UV,
                                                        const double2* __restrict__ dir, double alphaK0,
                                                        double* partials)
@@ -393,7 +467,7 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(TriSet T, const double2* 
             u0[k] = UV[vi[k]];
             d[k] = dir[vi[k]];
         }
-        const RestT<double> r = load_rest(T, t, isMesh);
+        const RestT<EP> r = rest_as<EP>(load_rest(T, t, isMesh));
         double alpha = alphaK0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -403,8 +477,17 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(TriSet T, const double2* 
                 p[q][0] = add_rn(u0[q].x, mul_rn(alpha, d[q].x));
                 p[q][1] = add_rn(u0[q].y, mul_rn(alpha, d[q].y));
             }
-            double dbA;
-            const double E = sd_energy<double>(p[0][0], p[0][1], p[1][0], p[1][1], p[2][0], p[2][1], r, dbA);
+            double E, dbA;
+            if constexpr (std::is_same<EP, double>::value) {
+                E = sd_energy<double>(p[0][0], p[0][1], p[1][0], p[1][1], p[2][0], p[2][1], r, dbA);
+            } else {
+                // FP32 energy on relative positions; the inversion test stays FP64
+                const double ax = p[1][0] - p[0][0], ay = p[1][1] - p[0][1];
+                const double bx = p[2][0] - p[0][0], by = p[2][1] - p[0][1];
+                EP dbAf;
+                E = (double)sd_energy<EP>(EP(0), EP(0), EP(ax), EP(ay), EP(bx), EP(by), r, dbAf);
+                dbA = ax * by - ay * bx;
+            }
             acc[k].s = isMesh ? E : 0.0;
             acc[K + k].s = isMesh ? 0.0 : E;
             acc[2 * K + k].s = (dbA < 0.0) ? 1.0 : 0.0;  // inversion count (exact small integers)
@@ -433,13 +516,17 @@ __global__ void __launch_bounds__(kBlock) k_finish_linesearch(const double* __re
 }
 
 void launch_linesearch(hipStream_t s, const TriSet& T, const double* UV, const double* dir, double alpha0, int k0,
-                       double* partials, double* out)
+                       double* partials, double* out, int f32)
 {
     double a = alpha0;
     for (int k = 0; k < k0; ++k) a = a / 2.0;
     const int grid = grid_for(T.nF);
-    hipLaunchKernelGGL(k_linesearch, dim3(grid), dim3(kBlock), 0, s, T, reinterpret_cast<const double2*>(UV),
-                       reinterpret_cast<const double2*>(dir), a, partials);
+    if (f32)
+        hipLaunchKernelGGL(k_linesearch<float>, dim3(grid), dim3(kBlock), 0, s, T,
+                           reinterpret_cast<const double2*>(UV), reinterpret_cast<const double2*>(dir), a, partials);
+    else
+        hipLaunchKernelGGL(k_linesearch<double>, dim3(grid), dim3(kBlock), 0, s, T,
+                           reinterpret_cast<const double2*>(UV), reinterpret_cast<const double2*>(dir), a, partials);
     hipLaunchKernelGGL(k_finish_linesearch, dim3(1), dim3(kBlock), 0, s, partials, grid, T.sMesh, T.sAir, out);
 }
 
@@ -572,15 +659,17 @@ void launch_pcg_init(hipStream_t s, int nV, const double* b, const double* Minv,
 }
 
 // q_v = sum_k A_vk (z_w + beta p_w) over the row's blocks, kSpmvLanes lanes per row.
+template <typename V4>
 __device__ __forceinline__ double2 row_apply(const int* __restrict__ rowPtr, const int* __restrict__ colIdx,
-                                             const double4* __restrict__ vals, const double2* __restrict__ z,
+                                             const V4* __restrict__ vals, const double2* __restrict__ z,
                                              const double2* __restrict__ pOld, double beta, int v, int lane)
 {
     double yx = 0.0, yy = 0.0;
     const int k1 = rowPtr[v + 1];
     for (int k = rowPtr[v] + lane; k < k1; k += kSpmvLanes) {
         const int w = colIdx[k];
-        const double4 a = vals[k];
+        const V4 av = vals[k];
+        const double4 a = make_double4((double)av.x, (double)av.y, (double)av.z, (double)av.w);
         const double2 zw = z[w];
         double px = zw.x, py = zw.y;
         if (pOld) {
@@ -599,9 +688,10 @@ __device__ __forceinline__ double2 row_apply(const int* __restrict__ rowPtr, con
     return make_double2(yx, yy);
 }
 
+template <typename V4>
 __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* __restrict__ rowPtr,
                                                      const int* __restrict__ colIdx,
-                                                     const double4* __restrict__ vals,
+                                                     const V4* __restrict__ vals,
                                                      const double2* __restrict__ z,
                                                      const double2* __restrict__ pOld, double2* pNew, double2* q,
                                                      const double* __restrict__ partRz, int nRz, int sRz,
@@ -649,14 +739,20 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* 
     if (threadIdx.x == 0) partS[blockIdx.x] = acc[0];
 }
 
-void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const double* vals,
-                     const double* z, const double* pOld, double* pNew, double* q, const double* partRz, int nRz,
-                     int sRz, const double* partRr, int nRr, int sRr, double* partS, PcgCtrl* ctrl)
+void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const void* vals,
+                     int vals32, const double* z, const double* pOld, double* pNew, double* q, const double* partRz,
+                     int nRz, int sRz, const double* partRr, int nRr, int sRr, double* partS, PcgCtrl* ctrl)
 {
-    hipLaunchKernelGGL(k_pcg_spmv, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, it, nV, rowPtr, colIdx,
-                       reinterpret_cast<const double4*>(vals), reinterpret_cast<const double2*>(z),
-                       reinterpret_cast<const double2*>(pOld), reinterpret_cast<double2*>(pNew),
-                       reinterpret_cast<double2*>(q), partRz, nRz, sRz, partRr, nRr, sRr, partS, ctrl);
+    if (vals32)
+        hipLaunchKernelGGL(k_pcg_spmv<float4>, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, it, nV, rowPtr, colIdx,
+                           reinterpret_cast<const float4*>(vals), reinterpret_cast<const double2*>(z),
+                           reinterpret_cast<const double2*>(pOld), reinterpret_cast<double2*>(pNew),
+                           reinterpret_cast<double2*>(q), partRz, nRz, sRz, partRr, nRr, sRr, partS, ctrl);
+    else
+        hipLaunchKernelGGL(k_pcg_spmv<double4>, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, it, nV, rowPtr, colIdx,
+                           reinterpret_cast<const double4*>(vals), reinterpret_cast<const double2*>(z),
+                           reinterpret_cast<const double2*>(pOld), reinterpret_cast<double2*>(pNew),
+                           reinterpret_cast<double2*>(q), partRz, nRz, sRz, partRr, nRr, sRr, partS, ctrl);
 }
 
 template <bool APPLY_M>
@@ -716,8 +812,9 @@ void launch_pcg_update(hipStream_t s, int it, int nV, const double* pNew, const 
 }
 
 // ----------------------------------------------------------------------------
+template <typename V4>
 __global__ void __launch_bounds__(kBlock) k_spmv(int nV, const int* __restrict__ rowPtr,
-                                                 const int* __restrict__ colIdx, const double4* __restrict__ vals,
+                                                 const int* __restrict__ colIdx, const V4* __restrict__ vals,
                                                  const double2* __restrict__ x, double2* y)
 {
     int r0, r1;
@@ -730,12 +827,17 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int nV, const int* __restrict__
     }
 }
 
-void launch_spmv(hipStream_t s, int nV, const int* rowPtr, const int* colIdx, const double* vals, const double* x,
-                 double* y)
+void launch_spmv(hipStream_t s, int nV, const int* rowPtr, const int* colIdx, const void* vals, int vals32,
+                 const double* x, double* y)
 {
-    hipLaunchKernelGGL(k_spmv, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, nV, rowPtr, colIdx,
-                       reinterpret_cast<const double4*>(vals), reinterpret_cast<const double2*>(x),
-                       reinterpret_cast<double2*>(y));
+    if (vals32)
+        hipLaunchKernelGGL(k_spmv<float4>, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, nV, rowPtr, colIdx,
+                           reinterpret_cast<const float4*>(vals), reinterpret_cast<const double2*>(x),
+                           reinterpret_cast<double2*>(y));
+    else
+        hipLaunchKernelGGL(k_spmv<double4>, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, nV, rowPtr, colIdx,
+                           reinterpret_cast<const double4*>(vals), reinterpret_cast<const double2*>(x),
+                           reinterpret_cast<double2*>(y));
 }
 
 __global__ void __launch_bounds__(kBlock) k_expand_upper(int nnz, const double* __restrict__ a,

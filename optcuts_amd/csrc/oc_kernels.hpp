@@ -36,9 +36,10 @@ void launch_rest_features(hipStream_t s, int nF, const int* f0, const int* f1, c
 // applied, block scale not applied), gradient contributions elemG[t*6..]
 // and projected Hessian upper blocks elemH[t*24..] (both multiplied by w
 // and the block scale). When esum is non-null, esum[0] = sum of mesh
-// energies, esum[1] = sum of air energies (compensated).
-void launch_element(hipStream_t s, const TriSet& T, const double* UV, double* Eelem, double* elemG, double* elemH,
-                    int project, double* partials, double* esum);
+// energies, esum[1] = sum of air energies (compensated). f32: the element
+// terms are evaluated in FP32 (elemH then holds floats); sums stay FP64.
+void launch_element(hipStream_t s, const TriSet& T, const double* UV, double* Eelem, double* elemG, void* elemH,
+                    int project, double* partials, double* esum, int f32 = 0);
 
 // Triplets in the reference order (SymDirichletEnergy.cpp:527-548 with
 // IglUtils::addBlockToMatrix), unscaled: triangle t writes at tripOff[t];
@@ -53,11 +54,12 @@ void launch_gather_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSr
 
 // Upper-block assembly into the full 2x2-block CSR; diagonal blocks also
 // produce the Jacobi preconditioner (blockJacobi: inverse 2x2 block,
-// else inverse diagonal entries).
+// else inverse diagonal entries). Sums are FP64; vals is float when vals32.
 template <typename HT>
 void launch_assemble(hipStream_t s, int nUB, const int* ubRow, const int* ubCol, const int* ubPosU,
                      const int* ubPosL, const int* ubCPtr, const int* ubCSrc, const unsigned char* fixedMask,
-                     const double* fixedDiag, const HT* elemH, double* vals, double* Minv, int blockJacobi);
+                     const double* fixedDiag, const HT* elemH, void* vals, int vals32, double* Minv,
+                     int blockJacobi);
 
 // Minimum over triangles of the injectivity step bound, folded into
 // *ordMin (order-preserving uint64 of a double; min is order-independent).
@@ -67,10 +69,10 @@ void launch_step_bound(hipStream_t s, const TriSet& T, const double* UV, const d
 // Line-search energies at K candidate steps alpha_k = alpha0 / 2^(k0+k),
 // k = 0..K-1: out[k] = sMesh*E_mesh + sAir*E_air, out[K+k] = E_mesh,
 // out[2K+k] = E_air, out[3K+k] = number of triangles with negative
-// doubled area.
+// doubled area (always FP64). f32: FP32 element energies, FP64 sums.
 constexpr int kLSCand = 8;
 void launch_linesearch(hipStream_t s, const TriSet& T, const double* UV, const double* dir, double alpha0, int k0,
-                       double* partials, double* out);
+                       double* partials, double* out, int f32 = 0);
 
 // UV[i] = UV[i] + alpha * d[i] for i < n (two roundings, as the reference's
 // stepForward, Optimizer.cpp:666-669).
@@ -96,15 +98,16 @@ int pcg_grid_spmv(int nV);
 // records of stride sRz and r.r over nRr records of stride sRr.
 void launch_pcg_init(hipStream_t s, int nV, const double* b, const double* Minv, double* x, double* r, double* z,
                      double* partU, PcgCtrl* ctrl);
-void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const double* vals,
-                     const double* z, const double* pOld, double* pNew, double* q, const double* partRz, int nRz,
-                     int sRz, const double* partRr, int nRr, int sRr, double* partS, PcgCtrl* ctrl);
+// vals: double4 blocks, or float4 blocks when vals32 (FP64 arithmetic either way).
+void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const void* vals,
+                     int vals32, const double* z, const double* pOld, double* pNew, double* q, const double* partRz,
+                     int nRz, int sRz, const double* partRr, int nRr, int sRr, double* partS, PcgCtrl* ctrl);
 void launch_pcg_update(hipStream_t s, int it, int nV, const double* pNew, const double* q, const double* Minv,
                        double* x, double* r, double* z, const double* partS, double* partUout, PcgCtrl* ctrl);
 
-// y = A x (2x2-block CSR).
-void launch_spmv(hipStream_t s, int nV, const int* rowPtr, const int* colIdx, const double* vals, const double* x,
-                 double* y);
+// y = A x (2x2-block CSR; float blocks when vals32).
+void launch_spmv(hipStream_t s, int nV, const int* rowPtr, const int* colIdx, const void* vals, int vals32,
+                 const double* x, double* y);
 
 // Expand the reference's upper-triangle CSR values (per DOF) into the full
 // block CSR: vals[dst[k]] = a[k] and, for off-diagonal-block or strictly

@@ -125,3 +125,44 @@ class ScaffoldOracle:
             halv += 1
             E = self.energy(X + step * D)[0]
         return X + step * D, E, step, halv, float(g @ g)
+
+
+def block_system(V, F, UV, fixed=(0,)):
+    """Newton matrix of the mesh-only objective as the full symmetric 2x2-block
+    CSR the device driver assembles: fixed rows keep an identity diagonal
+    block only, free rows drop fixed columns. Returns (rowPtr, colIdx, vals
+    (nnzb x 2 x 2), fixedMask)."""
+    import scipy.sparse as sp
+    nV = V.shape[0]
+    feat = O.RestFeatures(V, F)
+    H = O.element_hessians(F, UV, feat)
+    I, J, Vt = O.hessian_triplets(F, H)
+    A = sp.coo_matrix((Vt, (I, J)), shape=(2 * nV, 2 * nV)).tocsr()
+    fm = np.zeros(nV, np.uint8)
+    fm[list(fixed)] = 1
+    nbr = [set() for _ in range(nV)]
+    for t in F:
+        for a in t:
+            for b in t:
+                nbr[a].add(int(b))
+    rp, ci, vals = [0], [], []
+    A = A.tolil()
+    for u in range(nV):
+        cols = [u] if fm[u] else sorted(w for w in nbr[u] if w == u or not fm[w])
+        for w in cols:
+            ci.append(w)
+            if fm[u]:
+                vals.append(np.eye(2))
+            else:
+                vals.append(A[2 * u:2 * u + 2, 2 * w:2 * w + 2].toarray())
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(vals), fm
+
+
+def block_csr_to_sparse(rp, ci, vals):
+    import scipy.sparse as sp
+    nV = len(rp) - 1
+    row = np.repeat(np.arange(nV), np.diff(rp))
+    r = (2 * row[:, None, None] + np.arange(2)[None, :, None]).repeat(2, 2)
+    c = (2 * ci[:, None, None] + np.arange(2)[None, None, :]).repeat(2, 1)
+    return sp.csr_matrix((np.asarray(vals).ravel(), (r.ravel(), c.ravel())), shape=(2 * nV, 2 * nV))
