@@ -48,9 +48,9 @@ enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1, OC_PRECOND_AMG = 2, O
  * FP64: every element term, the Hessian values and the PCG in FP64 (the plain form; matches the reference's
  *   FP64 run).
  * MIXED: element energy, gradient and projected Hessian in FP32 (on vertex positions relative to each
- *   triangle's first vertex), energy and gradient sums in compensated FP64, the Hessian stored in FP32 and the
- *   multigrid hierarchy in FP32; the injectivity step-bound roots, the inversion test, the PCG vectors,
- *   residual recurrence and scalars stay FP64. */
+ *   triangle's first vertex), energy and gradient sums in compensated FP64, the multigrid hierarchy in FP32;
+ *   the assembled Hessian (summed in FP64 from the FP32 element blocks), the injectivity step-bound roots, the
+ *   inversion test, the PCG matrix products, vectors, residual recurrence and scalars stay FP64. */
 enum { OC_PRECISION_FP64 = 0, OC_PRECISION_MIXED = 1 };
 
 typedef struct oc_ctx oc_ctx;
@@ -147,9 +147,10 @@ typedef struct {
     int converged;    /* 1 if ||g||^2 < targetGRes (no step taken) */
     int pcg_iters;    /* PCG iterations of this Newton step */
     double pcg_rel_res;
-    int pcg_fallback; /* 0, or why the multigrid-preconditioned solve was abandoned for Jacobi-PCG on this
-                         step: 1 breakdown (r.z or p.Ap not positive), 2 iteration cap */
-    int pcg_iters_first; /* iterations of the first (multigrid) attempt when pcg_fallback != 0 */
+    int pcg_fallback; /* 0: the selected preconditioner solved the step. Otherwise its first attempt broke down
+                         (r.z or p.Ap not positive) or hit pcgMaxIter, and 1: multigrid rebuilt with its
+                         guaranteed smoother interval solved it; 2: Jacobi-PCG solved it */
+    int pcg_iters_first; /* iterations of the first attempt when pcg_fallback != 0 (pcg_iters counts all) */
 } oc_iter_stats;
 
 /* Optimizer::precompute (Optimizer.cpp:154-201) for energyParams = {a0}: sets targetGRes =

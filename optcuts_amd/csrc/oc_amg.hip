@@ -14,7 +14,7 @@ namespace {
 constexpr int NS = 3;              // near-null-space vectors: two translations and a rotation
 constexpr int kAmgMaxGrid = 1024;  // workgroups of a row kernel (bounds partial records)
 constexpr int kCoarseMaxDofs = 64;  // the coarsest inverse lives in LDS (64 x 64 doubles)
-constexpr int kLanczosSteps = 10;  // Lanczos steps of the lambda_max estimate
+constexpr int kLanczosSteps = 15;  // Lanczos steps of the lambda_max estimate
 constexpr int kMaxLevels = 10;
 
 template <int BS>
@@ -286,19 +286,47 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(int j, int n, const int* __r
     }
 }
 
-// W_{-1} = D^-1 h and the partial records of h^T D^-1 h = |W_{-1}|_D^2.
+// Start vector and Gershgorin bound. g_u = max over the vertex's rows of
+// sum_j |(D^-1 A)_ij| bounds lambda_max(D^-1 A) by max_u g_u. The largest
+// eigenvalues of D^-1 A on distorted meshes belong to eigenvectors localised
+// on a few vertices, which a uniform random start barely touches (Lanczos
+// then needs many more steps); weighting the start by g_u^4 puts its mass
+// where those vectors live. h = hash * g^4, W_{-1} = D^-1 h, partial records
+// of h^T D^-1 h = |W_{-1}|_D^2 and of max g (gPart).
 template <typename T, int BS>
-__global__ void __launch_bounds__(kBlock) k_lanczos_start(int n, const T* __restrict__ Dinv, double* W,
-                                                          double* partOut, double* coef)
+__global__ void __launch_bounds__(kBlock) k_lanczos_start(int n, const int* __restrict__ rowPtr,
+                                                          const T* __restrict__ vals, const T* __restrict__ Dinv,
+                                                          double* W, double* partOut, double* gPart, double* coef)
 {
     int r0, r1;
     chunk_rows(n, r0, r1);
     double acc[2] = {0.0, 0.0};
+    double gmax = 0.0;
     for (int u = r0 + (int)threadIdx.x; u < r1; u += kBlock) {
+        const T* D = Dinv + (size_t)u * BS * BS;
+        double rs[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) rs[i] = 0.0;
+        for (int k = rowPtr[u]; k < rowPtr[u + 1]; ++k) {
+            const T* a = vals + (size_t)k * BS * BS;
+#pragma unroll
+            for (int i = 0; i < BS; ++i)
+#pragma unroll
+                for (int j = 0; j < BS; ++j) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int c = 0; c < BS; ++c) s += (double)D[i * BS + c] * (double)a[c * BS + j];
+                    rs[i] += fabs(s);
+                }
+        }
+        double gu = 0.0;
+#pragma unroll
+        for (int i = 0; i < BS; ++i) gu = fmax(gu, rs[i]);
+        gmax = fmax(gmax, gu);
+        const double g2 = gu * gu, w4 = g2 * g2;
         double h[BS];
 #pragma unroll
-        for (int i = 0; i < BS; ++i) h[i] = hash_unit((unsigned)(u * BS + i));
-        const T* D = Dinv + (size_t)u * BS * BS;
+        for (int i = 0; i < BS; ++i) h[i] = hash_unit((unsigned)(u * BS + i)) * w4;
 #pragma unroll
         for (int i = 0; i < BS; ++i) {
             double s = 0.0;
@@ -309,17 +337,44 @@ __global__ void __launch_bounds__(kBlock) k_lanczos_start(int n, const T* __rest
         }
     }
     block_sum<2>(acc);
+    // block maximum of g (values are >= 0)
+    gmax = wave_max(gmax);
+    __shared__ double gsh[kBlock / kWave];
+    if (threadIdx.x % kWave == 0) gsh[threadIdx.x / kWave] = gmax;
+    __syncthreads();
     if (threadIdx.x == 0) {
         partOut[2 * blockIdx.x] = acc[0];
         partOut[2 * blockIdx.x + 1] = acc[1];
+        double m = gsh[0];
+        for (int w = 1; w < kBlock / kWave; ++w) m = fmax(m, gsh[w]);
+        gPart[blockIdx.x] = m;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) coef[2 * kLanczosSteps] = (double)kLanczosSteps;
 }
 
-// Largest eigenvalue of the Lanczos tridiagonal (Sturm-count bisection).
+// lam[1] = the Gershgorin bound; lam[0] = the largest eigenvalue of the
+// Lanczos tridiagonal (Sturm-count bisection), or, in the safe mode (no
+// Lanczos steps were run), the Gershgorin bound / 1.1 so that the smoother's
+// interval ends exactly at a guaranteed upper bound.
 __global__ void __launch_bounds__(kBlock) k_lanczos_finish(const double* __restrict__ partIn, int nPartIn,
-                                                           double* coef, double* lam)
+                                                           const double* __restrict__ gPart, int safe, double* coef,
+                                                           double* lam)
 {
+    {
+        __shared__ double gsh[kBlock];
+        double m = 0.0;
+        for (int b = threadIdx.x; b < nPartIn; b += blockDim.x) m = fmax(m, gPart[b]);
+        gsh[threadIdx.x] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double g = 0.0;
+            for (int t = 0; t < (int)blockDim.x; ++t) g = fmax(g, gsh[t]);
+            lam[1] = g;
+            if (safe) lam[0] = g / 1.1;
+        }
+        __syncthreads();
+        if (safe) return;
+    }
     int m = (int)coef[2 * kLanczosSteps];
     if (m == kLanczosSteps) {
         double acc[2] = {0.0, 0.0};
@@ -912,7 +967,7 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
         dalloc(A.r, (size_t)A.n * bs);
         dalloc(A.d, (size_t)A.n * bs);
         dalloc(A.d2, (size_t)A.n * bs);
-        dalloc(A.lam, 1);
+        dalloc(A.lam, 2);
         dalloc(A.lz, (size_t)5 * A.n * bs + 2 * kLanczosSteps + 1);
         if (l + 1 < L.size()) {
             up(s, A.dAggPtr, A.aggPtr);
@@ -949,7 +1004,7 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
     }
     dalloc(dCoarseInv, (size_t)nCoarse * nCoarse);
     nPartials = 2 * kAmgMaxGrid;  // two doubles per workgroup record
-    dalloc(dPartials, (size_t)nPartials * 2);
+    dalloc(dPartials, (size_t)nPartials * 2 + kAmgMaxGrid);  // two record buffers + Gershgorin records
     if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) {
         err = "AMG: device allocation or upload failed";
         release();
@@ -960,7 +1015,7 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
 }
 
 template <typename T>
-void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double* UV)
+void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double* UV, int safe)
 {
     for (size_t l = 0; l < L.size(); ++l) {
         Level& A = *L[l];
@@ -988,13 +1043,14 @@ void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double
         const size_t nv = (size_t)A.n * bs;
         double* coef = A.lz + 5 * nv;
         LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv}};
+        double* gPart = dPartials + 2 * (size_t)nPartials;
         if (bs == 2)
-            hipLaunchKernelGGL((k_lanczos_start<T, 2>), dim3(g), dim3(kBlock), 0, s, A.n, A.Dinv, lb.W[1], part[1],
-                               coef);
+            hipLaunchKernelGGL((k_lanczos_start<T, 2>), dim3(g), dim3(kBlock), 0, s, A.n, A.dRowPtr, A.vals, A.Dinv,
+                               lb.W[1], part[1], gPart, coef);
         else
-            hipLaunchKernelGGL((k_lanczos_start<T, 3>), dim3(g), dim3(kBlock), 0, s, A.n, A.Dinv, lb.W[1], part[1],
-                               coef);
-        for (int j = 0; j < kLanczosSteps; ++j) {
+            hipLaunchKernelGGL((k_lanczos_start<T, 3>), dim3(g), dim3(kBlock), 0, s, A.n, A.dRowPtr, A.vals, A.Dinv,
+                               lb.W[1], part[1], gPart, coef);
+        for (int j = 0; j < (safe ? 0 : kLanczosSteps); ++j) {
             if (bs == 2)
                 hipLaunchKernelGGL((k_lanczos<T, 2, Lanes<2>::value>), dim3(g), dim3(kBlock), 0, s, j, A.n,
                                    A.dRowPtr, A.dColIdx, A.vals, A.Dinv, lb, part[(j + 1) & 1], g, part[j & 1], coef);
@@ -1002,8 +1058,8 @@ void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double
                 hipLaunchKernelGGL((k_lanczos<T, 3, Lanes<3>::value>), dim3(g), dim3(kBlock), 0, s, j, A.n,
                                    A.dRowPtr, A.dColIdx, A.vals, A.Dinv, lb, part[(j + 1) & 1], g, part[j & 1], coef);
         }
-        hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(kBlock), 0, s, part[(kLanczosSteps + 1) & 1], g, coef,
-                           A.lam);
+        hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(kBlock), 0, s, part[(kLanczosSteps + 1) & 1], g, gPart,
+                           safe, coef, A.lam);
         // tentative prolongator, smoothing, Galerkin product
         Level& N = *L[l + 1];
         const int nP = (int)A.pCol.size(), nT = (int)A.tCol.size(), nAc = (int)N.colIdx.size();

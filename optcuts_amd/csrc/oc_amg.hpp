@@ -37,7 +37,10 @@ public:
     ~AmgBase() override = default;
     virtual int setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
                       const std::vector<unsigned char>& fixedMask, std::string& err) = 0;
-    virtual void numeric(hipStream_t s, const void* A0vals, int vals32, const double* UV) = 0;
+    // safe != 0: smoother interval and prolongator damping from the
+    // Gershgorin bound of D^-1 A (guaranteed, slower convergence) instead of
+    // the Lanczos estimate.
+    virtual void numeric(hipStream_t s, const void* A0vals, int vals32, const double* UV, int safe = 0) = 0;
     virtual int levels() const = 0;
     virtual long level_rows(int l) const = 0;
     virtual long level_blocks(int l) const = 0;
@@ -62,7 +65,7 @@ public:
     // Numeric hierarchy from the fine-level block values (same layout as the
     // pattern; double, or float when vals32) and the current UV (for the
     // rigid near-null space).
-    void numeric(hipStream_t s, const void* A0vals, int vals32, const double* UV) override;
+    void numeric(hipStream_t s, const void* A0vals, int vals32, const double* UV, int safe = 0) override;
 
     // z = M r (one V-cycle) for FP64 vectors of 2*nV; writes per-workgroup
     // partial records of r.z into rzPart (rz_records() of them). Every

@@ -526,7 +526,11 @@ int build_system(oc_ctx* c)
     build_vertex_lists(nVt, nFt, Ft.data(), c->vcPtr, c->vcSrc);
 
     if (upload_pattern(c, c->sys)) return OC_ERR_HIP;
-    c->sys.vals32 = c->sysPrecision == OC_PRECISION_MIXED;
+    // The PCG matrix stays FP64 in both precisions: rounding the assembled
+    // Hessian to FP32 can make it indefinite along its near-null directions
+    // (rigid rotation about the fixed vertex), which breaks CG. The mixed
+    // form's FP32 lives in the element terms and the preconditioner.
+    c->sys.vals32 = 0;
     if (upload(c, c->dUbRow, c->ubRow) || upload(c, c->dUbCol, c->ubCol) || upload(c, c->dUbPosU, c->ubPosU) ||
         upload(c, c->dUbPosL, c->ubPosL) || upload(c, c->dUbCPtr, c->ubCPtr) || upload(c, c->dUbCSrc, c->ubCSrc) ||
         upload(c, c->dVcPtr, c->vcPtr) || upload(c, c->dVcSrc, c->vcSrc) ||
@@ -1253,13 +1257,24 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     double rr = 0.0;
     int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext);
     if (c->ext && st != OC_OK) {
-        // the multigrid-preconditioned solve broke down or hit its cap: this
-        // Newton step falls back to Jacobi-PCG (same tolerance)
-        S->pcg_fallback = st == OC_ERR_BREAKDOWN ? 1 : 2;
+        // The preconditioned solve broke down or hit its cap. Multigrid first
+        // retries with its guaranteed (Gershgorin) smoother interval; what
+        // still fails falls back to Jacobi-PCG at the same tolerance.
         S->pcg_iters_first = it;
-        int it2 = 0;
-        st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, std::max(c->pcgMaxIter, 200000), it2, rr, nullptr);
-        it += it2;
+        S->pcg_fallback = 2;
+        if (c->amgValid) {
+            c->amg->numeric(s, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>(), 1);
+            int it2 = 0;
+            st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, c->ext);
+            it += it2;
+            if (st == OC_OK) S->pcg_fallback = 1;
+        }
+        if (st != OC_OK) {
+            int it2 = 0;
+            st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, std::max(c->pcgMaxIter, 200000), it2, rr,
+                         nullptr);
+            it += it2;
+        }
     }
     S->pcg_iters = it;
     S->pcg_rel_res = rr;

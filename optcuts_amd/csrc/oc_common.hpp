@@ -56,6 +56,13 @@ __device__ __forceinline__ float wave_sumf(float v)
     return v;
 }
 
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
 __device__ __forceinline__ double wave_min(double v)
 {
 #pragma unroll

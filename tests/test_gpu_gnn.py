@@ -124,9 +124,11 @@ def test_loss_and_gradient_match_oracle(net, system):
     assert np.allclose(net.get_grads(), 2 * grad, rtol=1e-5, atol=1e-6 * np.abs(grad).max())
 
 
-def test_adam_matches_oracle(net, system):
+def test_adam_matches_oracle(system):
     rp, ci, va, fm = system
     th = _params(7)
+    net = capi.Gnn(0)  # fresh optimizer state
+    request_close = net.close
     net.set_params(th)
     net.set_system(rp, ci, va, fm)
     net.zero_grad()
@@ -134,7 +136,10 @@ def test_adam_matches_oracle(net, system):
     grad = net.get_grads().astype(np.float64)
     net.adam_step(1e-3, 1)
     th1, _, _ = G.adam(th.astype(np.float64), grad, np.zeros_like(grad), np.zeros_like(grad), 1, 1e-3)
-    assert np.abs(net.get_params() - th1).max() <= 1e-6
+    try:
+        assert np.abs(net.get_params() - th1).max() <= 1e-6
+    finally:
+        request_close()
 
 
 def test_training_beats_untrained_network():
