@@ -50,7 +50,10 @@ enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1, OC_PRECOND_AMG = 2, O
  * MIXED: element energy, gradient and projected Hessian in FP32 (on vertex positions relative to each
  *   triangle's first vertex), energy and gradient sums in compensated FP64, the multigrid hierarchy in FP32;
  *   the assembled Hessian (summed in FP64 from the FP32 element blocks), the injectivity step-bound roots, the
- *   inversion test, the PCG matrix products, vectors, residual recurrence and scalars stay FP64. */
+ *   inversion test, the PCG matrix products, vectors, residual recurrence and scalars stay FP64. The FP32
+ *   element Hessians carry rounding noise of ~1e-7 of each block; Newton matrices of distorted maps reach
+ *   condition numbers near 1e15, so that noise perturbs their smooth spectrum and can make them indefinite:
+ *   long Krylov runs may then break down (the driver falls back as pcg_fallback reports). */
 enum { OC_PRECISION_FP64 = 0, OC_PRECISION_MIXED = 1 };
 
 typedef struct oc_ctx oc_ctx;
