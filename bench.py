@@ -32,6 +32,25 @@ def spmv_bytes(nnzb, nV):
     return 36 * nnzb + 4 * (nV + 1) + 64 * nV
 
 
+def lanczos0_bytes(nnzb, nV):
+    """Algorithmic bytes of one fine-level Lanczos step of the multigrid set-up
+    (k_lanczos on the 2x2-block CSR): values 32 B + column index 4 B per block,
+    row pointers 4 B per row, per vertex the inverse diagonal block (32 B),
+    W_{j-1}, v_{j-1}, v_{j-2} read once (48 B) and v_j, W_j written (32 B)."""
+    return 36 * nnzb + 4 * (nV + 1) + 112 * nV
+
+
+# HBM traffic per launch of the roofline kernels at the 200k-face workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), FETCH_SIZE
+# scaled by 2.0 and WRITE_SIZE by 1.0: the factors that make the same counters
+# read the known size of a plain 512 MiB device copy. Summaries with commands:
+# profiles/pmc_traffic_chosen_200k.md.
+TRAFFIC_200K = {
+    "k_lanczos_fine": (2.0 * 21601.4 + 3317.9) * 1024,
+    "k_pcg_spmv": (2.0 * 13153.8 + 2428.7) * 1024,
+}
+
+
 def max_iter_for(precond):
     from optcuts_amd import capi
     return 1000 if precond == capi.PRECOND_AMG else 200000
@@ -112,7 +131,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tte", action="store_true", help="skip the time-to-E_SD run")
     ap.add_argument("--tte-max-iter", type=int, default=60)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_per_launch": a.numel() * 8}), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -157,6 +187,7 @@ def main():
         r = {"newton_iters_per_s": args.steps / el, "ms_per_step": 1e3 * el / args.steps,
              "pcg_iters_per_newton": float(np.mean([s["pcg_iters"] for s in timed])),
              "pcg_fallback_codes": [s["pcg_fallback"] for s in timed],
+             "precond_setups": [s["precond_setups"] for s in timed],
              "precond": {capi.PRECOND_JACOBI: "jacobi", capi.PRECOND_AMG: "sa-amg",
                          capi.PRECOND_LEARNED: "learned-factor"}[precond],
              "precision": {capi.PRECISION_FP64: "fp64", capi.PRECISION_MIXED: "mixed"}[precision],
@@ -169,6 +200,8 @@ def main():
         if precond in (capi.PRECOND_AMG, capi.PRECOND_LEARNED):
             r["kernel_ms"]["precond_apply"] = ctx.time_kernel(7, reps=10)
             r["kernel_ms"]["precond_setup"] = ctx.time_kernel(8, reps=5)
+        if precond == capi.PRECOND_AMG:
+            r["kernel_ms"]["lanczos_fine_step"] = ctx.time_kernel(9, reps=50)
         if e_star is not None and not args.no_tte:
             r["time_to_E_SD_1e-6"] = time_to_target(ctx, V, F, UV, tol, precond, precision, e_star,
                                                     args.tte_max_iter)
@@ -179,9 +212,22 @@ def main():
 
     head = res["chosen"] if "chosen" in res else res["plain"]
     plain = res.get("plain")
-    b = spmv_bytes(nnzb, nVt)
-    ms_spmv = (plain or head)["kernel_ms"]["pcg_spmv"]
-    achieved = b / (ms_spmv * 1e-3) / 1e9
+    is_200k = args.grid == 317 and args.seed == 0
+
+    def roofline(kernel, nbytes, ms, traffic_key):
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_200K[traffic_key] if is_200k else None,
+                "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": ms}
+
+    if "chosen" in res:
+        # the chosen form's dominant kernel (kernel trace: k_lanczos, fine level)
+        rl = roofline("k_lanczos<double,2,8> (fine-level Lanczos step of the multigrid set-up)",
+                      lanczos0_bytes(nnzb, nVt), res["chosen"]["kernel_ms"]["lanczos_fine_step"], "k_lanczos_fine")
+    else:
+        rl = roofline("k_pcg_spmv", spmv_bytes(nnzb, nVt), head["kernel_ms"]["pcg_spmv"], "k_pcg_spmv")
+    rl_plain = roofline("k_pcg_spmv", spmv_bytes(nnzb, nVt), plain["kernel_ms"]["pcg_spmv"],
+                        "k_pcg_spmv") if plain else None
     speeds = {}
     if "plain" in res and e_star is not None and not args.no_tte:
         tp = res["plain"]["time_to_E_SD_1e-6"]["seconds"]
@@ -210,9 +256,8 @@ def main():
         "forms": res,
         "time_to_E_SD_speedup_chosen_vs_plain": speed,
         "time_to_E_SD_speedup_vs_plain": speeds,
-        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": b},
+        "roofline": rl,
+        "roofline_plain_form": rl_plain,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

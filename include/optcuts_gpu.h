@@ -154,6 +154,7 @@ typedef struct {
                          (r.z or p.Ap not positive) or hit pcgMaxIter, and 1: multigrid rebuilt with its
                          guaranteed smoother interval solved it; 2: Jacobi-PCG solved it */
     int pcg_iters_first; /* iterations of the first attempt when pcg_fallback != 0 (pcg_iters counts all) */
+    int precond_setups;  /* multigrid hierarchies built on this step (1, or 2 after a safe-interval retry) */
 } oc_iter_stats;
 
 /* Optimizer::precompute (Optimizer.cpp:154-201) for energyParams = {a0}: sets targetGRes =
@@ -175,10 +176,12 @@ int oc_newton_set_learned(oc_ctx* ctx, oc_gnn* net);
 int oc_newton_step(oc_ctx* ctx, oc_iter_stats* stats, int* done);
 
 /* ---- timing of the dominant kernels (HIP events on the context stream) ---- */
-/* Average device time in ms (HIP events around each launch) of `reps` launches of: 0 element pass (E, g,
+/* Average device time in ms (HIP events around `reps` back-to-back launches) of: 0 element pass (E, g,
  * projected H), 1 block assembly, 2 one PCG iteration (both kernels), 3 line-search candidates, 4 step bound,
- * 5 PCG SpMV kernel, 6 PCG update kernel, 7 one AMG V-cycle, 8 AMG numeric set-up. Needs oc_newton_init (7 and
- * 8 with OC_PRECOND_AMG and one oc_newton_step); the current search direction is kept. */
+ * 5 PCG SpMV kernel, 6 PCG update kernel, 7 one application of the multigrid or learned preconditioner, 8 its
+ * set-up (multigrid hierarchy or network forward pass), 9 one fine-level Lanczos step of the multigrid set-up
+ * (the lambda_max estimate). Needs oc_newton_init (7-9 with that preconditioner and one oc_newton_step); the
+ * current search direction is kept. */
 int oc_time_kernel(oc_ctx* ctx, int which, int reps, double* ms);
 /* Sizes used for algorithmic-byte accounting: triangles, vertices (combined), stored blocks, upper blocks. */
 int oc_system_sizes(oc_ctx* ctx, long long* nF, long long* nV, long long* nnzBlocks, long long* nUpperBlocks);

@@ -38,7 +38,7 @@ class IterStats(C.Structure):
     _fields_ = [("E_total", C.c_double), ("E_SD", C.c_double), ("E_scaf", C.c_double), ("grad_sq", C.c_double),
                 ("step", C.c_double), ("halvings", C.c_int), ("stopped", C.c_int), ("converged", C.c_int),
                 ("pcg_iters", C.c_int), ("pcg_rel_res", C.c_double), ("pcg_fallback", C.c_int),
-                ("pcg_iters_first", C.c_int)]
+                ("pcg_iters_first", C.c_int), ("precond_setups", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

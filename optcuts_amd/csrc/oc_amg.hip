@@ -360,22 +360,23 @@ __global__ void __launch_bounds__(kBlock) k_lanczos_finish(const double* __restr
                                                            const double* __restrict__ gPart, int safe, double* coef,
                                                            double* lam)
 {
+    __shared__ double red[kBlock];
     {
-        __shared__ double gsh[kBlock];
-        double m = 0.0;
-        for (int b = threadIdx.x; b < nPartIn; b += blockDim.x) m = fmax(m, gPart[b]);
-        gsh[threadIdx.x] = m;
+        double g = 0.0;
+        for (int b = threadIdx.x; b < nPartIn; b += blockDim.x) g = fmax(g, gPart[b]);
+        red[threadIdx.x] = g;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double g = 0.0;
-            for (int t = 0; t < (int)blockDim.x; ++t) g = fmax(g, gsh[t]);
-            lam[1] = g;
-            if (safe) lam[0] = g / 1.1;
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+            __syncthreads();
         }
-        __syncthreads();
+        if (threadIdx.x == 0) {
+            lam[1] = red[0];
+            if (safe) lam[0] = red[0] / 1.1;
+        }
         if (safe) return;
     }
-    int m = (int)coef[2 * kLanczosSteps];
+    const int m = (int)coef[2 * kLanczosSteps];
     if (m == kLanczosSteps) {
         double acc[2] = {0.0, 0.0};
         for (int b = threadIdx.x; b < nPartIn; b += blockDim.x) {
@@ -386,36 +387,55 @@ __global__ void __launch_bounds__(kBlock) k_lanczos_finish(const double* __restr
         if (threadIdx.x == 0) coef[kLanczosSteps - 1] = acc[0];
         __syncthreads();
     }
-    if (threadIdx.x != 0) return;
     if (m == 0) {  // start vector in the null space of A: no information
-        *lam = 1.0;
+        if (threadIdx.x == 0) lam[0] = 1.0;
         return;
     }
-    const double* al = coef;
-    const double* be = coef + kLanczosSteps;  // be[i] couples rows i-1 and i (i >= 1)
-    double lo = al[0], hi = al[0];
-    for (int i = 0; i < m; ++i) {
-        const double r = (i > 0 ? fabs(be[i]) : 0.0) + (i + 1 < m ? fabs(be[i + 1]) : 0.0);
-        lo = fmin(lo, al[i] - r);
-        hi = fmax(hi, al[i] + r);
+    // T: al[0..m), be[i] couples rows i-1 and i (i >= 1)
+    __shared__ double al[kLanczosSteps], be[kLanczosSteps];
+    __shared__ double lohi[2];
+    __shared__ int hit[kWave];
+    if ((int)threadIdx.x < m) {
+        al[threadIdx.x] = coef[threadIdx.x];
+        be[threadIdx.x] = coef[kLanczosSteps + threadIdx.x];
     }
-    // count(x) = number of eigenvalues of T below x; lambda_max = sup{x : count(x) < m}
-    for (int it = 0; it < 100 && hi - lo > 1e-13 * fmax(fabs(hi), 1e-300); ++it) {
-        const double x = 0.5 * (lo + hi);
-        int cnt = 0;
-        double d = 1.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double lo = al[0], hi = al[0];
         for (int i = 0; i < m; ++i) {
-            const double b2 = i > 0 ? be[i] * be[i] : 0.0;
-            d = al[i] - x - (i > 0 ? b2 / d : 0.0);
-            if (d == 0.0) d = -1e-300;
-            if (d < 0.0) ++cnt;
+            const double r = (i > 0 ? fabs(be[i]) : 0.0) + (i + 1 < m ? fabs(be[i + 1]) : 0.0);
+            lo = fmin(lo, al[i] - r);
+            hi = fmax(hi, al[i] + r);
         }
-        if (cnt == m)
-            hi = x;
-        else
-            lo = x;
+        lohi[0] = lo;
+        lohi[1] = hi;
     }
-    *lam = hi;
+    __syncthreads();
+    // Multisection on the Sturm count (number of eigenvalues of T below x):
+    // 64 points per round shrink [lo, hi] around lambda_max by 65x.
+    for (int round = 0; round < 6; ++round) {
+        const double lo = lohi[0], h = (lohi[1] - lohi[0]) / (kWave + 1);
+        if ((int)threadIdx.x < kWave) {
+            const double x = lo + (threadIdx.x + 1) * h;
+            int cnt = 0;
+            double d = 1.0;
+            for (int i = 0; i < m; ++i) {
+                d = al[i] - x - (i > 0 ? be[i] * be[i] / d : 0.0);
+                if (d == 0.0) d = -1e-300;
+                if (d < 0.0) ++cnt;
+            }
+            hit[threadIdx.x] = cnt == m;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int k = 0;
+            while (k < kWave && !hit[k]) ++k;  // counts grow with x: the first hit brackets lambda_max
+            lohi[0] = lo + k * h;
+            if (k < kWave) lohi[1] = lo + (k + 1) * h;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lam[0] = lohi[1];
 }
 
 // Tentative prolongator: per aggregate, B (rows of its members) = Q R by
@@ -932,6 +952,34 @@ template <typename T>
 long Amg<T>::level_rows(int l) const
 {
     return (long)L[l]->n * L[l]->bs;
+}
+
+template <typename T>
+void Amg<T>::lanczos0_prepare(hipStream_t s)
+{
+    Level& A = *L[0];
+    const int g = rows_grid(A.n, kBlock / Lanes<2>::value);
+    double* part[2] = {dPartials, dPartials + nPartials};
+    const size_t nv = (size_t)A.n * 2;
+    double* coef = A.lz + 5 * nv;
+    LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv}};
+    hipLaunchKernelGGL((k_lanczos_start<T, 2>), dim3(g), dim3(kBlock), 0, s, A.n, A.dRowPtr, A.vals, A.Dinv, lb.W[1],
+                       part[1], dPartials + 2 * (size_t)nPartials, coef);
+    for (int j = 0; j < 2; ++j)
+        hipLaunchKernelGGL((k_lanczos<T, 2, Lanes<2>::value>), dim3(g), dim3(kBlock), 0, s, j, A.n, A.dRowPtr,
+                           A.dColIdx, A.vals, A.Dinv, lb, part[(j + 1) & 1], g, part[j & 1], coef);
+}
+
+template <typename T>
+void Amg<T>::lanczos0_step(hipStream_t s)
+{
+    // step 2 reads part[1] (written by step 1) and writes part[0]: repeatable
+    Level& A = *L[0];
+    const int g = rows_grid(A.n, kBlock / Lanes<2>::value);
+    const size_t nv = (size_t)A.n * 2;
+    LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv}};
+    hipLaunchKernelGGL((k_lanczos<T, 2, Lanes<2>::value>), dim3(g), dim3(kBlock), 0, s, 2, A.n, A.dRowPtr, A.dColIdx,
+                       A.vals, A.Dinv, lb, dPartials + nPartials, g, dPartials, A.lz + 5 * nv);
 }
 
 template <typename T>

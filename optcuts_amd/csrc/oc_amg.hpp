@@ -44,6 +44,12 @@ public:
     virtual int levels() const = 0;
     virtual long level_rows(int l) const = 0;
     virtual long level_blocks(int l) const = 0;
+    // Prepares the level-0 Lanczos recurrence (start and two steps) of the
+    // current hierarchy; after it, lanczos0_step() launches one further
+    // level-0 step that reads and writes the same buffers every time (for
+    // timing). Call after numeric().
+    virtual void lanczos0_prepare(hipStream_t s) = 0;
+    virtual void lanczos0_step(hipStream_t s) = 0;
     virtual bool ready() const = 0;
 };
 
@@ -76,6 +82,8 @@ public:
     int levels() const override { return (int)L.size(); }
     long level_rows(int l) const override;
     long level_blocks(int l) const override;
+    void lanczos0_prepare(hipStream_t s) override;
+    void lanczos0_step(hipStream_t s) override;
     bool ready() const override { return ready_; }
 
 private:

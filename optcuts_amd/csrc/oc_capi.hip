@@ -1251,7 +1251,14 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     // Hessian assembly + direction: H d = -g
     system_assemble(c);
     oc::launch_scale_copy(s, 2L * nVt, -1.0, c->dG.as<double>(), c->dB.as<double>());
-    if (c->amgValid) c->amg->numeric(s, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>());
+    // The multigrid hierarchy is rebuilt for every Newton matrix: reusing the
+    // previous one stays a valid (SPD) preconditioner but, measured at 200k
+    // faces, early Newton steps change the matrix enough to multiply the PCG
+    // iterations (12 -> 733), which costs more than the set-up saved.
+    if (c->amgValid) {
+        c->amg->numeric(s, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>());
+        ++S->precond_setups;
+    }
     if (c->nPrecond == OC_PRECOND_LEARNED) c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
     int it = 0;
     double rr = 0.0;
@@ -1264,6 +1271,7 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
         S->pcg_fallback = 2;
         if (c->amgValid) {
             c->amg->numeric(s, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>(), 1);
+            ++S->precond_setups;
             int it2 = 0;
             st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, c->ext);
             it += it2;
@@ -1369,9 +1377,13 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
 int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
 {
     if (!c) return OC_ERR_ARG;
-    OC_ARG(c, ms && reps > 0 && which >= 0 && which <= 8, "oc_time_kernel: bad args");
+    OC_ARG(c, ms && reps > 0 && which >= 0 && which <= 9, "oc_time_kernel: bad args");
     if ((which == 7 || which == 8) && !c->ext) {
         c->err = "oc_time_kernel: no multigrid or learned preconditioner is active";
+        return OC_ERR_STATE;
+    }
+    if (which == 9 && !c->amgValid) {
+        c->err = "oc_time_kernel: the multigrid preconditioner is not active";
         return OC_ERR_STATE;
     }
     if (!c->sysReady) {
@@ -1435,9 +1447,13 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
             else
                 c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
             break;
+        case 9:
+            c->amg->lanczos0_step(s);
+            break;
         }
     };
     if (which == 7) OC_HIP(c, c->dTmpVec.ensure(sizeof(double) * 2 * (size_t)c->nVt));
+    if (which == 9) c->amg->lanczos0_prepare(s);
     if (which == 2 || which == 5 || which == 6) {
         // Keep the PCG kernels doing full work on the current system: a
         // private copy of x (dTmpVec) so the last direction is not
@@ -1468,17 +1484,16 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
     OC_HIP(c, hipEventCreate(&e0));
     OC_HIP(c, hipEventCreate(&e1));
     launch();  // warm
-    double total = 0.0;
-    for (int r = 0; r < reps; ++r) {
-        OC_HIP(c, hipEventRecord(e0, s));
-        launch();
-        OC_HIP(c, hipEventRecord(e1, s));
-        OC_HIP(c, hipEventSynchronize(e1));
-        float t = 0.f;
-        OC_HIP(c, hipEventElapsedTime(&t, e0, e1));
-        total += t;
-    }
-    *ms = total / reps;
+    OC_HIP(c, hipStreamSynchronize(s));
+    // back-to-back launches between one pair of events: the per-launch
+    // average then excludes the event round trips
+    OC_HIP(c, hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) launch();
+    OC_HIP(c, hipEventRecord(e1, s));
+    OC_HIP(c, hipEventSynchronize(e1));
+    float t = 0.f;
+    OC_HIP(c, hipEventElapsedTime(&t, e0, e1));
+    *ms = (double)t / reps;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     OC_HIP(c, hipGetLastError());
