@@ -175,6 +175,21 @@ int oc_newton_set_learned(oc_ctx* ctx, oc_gnn* net);
  * step. *done = 1 when the solve returned 1 (gradient tolerance reached or line search stopped). */
 int oc_newton_step(oc_ctx* ctx, oc_iter_stats* stats, int* done);
 
+/* ---- batched charts ----------------------------------------------------------------------------------
+ * Many independent charts in one context: after oc_set_mesh with the union of the charts (chart k owns the
+ * vertices [vertexStart[k], vertexStart[k+1]) and the triangles [faceStart[k], faceStart[k+1]), triangles
+ * referencing only their own chart's vertices; fixed vertices as usual), oc_set_charts makes every later
+ * oc_newton_init / oc_newton_step treat each chart as its own problem: its own surface-area weights, gradient
+ * tolerance, injectivity bound, line search, step and stop tests — the results of separate
+ * Optimizer::solve(1) calls per chart (Optimizer.cpp:206-231, 575-652) — while each kernel launch and the
+ * (block-diagonal) Newton solve cover all charts at once. The step reports sums over charts; *done = 1 once
+ * no chart is active. oc_set_mesh clears the charts; an air mesh is not supported with charts. No reference
+ * counterpart (the reference runs charts one at a time). */
+int oc_set_charts(oc_ctx* ctx, int nCharts, const int* vertexStart, const int* faceStart);
+/* Per chart after the last oc_newton_init / oc_newton_step: E_SD, |g|^2, step size, state (0 active, 1 gradient
+ * tolerance reached, 2 line search stopped). Any pointer may be null; arrays of nCharts. */
+int oc_charts_state(oc_ctx* ctx, double* E_SD, double* gradSq, double* step, int* state);
+
 /* ---- timing of the dominant kernels (HIP events on the context stream) ---- */
 /* Average device time in ms (HIP events around `reps` back-to-back launches) of: 0 element pass (E, g,
  * projected H), 1 block assembly, 2 one PCG iteration (both kernels), 3 line-search candidates, 4 step bound,

@@ -79,6 +79,8 @@ _SIGS = {
     "oc_solver_set_params": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int]),
     "oc_solver_last_stats": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "oc_newton_get_system": (C.c_int, [_ctx, _vp, _vp, _vp, _vp, _vp]),
+    "oc_set_charts": (C.c_int, [_ctx, C.c_int, _ip, _ip]),
+    "oc_charts_state": (C.c_int, [_ctx, _vp, _vp, _vp, _vp]),
     "oc_newton_set_learned": (C.c_int, [_ctx, _vp]),
     "oc_gnn_num_params": (C.c_int, []),
     "oc_gnn_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
@@ -276,6 +278,19 @@ class Context:
         return it.value, rr.value
 
     # ---- Newton driver -------------------------------------------------------
+    def set_charts(self, vertex_start, face_start):
+        """Batched charts: contiguous vertex / triangle ranges of the mesh set by set_mesh."""
+        vs, fs = _i32(vertex_start), _i32(face_start)
+        self._chk(self.L.oc_set_charts(self.h, len(vs) - 1, vs, fs))
+        self.n_charts = len(vs) - 1
+
+    def charts_state(self):
+        n = self.n_charts
+        E, g, a = np.zeros(n), np.zeros(n), np.zeros(n)
+        st = np.zeros(n, np.int32)
+        self._chk(self.L.oc_charts_state(self.h, E.ctypes.data, g.ctypes.data, a.ctypes.data, st.ctypes.data))
+        return {"E_SD": E, "grad_sq": g, "step": a, "state": st}
+
     def set_learned(self, net):
         """Bind a Gnn for PRECOND_LEARNED (None unbinds); the context keeps a reference."""
         self._learned = net

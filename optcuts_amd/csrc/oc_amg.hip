@@ -1045,12 +1045,14 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
     }
     const Level& C = *L.back();
     nCoarse = C.n * C.bs;
-    if (nCoarse > kCoarseMaxDofs) {
+    coarseDiag = true;
+    for (int u = 0; u < C.n; ++u) coarseDiag = coarseDiag && C.rowPtr[u + 1] - C.rowPtr[u] == 1;
+    if (nCoarse > kCoarseMaxDofs && !coarseDiag) {
         err = "AMG: coarsening stalled above the dense coarsest size";
         release();
         return -1;
     }
-    dalloc(dCoarseInv, (size_t)nCoarse * nCoarse);
+    if (!coarseDiag) dalloc(dCoarseInv, (size_t)nCoarse * nCoarse);
     nPartials = 2 * kAmgMaxGrid;  // two doubles per workgroup record
     dalloc(dPartials, (size_t)nPartials * 2 + kAmgMaxGrid);  // two record buffers + Gershgorin records
     if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) {
@@ -1133,8 +1135,39 @@ void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double
         }
     }
     const Level& C = *L.back();
-    hipLaunchKernelGGL(k_dense_inverse<T>, dim3(1), dim3(256), 0, s, C.n, C.bs, C.dRowPtr, C.dColIdx, C.vals,
-                       dCoarseInv);
+    if (!coarseDiag)
+        hipLaunchKernelGGL(k_dense_inverse<T>, dim3(1), dim3(256), 0, s, C.n, C.bs, C.dRowPtr, C.dColIdx, C.vals,
+                           dCoarseInv);
+}
+
+// x = D^-1 b on a block-diagonal coarsest level (exact)
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_coarse_diag(const int* __restrict__ stop, int n, const T* __restrict__ Dinv,
+                                                        const T* __restrict__ b, T* x)
+{
+    if (stop && *stop) return;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const T* D = Dinv + (size_t)u * BS * BS;
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < BS; ++j) acc += (double)D[i * BS + j] * (double)b[(size_t)u * BS + j];
+        x[(size_t)u * BS + i] = (T)acc;
+    }
+}
+
+template <typename T>
+void Amg<T>::coarse_solve(hipStream_t s, const int* stop)
+{
+    Level& C = *L.back();
+    if (!coarseDiag)
+        hipLaunchKernelGGL(k_dense_apply<T>, dim3(1), dim3(256), 0, s, stop, nCoarse, dCoarseInv, C.b, C.x);
+    else if (C.bs == 2)
+        hipLaunchKernelGGL((k_coarse_diag<T, 2>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, stop, C.n, C.Dinv, C.b, C.x);
+    else
+        hipLaunchKernelGGL((k_coarse_diag<T, 3>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, stop, C.n, C.Dinv, C.b, C.x);
 }
 
 template <typename T>
@@ -1142,7 +1175,7 @@ void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
 {
     Level& A = *L[l];
     if (l + 1 == (int)L.size()) {
-        hipLaunchKernelGGL(k_dense_apply<T>, dim3(1), dim3(256), 0, s, stop, nCoarse, dCoarseInv, A.b, A.x);
+        coarse_solve(s, stop);
         return;
     }
     Level& N = *L[l + 1];
@@ -1195,7 +1228,7 @@ void Amg<T>::apply(hipStream_t s, const double* r, double* z, double* rzPart, co
     if (L.size() == 1) {
         // single level: the dense inverse is the whole preconditioner
         hipLaunchKernelGGL((k_cvt<double, T>), dim3(grid_for((long)A.n * 2)), dim3(kBlock), 0, s, (long)A.n * 2, r, A.b);
-        hipLaunchKernelGGL(k_dense_apply<T>, dim3(1), dim3(256), 0, s, stop, nCoarse, dCoarseInv, A.b, A.x);
+        coarse_solve(s, stop);
         hipLaunchKernelGGL(k_finish_z<T>, dim3(g), dim3(kBlock), 0, s, stop, A.n * 2, A.x, r, z, rzPart);
         return;
     }

@@ -89,13 +89,15 @@ public:
 private:
     struct Level;
     std::vector<Level*> L;
-    int nCoarse = 0;           // DOFs of the dense coarsest level
+    int nCoarse = 0;           // DOFs of the coarsest level
+    bool coarseDiag = false;   // coarsest level block diagonal: solved by its diagonal-block inverses
     double* dCoarseInv = nullptr;  // nCoarse x nCoarse, double
     double* dPartials = nullptr;
     int nPartials = 0;
     bool ready_ = false;
     void release();
     void vcycle(hipStream_t s, int l, const int* stop);
+    void coarse_solve(hipStream_t s, const int* stop);
 };
 
 extern template class Amg<float>;

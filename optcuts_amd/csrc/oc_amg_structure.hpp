@@ -94,9 +94,21 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
                 return -1;
             }
         if ((long)A.n * A.bs <= coarseMaxDofs || (int)L.size() >= maxLevels) break;
+        // Isolated nodes (only a diagonal block: a disconnected component
+        // already reduced to one node, e.g. one of many independent charts)
+        // stay singleton aggregates; once every node is isolated the level is
+        // block diagonal and its diagonal-block inverse is the exact coarse
+        // solve.
+        int nIso = 0, nIn = 0;
+        for (int u = 0; u < A.n; ++u) {
+            if (A.excluded[u]) continue;
+            ++nIn;
+            nIso += A.rowPtr[u + 1] - A.rowPtr[u] == 1;
+        }
+        if (nIso == nIn) break;
         std::vector<int> agg;
         const int nc = aggregate_nodes(A.n, A.rowPtr, A.colIdx, A.excluded, agg);
-        if (nc == 0 || nc * 10 > A.n * 9) break;  // coarsening stalled
+        if (nc == 0 || (long)(nc - nIso) * 10 > (long)(nIn - nIso) * 9) break;  // coupled part stopped coarsening
         A.nc = nc;
         // aggregate members
         A.aggPtr.assign(nc + 1, 0);

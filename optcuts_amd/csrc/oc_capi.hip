@@ -132,6 +132,15 @@ struct oc_ctx {
     int lastIters = 0;
     double lastRelRes = 0.0;
 
+    // ---- batched charts (contiguous vertex / triangle ranges of the mesh) ----------
+    int nCharts = 0;  // 0: one problem
+    std::vector<int> chartV, chartF;        // nCharts + 1 range starts
+    std::vector<int> chartBlk;              // chart-aligned triangle blocks of chart c: [chartBlk[c], chartBlk[c+1])
+    std::vector<double> chartTarget, chartLastE, chartGsq, chartStep;
+    std::vector<int> chartState;            // 0 active, 1 gradient converged, 2 line search stopped
+    DevBuf dChartV, dChartF, dChartBlk, dBlkRange, dBlkChart, dVChart, dTriChart, dChartArea, dChartActive,
+        dChartAlpha, dChartOrdMin, dChartOut, dChartGsq;
+
     // ---- LinSysSolver drop-in ---------------------------------------------------
     BlockPattern sol;
     int solN = 0;
@@ -211,6 +220,10 @@ TriSet mesh_set(oc_ctx* c, int uniform)
     T.E1 = c->dE1.as<double>();
     T.E01 = c->dE01.as<double>();
     T.surfaceArea = c->dMeshSums.as<double>();
+    if (c->nCharts > 0) {
+        T.surfaceArea = c->dChartArea.as<double>();
+        T.triChart = c->dTriChart.as<int>();
+    }
     T.uniformMesh = uniform;
     T.sMesh = 1.0;
     T.sAir = 1.0;
@@ -652,6 +665,7 @@ int oc_set_mesh(oc_ctx* c, int nV, const double* Vrest, int nF, const int* F, in
     c->nBnd = 0;
     c->FairGlobal.clear();
     c->airFixedGlobal.clear();
+    c->nCharts = 0;
     c->nV = nV;
     c->nF = nF;
     c->Fh.assign(F, F + 3 * (size_t)nF);
@@ -847,6 +861,7 @@ int oc_set_air_mesh(oc_ctx* c, int nVair, const double* airUV, int nFair, const 
 {
     if (!c) return OC_ERR_ARG;
     OC_ARG(c, c->nV > 0, "oc_set_air_mesh: no mesh");
+    OC_ARG(c, c->nCharts == 0 || nFair == 0, "oc_set_air_mesh: not available with batched charts");
     (void)hipSetDevice(c->device);
     c->sysReady = false;
     if (nFair == 0) {
@@ -1138,6 +1153,87 @@ int oc_solver_last_stats(oc_ctx* c, int* iters, double* relResidual)
     return OC_OK;
 }
 
+// ---- batched charts -------------------------------------------------------------------
+int oc_set_charts(oc_ctx* c, int nCharts, const int* vertexStart, const int* faceStart)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0, "oc_set_charts: no mesh");
+    OC_ARG(c, c->nFair == 0, "oc_set_charts: not available with an air mesh");
+    OC_ARG(c, nCharts >= 1 && vertexStart && faceStart, "oc_set_charts: bad arguments");
+    OC_ARG(c, vertexStart[0] == 0 && vertexStart[nCharts] == c->nV && faceStart[0] == 0 &&
+                  faceStart[nCharts] == c->nF,
+           "oc_set_charts: ranges must cover all vertices and triangles");
+    for (int k = 0; k < nCharts; ++k)
+        OC_ARG(c, vertexStart[k + 1] > vertexStart[k] && faceStart[k + 1] > faceStart[k],
+               "oc_set_charts: every chart needs vertices and triangles");
+    std::vector<int> vChart(c->nV), triChart(c->nF);
+    for (int k = 0; k < nCharts; ++k) {
+        for (int v = vertexStart[k]; v < vertexStart[k + 1]; ++v) vChart[v] = k;
+        for (int t = faceStart[k]; t < faceStart[k + 1]; ++t) {
+            triChart[t] = k;
+            for (int j = 0; j < 3; ++j) {
+                const int v = c->Fh[3 * (size_t)t + j];
+                OC_ARG(c, v >= vertexStart[k] && v < vertexStart[k + 1],
+                       "oc_set_charts: a triangle uses a vertex of another chart");
+            }
+        }
+    }
+    (void)hipSetDevice(c->device);
+    c->sysReady = false;
+    c->chartV.assign(vertexStart, vertexStart + nCharts + 1);
+    c->chartF.assign(faceStart, faceStart + nCharts + 1);
+    // chart-aligned triangle blocks
+    std::vector<int2> blk;
+    std::vector<int> blkChart;
+    c->chartBlk.assign(nCharts + 1, 0);
+    for (int k = 0; k < nCharts; ++k) {
+        for (int t0 = faceStart[k]; t0 < faceStart[k + 1]; t0 += oc::kBlock) {
+            int2 r;
+            r.x = t0;
+            r.y = std::min(t0 + oc::kBlock, faceStart[k + 1]);
+            blk.push_back(r);
+            blkChart.push_back(k);
+        }
+        c->chartBlk[k + 1] = (int)blk.size();
+    }
+    if (upload(c, c->dChartV, c->chartV) || upload(c, c->dChartF, c->chartF) || upload(c, c->dChartBlk, c->chartBlk) ||
+        upload(c, c->dBlkChart, blkChart) || upload(c, c->dVChart, vChart) || upload(c, c->dTriChart, triChart))
+        return OC_ERR_HIP;
+    OC_HIP(c, c->dBlkRange.ensure(sizeof(int2) * blk.size()));
+    OC_HIP(c, hipMemcpyAsync(c->dBlkRange.p, blk.data(), sizeof(int2) * blk.size(), hipMemcpyHostToDevice, c->stream));
+    const size_t nC = (size_t)nCharts;
+    OC_HIP(c, c->dChartArea.ensure(sizeof(double) * nC));
+    OC_HIP(c, c->dChartActive.ensure(nC));
+    OC_HIP(c, c->dChartAlpha.ensure(sizeof(double) * nC));
+    OC_HIP(c, c->dChartOrdMin.ensure(sizeof(unsigned long long) * nC));
+    OC_HIP(c, c->dChartOut.ensure(sizeof(double) * 2 * oc::kLSCand * nC));
+    OC_HIP(c, c->dChartGsq.ensure(sizeof(double) * nC));
+    if (ensure_scratch(c, (long)c->nF + (long)oc::kBlock * nCharts)) return OC_ERR_HIP;
+    // per-chart surface areas (the weight of chart c's triangles is area / its chart's area)
+    oc::launch_chart_seg_sum(c->stream, nCharts, c->dChartF.as<int>(), c->dTriArea.as<double>(),
+                             c->dChartArea.as<double>());
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->nCharts = nCharts;
+    return OC_OK;
+}
+
+int oc_charts_state(oc_ctx* c, double* E_SD, double* gradSq, double* step, int* state)
+{
+    if (!c) return OC_ERR_ARG;
+    if (c->nCharts == 0 || c->chartState.empty()) {
+        c->err = "oc_charts_state: no batched charts initialised (oc_set_charts, oc_newton_init)";
+        return OC_ERR_STATE;
+    }
+    for (int k = 0; k < c->nCharts; ++k) {
+        if (E_SD) E_SD[k] = c->chartLastE[k] / c->a0;
+        if (gradSq) gradSq[k] = c->chartGsq[k];
+        if (step) step[k] = c->chartStep[k];
+        if (state) state[k] = c->chartState[k];
+    }
+    return OC_OK;
+}
+
 // ---- Newton driver ------------------------------------------------------------------
 int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, int precond, double pcgRelTol,
                    int pcgMaxIter, double* E0)
@@ -1167,6 +1263,35 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     if (int st = build_system(c)) return st;
     c->targetGRes = a0 * (double)(c->nV - (int)c->fixedSorted.size()) / (double)c->nV * relGL2Tol;
     if (int st = system_energy(c, &c->lastE, &c->lastESD, &c->lastEScaf)) return st;
+    if (c->nCharts > 0) {
+        // per-chart targetGRes (Optimizer.cpp:675-678) and energies
+        const int nC = c->nCharts;
+        c->chartTarget.assign(nC, 0.0);
+        for (int k = 0; k < nC; ++k) {
+            int nFix = 0;
+            for (int v : c->fixedSorted) nFix += v >= c->chartV[k] && v < c->chartV[k + 1];
+            const int nVc = c->chartV[k + 1] - c->chartV[k];
+            c->chartTarget[k] = a0 * (double)(nVc - nFix) / (double)nVc * relGL2Tol;
+        }
+        oc::launch_element(c->stream, mesh_set(c, 0), c->dUV.as<double>(), c->dEelem.as<double>(), nullptr, nullptr, 0,
+                           nullptr, nullptr);
+        oc::launch_chart_seg_sum(c->stream, nC, c->dChartF.as<int>(), c->dEelem.as<double>(),
+                                 c->dChartGsq.as<double>());
+        OC_HIP(c, hipGetLastError());
+        c->chartLastE.assign(nC, 0.0);
+        OC_HIP(c, hipMemcpyAsync(c->chartLastE.data(), c->dChartGsq.p, sizeof(double) * nC, hipMemcpyDeviceToHost,
+                                 c->stream));
+        OC_HIP(c, hipStreamSynchronize(c->stream));
+        double e = 0.0;
+        for (double& x : c->chartLastE) {
+            x *= a0;
+            e += x;
+        }
+        c->chartState.assign(nC, 0);
+        c->chartGsq.assign(nC, 0.0);
+        c->chartStep.assign(nC, 0.0);
+        c->lastE = e;
+    }
     if (E0) *E0 = c->lastE;
     return OC_OK;
 }
@@ -1198,59 +1323,11 @@ void system_element_pass(oc_ctx* c, const TriSet& T)
                        nullptr, f32);
 }
 
-void system_assemble(oc_ctx* c)
+// Newton direction H d = b (b in c->dB, d in c->dX) with the selected
+// preconditioner and its fallbacks; fills the PCG fields of S.
+int solve_direction(oc_ctx* c, oc_iter_stats* S)
 {
-    const int nUB = (int)c->ubRow.size();
-    const int bj = c->nPrecond == OC_PRECOND_BLOCK_JACOBI;
-    if (c->sysPrecision == OC_PRECISION_MIXED)
-        oc::launch_assemble<float>(c->stream, nUB, c->dUbRow.as<int>(), c->dUbCol.as<int>(), c->dUbPosU.as<int>(),
-                                   c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(), c->dUbCSrc.as<int>(),
-                                   c->dFixedMaskSys.as<unsigned char>(), c->dFixedDiag.as<double>(),
-                                   c->dElemH.as<float>(), c->sys.dVals.p, c->sys.vals32, c->sys.dMinv.as<double>(), bj);
-    else
-        oc::launch_assemble<double>(c->stream, nUB, c->dUbRow.as<int>(), c->dUbCol.as<int>(), c->dUbPosU.as<int>(),
-                                    c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(), c->dUbCSrc.as<int>(),
-                                    c->dFixedMaskSys.as<unsigned char>(), c->dFixedDiag.as<double>(),
-                                    c->dElemH.as<double>(), c->sys.dVals.p, c->sys.vals32, c->sys.dMinv.as<double>(),
-                                    bj);
-}
-}  // namespace
-
-int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
-{
-    if (!c) return OC_ERR_ARG;
-    OC_ARG(c, S && done, "oc_newton_step: null output");
-    if (!c->sysReady) {
-        c->err = "oc_newton_step: call oc_newton_init first";
-        return OC_ERR_STATE;
-    }
-    (void)hipSetDevice(c->device);
-    std::memset(S, 0, sizeof(*S));
-    *done = 0;
-    const TriSet T = system_set(c);
-    const int nVt = c->nVt;
     hipStream_t s = c->stream;
-    const int f32 = c->sysPrecision == OC_PRECISION_MIXED;
-    // gradient (and element Hessians in the same pass)
-    system_element_pass(c, T);
-    oc::launch_gather_grad(s, nVt, c->dVcPtr.as<int>(), c->dVcSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
-                           c->dElemG.as<double>(), c->dG.as<double>(), c->dPartials.as<double>(), scalar(c, 6));
-    OC_HIP(c, hipGetLastError());
-    double gsq;
-    OC_HIP(c, hipMemcpyAsync(&gsq, scalar(c, 6), sizeof(double), hipMemcpyDeviceToHost, s));
-    OC_HIP(c, hipStreamSynchronize(s));
-    S->grad_sq = gsq;
-    S->E_total = c->lastE;
-    S->E_SD = c->lastESD;
-    S->E_scaf = c->lastEScaf;
-    if (gsq < c->targetGRes) {  // Optimizer.cpp:215-221
-        S->converged = 1;
-        *done = 1;
-        return OC_OK;
-    }
-    // Hessian assembly + direction: H d = -g
-    system_assemble(c);
-    oc::launch_scale_copy(s, 2L * nVt, -1.0, c->dG.as<double>(), c->dB.as<double>());
     // The multigrid hierarchy is rebuilt for every Newton matrix: reusing the
     // previous one stays a valid (SPD) preconditioner but, measured at 200k
     // faces, early Newton steps change the matrix enough to multiply the PCG
@@ -1287,6 +1364,199 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     S->pcg_iters = it;
     S->pcg_rel_res = rr;
     if (st != OC_OK && st != OC_ERR_NOT_CONVERGED) return st;
+    return OC_OK;
+}
+
+void system_assemble(oc_ctx* c);
+// One Newton iteration of every active chart (batched charts): the element
+// pass, assembly and PCG run once over the union of the charts (the system is
+// block diagonal, so the solve is the per-chart solves at once); gradient
+// norms, step bounds, line-search energies, step sizes, commits and stop
+// tests are per chart, exactly as separate Optimizer::solve(1) calls would
+// make them (Optimizer.cpp:206-231, 575-652).
+int newton_step_charts(oc_ctx* c, oc_iter_stats* S, int* done)
+{
+    const TriSet T = system_set(c);
+    const int nVt = c->nVt, nC = c->nCharts;
+    hipStream_t s = c->stream;
+    system_element_pass(c, T);
+    oc::launch_gather_grad(s, nVt, c->dVcPtr.as<int>(), c->dVcSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
+                           c->dElemG.as<double>(), c->dG.as<double>(), c->dPartials.as<double>(), scalar(c, 6));
+    oc::launch_chart_gsq(s, nC, c->dChartV.as<int>(), c->dG.as<double>(), c->dChartGsq.as<double>());
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipMemcpyAsync(c->chartGsq.data(), c->dChartGsq.p, sizeof(double) * nC, hipMemcpyDeviceToHost, s));
+    OC_HIP(c, hipStreamSynchronize(s));
+    std::vector<unsigned char> act(nC, 0);
+    int nAct = 0;
+    double gsum = 0.0, esum = 0.0;
+    for (int k = 0; k < nC; ++k) {
+        gsum += c->chartGsq[k];
+        esum += c->chartLastE[k];
+        if (c->chartState[k] == 0 && c->chartGsq[k] < c->chartTarget[k]) c->chartState[k] = 1;  // :215-221
+        act[k] = c->chartState[k] == 0;
+        nAct += act[k];
+    }
+    S->grad_sq = gsum;
+    S->E_total = esum;
+    S->E_SD = esum / c->a0;
+    if (nAct == 0) {
+        S->converged = 1;
+        *done = 1;
+        return OC_OK;
+    }
+    OC_HIP(c, hipMemcpyAsync(c->dChartActive.p, act.data(), nC, hipMemcpyHostToDevice, s));
+    system_assemble(c);
+    oc::launch_chart_rhs(s, nVt, c->dVChart.as<int>(), c->dChartActive.as<unsigned char>(), c->dG.as<double>(),
+                         c->dB.as<double>());
+    if (int st = solve_direction(c, S)) return st;
+    const double* dir = c->dX.as<double>();
+    // per-chart injectivity bound (cap 1)
+    const int nBlk = c->chartBlk[nC];
+    std::vector<unsigned long long> om(nC, oc::dbl_to_ordered_host(1.0));
+    OC_HIP(c, hipMemcpyAsync(c->dChartOrdMin.p, om.data(), sizeof(unsigned long long) * nC, hipMemcpyHostToDevice,
+                             s));
+    oc::launch_chart_step_bound(s, T, c->dUV.as<double>(), dir, nBlk, c->dBlkRange.as<int2>(),
+                                c->dBlkChart.as<int>(), c->dChartOrdMin.as<unsigned long long>(), 1.0);
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipMemcpyAsync(om.data(), c->dChartOrdMin.p, sizeof(unsigned long long) * nC, hipMemcpyDeviceToHost,
+                             s));
+    OC_HIP(c, hipStreamSynchronize(s));
+    // line search, each chart with its own rule state
+    constexpr int K = oc::kLSCand;
+    std::vector<double> alpha(nC, 0.0), alphaCommit(nC, 0.0), testE(nC, 0.0), base(nC, 0.0);
+    std::vector<int> kNext(nC, 0), phase(nC, 0), fin(nC, 1), stopped(nC, 0);
+    int halvings = 0;
+    for (int k = 0; k < nC; ++k)
+        if (act[k]) {
+            alpha[k] = alphaCommit[k] = oc::ordered_to_dbl_host(om[k]) * 0.99;
+            fin[k] = 0;
+        }
+    std::vector<double> out(2 * K * (size_t)nC), aDev(nC);
+    for (;;) {
+        bool any = false;
+        for (int k = 0; k < nC; ++k) {
+            // each unfinished chart fetches K candidates from its next index
+            aDev[k] = fin[k] ? 0.0 : alpha[k];
+            any = any || !fin[k];
+        }
+        if (!any) break;
+        OC_HIP(c, hipMemcpyAsync(c->dChartAlpha.p, aDev.data(), sizeof(double) * nC, hipMemcpyHostToDevice, s));
+        oc::launch_chart_linesearch(s, T, c->dUV.as<double>(), dir, nBlk, c->dBlkRange.as<int2>(),
+                                    c->dBlkChart.as<int>(), c->dChartAlpha.as<double>(), nC, c->dChartBlk.as<int>(),
+                                    c->dPartials.as<double>(), c->dChartOut.as<double>());
+        OC_HIP(c, hipGetLastError());
+        OC_HIP(c, hipMemcpyAsync(out.data(), c->dChartOut.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s));
+        OC_HIP(c, hipStreamSynchronize(s));
+        for (int k = 0; k < nC; ++k) {
+            if (fin[k]) continue;
+            const double lastE = c->chartLastE[k];
+            for (int j = 0; j < K && !fin[k]; ++j) {
+                // candidate j is alpha[k] / 2^j, the value alpha[k] takes after j halvings
+                const double E = c->a0 * out[(size_t)k * 2 * K + j];
+                const bool inv = out[(size_t)k * 2 * K + K + j] > 0.0;
+                testE[k] = E;
+                if (phase[k] == 0 && !(E > lastE)) phase[k] = 1;  // energy test passed: inversion guard next
+                if (phase[k] == 1 && !inv) {                       // (:615-629)
+                    fin[k] = 1;
+                    break;
+                }
+                const double na = alpha[k] / 2.0;
+                ++halvings;
+                if (na == 0.0) {
+                    stopped[k] = 1;
+                    fin[k] = 1;
+                    alpha[k] = 0.0;
+                    break;
+                }
+                alpha[k] = na;
+                alphaCommit[k] = na;
+            }
+        }
+    }
+    for (int k = 0; k < nC; ++k) aDev[k] = act[k] ? alphaCommit[k] : 0.0;
+    OC_HIP(c, hipMemcpyAsync(c->dChartAlpha.p, aDev.data(), sizeof(double) * nC, hipMemcpyHostToDevice, s));
+    oc::launch_chart_commit(s, nVt, c->dVChart.as<int>(), c->dChartAlpha.as<double>(), dir, c->dUV.as<double>());
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipStreamSynchronize(s));
+    int nLeft = 0;
+    double esumNew = 0.0, amax = 0.0;
+    for (int k = 0; k < nC; ++k) {
+        c->chartStep[k] = act[k] ? alpha[k] : 0.0;
+        if (act[k]) {
+            const double dec = c->chartLastE[k] - testE[k];
+            bool st = stopped[k] != 0;
+            if (c->allowEDecRelTol && (dec / c->chartLastE[k] < 1.0e-6 * alpha[k]) && (alpha[k] > 1.0e-3)) st = true;
+            c->chartLastE[k] = testE[k];
+            if (st) c->chartState[k] = 2;
+            amax = std::max(amax, alpha[k]);
+        }
+        esumNew += c->chartLastE[k];
+        nLeft += c->chartState[k] == 0;
+    }
+    S->E_total = esumNew;
+    S->E_SD = esumNew / c->a0;
+    S->step = amax;
+    S->halvings = halvings;
+    S->stopped = nLeft == 0;
+    *done = nLeft == 0;
+    return OC_OK;
+}
+
+void system_assemble(oc_ctx* c)
+{
+    const int nUB = (int)c->ubRow.size();
+    const int bj = c->nPrecond == OC_PRECOND_BLOCK_JACOBI;
+    if (c->sysPrecision == OC_PRECISION_MIXED)
+        oc::launch_assemble<float>(c->stream, nUB, c->dUbRow.as<int>(), c->dUbCol.as<int>(), c->dUbPosU.as<int>(),
+                                   c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(), c->dUbCSrc.as<int>(),
+                                   c->dFixedMaskSys.as<unsigned char>(), c->dFixedDiag.as<double>(),
+                                   c->dElemH.as<float>(), c->sys.dVals.p, c->sys.vals32, c->sys.dMinv.as<double>(), bj);
+    else
+        oc::launch_assemble<double>(c->stream, nUB, c->dUbRow.as<int>(), c->dUbCol.as<int>(), c->dUbPosU.as<int>(),
+                                    c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(), c->dUbCSrc.as<int>(),
+                                    c->dFixedMaskSys.as<unsigned char>(), c->dFixedDiag.as<double>(),
+                                    c->dElemH.as<double>(), c->sys.dVals.p, c->sys.vals32, c->sys.dMinv.as<double>(),
+                                    bj);
+}
+}  // namespace
+
+int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, S && done, "oc_newton_step: null output");
+    if (!c->sysReady) {
+        c->err = "oc_newton_step: call oc_newton_init first";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    std::memset(S, 0, sizeof(*S));
+    *done = 0;
+    if (c->nCharts > 0) return newton_step_charts(c, S, done);
+    const TriSet T = system_set(c);
+    const int nVt = c->nVt;
+    hipStream_t s = c->stream;
+    const int f32 = c->sysPrecision == OC_PRECISION_MIXED;
+    // gradient (and element Hessians in the same pass)
+    system_element_pass(c, T);
+    oc::launch_gather_grad(s, nVt, c->dVcPtr.as<int>(), c->dVcSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
+                           c->dElemG.as<double>(), c->dG.as<double>(), c->dPartials.as<double>(), scalar(c, 6));
+    OC_HIP(c, hipGetLastError());
+    double gsq;
+    OC_HIP(c, hipMemcpyAsync(&gsq, scalar(c, 6), sizeof(double), hipMemcpyDeviceToHost, s));
+    OC_HIP(c, hipStreamSynchronize(s));
+    S->grad_sq = gsq;
+    S->E_total = c->lastE;
+    S->E_SD = c->lastESD;
+    S->E_scaf = c->lastEScaf;
+    if (gsq < c->targetGRes) {  // Optimizer.cpp:215-221
+        S->converged = 1;
+        *done = 1;
+        return OC_OK;
+    }
+    // Hessian assembly + direction: H d = -g
+    system_assemble(c);
+    oc::launch_scale_copy(s, 2L * nVt, -1.0, c->dG.as<double>(), c->dB.as<double>());
+    if (int st = solve_direction(c, S)) return st;
     const double* dir = c->dX.as<double>();
     // line search (Optimizer.cpp:575-652)
     double bound;

@@ -121,7 +121,7 @@ void launch_rest_features(hipStream_t s, int nF, const int* f0, const int* f1, c
 __device__ __forceinline__ RestT<double> load_rest(const TriSet& T, int t, bool isMesh)
 {
     RestT<double> r;
-    r.w = (isMesh && !T.uniformMesh) ? T.triArea[t] / *T.surfaceArea : 1.0;
+    r.w = (isMesh && !T.uniformMesh) ? T.triArea[t] / T.surfaceArea[T.triChart ? T.triChart[t] : 0] : 1.0;
     r.A2 = T.A2[t];
     r.E0 = T.E0[t];
     r.E1 = T.E1[t];
@@ -913,6 +913,189 @@ __global__ void __launch_bounds__(kBlock) k_scale_copy(long n, double sign, cons
 void launch_scale_copy(hipStream_t s, long n, double sign, const double* g, double* b)
 {
     hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(n)), dim3(kBlock), 0, s, n, sign, g, b);
+}
+
+}  // namespace oc
+
+// ============================================================================ batched charts
+namespace oc {
+
+// out[s] = compensated sum of vals over [segStart[s], segStart[s+1]) (one block per segment)
+__global__ void __launch_bounds__(kBlock) k_chart_seg_sum(const int* __restrict__ segStart,
+                                                          const double* __restrict__ vals, double* out)
+{
+    const int sg = blockIdx.x;
+    CSum acc[1] = {{0.0, 0.0}};
+    for (int i = segStart[sg] + threadIdx.x; i < segStart[sg + 1]; i += blockDim.x)
+        acc[0] = csum_add(acc[0], CSum{vals[i], 0.0});
+    block_csum<1>(acc);
+    if (threadIdx.x == 0) out[sg] = acc[0].s + acc[0].c;
+}
+
+void launch_chart_seg_sum(hipStream_t s, int nSeg, const int* segStart, const double* vals, double* out)
+{
+    hipLaunchKernelGGL(k_chart_seg_sum, dim3(nSeg), dim3(kBlock), 0, s, segStart, vals, out);
+}
+
+// out[c] = |g|^2 over chart c's vertices (one block per chart)
+__global__ void __launch_bounds__(kBlock) k_chart_gsq(const int* __restrict__ vStart, const double2* __restrict__ g,
+                                                      double* out)
+{
+    const int c = blockIdx.x;
+    double acc[1] = {0.0};
+    for (int v = vStart[c] + threadIdx.x; v < vStart[c + 1]; v += blockDim.x) {
+        const double2 x = g[v];
+        acc[0] += x.x * x.x + x.y * x.y;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) out[c] = acc[0];
+}
+
+void launch_chart_gsq(hipStream_t s, int nCharts, const int* vStart, const double* g, double* out)
+{
+    hipLaunchKernelGGL(k_chart_gsq, dim3(nCharts), dim3(kBlock), 0, s, vStart, reinterpret_cast<const double2*>(g),
+                       out);
+}
+
+// Line-search energies of the mesh triangles of chart-aligned blocks at K
+// halvings of each chart's own step; block records of (E_k, inverted_k).
+__global__ void __launch_bounds__(kBlock) k_chart_linesearch(TriSet T, const double2* __restrict__ UV,
+                                                             const double2* __restrict__ dir,
+                                                             const int2* __restrict__ blkRange,
+                                                             const int* __restrict__ blkChart,
+                                                             const double* __restrict__ alphaChart, double* partials)
+{
+    constexpr int K = kLSCand;
+    const int2 rg = blkRange[blockIdx.x];
+    const int t = rg.x + threadIdx.x;
+    CSum acc[2 * K];
+#pragma unroll
+    for (int k = 0; k < 2 * K; ++k) acc[k] = CSum{0.0, 0.0};
+    if (t < rg.y) {
+        const int vi[3] = {T.f0[t], T.f1[t], T.f2[t]};
+        double2 u0[3], d[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            u0[k] = UV[vi[k]];
+            d[k] = dir[vi[k]];
+        }
+        const RestT<double> r = load_rest(T, t, true);
+        double alpha = alphaChart[blkChart[blockIdx.x]];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double p[3][2];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                p[q][0] = add_rn(u0[q].x, mul_rn(alpha, d[q].x));
+                p[q][1] = add_rn(u0[q].y, mul_rn(alpha, d[q].y));
+            }
+            double dbA;
+            acc[k].s = sd_energy<double>(p[0][0], p[0][1], p[1][0], p[1][1], p[2][0], p[2][1], r, dbA);
+            acc[K + k].s = dbA < 0.0 ? 1.0 : 0.0;
+            alpha = alpha / 2.0;
+        }
+    }
+    publish_csum<2 * K>(acc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) k_chart_linesearch_finish(const double* __restrict__ partials,
+                                                                    const int* __restrict__ chartBlk, double* out)
+{
+    constexpr int K = kLSCand;
+    const int c = blockIdx.x;
+    const int b0 = chartBlk[c], nb = chartBlk[c + 1] - chartBlk[c];
+    __shared__ double res[2 * K];
+    finish_csum<2 * K>(partials + (size_t)b0 * 4 * K, nb, res);
+    __syncthreads();
+    if (threadIdx.x < 2 * K) out[(size_t)c * 2 * K + threadIdx.x] = res[threadIdx.x];
+}
+
+void launch_chart_linesearch(hipStream_t s, const TriSet& T, const double* UV, const double* dir, int nBlk,
+                             const int2* blkRange, const int* blkChart, const double* alphaChart, int nCharts,
+                             const int* chartBlk, double* partials, double* out)
+{
+    hipLaunchKernelGGL(k_chart_linesearch, dim3(nBlk), dim3(kBlock), 0, s, T, reinterpret_cast<const double2*>(UV),
+                       reinterpret_cast<const double2*>(dir), blkRange, blkChart, alphaChart, partials);
+    hipLaunchKernelGGL(k_chart_linesearch_finish, dim3(nCharts), dim3(kBlock), 0, s, partials, chartBlk, out);
+}
+
+// per-chart minimum of the injectivity bound (one atomicMin per block)
+__global__ void __launch_bounds__(kBlock) k_chart_step_bound(TriSet T, const double2* __restrict__ UV,
+                                                             const double2* __restrict__ dir,
+                                                             const int2* __restrict__ blkRange,
+                                                             const int* __restrict__ blkChart,
+                                                             unsigned long long* ordMin, double cap)
+{
+    const int2 rg = blkRange[blockIdx.x];
+    const int t = rg.x + threadIdx.x;
+    double bound = cap;
+    if (t < rg.y) {
+        const int vi[3] = {T.f0[t], T.f1[t], T.f2[t]};
+        double U[3][2], Vd[3][2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double2 u = UV[vi[k]], d = dir[vi[k]];
+            U[k][0] = u.x;
+            U[k][1] = u.y;
+            Vd[k][0] = d.x;
+            Vd[k][1] = d.y;
+        }
+        bound = sd_step_bound(U, Vd, cap);
+    }
+    bound = wave_min(bound != bound ? cap : bound);
+    __shared__ double wmin[kBlock / kWave];
+    if (threadIdx.x % kWave == 0) wmin[threadIdx.x / kWave] = bound;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+        for (int w = 1; w < kBlock / kWave; ++w) m = fmin(m, wmin[w]);
+        if (m < cap) atomicMin(ordMin + blkChart[blockIdx.x], dbl_to_ordered(m));
+    }
+}
+
+void launch_chart_step_bound(hipStream_t s, const TriSet& T, const double* UV, const double* dir, int nBlk,
+                             const int2* blkRange, const int* blkChart, unsigned long long* ordMin, double cap)
+{
+    hipLaunchKernelGGL(k_chart_step_bound, dim3(nBlk), dim3(kBlock), 0, s, T, reinterpret_cast<const double2*>(UV),
+                       reinterpret_cast<const double2*>(dir), blkRange, blkChart, ordMin, cap);
+}
+
+__global__ void __launch_bounds__(kBlock) k_chart_rhs(int nV, const int* __restrict__ vChart,
+                                                      const unsigned char* __restrict__ active,
+                                                      const double2* __restrict__ g, double2* b)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    const double2 x = g[v];
+    b[v] = active[vChart[v]] ? make_double2(-x.x, -x.y) : make_double2(0.0, 0.0);
+}
+
+void launch_chart_rhs(hipStream_t s, int nV, const int* vChart, const unsigned char* active, const double* g,
+                      double* b)
+{
+    hipLaunchKernelGGL(k_chart_rhs, dim3(grid_for(nV)), dim3(kBlock), 0, s, nV, vChart, active,
+                       reinterpret_cast<const double2*>(g), reinterpret_cast<double2*>(b));
+}
+
+__global__ void __launch_bounds__(kBlock) k_chart_commit(int nV, const int* __restrict__ vChart,
+                                                         const double* __restrict__ alphaChart,
+                                                         const double2* __restrict__ d, double2* UV)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    const double a = alphaChart[vChart[v]];
+    const double2 dv = d[v];
+    double2 u = UV[v];
+    u.x = add_rn(u.x, mul_rn(a, dv.x));
+    u.y = add_rn(u.y, mul_rn(a, dv.y));
+    UV[v] = u;
+}
+
+void launch_chart_commit(hipStream_t s, int nV, const int* vChart, const double* alphaChart, const double* d,
+                         double* UV)
+{
+    hipLaunchKernelGGL(k_chart_commit, dim3(grid_for(nV)), dim3(kBlock), 0, s, nV, vChart, alphaChart,
+                       reinterpret_cast<const double2*>(d), reinterpret_cast<double2*>(UV));
 }
 
 }  // namespace oc

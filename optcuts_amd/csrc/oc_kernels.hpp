@@ -18,7 +18,8 @@ struct TriSet {
     const double* E0 = nullptr;
     const double* E1 = nullptr;
     const double* E01 = nullptr;
-    const double* surfaceArea = nullptr;  // device scalar (mesh block)
+    const double* surfaceArea = nullptr;  // device scalar (mesh block), or per-chart array with triChart
+    const int* triChart = nullptr;        // batched charts: chart of each mesh triangle (w = area / its chart's area)
     int uniformMesh = 0;                  // mesh triangles use w = 1
     double sMesh = 1.0, sAir = 1.0;
 };
@@ -115,6 +116,26 @@ void launch_spmv(hipStream_t s, int nV, const int* rowPtr, const int* colIdx, co
 void launch_expand_upper(hipStream_t s, int nnz, const double* a, const int* dst, const int* mir, double* vals);
 void launch_jacobi_from_bsr(hipStream_t s, int nV, const int* diagPos, const double* vals, double* Minv,
                             int blockJacobi);
+
+// ---- batched charts ----------------------------------------------------------
+// Charts are contiguous vertex and triangle ranges of one combined mesh. Chart
+// kernels use a chart-aligned block map: block b covers items
+// [blkRange[b].x, blkRange[b].y) of chart blkChart[b] (at most kBlock items),
+// the blocks of chart c are [chartBlk[c], chartBlk[c+1]).
+void launch_chart_seg_sum(hipStream_t s, int nSeg, const int* segStart, const double* vals, double* out);
+void launch_chart_gsq(hipStream_t s, int nCharts, const int* vStart, const double* g, double* out);
+// out[c*2K + k] = E_mesh of chart c at alpha_c / 2^k, out[c*2K + K + k] = inverted triangles
+void launch_chart_linesearch(hipStream_t s, const TriSet& T, const double* UV, const double* dir, int nBlk,
+                             const int2* blkRange, const int* blkChart, const double* alphaChart, int nCharts,
+                             const int* chartBlk, double* partials, double* out);
+void launch_chart_step_bound(hipStream_t s, const TriSet& T, const double* UV, const double* dir, int nBlk,
+                             const int2* blkRange, const int* blkChart, unsigned long long* ordMin, double cap);
+// b[v] = active[chart(v)] ? -g[v] : 0 (per DOF pair)
+void launch_chart_rhs(hipStream_t s, int nV, const int* vChart, const unsigned char* active, const double* g,
+                      double* b);
+// UV[v] += alpha[chart(v)] * d[v] (two roundings, as the single-chart commit)
+void launch_chart_commit(hipStream_t s, int nV, const int* vChart, const double* alphaChart, const double* d,
+                         double* UV);
 
 }  // namespace oc
 

@@ -67,3 +67,25 @@ def batched_disc_charts(n_charts=64, lo=500, hi=20000, base_seed=1000):
         m = max(3, int(round(np.sqrt(fk / 2.0))) + 1)
         charts.append(height_field_disc(m, seed=base_seed + k))
     return charts
+
+
+def union_of_charts(charts):
+    """Concatenate charts into one mesh for batched solving.
+
+    Returns (V, F, UV, vertexStart, faceStart, fixed): chart k owns vertices
+    [vertexStart[k], vertexStart[k+1]) and triangles [faceStart[k],
+    faceStart[k+1]); fixed lists each chart's vertex 0 (the reference fixes
+    vertex 0 of a mesh, TriMesh.cpp:325-328).
+    """
+    vs, fs = [0], [0]
+    Vs, Fs, UVs, fixed = [], [], [], []
+    for V, F, UV in charts:
+        off = vs[-1]
+        Vs.append(V)
+        Fs.append(F + off)
+        UVs.append(UV)
+        fixed.append(off)
+        vs.append(off + V.shape[0])
+        fs.append(fs[-1] + F.shape[0])
+    return (np.concatenate(Vs), np.concatenate(Fs).astype(np.int32), np.concatenate(UVs),
+            np.array(vs, np.int32), np.array(fs, np.int32), np.array(fixed, np.int32))
