@@ -105,6 +105,52 @@ def time_to_target(ctx, V, F, UV, pcg_tol, precond, precision, e_star, max_iter)
             "driver_energy_minus_fp64": s["E_SD"] - e64, "pcg_fallbacks": fb}
 
 
+def charts_bench(ctx, max_iter=60):
+    """The 64-chart workload (height-field discs of 500-20000 faces, log-uniform
+    sizes, seeds 1000+i): all charts batched in one context versus the same
+    charts solved one after another, chosen solve form, each chart run until
+    its own stop (gradient tolerance or line-search stop) or max_iter."""
+    from optcuts_amd import capi
+    from optcuts_amd.meshgen import batched_disc_charts, union_of_charts
+    charts = batched_disc_charts(64, 500, 20000, base_seed=1000)
+    V, F, UV, vs, fs, fixed = union_of_charts(charts)
+    kw = dict(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG, pcg_rel_tol=1e-4,
+              pcg_max_iter=1000)
+    ctx.set_mesh(V, F, fixed=fixed)
+    ctx.set_uv(UV)
+    ctx.set_charts(vs, fs)
+    t0 = time.perf_counter()
+    ctx.newton_init(**kw)
+    for it in range(1, max_iter + 1):
+        s, done = ctx.newton_step()
+        if done:
+            break
+    tb = time.perf_counter() - t0
+    st = ctx.charts_state()
+    e_b = st["E_SD"].copy()
+    t0 = time.perf_counter()
+    e_s = []
+    its_s = 0
+    for Vk, Fk, UVk in charts:
+        ctx.set_mesh(Vk, Fk, fixed=(0,))
+        ctx.set_uv(UVk)
+        ctx.newton_init(**kw)
+        for k in range(1, max_iter + 1):
+            s, done = ctx.newton_step()
+            if done:
+                break
+        its_s += k
+        e_s.append(s["E_SD"])
+    ts = time.perf_counter() - t0
+    e_s = __import__("numpy").array(e_s)
+    return {"charts": len(charts), "faces": int(F.shape[0]), "vertices": int(V.shape[0]),
+            "batched_seconds": tb, "batched_newton_iterations": it,
+            "sequential_seconds": ts, "sequential_newton_iterations_total": its_s,
+            "speedup_batched_vs_sequential": ts / tb,
+            "charts_converged": int((st["state"] != 0).sum()),
+            "max_rel_diff_final_E_SD": float(abs(e_b - e_s).max() / e_s.min())}
+
+
 def cpu_baseline(V, F, UV, iters=3):
     from oracle import oracle as O
     t = time.perf_counter()
@@ -129,6 +175,7 @@ def main():
                     default="all", help="all = every form (learned ones when trained weights exist); "
                                         "both = plain and chosen")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-charts", action="store_true", help="skip the 64-chart batched run")
     ap.add_argument("--no-tte", action="store_true", help="skip the time-to-E_SD run")
     ap.add_argument("--tte-max-iter", type=int, default=60)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -206,6 +253,7 @@ def main():
             r["time_to_E_SD_1e-6"] = time_to_target(ctx, V, F, UV, tol, precond, precision, e_star,
                                                     args.tte_max_iter)
         res[name] = r
+    charts = None if args.no_charts else charts_bench(ctx)
     ctx.close()
     if net is not None:
         net.close()
@@ -256,6 +304,7 @@ def main():
         "forms": res,
         "time_to_E_SD_speedup_chosen_vs_plain": speed,
         "time_to_E_SD_speedup_vs_plain": speeds,
+        "batched_charts": charts,
         "roofline": rl,
         "roofline_plain_form": rl_plain,
         "cpu_baseline": None,
