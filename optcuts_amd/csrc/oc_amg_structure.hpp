@@ -4,12 +4,127 @@
 #pragma once
 
 #include <algorithm>
+#include <thread>
 #include <numeric>
 #include <string>
 #include <utility>
 #include <vector>
 
 namespace oc {
+
+// Row-block parallel loop for the symbolic set-up: fn(chunk, r0, r1) on
+// nChunks contiguous row ranges, each on its own host thread. Callers write
+// chunk-local outputs and concatenate them in chunk order, so the result does
+// not depend on the thread count.
+template <typename Fn>
+inline void parallel_row_chunks(int n, int nChunks, Fn&& fn)
+{
+    std::vector<std::thread> th;
+    for (int c = 0; c < nChunks; ++c) {
+        const int r0 = (int)((long)n * c / nChunks), r1 = (int)((long)n * (c + 1) / nChunks);
+        th.emplace_back([&fn, c, r0, r1] { fn(c, r0, r1); });
+    }
+    for (auto& t : th) t.join();
+}
+
+inline int setup_threads(long work)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int t = (int)std::min<unsigned>(hw ? hw : 1, 16);
+    return work < 20000 ? 1 : t;
+}
+
+// Symbolic product rows: for each output row, the distinct columns (sorted)
+// and the contributing (left, right) pairs grouped by column in generation
+// order. gen(row, emit) calls emit(col, left, right) for every pair of the row
+// in a fixed order; rows [r0, r1).
+struct SymRows {
+    std::vector<int> nCols;   // per row
+    std::vector<int> cols;    // concatenated
+    std::vector<int> grpEnd;  // per column entry: end of its pair group (row-local offsets)
+    std::vector<int> a, b;    // pairs
+};
+
+template <typename Gen>
+inline void symbolic_rows(int r0, int r1, int ncols, Gen&& gen, SymRows& out)
+{
+    struct Item {
+        int c, x, y;
+    };
+    std::vector<Item> buf;
+    std::vector<int> slot(ncols, -1), cols, cnt, fill;
+    for (int r = r0; r < r1; ++r) {
+        buf.clear();
+        gen(r, [&](int c, int x, int y) { buf.push_back(Item{c, x, y}); });
+        cols.clear();
+        for (const Item& it : buf)
+            if (slot[it.c] < 0) {
+                slot[it.c] = 0;
+                cols.push_back(it.c);
+            }
+        std::sort(cols.begin(), cols.end());
+        const int k = (int)cols.size();
+        cnt.assign(k + 1, 0);
+        for (int q = 0; q < k; ++q) slot[cols[q]] = q;
+        for (const Item& it : buf) cnt[slot[it.c] + 1]++;
+        for (int q = 0; q < k; ++q) cnt[q + 1] += cnt[q];
+        const size_t base = out.a.size();
+        out.a.resize(base + buf.size());
+        out.b.resize(base + buf.size());
+        fill.assign(cnt.begin(), cnt.end() - 1);
+        for (const Item& it : buf) {
+            const int q = fill[slot[it.c]]++;
+            out.a[base + q] = it.x;
+            out.b[base + q] = it.y;
+        }
+        out.nCols.push_back(k);
+        for (int q = 0; q < k; ++q) {
+            out.cols.push_back(cols[q]);
+            out.grpEnd.push_back(cnt[q + 1]);
+            slot[cols[q]] = -1;
+        }
+    }
+}
+
+// Concatenates chunk results into CSR row pointers, columns, pair-group
+// pointers and pairs.
+inline void concat_rows(int n, std::vector<SymRows>& parts, std::vector<int>& rowPtr, std::vector<int>& col,
+                        std::vector<int>& grpPtr, std::vector<int>& pa, std::vector<int>& pb)
+{
+    rowPtr.assign(n + 1, 0);
+    col.clear();
+    grpPtr.assign(1, 0);
+    pa.clear();
+    pb.clear();
+    size_t nc = 0, np = 0;
+    for (auto& P : parts) {
+        nc += P.cols.size();
+        np += P.a.size();
+    }
+    col.reserve(nc);
+    grpPtr.reserve(nc + 1);
+    pa.reserve(np);
+    pb.reserve(np);
+    int r = 0;
+    for (auto& P : parts) {
+        size_t ci = 0, pbase = 0;
+        for (int k : P.nCols) {
+            // pairs of this row start at pbase in P.a; group ends are row-local
+            int rowPairs = 0;
+            for (int q = 0; q < k; ++q) {
+                col.push_back(P.cols[ci + q]);
+                grpPtr.push_back((int)(pa.size() + P.grpEnd[ci + q]));
+                rowPairs = P.grpEnd[ci + q];
+            }
+            pa.insert(pa.end(), P.a.begin() + pbase, P.a.begin() + pbase + rowPairs);
+            pb.insert(pb.end(), P.b.begin() + pbase, P.b.begin() + pbase + rowPairs);
+            pbase += rowPairs;
+            ci += k;
+            rowPtr[r + 1] = rowPtr[r] + k;
+            ++r;
+        }
+    }
+}
 
 constexpr int kAmgNullDim = 3;  // two translations and a rotation of the UV plane
 
@@ -121,56 +236,34 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
             for (int u = 0; u < A.n; ++u)
                 if (agg[u] >= 0) A.aggMem[f[agg[u]]++] = u;
         }
-        // P pattern: row u -> aggregates of its (non-excluded) closed neighbourhood
-        A.pPtr.assign(A.n + 1, 0);
-        A.pCol.clear();
-        A.pRow.clear();
-        A.pIsTent.clear();
-        for (int u = 0; u < A.n; ++u) {
-            if (!A.excluded[u]) {
-                std::vector<int> cs;
-                for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
-                    const int w = A.colIdx[k];
-                    if (!A.excluded[w]) cs.push_back(agg[w]);
-                }
-                std::sort(cs.begin(), cs.end());
-                cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
-                for (int c : cs) {
-                    A.pCol.push_back(c);
-                    A.pRow.push_back(u);
-                    A.pIsTent.push_back(c == agg[u] ? 1 : 0);
-                }
-            }
-            A.pPtr[u + 1] = (int)A.pCol.size();
+        // P pattern: row u -> aggregates of its (non-excluded) closed
+        // neighbourhood; with it the smoothing contributions (the A blocks of
+        // row u grouped by the aggregate of their column, k ascending).
+        {
+            const int nt = setup_threads((long)A.colIdx.size());
+            std::vector<SymRows> parts(nt);
+            parallel_row_chunks(A.n, nt, [&](int ch, int r0, int r1) {
+                symbolic_rows(r0, r1, nc,
+                              [&](int u, auto&& emit) {
+                                  if (A.excluded[u]) return;
+                                  for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
+                                      const int w = A.colIdx[k];
+                                      if (!A.excluded[w]) emit(agg[w], k, 0);
+                                  }
+                              },
+                              parts[ch]);
+            });
+            std::vector<int> unused;
+            concat_rows(A.n, parts, A.pPtr, A.pCol, A.psPtr, A.psA, unused);
         }
         const int nP = (int)A.pCol.size();
-        auto pfind = [&](int u, int c) {
-            const int* b = A.pCol.data() + A.pPtr[u];
-            const int* e = A.pCol.data() + A.pPtr[u + 1];
-            const int* p = std::lower_bound(b, e, c);
-            return (p != e && *p == c) ? int(p - A.pCol.data()) : -1;
-        };
-        // smoothing contributions: A blocks of row u bucketed by aggregate of their column
-        A.psPtr.assign(nP + 1, 0);
-        for (int u = 0; u < A.n; ++u) {
-            if (A.excluded[u]) continue;
-            for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
-                const int w = A.colIdx[k];
-                if (!A.excluded[w]) A.psPtr[pfind(u, agg[w]) + 1]++;
+        A.pRow.assign(nP, 0);
+        A.pIsTent.assign(nP, 0);
+        for (int u = 0; u < A.n; ++u)
+            for (int j = A.pPtr[u]; j < A.pPtr[u + 1]; ++j) {
+                A.pRow[j] = u;
+                A.pIsTent[j] = A.pCol[j] == agg[u] ? 1 : 0;
             }
-        }
-        for (int j = 0; j < nP; ++j) A.psPtr[j + 1] += A.psPtr[j];
-        A.psA.assign(A.psPtr[nP], 0);
-        {
-            std::vector<int> f(A.psPtr.begin(), A.psPtr.end() - 1);
-            for (int u = 0; u < A.n; ++u) {
-                if (A.excluded[u]) continue;
-                for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
-                    const int w = A.colIdx[k];
-                    if (!A.excluded[w]) A.psA[f[pfind(u, agg[w])]++] = k;
-                }
-            }
-        }
         // restriction: P blocks by column
         A.rPtr.assign(nc + 1, 0);
         for (int j = 0; j < nP; ++j) A.rPtr[A.pCol[j] + 1]++;
@@ -180,75 +273,46 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
             std::vector<int> f(A.rPtr.begin(), A.rPtr.end() - 1);
             for (int j = 0; j < nP; ++j) A.rP[f[A.pCol[j]]++] = j;
         }
-        // T = A P: pattern and (A block, P block) pairs
-        A.tPtr.assign(A.n + 1, 0);
-        A.tCol.clear();
-        A.tcPtr.assign(1, 0);
-        A.tcA.clear();
-        A.tcP.clear();
-        for (int u = 0; u < A.n; ++u) {
-            std::vector<std::pair<int, std::pair<int, int>>> items;  // (coarse col, (k, j))
-            for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
-                const int w = A.colIdx[k];
-                for (int j = A.pPtr[w]; j < A.pPtr[w + 1]; ++j) items.push_back({A.pCol[j], {k, j}});
-            }
-            std::stable_sort(items.begin(), items.end(),
-                             [](const auto& a, const auto& b) { return a.first < b.first; });
-            for (size_t q = 0; q < items.size(); ++q) {
-                if (q == 0 || items[q].first != items[q - 1].first) {
-                    if (q) A.tcPtr.push_back((int)A.tcA.size());
-                    A.tCol.push_back(items[q].first);
-                }
-                A.tcA.push_back(items[q].second.first);
-                A.tcP.push_back(items[q].second.second);
-            }
-            if (!items.empty()) A.tcPtr.push_back((int)A.tcA.size());
-            A.tPtr[u + 1] = (int)A.tCol.size();
+        // T = A P: pattern and (A block, P block) pairs, per row grouped by
+        // coarse column in generation order (k ascending, then j).
+        {
+            const int nt = setup_threads((long)A.pCol.size() * 8);
+            std::vector<SymRows> parts(nt);
+            parallel_row_chunks(A.n, nt, [&](int ch, int r0, int r1) {
+                symbolic_rows(r0, r1, nc,
+                              [&](int u, auto&& emit) {
+                                  for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
+                                      const int w = A.colIdx[k];
+                                      for (int j = A.pPtr[w]; j < A.pPtr[w + 1]; ++j) emit(A.pCol[j], k, j);
+                                  }
+                              },
+                              parts[ch]);
+            });
+            concat_rows(A.n, parts, A.tPtr, A.tCol, A.tcPtr, A.tcA, A.tcP);
         }
-        // coarse A = P^T T: pattern and (P block, T block) pairs, rows by coarse node
+        // coarse A = P^T T: pattern and (P block, T block) pairs, rows by coarse
+        // node: over the P blocks (u, c1) in increasing block order (the
+        // restriction lists), the T entries of row u.
         AmgHostLevel nxl;
         AmgHostLevel* nx = &nxl;
         nx->n = nc;
         nx->bs = kAmgNullDim;
         nx->excluded.assign(nc, 0);
         {
-            std::vector<int> cnt(nc + 1, 0);
-            for (int u = 0; u < A.n; ++u)
-                for (int j = A.pPtr[u]; j < A.pPtr[u + 1]; ++j) cnt[A.pCol[j] + 1] += A.tPtr[u + 1] - A.tPtr[u];
-            for (int c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
-            std::vector<int> c2(cnt[nc]), jp(cnt[nc]), tp(cnt[nc]);
-            std::vector<int> f(cnt.begin(), cnt.end() - 1);
-            for (int u = 0; u < A.n; ++u)
-                for (int j = A.pPtr[u]; j < A.pPtr[u + 1]; ++j) {
-                    const int c1 = A.pCol[j];
-                    for (int t = A.tPtr[u]; t < A.tPtr[u + 1]; ++t) {
-                        c2[f[c1]] = A.tCol[t];
-                        jp[f[c1]] = j;
-                        tp[f[c1]] = t;
-                        ++f[c1];
-                    }
-                }
-            nx->rowPtr.assign(nc + 1, 0);
-            nx->colIdx.clear();
-            A.acPtr.assign(1, 0);
-            A.acP.clear();
-            A.acT.clear();
-            std::vector<int> idx;
-            for (int c1 = 0; c1 < nc; ++c1) {
-                idx.resize(cnt[c1 + 1] - cnt[c1]);
-                std::iota(idx.begin(), idx.end(), cnt[c1]);
-                std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return c2[a] < c2[b]; });
-                for (size_t q = 0; q < idx.size(); ++q) {
-                    if (q == 0 || c2[idx[q]] != c2[idx[q - 1]]) {
-                        if (q) A.acPtr.push_back((int)A.acP.size());
-                        nx->colIdx.push_back(c2[idx[q]]);
-                    }
-                    A.acP.push_back(jp[idx[q]]);
-                    A.acT.push_back(tp[idx[q]]);
-                }
-                if (!idx.empty()) A.acPtr.push_back((int)A.acP.size());
-                nx->rowPtr[c1 + 1] = (int)nx->colIdx.size();
-            }
+            const int nt = setup_threads((long)A.tCol.size() * 4);
+            std::vector<SymRows> parts(nt);
+            parallel_row_chunks(nc, nt, [&](int ch, int r0, int r1) {
+                symbolic_rows(r0, r1, nc,
+                              [&](int c1, auto&& emit) {
+                                  for (int q = A.rPtr[c1]; q < A.rPtr[c1 + 1]; ++q) {
+                                      const int j = A.rP[q];
+                                      const int u = A.pRow[j];
+                                      for (int t = A.tPtr[u]; t < A.tPtr[u + 1]; ++t) emit(A.tCol[t], j, t);
+                                  }
+                              },
+                              parts[ch]);
+            });
+            concat_rows(nc, parts, nx->rowPtr, nx->colIdx, A.acPtr, A.acP, A.acT);
         }
         L.push_back(std::move(nxl));
     }

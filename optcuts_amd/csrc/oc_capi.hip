@@ -3,6 +3,7 @@
 // device-resident Newton step. Kernels live in oc_kernels.hip.
 #include "../../include/optcuts_gpu.h"
 #include "oc_amg.hpp"
+#include "oc_amg_structure.hpp"
 #include "oc_common.hpp"
 #include "oc_gnn.hpp"
 #include "oc_kernels.hpp"
@@ -477,66 +478,111 @@ int build_system(oc_ctx* c)
         c->fixedDiag[v] = c->a0;
         if (c->nFair && !c->airFixedGlobal.empty() && c->airFixedGlobal[v]) c->fixedDiag[v] += sAir;
     }
-    // neighbour rows (with diagonal)
-    std::vector<std::vector<int>> rows(nVt);
-    for (int t = 0; t < nFt; ++t)
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) rows[Ft[3 * t + a]].push_back(Ft[3 * t + b]);
-    for (int v = 0; v < nVt; ++v) {
-        auto& r = rows[v];
-        r.push_back(v);
-        std::sort(r.begin(), r.end());
-        r.erase(std::unique(r.begin(), r.end()), r.end());
-    }
-    build_block_pattern(c->sys, nVt, rows, c->fixedMaskSys);
-    rows.clear();
-    rows.shrink_to_fit();
-    // upper blocks
-    c->ubRow.clear();
-    c->ubCol.clear();
-    c->ubPosU.clear();
-    c->ubPosL.clear();
-    std::vector<int> posToUB(c->sys.nnzb(), -1);
-    for (int u = 0; u < nVt; ++u)
-        for (int k = c->sys.rowPtr[u]; k < c->sys.rowPtr[u + 1]; ++k) {
-            const int w = c->sys.colIdx[k];
-            if (w < u) continue;
-            posToUB[k] = (int)c->ubRow.size();
-            c->ubRow.push_back(u);
-            c->ubCol.push_back(w);
-            c->ubPosU.push_back(k);
-            c->ubPosL.push_back(w == u ? -1 : c->sys.find(w, u));
-        }
-    const int nUB = (int)c->ubRow.size();
-    // contributions per upper block, triangle order
-    c->ubCPtr.assign(nUB + 1, 0);
-    auto for_each_contrib = [&](auto&& fn) {
-        for (int t = 0; t < nFt; ++t) {
-            const int* f = &Ft[3 * (size_t)t];
-            for (int a = 0; a < 3; ++a) {
-                if (c->fixedMaskSys[f[a]]) continue;
-                for (int b = 0; b < 3; ++b) {
-                    if (c->fixedMaskSys[f[b]]) continue;
-                    if (f[a] > f[b]) continue;
-                    if (f[a] == f[b] && a != b) continue;
-                    // element block (a,b) oriented rows f[a], cols f[b]
-                    const int lo = a < b ? a : b, hi = a < b ? b : a;
-                    const int id = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
-                    const unsigned int code = (unsigned int)(6 * t + id) | (a > b ? 0x80000000u : 0u);
-                    const int ub = posToUB[c->sys.find(f[a], f[b])];
-                    fn(ub, code);
+    // vertex -> (triangle*3 + corner) lists in triangle order (the gradient
+    // gathers use them, and so does everything below)
+    build_vertex_lists(nVt, nFt, Ft.data(), c->vcPtr, c->vcSrc);
+    const unsigned char* fixed = c->fixedMaskSys.data();
+    const int nth = oc::setup_threads(3L * nFt);
+    // Block pattern, one row per vertex from its triangles: fixed rows keep
+    // only the diagonal, free rows drop fixed columns. Row chunks in parallel,
+    // concatenated in order.
+    {
+        std::vector<std::vector<int>> part(nth);
+        std::vector<int> len(nVt);
+        oc::parallel_row_chunks(nVt, nth, [&](int ch, int r0, int r1) {
+            std::vector<int> loc;
+            auto& out = part[ch];
+            for (int v = r0; v < r1; ++v) {
+                const size_t n0 = out.size();
+                if (fixed[v]) {
+                    out.push_back(v);
+                } else {
+                    loc.assign(1, v);
+                    for (int q = c->vcPtr[v]; q < c->vcPtr[v + 1]; ++q) {
+                        const int t = c->vcSrc[q] / 3;
+                        for (int k = 0; k < 3; ++k) loc.push_back(Ft[3 * (size_t)t + k]);
+                    }
+                    std::sort(loc.begin(), loc.end());
+                    loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+                    for (int w : loc)
+                        if (w == v || !fixed[w]) out.push_back(w);
                 }
+                len[v] = (int)(out.size() - n0);
+            }
+        });
+        BlockPattern& P = c->sys;
+        P.nV = nVt;
+        P.rowPtr.assign(nVt + 1, 0);
+        for (int v = 0; v < nVt; ++v) P.rowPtr[v + 1] = P.rowPtr[v] + len[v];
+        P.colIdx.clear();
+        P.colIdx.reserve(P.rowPtr[nVt]);
+        for (auto& x : part) P.colIdx.insert(P.colIdx.end(), x.begin(), x.end());
+        P.diagPos.assign(nVt, -1);
+        for (int v = 0; v < nVt; ++v)
+            for (int k = P.rowPtr[v]; k < P.rowPtr[v + 1]; ++k)
+                if (P.colIdx[k] == v) P.diagPos[v] = k;
+    }
+    const BlockPattern& P = c->sys;
+    // upper blocks (w >= u) in row order, and each position's upper block
+    std::vector<int> ubStart(nVt + 1, 0), posToUB(P.nnzb(), -1);
+    for (int u = 0; u < nVt; ++u) {
+        int n = 0;
+        for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k) n += P.colIdx[k] >= u;
+        ubStart[u + 1] = ubStart[u] + n;
+    }
+    const int nUB = ubStart[nVt];
+    c->ubRow.assign(nUB, 0);
+    c->ubCol.assign(nUB, 0);
+    c->ubPosU.assign(nUB, 0);
+    c->ubPosL.assign(nUB, 0);
+    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
+        for (int u = r0; u < r1; ++u) {
+            int b = ubStart[u];
+            for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k) {
+                const int w = P.colIdx[k];
+                if (w < u) continue;
+                posToUB[k] = b;
+                c->ubRow[b] = u;
+                c->ubCol[b] = w;
+                c->ubPosU[b] = k;
+                c->ubPosL[b] = w == u ? -1 : P.find(w, u);
+                ++b;
+            }
+        }
+    });
+    // Contributions per upper block in triangle order. Block (u, w), u <= w,
+    // receives the element block (a, b) of every triangle t with f[a] = u and
+    // f[b] = w (for u == w only a == b); walking u's triangles in order gives
+    // each of its blocks its contributions in triangle order.
+    c->ubCPtr.assign(nUB + 1, 0);
+    auto row_contribs = [&](int u, auto&& fn) {
+        if (fixed[u]) return;
+        for (int q = c->vcPtr[u]; q < c->vcPtr[u + 1]; ++q) {
+            const int t = c->vcSrc[q] / 3, a = c->vcSrc[q] % 3;
+            const int* f = &Ft[3 * (size_t)t];
+            for (int b = 0; b < 3; ++b) {
+                const int w = f[b];
+                if (fixed[w] || w < u || (w == u && a != b)) continue;
+                const int lo = a < b ? a : b, hi = a < b ? b : a;
+                const int id = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+                const unsigned int code = (unsigned int)(6 * t + id) | (a > b ? 0x80000000u : 0u);
+                fn(posToUB[P.find(u, w)], code);
             }
         }
     };
-    for_each_contrib([&](int ub, unsigned int) { c->ubCPtr[ub + 1]++; });
+    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
+        for (int u = r0; u < r1; ++u) row_contribs(u, [&](int ub, unsigned int) { c->ubCPtr[ub + 1]++; });
+    });
     for (int b = 0; b < nUB; ++b) c->ubCPtr[b + 1] += c->ubCPtr[b];
     c->ubCSrc.assign(c->ubCPtr[nUB], 0);
-    {
-        std::vector<int> fill(c->ubCPtr.begin(), c->ubCPtr.end() - 1);
-        for_each_contrib([&](int ub, unsigned int code) { c->ubCSrc[fill[ub]++] = (int)code; });
-    }
-    build_vertex_lists(nVt, nFt, Ft.data(), c->vcPtr, c->vcSrc);
+    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
+        std::vector<int> fill;
+        for (int u = r0; u < r1; ++u) {
+            const int b0 = ubStart[u], b1 = ubStart[u + 1];
+            fill.assign(c->ubCPtr.begin() + b0, c->ubCPtr.begin() + b1);
+            row_contribs(u, [&](int ub, unsigned int code) { c->ubCSrc[fill[ub - b0]++] = (int)code; });
+        }
+    });
 
     if (upload_pattern(c, c->sys)) return OC_ERR_HIP;
     // The PCG matrix stays FP64 in both precisions: rounding the assembled
