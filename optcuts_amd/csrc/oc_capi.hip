@@ -118,6 +118,7 @@ struct oc_ctx {
     int nPrecond = OC_PRECOND_JACOBI;
     double pcgTol = 1e-10;
     int pcgMaxIter = 10000;
+    int pcgHint = 0;  // iterations of the previous Newton step's solve (first-batch size)
 
     int precision = OC_PRECISION_FP64;  // requested for the next oc_newton_init
     int sysPrecision = OC_PRECISION_FP64;
@@ -318,7 +319,7 @@ int ensure_pcg(oc_ctx* c, int nV)
 // PCG on pattern P for right-hand side b (device), solution in c->dX.
 // Iterations are queued in batches; ctrl is read back between batches.
 int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxIter, int& iters, double& relRes,
-            oc::Preconditioner* amg = nullptr)
+            oc::Preconditioner* amg = nullptr, int hint = 0)
 {
     const int nV = P.nV;
     PcgCtrl* ctrl = c->dCtrl.as<PcgCtrl>();
@@ -337,7 +338,12 @@ int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxI
     if (amg) amg->apply(c->stream, c->dR.as<double>(), c->dZ.as<double>(), partRZ, &ctrl->done);
     double* pbuf[2] = {c->dP0.as<double>(), c->dP1.as<double>()};
     int it = 0;
-    int batch = amg ? 8 : 32;
+    // Iterations are queued in batches and the control record read back after
+    // each. Queued iterations past convergence still launch (and return at
+    // once), so the first batch is sized by the caller's hint (the previous
+    // solve's iteration count, +1 for the converging check), then grows.
+    int batch = hint > 0 ? hint + 1 : (amg ? 8 : 32);
+    int next = amg ? 2 : 32;
     const int maxBatch = amg ? 32 : 512;
     for (;;) {
         for (int k = 0; k < batch; ++k, ++it) {
@@ -360,7 +366,8 @@ int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxI
         OC_HIP(c, hipMemcpyAsync(&h, ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, c->stream));
         OC_HIP(c, hipStreamSynchronize(c->stream));
         if (h.done) break;
-        if (batch < maxBatch) batch *= 2;
+        batch = next;
+        if (next < maxBatch) next *= 2;
     }
     iters = h.iter;
     relRes = h.bb > 0.0 ? std::sqrt(h.rr / h.bb) : 0.0;
@@ -1297,6 +1304,7 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     c->nPrecond = precond;
     c->pcgTol = pcgRelTol;
     c->pcgMaxIter = pcgMaxIter;
+    c->pcgHint = 0;
     c->sysPrecision = c->precision;
     // Optimizer constructor: exit(-1) on an inverted initial map (Optimizer.cpp:65-67)
     if (ensure_scratch(c, c->nF + c->nFair)) return OC_ERR_HIP;
@@ -1385,7 +1393,8 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S)
     if (c->nPrecond == OC_PRECOND_LEARNED) c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
     int it = 0;
     double rr = 0.0;
-    int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext);
+    int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext, c->pcgHint);
+    c->pcgHint = st == OC_OK ? it : 0;
     if (c->ext && st != OC_OK) {
         // The preconditioned solve broke down or hit its cap. Multigrid first
         // retries with its guaranteed (Gershgorin) smoother interval; what
