@@ -175,6 +175,25 @@ int oc_newton_set_learned(oc_ctx* ctx, oc_gnn* net);
  * step. *done = 1 when the solve returned 1 (gradient tolerance reached or line search stopped). */
 int oc_newton_step(oc_ctx* ctx, oc_iter_stats* stats, int* done);
 
+/* ---- initial map ------------------------------------------------------------------------------------
+ * Tutte's embedding as the reference initialises UV (main.cpp:1313-1364 and 1380-1395, 1486-1532). The
+ * pinned loops are
+ *   OC_TUTTE_COMPONENTS (mesh without input UV): per edge-connected triangle component (igl::facet_components,
+ *     numbered by lowest triangle), the first longest loop of igl::boundary_loop over its triangles;
+ *   OC_TUTTE_ALL_LOOPS (input UV re-initialised): every loop of igl::boundary_loop over the whole mesh.
+ * Loop k is placed on the unit circle by arc length of the rest positions (igl::map_vertices_to_circle) and
+ * shifted by (k % d, k / d) * 2.1, d the smallest integer with d*d >= number of loops; the rest solves the
+ * harmonic system of the uniform Laplacian (IglUtils::computeUniformLaplacian, IglUtils.cpp:39-60;
+ * igl::harmonic with k = 1) — here by PCG on the GPU (OC_PRECOND_JACOBI or OC_PRECOND_AMG, relative
+ * tolerance relTol) instead of a sparse Cholesky; pinned vertices take their circle positions exactly. The
+ * map becomes the context's UV (as oc_set_uv) and is copied to UVout (nV x 2) when non-null; *nLoops = loops
+ * pinned, *nBnd = vertices pinned, *iters = PCG iterations. Errors: a component without boundary
+ * (OC_ERR_STATE), an edge of three or more triangles, a vertex on two pinned loops or in no triangle
+ * (OC_ERR_ARG). Call oc_newton_init afterwards. */
+enum { OC_TUTTE_COMPONENTS = 0, OC_TUTTE_ALL_LOOPS = 1 };
+int oc_tutte(oc_ctx* ctx, int layout, int precond, double relTol, int maxIter, double* UVout, int* nLoops,
+             int* nBnd, int* iters);
+
 /* ---- batched charts ----------------------------------------------------------------------------------
  * Many independent charts in one context: after oc_set_mesh with the union of the charts (chart k owns the
  * vertices [vertexStart[k], vertexStart[k+1]) and the triangles [faceStart[k], faceStart[k+1]), triangles

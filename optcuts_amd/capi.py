@@ -26,6 +26,8 @@ PRECOND_AMG = 2
 PRECOND_LEARNED = 3
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+TUTTE_COMPONENTS = 0
+TUTTE_ALL_LOOPS = 1
 
 
 class OCError(RuntimeError):
@@ -79,6 +81,8 @@ _SIGS = {
     "oc_solver_set_params": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int]),
     "oc_solver_last_stats": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "oc_newton_get_system": (C.c_int, [_ctx, _vp, _vp, _vp, _vp, _vp]),
+    "oc_tutte": (C.c_int, [_ctx, C.c_int, C.c_int, C.c_double, C.c_int, _vp, C.POINTER(C.c_int),
+                           C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "oc_set_charts": (C.c_int, [_ctx, C.c_int, _ip, _ip]),
     "oc_charts_state": (C.c_int, [_ctx, _vp, _vp, _vp, _vp]),
     "oc_newton_set_learned": (C.c_int, [_ctx, _vp]),
@@ -278,6 +282,15 @@ class Context:
         return it.value, rr.value
 
     # ---- Newton driver -------------------------------------------------------
+    def tutte(self, layout=TUTTE_COMPONENTS, precond=PRECOND_JACOBI, rel_tol=1e-12, max_iter=200000):
+        """Tutte initial map of the current mesh (becomes the context's UV). Returns (UV, loops pinned,
+        vertices pinned, PCG iterations)."""
+        out = np.zeros((self.nV, 2))
+        nl, nb, it = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.L.oc_tutte(self.h, int(layout), int(precond), float(rel_tol), int(max_iter),
+                                  out.ctypes.data, C.byref(nl), C.byref(nb), C.byref(it)))
+        return out, nl.value, nb.value, it.value
+
     def set_charts(self, vertex_start, face_start):
         """Batched charts: contiguous vertex / triangle ranges of the mesh set by set_mesh."""
         vs, fs = _i32(vertex_start), _i32(face_start)

@@ -76,6 +76,7 @@ struct oc_ctx {
     // ---- mesh block (block 0) -----------------------------------------------
     int nV = 0, nF = 0;
     std::vector<int> Fh;            // nF x 3
+    std::vector<double> Vh;         // nV x 3 rest positions
     std::vector<int> fixedSorted;   // mesh fixed vertices, ascending
     std::vector<unsigned char> fixedMaskMesh;
     double surfaceArea = 0.0, avgEdgeLen = 0.0;
@@ -142,6 +143,10 @@ struct oc_ctx {
     std::vector<int> chartState;            // 0 active, 1 gradient converged, 2 line search stopped
     DevBuf dChartV, dChartF, dChartBlk, dBlkRange, dBlkChart, dVChart, dTriChart, dChartArea, dChartActive,
         dChartAlpha, dChartOrdMin, dChartOut, dChartGsq;
+
+    // ---- Tutte initial map --------------------------------------------------------
+    BlockPattern tut;
+    oc::Amg<double> amgTut;
 
     // ---- LinSysSolver drop-in ---------------------------------------------------
     BlockPattern sol;
@@ -722,6 +727,7 @@ int oc_set_mesh(oc_ctx* c, int nV, const double* Vrest, int nF, const int* F, in
     c->nV = nV;
     c->nF = nF;
     c->Fh.assign(F, F + 3 * (size_t)nF);
+    c->Vh.assign(Vrest, Vrest + 3 * (size_t)nV);
     c->fixedSorted.assign(fixedV, fixedV + nFixed);
     std::sort(c->fixedSorted.begin(), c->fixedSorted.end());
     c->fixedSorted.erase(std::unique(c->fixedSorted.begin(), c->fixedSorted.end()), c->fixedSorted.end());
@@ -1203,6 +1209,321 @@ int oc_solver_last_stats(oc_ctx* c, int* iters, double* relResidual)
     if (!c) return OC_ERR_ARG;
     if (iters) *iters = c->lastIters;
     if (relResidual) *relResidual = c->lastRelRes;
+    return OC_OK;
+}
+
+// ---- Tutte initial map (main.cpp:1317-1364) -----------------------------------------
+namespace {
+
+// Boundary topology of a triangle mesh for the Tutte map: triangle edge
+// (F(f,i), F(f,i+1)) is a boundary edge when no other triangle holds it
+// (triangle_triangle_adjacency.cpp); vertex -> triangles in ascending order
+// (vertex_triangle_adjacency.cpp); edge-connected components numbered in order
+// of their lowest triangle (facet_components.cpp).
+struct LoopTopology {
+    int nV = 0, nF = 0, nComp = 0;
+    const int* F = nullptr;
+    std::vector<int> tt;           // 3 per triangle: neighbour across edge i, -1 on the boundary
+    std::vector<char> faceHasBnd;  // triangle has a boundary edge
+    std::vector<int> vfPtr, vf;
+    std::vector<int> comp;  // per triangle
+
+    // 0, or -1 for an edge held by three or more triangles
+    int build(int nV_, int nF_, const int* F_)
+    {
+        nV = nV_;
+        nF = nF_;
+        F = F_;
+        struct E {
+            int a, b, f, i;
+        };
+        std::vector<E> es;
+        es.reserve(3 * (size_t)nF);
+        for (int f = 0; f < nF; ++f)
+            for (int i = 0; i < 3; ++i) {
+                int a = F[3 * (size_t)f + i], b = F[3 * (size_t)f + (i + 1) % 3];
+                if (a > b) std::swap(a, b);
+                es.push_back(E{a, b, f, i});
+            }
+        std::sort(es.begin(), es.end(), [](const E& x, const E& y) {
+            return x.a != y.a ? x.a < y.a : (x.b != y.b ? x.b < y.b : (x.f != y.f ? x.f < y.f : x.i < y.i));
+        });
+        tt.assign(3 * (size_t)nF, -1);
+        for (size_t k = 0; k < es.size();) {
+            size_t e = k + 1;
+            while (e < es.size() && es[e].a == es[k].a && es[e].b == es[k].b) ++e;
+            if (e - k > 2) return -1;
+            if (e - k == 2) {
+                tt[3 * (size_t)es[k].f + es[k].i] = es[k + 1].f;
+                tt[3 * (size_t)es[k + 1].f + es[k + 1].i] = es[k].f;
+            }
+            k = e;
+        }
+        faceHasBnd.assign(nF, 0);
+        for (int f = 0; f < nF; ++f)
+            for (int i = 0; i < 3; ++i)
+                if (tt[3 * (size_t)f + i] < 0) faceHasBnd[f] = 1;
+        vfPtr.assign(nV + 1, 0);
+        vf.resize(3 * (size_t)nF);
+        for (int f = 0; f < nF; ++f)
+            for (int i = 0; i < 3; ++i) vfPtr[F[3 * (size_t)f + i] + 1]++;
+        for (int v = 0; v < nV; ++v) vfPtr[v + 1] += vfPtr[v];
+        std::vector<int> fill(vfPtr.begin(), vfPtr.end() - 1);
+        for (int f = 0; f < nF; ++f)
+            for (int i = 0; i < 3; ++i) vf[fill[F[3 * (size_t)f + i]]++] = f;
+        comp.assign(nF, -1);
+        nComp = 0;
+        std::vector<int> queue;
+        for (int g = 0; g < nF; ++g) {
+            if (comp[g] >= 0) continue;
+            comp[g] = nComp;
+            queue.assign(1, g);
+            while (!queue.empty()) {
+                const int f = queue.back();
+                queue.pop_back();
+                for (int i = 0; i < 3; ++i) {
+                    const int h = tt[3 * (size_t)f + i];
+                    if (h >= 0 && comp[h] < 0) {
+                        comp[h] = nComp;
+                        queue.push_back(h);
+                    }
+                }
+            }
+            ++nComp;
+        }
+        return 0;
+    }
+
+    // Loops of the triangles of component `which` (-1: all triangles) as
+    // libigl's boundary_loop walks them (boundary_loop.cpp): a loop starts at
+    // the smallest unvisited border vertex and moves on through the first
+    // triangle (ascending) whose boundary edge leaves the current vertex to an
+    // unvisited one. `starts` lists the border vertices of those triangles in
+    // ascending order; `unvisited` is all zero on entry and on return.
+    void walk(int which, const std::vector<int>& starts, std::vector<char>& unvisited,
+              std::vector<std::vector<int>>& loops) const
+    {
+        for (int v : starts) unvisited[v] = 1;
+        for (int start : starts) {
+            if (!unvisited[start]) continue;
+            std::vector<int> loop(1, start);
+            unvisited[start] = 0;
+            for (;;) {
+                const int v = loop.back();
+                int next = -1;
+                for (int q = vfPtr[v]; q < vfPtr[v + 1] && next < 0; ++q) {
+                    const int f = vf[q];
+                    if (!faceHasBnd[f] || (which >= 0 && comp[f] != which)) continue;
+                    int loc = 0;
+                    while (F[3 * (size_t)f + loc] != v) ++loc;
+                    const int vn = F[3 * (size_t)f + (loc + 1) % 3];
+                    if (unvisited[vn] && tt[3 * (size_t)f + loc] < 0) next = vn;
+                }
+                if (next < 0) break;
+                loop.push_back(next);
+                unvisited[next] = 0;
+            }
+            loops.push_back(std::move(loop));
+        }
+    }
+
+    // The loops the Tutte map pins: layout 0 = the first longest loop of each
+    // component (main.cpp:1486-1512), layout 1 = every loop (main.cpp:1313-1317).
+    // Returns -1 - k when component k has no boundary.
+    int tutte_loops(int layout, std::vector<std::vector<int>>& out) const
+    {
+        out.clear();
+        std::vector<char> unvisited(nV, 0);
+        if (layout == 1) {
+            std::vector<char> mark(nV, 0);
+            for (int f = 0; f < nF; ++f)
+                for (int i = 0; i < 3; ++i)
+                    if (tt[3 * (size_t)f + i] < 0)
+                        mark[F[3 * (size_t)f + i]] = mark[F[3 * (size_t)f + (i + 1) % 3]] = 1;
+            std::vector<int> starts;
+            for (int v = 0; v < nV; ++v)
+                if (mark[v]) starts.push_back(v);
+            walk(-1, starts, unvisited, out);
+            return out.empty() ? -1 : 0;
+        }
+        std::vector<std::vector<int>> starts(nComp);
+        for (int f = 0; f < nF; ++f)
+            for (int i = 0; i < 3; ++i)
+                if (tt[3 * (size_t)f + i] < 0) {
+                    starts[comp[f]].push_back(F[3 * (size_t)f + i]);
+                    starts[comp[f]].push_back(F[3 * (size_t)f + (i + 1) % 3]);
+                }
+        std::vector<std::vector<int>> loops;
+        for (int k = 0; k < nComp; ++k) {
+            auto& st = starts[k];
+            std::sort(st.begin(), st.end());
+            st.erase(std::unique(st.begin(), st.end()), st.end());
+            loops.clear();
+            walk(k, st, unvisited, loops);
+            if (loops.empty()) return -1 - k;
+            size_t best = 0;
+            for (size_t b = 1; b < loops.size(); ++b)
+                if (loops[best].size() < loops[b].size()) best = b;
+            out.push_back(std::move(loops[best]));
+        }
+        return 0;
+    }
+};
+
+}  // namespace
+
+int oc_tutte(oc_ctx* c, int layout, int precond, double relTol, int maxIter, double* UVout, int* nLoops,
+             int* nBnd, int* iters)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0, "oc_tutte: no mesh");
+    OC_ARG(c, (layout == OC_TUTTE_COMPONENTS || layout == OC_TUTTE_ALL_LOOPS) && relTol > 0.0 && maxIter > 0 &&
+                  (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_AMG),
+           "oc_tutte: bad parameters");
+    (void)hipSetDevice(c->device);
+    const int nV = c->nV, nF = c->nF;
+    LoopTopology topo;
+    if (topo.build(nV, nF, c->Fh.data()) != 0) {
+        c->err = "oc_tutte: an edge is shared by more than two triangles";
+        return OC_ERR_ARG;
+    }
+    std::vector<std::vector<int>> loops;
+    const int lr = topo.tutte_loops(layout, loops);
+    if (lr != 0) {
+        c->err = layout == OC_TUTTE_ALL_LOOPS
+                     ? std::string("oc_tutte: the mesh has no boundary")
+                     : "oc_tutte: component " + std::to_string(-1 - lr) + " has no boundary";
+        return OC_ERR_STATE;
+    }
+    // loop k on the unit circle by arc length of the rest positions
+    // (map_vertices_to_circle.cpp), shifted by (k % d, k / d) * 2.1 with d the
+    // smallest integer with d * d >= number of loops (main.cpp:1335-1357)
+    auto dist = [&](int a, int b) {
+        double s2 = 0.0;
+        for (int j = 0; j < 3; ++j) {
+            const double d = c->Vh[3 * (size_t)a + j] - c->Vh[3 * (size_t)b + j];
+            s2 += d * d;
+        }
+        return std::sqrt(s2);
+    };
+    int grid = 0;
+    do {
+        ++grid;
+    } while (grid * grid < (int)loops.size());
+    std::vector<double> bc(2 * (size_t)nV, 0.0);
+    std::vector<unsigned char> isB(nV, 0);
+    int nb = 0;
+    for (size_t k = 0; k < loops.size(); ++k) {
+        const std::vector<int>& bnd = loops[k];
+        const int n = (int)bnd.size();
+        std::vector<double> len(n, 0.0);
+        for (int i = 1; i < n; ++i) len[i] = len[i - 1] + dist(bnd[i - 1], bnd[i]);
+        const double total = len[n - 1] + dist(bnd[0], bnd[n - 1]);
+        const double xo = (double)(k % grid) * 2.1, yo = (double)(k / grid) * 2.1;
+        for (int i = 0; i < n; ++i) {
+            if (isB[bnd[i]]) {
+                c->err = "oc_tutte: vertex " + std::to_string(bnd[i]) + " lies on two boundary loops";
+                return OC_ERR_ARG;
+            }
+            const double a = len[i] * 2.0 * M_PI / total;
+            bc[2 * (size_t)bnd[i]] = std::cos(a) + xo;
+            bc[2 * (size_t)bnd[i] + 1] = std::sin(a) + yo;
+            isB[bnd[i]] = 1;
+        }
+        nb += n;
+    }
+    // Q = -L for the uniform Laplacian (IglUtils.cpp:39-60): per triangle +1 on
+    // both directions of each edge, -2 per corner, negated; igl::harmonic with
+    // k = 1 solves Q_II W_I = -Q_IB bc (harmonic.cpp, min_quad_with_fixed).
+    std::vector<std::vector<int>> rows(nV);
+    for (int t = 0; t < nF; ++t)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) rows[c->Fh[3 * (size_t)t + a]].push_back(c->Fh[3 * (size_t)t + b]);
+    for (int v = 0; v < nV; ++v) {
+        auto& r = rows[v];
+        r.push_back(v);
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+    }
+    BlockPattern& P = c->tut;
+    build_block_pattern(P, nV, rows, isB);
+    rows.clear();
+    std::vector<double> q(P.nnzb(), 0.0), b(2 * (size_t)nV, 0.0);
+    for (int t = 0; t < nF; ++t) {
+        const int* f = &c->Fh[3 * (size_t)t];
+        for (int e = 0; e < 3; ++e) {
+            const int u = f[e], w = f[(e + 1) % 3];
+            for (int dir = 0; dir < 2; ++dir) {
+                const int r = dir ? w : u, col = dir ? u : w;
+                if (isB[r]) continue;
+                if (isB[col]) {  // known boundary value: to the right-hand side
+                    b[2 * (size_t)r] += bc[2 * (size_t)col];
+                    b[2 * (size_t)r + 1] += bc[2 * (size_t)col + 1];
+                } else {
+                    q[P.find(r, col)] -= 1.0;
+                }
+            }
+            if (!isB[u]) q[P.diagPos[u]] += 2.0;
+        }
+    }
+    for (int v = 0; v < nV; ++v)
+        if (!isB[v] && q[P.diagPos[v]] == 0.0) {
+            c->err = "oc_tutte: vertex " + std::to_string(v) + " is in no triangle";
+            return OC_ERR_ARG;
+        }
+    for (int v = 0; v < nV; ++v)
+        if (isB[v]) {
+            q[P.diagPos[v]] = 1.0;
+            b[2 * (size_t)v] = bc[2 * (size_t)v];
+            b[2 * (size_t)v + 1] = bc[2 * (size_t)v + 1];
+        }
+    std::vector<double> vals(4 * (size_t)P.nnzb());
+    for (long k = 0; k < P.nnzb(); ++k) {
+        vals[4 * k] = q[k];
+        vals[4 * k + 1] = 0.0;
+        vals[4 * k + 2] = 0.0;
+        vals[4 * k + 3] = q[k];
+    }
+    if (upload_pattern(c, P)) return OC_ERR_HIP;
+    P.vals32 = 0;
+    OC_HIP(c, hipMemcpyAsync(P.dVals.p, vals.data(), sizeof(double) * vals.size(), hipMemcpyHostToDevice, c->stream));
+    if (ensure_pcg(c, nV)) return OC_ERR_HIP;
+    OC_HIP(c, hipMemcpyAsync(c->dB.p, b.data(), sizeof(double) * b.size(), hipMemcpyHostToDevice, c->stream));
+    oc::launch_jacobi_from_bsr(c->stream, nV, P.dDiagPos.as<int>(), P.dVals.as<double>(), P.dMinv.as<double>(), 0);
+    oc::Preconditioner* pre = nullptr;
+    if (precond == OC_PRECOND_AMG) {
+        std::string e;
+        if (c->amgTut.setup(c->stream, nV, P.rowPtr, P.colIdx, isB, e) != 0) {
+            c->err = e;
+            return OC_ERR_HIP;
+        }
+        // rigid-motion near-null space at the boundary-initialised positions,
+        // staged in the solution buffer (the solve starts from zero after it)
+        OC_HIP(c, hipMemcpyAsync(c->dX.p, bc.data(), sizeof(double) * bc.size(), hipMemcpyHostToDevice, c->stream));
+        c->amgTut.numeric(c->stream, P.dVals.p, 0, c->dX.as<double>());
+        pre = &c->amgTut;
+    }
+    OC_HIP(c, hipGetLastError());
+    int it = 0;
+    double rr = 0.0;
+    int st = run_pcg(c, P, c->dB.as<double>(), relTol, maxIter, it, rr, pre);
+    if (iters) *iters = it;
+    if (st != OC_OK) return st;
+    // boundary values exactly as placed (the solve meets them to its tolerance)
+    std::vector<double> uv(2 * (size_t)nV);
+    OC_HIP(c, hipMemcpyAsync(uv.data(), c->dX.p, sizeof(double) * uv.size(), hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    for (int v = 0; v < nV; ++v)
+        if (isB[v]) {
+            uv[2 * (size_t)v] = bc[2 * (size_t)v];
+            uv[2 * (size_t)v + 1] = bc[2 * (size_t)v + 1];
+        }
+    OC_HIP(c, hipMemcpyAsync(c->dUV.p, uv.data(), sizeof(double) * uv.size(), hipMemcpyHostToDevice, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    if (UVout) std::copy(uv.begin(), uv.end(), UVout);
+    if (nBnd) *nBnd = nb;
+    if (nLoops) *nLoops = (int)loops.size();
+    c->sysReady = false;
     return OC_OK;
 }
 

@@ -103,6 +103,8 @@ def ref():
         R.ref_solve.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         R.ref_make_pd6.argtypes = [_dp]
         R.ref_eigvals6.argtypes = [_dp, _dp]
+        R.ref_tutte.restype = C.c_int
+        R.ref_tutte.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_int, _dp, _ip, _ip]
         _ref = R
     return _ref
 
@@ -337,3 +339,175 @@ def ref_eigvals6(H):
     lam = np.zeros(6)
     ref().ref_eigvals6(_f64(H).reshape(-1), lam)
     return lam
+
+
+def ref_tutte(V, F, layout=0):
+    """The reference's Tutte initial map through the vendored libigl
+    (main.cpp:1313-1364 for layout 1, 1380-1395 + 1486-1532 for layout 0).
+    Returns (UV, list of boundary loops)."""
+    V, F = _f64(V), _i32(F)
+    UV = np.zeros((V.shape[0], 2))
+    bnd = np.zeros(V.shape[0] + 1, np.int32)
+    lens = np.zeros(V.shape[0] + 1, np.int32)
+    n = ref().ref_tutte(V.shape[0], V.reshape(-1), F.shape[0], F.reshape(-1), int(layout), UV.reshape(-1),
+                        bnd, lens)
+    if n < 0:
+        raise ValueError("a component has no boundary")
+    cut = np.concatenate([[0], np.cumsum(lens[:n])])
+    return UV, [bnd[cut[k]:cut[k + 1]].copy() for k in range(n)]
+
+
+def _edge_counts(F):
+    count = {}
+    for f in range(F.shape[0]):
+        for i in range(3):
+            a, b = int(F[f, i]), int(F[f, (i + 1) % 3])
+            key = (min(a, b), max(a, b))
+            count[key] = count.get(key, 0) + 1
+    return count
+
+
+def facet_components(F):
+    """Edge-connected triangle components (igl/facet_components.cpp), numbered in
+    order of their lowest triangle index."""
+    F = np.asarray(F)
+    owners = {}
+    for f in range(F.shape[0]):
+        for i in range(3):
+            a, b = int(F[f, i]), int(F[f, (i + 1) % 3])
+            owners.setdefault((min(a, b), max(a, b)), []).append(f)
+    C = -np.ones(F.shape[0], np.int64)
+    nc = 0
+    for g in range(F.shape[0]):
+        if C[g] >= 0:
+            continue
+        C[g] = nc
+        stack = [g]
+        while stack:
+            f = stack.pop()
+            for i in range(3):
+                a, b = int(F[f, i]), int(F[f, (i + 1) % 3])
+                for h in owners[(min(a, b), max(a, b))]:
+                    if C[h] < 0:
+                        C[h] = nc
+                        stack.append(h)
+        nc += 1
+    return C
+
+
+def boundary_loops(F, nV):
+    """Every boundary loop as libigl's boundary_loop walks them (boundary_loop.cpp,
+    triangle_triangle_adjacency.cpp, is_border_vertex.cpp): boundary edges are
+    triangle edges (F[f,i], F[f,i+1]) whose undirected edge no other triangle
+    has; each loop starts at the smallest unvisited border vertex and moves to
+    the next vertex of the first incident triangle (ascending) whose boundary
+    edge leaves the current one to an unvisited vertex."""
+    F = np.asarray(F)
+    count = _edge_counts(F)
+    bedge = np.zeros((F.shape[0], 3), bool)
+    border = np.zeros(nV, bool)
+    for f in range(F.shape[0]):
+        for i in range(3):
+            a, b = int(F[f, i]), int(F[f, (i + 1) % 3])
+            if count[(min(a, b), max(a, b))] == 1:
+                bedge[f, i] = True
+                border[a] = border[b] = True
+    vf = [[] for _ in range(nV)]
+    for f in range(F.shape[0]):
+        for i in range(3):
+            vf[int(F[f, i])].append(f)
+    unvisited = border.copy()
+    loops = []
+    for start in range(nV):
+        if not unvisited[start]:
+            continue
+        loop = [start]
+        unvisited[start] = False
+        while True:
+            v = loop[-1]
+            nxt = -1
+            for f in vf[v]:
+                if not bedge[f].any():
+                    continue
+                loc = int(np.nonzero(F[f] == v)[0][0])
+                vn = int(F[f, (loc + 1) % 3])
+                if unvisited[vn] and bedge[f, loc]:
+                    nxt = vn
+                    break
+            if nxt < 0:
+                break
+            loop.append(nxt)
+            unvisited[nxt] = False
+        loops.append(np.array(loop, np.int32))
+    return loops
+
+
+def boundary_loop(F, nV):
+    """The first longest boundary loop (igl::boundary_loop's single-loop form)."""
+    best = np.zeros(0, np.int32)
+    for l in boundary_loops(F, nV):
+        if len(l) > len(best):
+            best = l
+    return best
+
+
+def tutte_loops(F, nV, layout=0):
+    """Loops pinned by the Tutte map: layout 0 = the first longest loop of each
+    facet component, walked over that component's triangles (main.cpp:1486-1512);
+    layout 1 = every loop of the mesh (main.cpp:1313-1317)."""
+    F = np.asarray(F)
+    if layout == 1:
+        loops = boundary_loops(F, nV)
+        if not loops:
+            raise ValueError("a component has no boundary")
+        return loops
+    C = facet_components(F)
+    loops = []
+    for k in range(int(C.max()) + 1):
+        l = boundary_loop(F[C == k], nV)
+        if len(l) == 0:
+            raise ValueError("a component has no boundary")
+        loops.append(l)
+    return loops
+
+
+def tutte_uv(V, F, layout=0):
+    """Tutte initial map (main.cpp:1313-1364, 1486-1532): loop k on the unit circle
+    by arc length of the rest positions (map_vertices_to_circle.cpp), shifted by
+    (k % d, k // d) * 2.1 with d the smallest integer with d*d >= number of loops;
+    the interior harmonic for the uniform Laplacian of IglUtils.cpp:39-60
+    (igl::harmonic with k = 1: Q = -L, Q_II W_I = -Q_IB bc), solved directly.
+    Returns (UV, loops)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    V, F = _f64(V), np.asarray(F)
+    nV = V.shape[0]
+    loops = tutte_loops(F, nV, layout)
+    d = 1
+    while d * d < len(loops):
+        d += 1
+    bnd = np.concatenate(loops)
+    bcs = []
+    for k, l in enumerate(loops):
+        seg = np.linalg.norm(V[l[1:]] - V[l[:-1]], axis=1)
+        length = np.concatenate([[0.0], np.cumsum(seg)])
+        total = length[-1] + np.linalg.norm(V[l[0]] - V[l[-1]])
+        ang = length * 2.0 * np.pi / total
+        bcs.append(np.stack([np.cos(ang) + (k % d) * 2.1, np.sin(ang) + (k // d) * 2.1], 1))
+    bc = np.concatenate(bcs)
+    r, c, v = [], [], []
+    for (a, b) in ((0, 1), (1, 0), (1, 2), (2, 1), (2, 0), (0, 2)):
+        r.append(F[:, a]); c.append(F[:, b]); v.append(np.ones(F.shape[0]))
+    for a in range(3):
+        r.append(F[:, a]); c.append(F[:, a]); v.append(-2.0 * np.ones(F.shape[0]))
+    L = sp.csr_matrix((np.concatenate(v), (np.concatenate(r), np.concatenate(c))), shape=(nV, nV))
+    Q = (-L).tocsr()
+    isb = np.zeros(nV, bool)
+    isb[bnd] = True
+    I = np.nonzero(~isb)[0]
+    W = np.zeros((nV, 2))
+    W[bnd] = bc
+    if len(I):
+        rhs = -(Q[I][:, bnd] @ bc)
+        W[I] = spla.spsolve(Q[I][:, I].tocsc(), rhs).reshape(-1, 2)
+    return W, loops

@@ -6,13 +6,25 @@
 //     builder (set_pattern), triplet accumulation (update_a) and the Eigen
 //     SimplicialLDLT analyze/factorize/solve (EigenLibSolver.cpp:21-108);
 //   - src/Utils/IglUtils.hpp: IglUtils::makePD (IglUtils.hpp:71-90),
-//     header-only, on Eigen's SelfAdjointEigenSolver.
+//     header-only, on Eigen's SelfAdjointEigenSolver;
+//   - the vendored libigl's header-only igl::facet_components,
+//     igl::boundary_loop, igl::map_vertices_to_circle and igl::harmonic, called
+//     as the reference's Tutte initialisation calls them (main.cpp:1313-1364,
+//     1380-1395, 1486-1532). Its Laplacian
+//     comes from IglUtils::computeUniformLaplacian (IglUtils.cpp:39-60), a
+//     translation unit that needs TBB; ref_tutte assembles the same triplets
+//     (1 per directed triangle edge, -2 per corner) itself.
 // The remaining hot-path translation units (SymDirichletEnergy.cpp,
 // IglUtils.cpp, TriMesh.cpp, Scaffold.cpp) include <tbb/tbb.h>, which this
 // image lacks, so they are not built.
 
 #include "EigenLibSolver.hpp"
 #include "IglUtils.hpp"
+
+#include <igl/boundary_loop.h>
+#include <igl/facet_components.h>
+#include <igl/harmonic.h>
+#include <igl/map_vertices_to_circle.h>
 
 #include <cstring>
 #include <set>
@@ -102,6 +114,98 @@ void ref_eigvals6(const double* H, double* lam)
         for (int c = 0; c < 6; ++c) M(r, c) = H[6 * r + c];
     Eigen::SelfAdjointEigenSolver<Eigen::Matrix<double, 6, 6>> es(M);
     for (int k = 0; k < 6; ++k) lam[k] = es.eigenvalues()[k];
+}
+
+// Tutte initial UV, laid out as the reference does it:
+//   layout 0 (main.cpp:1380-1395,1486-1532, no input UV): facet components by
+//     igl::facet_components, per component the first longest loop of
+//     igl::boundary_loop over that component's triangles (in mesh order);
+//   layout 1 (main.cpp:1313-1364, input UV re-initialised): every loop of
+//     igl::boundary_loop over the whole mesh.
+// Loop k goes on the unit circle (igl::map_vertices_to_circle, arc length of the
+// rest positions) shifted by (k % d, k / d) * 2.1 with d the smallest integer
+// with d*d >= number of loops; the interior is igl::harmonic (k = 1) of the
+// uniform Laplacian over the whole mesh. UV: nV x 2 row-major. bndOut receives
+// the stacked loop vertices, loopLen the loop lengths. Returns the number of
+// loops (-1: a component without boundary).
+int ref_tutte(int nV, const double* Vrest, int nF, const int* F, int layout, double* UV, int* bndOut,
+              int* loopLen)
+{
+    Eigen::MatrixXd V(nV, 3);
+    for (int i = 0; i < nV; ++i)
+        for (int j = 0; j < 3; ++j) V(i, j) = Vrest[3 * i + j];
+    Eigen::MatrixXi Fm(nF, 3);
+    for (int t = 0; t < nF; ++t)
+        for (int j = 0; j < 3; ++j) Fm(t, j) = F[3 * t + j];
+    std::vector<std::vector<int>> loops;
+    if (layout == 0) {
+        Eigen::VectorXi C;
+        igl::facet_components(Fm, C);
+        const int nc = C.maxCoeff() + 1;
+        std::vector<std::vector<int>> rows(nc);
+        for (int t = 0; t < nF; ++t) rows[C[t]].push_back(t);
+        for (int k = 0; k < nc; ++k) {
+            Eigen::MatrixXi Fc((int)rows[k].size(), 3);
+            for (int r = 0; r < (int)rows[k].size(); ++r) Fc.row(r) = Fm.row(rows[k][r]);
+            std::vector<std::vector<int>> all;
+            igl::boundary_loop(Fc, all);
+            if (all.empty()) return -1;
+            int best = 0;
+            for (int b = 1; b < (int)all.size(); ++b)
+                if (all[best].size() < all[b].size()) best = b;
+            loops.push_back(all[best]);
+        }
+    } else {
+        igl::boundary_loop(Fm, loops);
+        if (loops.empty()) return -1;
+    }
+    int d = 0;
+    do {
+        ++d;
+    } while (d * d < (int)loops.size());
+    int nb = 0;
+    for (auto& l : loops) nb += (int)l.size();
+    Eigen::VectorXi bnd(nb);
+    Eigen::MatrixXd bnd_uv(nb, 2);
+    int off = 0;
+    for (int k = 0; k < (int)loops.size(); ++k) {
+        const int n = (int)loops[k].size();
+        for (int i = 0; i < n; ++i) bnd[off + i] = loops[k][i];
+        Eigen::MatrixXd circ;
+        igl::map_vertices_to_circle(V, bnd.segment(off, n), circ);
+        const double xo = k % d * 2.1, yo = k / d * 2.1;
+        for (int i = 0; i < n; ++i) {
+            bnd_uv(off + i, 0) = circ(i, 0) + xo;
+            bnd_uv(off + i, 1) = circ(i, 1) + yo;
+        }
+        if (loopLen) loopLen[k] = n;
+        off += n;
+    }
+    std::vector<Eigen::Triplet<double>> trip;
+    trip.reserve(9 * (size_t)nF);
+    for (int t = 0; t < nF; ++t) {
+        const int a = Fm(t, 0), b = Fm(t, 1), c = Fm(t, 2);
+        trip.emplace_back(a, b, 1.0);
+        trip.emplace_back(b, a, 1.0);
+        trip.emplace_back(b, c, 1.0);
+        trip.emplace_back(c, b, 1.0);
+        trip.emplace_back(c, a, 1.0);
+        trip.emplace_back(a, c, 1.0);
+        trip.emplace_back(a, a, -2.0);
+        trip.emplace_back(b, b, -2.0);
+        trip.emplace_back(c, c, -2.0);
+    }
+    Eigen::SparseMatrix<double> A(nV, nV), M;
+    A.setFromTriplets(trip.begin(), trip.end());
+    Eigen::MatrixXd W;
+    igl::harmonic(A, M, bnd, bnd_uv, 1, W);
+    for (int i = 0; i < nV; ++i) {
+        UV[2 * i] = W(i, 0);
+        UV[2 * i + 1] = W(i, 1);
+    }
+    if (bndOut)
+        for (int i = 0; i < nb; ++i) bndOut[i] = bnd[i];
+    return (int)loops.size();
 }
 
 }  // extern "C"

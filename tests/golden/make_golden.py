@@ -72,6 +72,30 @@ def main():
     Hpd = np.stack([O.ref_make_pd6(h) for h in Hin])
     lam = np.stack([O.ref_eigvals6(h) for h in Hin])
     out.update({"pd_in": Hin, "pd_out": Hpd, "pd_eigvals": lam})
+    # --- Tutte initial maps (main.cpp:1313-1364, 1486-1532): discs, a disc with a hole,
+    # three charts in one mesh with shuffled triangles; layout 0 = longest loop per
+    # component, layout 1 = every loop
+    from optcuts_amd.meshgen import union_of_charts
+    meshes = {}
+    for name, (m, seed) in {"tutte11": (11, 12), "tutte21": (21, 13)}.items():
+        meshes[name] = height_field_disc(m, seed=seed)[:2]
+    V, F, _ = height_field_disc(21, seed=14)
+    cen = V[F].mean(1)[:, :2]
+    F = F[np.linalg.norm(cen - 0.5, axis=1) > 0.18]
+    used = np.unique(F)
+    remap = -np.ones(V.shape[0], np.int64)
+    remap[used] = np.arange(len(used))
+    meshes["tutte21h"] = (V[used], remap[F].astype(np.int32))
+    V, F, _, _, _, _ = union_of_charts([height_field_disc(m, seed=s) for m, s in ((7, 15), (9, 16), (5, 17))])
+    F = F[np.random.default_rng(18).permutation(F.shape[0])]
+    meshes["tutte3c"] = (V, F)
+    for name, (V, F) in meshes.items():
+        out.update({name + "_Vrest": V, name + "_F": F})
+        for layout in (0, 1):
+            UVt, loops = O.ref_tutte(V, F, layout)
+            key = "%s_l%d" % (name, layout)
+            out.update({key + "_UV": UVt, key + "_bnd": np.concatenate(loops),
+                        key + "_loops": np.array([len(l) for l in loops], np.int32)})
     path = os.path.join(HERE, "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

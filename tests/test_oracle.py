@@ -405,3 +405,104 @@ def test_seam_sparsity_hand_case():
     UV2 = UV.copy(); UV2[4:] = UV[:2]
     assert O.seam_sparsity(cohE, Vrest, UV2, 1.0, triSoup=True) == 0.0
     assert O.seam_sparsity(cohE, Vrest, UV, 1.0, triSoup=True) == pytest.approx(1.0)
+
+
+# ---- Tutte initial map (main.cpp:1313-1364, 1486-1532) -------------------------------
+
+TUTTE_CASES = [(n, l) for n in ("tutte11", "tutte21", "tutte21h", "tutte3c") for l in (0, 1)]
+
+
+@pytest.mark.parametrize("name,layout", TUTTE_CASES)
+def test_tutte_matches_reference_golden(golden, name, layout):
+    """Pinned loops bit-exact and UV to 1e-12 against the reference's libigl path
+    (facet_components, boundary_loop, map_vertices_to_circle, harmonic with the
+    uniform Laplacian), both layouts."""
+    V, F = golden[name + "_Vrest"], golden[name + "_F"]
+    key = "%s_l%d" % (name, layout)
+    W, loops = O.tutte_uv(V, F, layout)
+    np.testing.assert_array_equal([len(l) for l in loops], golden[key + "_loops"])
+    np.testing.assert_array_equal(np.concatenate(loops), golden[key + "_bnd"])
+    np.testing.assert_allclose(W, golden[key + "_UV"], rtol=0, atol=1e-12)
+
+
+def _edge_weights(F, nV):
+    """Per vertex: {neighbour: number of triangles holding the edge}."""
+    nb = [dict() for _ in range(nV)]
+    for f in F:
+        for a in range(3):
+            for b in range(3):
+                if a != b:
+                    u, w = int(f[a]), int(f[b])
+                    nb[u][w] = nb[u].get(w, 0) + 1
+    return nb
+
+
+@pytest.mark.parametrize("name,layout", TUTTE_CASES)
+def test_tutte_properties(golden, name, layout):
+    """What the construction fixes, independent of the reference: every free vertex is
+    the average of its neighbours weighted by the number of triangles on each edge
+    (two inside the mesh, one along an unpinned hole); loop k lies on the unit circle
+    around ((k % d) * 2.1, (k // d) * 2.1) with angle steps proportional to rest edge
+    lengths; each loop is a cycle of boundary edges; discs keep every triangle's
+    orientation (Tutte's theorem for a convex boundary)."""
+    V, F = golden[name + "_Vrest"], golden[name + "_F"]
+    W, loops = O.tutte_uv(V, F, layout)
+    nV = V.shape[0]
+    nb = _edge_weights(F, nV)
+    isb = np.zeros(nV, bool)
+    isb[np.concatenate(loops)] = True
+    for v in np.nonzero(~isb)[0]:
+        js, ws = list(nb[v].keys()), np.array(list(nb[v].values()), float)
+        assert np.abs(W[v] - (ws[:, None] * W[js]).sum(0) / ws.sum()).max() < 1e-12
+    d = int(np.ceil(np.sqrt(len(loops))))
+    edges = {}
+    for f in F:
+        for i in range(3):
+            a, b = sorted((int(f[i]), int(f[(i + 1) % 3])))
+            edges[(a, b)] = edges.get((a, b), 0) + 1
+    for k, bnd in enumerate(loops):
+        centre = np.array([(k % d) * 2.1, (k // d) * 2.1])
+        P = W[bnd] - centre
+        np.testing.assert_allclose(np.linalg.norm(P, axis=1), 1.0, atol=1e-14)
+        assert np.abs(P[0] - [1.0, 0.0]).max() < 1e-15
+        ang = np.unwrap(np.arctan2(P[:, 1], P[:, 0]))
+        ang = np.append(ang - ang[0], 2 * np.pi)
+        seg = np.linalg.norm(V[np.roll(bnd, -1)] - V[bnd], axis=1)
+        np.testing.assert_allclose(np.diff(ang), seg * 2 * np.pi / seg.sum(), atol=1e-12)
+        for a, b in zip(bnd, np.roll(bnd, -1)):
+            assert edges[tuple(sorted((int(a), int(b))))] == 1
+    if name != "tutte21h":
+        assert O.check_inversion(F, W)
+        assert sum(len(l) for l in loops) == sum(1 for k in edges.values() if k == 1)
+
+
+def test_tutte_loops_and_components():
+    """Hand-checkable topology: a disc with a hole has two loops, the outer (longer)
+    one first and pinned alone in the per-component layout; three charts with
+    shuffled triangles number components by lowest triangle and loops by lowest
+    vertex, so the two layouts order the same three loops differently."""
+    V, F, _ = height_field_disc(21, seed=14)
+    cen = V[F].mean(1)[:, :2]
+    F = F[np.linalg.norm(cen - 0.5, axis=1) > 0.18]
+    all_loops = O.boundary_loops(F, V.shape[0])
+    assert [len(l) for l in all_loops] == [80, 26]
+    assert all_loops[0][0] == 0
+    assert np.abs(V[all_loops[0], :2] - 0.5).max(1).min() > 0.49  # outer loop on the square's rim
+    assert np.abs(V[all_loops[1], :2] - 0.5).max(1).max() < 0.3   # inner loop around the hole
+    assert [len(l) for l in O.tutte_loops(F, V.shape[0], 0)] == [80]
+    from optcuts_amd.meshgen import union_of_charts
+    V3, F3, _, vs, fs, _ = union_of_charts([height_field_disc(m, seed=s) for m, s in ((7, 15), (9, 16), (5, 17))])
+    perm = np.random.default_rng(18).permutation(F3.shape[0])
+    F3p = F3[perm]
+    C = O.facet_components(F3p)
+    chart_of_face = np.searchsorted(fs, perm, side="right") - 1
+    first = [chart_of_face[np.nonzero(C == k)[0][0]] for k in range(3)]
+    assert sorted(first) == [0, 1, 2]
+    for k in range(3):
+        assert np.all(chart_of_face[C == k] == first[k])
+    assert np.all(np.diff([np.nonzero(C == k)[0][0] for k in range(3)]) > 0)
+    l0 = O.tutte_loops(F3p, V3.shape[0], 0)
+    l1 = O.tutte_loops(F3p, V3.shape[0], 1)
+    assert [int(np.searchsorted(vs, l[0], side="right") - 1) for l in l1] == [0, 1, 2]
+    assert [int(np.searchsorted(vs, l[0], side="right") - 1) for l in l0] == first
+    assert [len(l) for l in l1] == [24, 32, 16]
