@@ -34,10 +34,11 @@ def spmv_bytes(nnzb, nV):
 
 def lanczos0_bytes(nnzb, nV):
     """Algorithmic bytes of one fine-level Lanczos step of the multigrid set-up
-    (k_lanczos on the 2x2-block CSR): values 32 B + column index 4 B per block,
-    row pointers 4 B per row, per vertex the inverse diagonal block (32 B),
-    W_{j-1}, v_{j-1}, v_{j-2} read once (48 B) and v_j, W_j written (32 B)."""
-    return 36 * nnzb + 4 * (nV + 1) + 112 * nV
+    (k_lanczos on the 2x2-block CSR): values 32 B + column index 4 B per block
+    (the diagonal block, re-read for D and its inverse, is one of them), row
+    pointers 4 B per row, per vertex the diagonal position (4 B), W_{j-1},
+    W_{j-2}, v_{j-1}, v_{j-2} read once (64 B) and v_j, W_j written (32 B)."""
+    return 36 * nnzb + 4 * (nV + 1) + 100 * nV
 
 
 # HBM traffic per launch of the roofline kernels at the 200k-face workload, from

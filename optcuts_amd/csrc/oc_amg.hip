@@ -14,7 +14,8 @@ namespace {
 constexpr int NS = 3;              // near-null-space vectors: two translations and a rotation
 constexpr int kAmgMaxGrid = 1024;  // workgroups of a row kernel (bounds partial records)
 constexpr int kCoarseMaxDofs = 64;  // the coarsest inverse lives in LDS (64 x 64 doubles)
-constexpr int kLanczosSteps = 15;  // Lanczos steps of the lambda_max estimate
+constexpr int kLanczosSteps = 15;       // Lanczos steps of the lambda_max estimate (fine level; array size)
+constexpr int kLanczosStepsCoarse = 10;  // on coarse levels
 constexpr int kMaxLevels = 10;
 
 template <int BS>
@@ -94,25 +95,11 @@ __global__ void __launch_bounds__(kBlock) k_cvt(long n, const TI* __restrict__ i
     if (i < n) out[i] = (T)in[i];
 }
 
-// Inverse of each diagonal block; an exactly zero diagonal entry (a coarse
-// DOF whose prolongator column vanished) is set to 1 in A as well.
-template <typename T, int BS>
-__global__ void __launch_bounds__(kBlock) k_blockinv(int n, const int* __restrict__ diagPos, T* vals, T* Dinv)
+// Inverse of a small SPD block in double: closed form for 2x2, Gauss-Jordan
+// without pivoting otherwise.
+template <int BS>
+__device__ __forceinline__ void block_inverse(const double (&m)[BS][BS], double (&inv)[BS][BS])
 {
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n) return;
-    T* a = vals + (size_t)diagPos[u] * BS * BS;
-    double m[BS][BS];
-#pragma unroll
-    for (int i = 0; i < BS; ++i) {
-#pragma unroll
-        for (int j = 0; j < BS; ++j) m[i][j] = (double)a[i * BS + j];
-        if (m[i][i] == 0.0) {
-            m[i][i] = 1.0;
-            a[i * BS + i] = T(1);
-        }
-    }
-    double inv[BS][BS];
     if (BS == 2) {
         const double det = m[0][0] * m[1][1] - m[0][1] * m[1][0];
         inv[0][0] = m[1][1] / det;
@@ -120,7 +107,6 @@ __global__ void __launch_bounds__(kBlock) k_blockinv(int n, const int* __restric
         inv[1][0] = -m[1][0] / det;
         inv[1][1] = m[0][0] / det;
     } else {
-        // Gauss-Jordan on the small block (SPD: no pivoting needed)
         double w[BS][2 * BS];
 #pragma unroll
         for (int i = 0; i < BS; ++i)
@@ -144,6 +130,28 @@ __global__ void __launch_bounds__(kBlock) k_blockinv(int n, const int* __restric
 #pragma unroll
             for (int j = 0; j < BS; ++j) inv[i][j] = w[i][BS + j];
     }
+}
+
+// Inverse of each diagonal block; an exactly zero diagonal entry (a coarse
+// DOF whose prolongator column vanished) is set to 1 in A as well.
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_blockinv(int n, const int* __restrict__ diagPos, T* vals, T* Dinv)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    T* a = vals + (size_t)diagPos[u] * BS * BS;
+    double m[BS][BS];
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+#pragma unroll
+        for (int j = 0; j < BS; ++j) m[i][j] = (double)a[i * BS + j];
+        if (m[i][i] == 0.0) {
+            m[i][i] = 1.0;
+            a[i * BS + i] = T(1);
+        }
+    }
+    double inv[BS][BS];
+    block_inverse<BS>(m, inv);
     T* o = Dinv + (size_t)u * BS * BS;
 #pragma unroll
     for (int i = 0; i < BS; ++i)
@@ -167,20 +175,108 @@ __device__ __forceinline__ double hash_unit(unsigned int i)
 // power iteration does when the top of the spectrum is clustered (large
 // meshes), and they never exceed lambda_max.
 //
-// One kernel per step j with every grid reduction folded into the next step:
-// W_{j-1} = D^-1 A v_{j-1} (W_{-1} = D^-1 h for a fixed pseudo-random h),
-// alpha_{j-1} = v_{j-1}^T A v_{j-1}, gamma_{j-1} = |W_{j-1}|_D^2 and
+// One kernel per step j with every grid reduction folded into the next step.
+// With M = D^-1 A, W_{j-1} = M v_{j-1} (W_{-1} = D^-1 h for a fixed
+// pseudo-random h), alpha_{j-1} = v_{j-1}^T A v_{j-1} and gamma_{j-1} =
+// |W_{j-1}|_D^2 (partial records of the previous step):
 //   beta_j^2 = gamma_{j-1} - alpha_{j-1}^2 - beta_{j-1}^2,
 //   v_j = (W_{j-1} - alpha_{j-1} v_{j-1} - beta_{j-1} v_{j-2}) / beta_j,
-// where the v_j entries of neighbour rows are formed on the fly from the
-// stored W_{j-1}, v_{j-1}, v_{j-2}. Every workgroup reduces the previous
-// step's partial records itself (same order, same result); workgroup 0 keeps
-// the coefficients. coef: alpha[kLanczosSteps], beta[kLanczosSteps], then the
-// number of valid steps (fewer after an invariant subspace is found).
+//   W_j = M v_j = (M W_{j-1} - alpha_{j-1} W_{j-1} - beta_{j-1} W_{j-2}) / beta_j,
+// so the product gathers the single vector W_{j-1}; everything else is the
+// row's own entries. A v_j = D W_j gives this step's records v_j^T D W_j and
+// W_j^T D W_j. Every workgroup reduces the previous step's records itself
+// (same order, same result) after issuing its row loads, so the reduction
+// overlaps their latency; workgroup 0 keeps the coefficients. coef:
+// alpha[kLanczosSteps], beta[kLanczosSteps], then the number of valid steps
+// (fewer after an invariant subspace is found).
 struct LanczosBufs {
     double* v[3];  // v_j in v[j % 3]
-    double* W[2];  // W_j in W[(j + 2) % 2] (W_{-1} in W[1])
+    double* W[3];  // W_j in W[(j + 1) % 3] (W_{-1} in W[0])
 };
+
+// lane-group shape of the Lanczos step: lanes per row, rows per lane group per pass
+template <int BS>
+struct LzShape;
+template <>
+struct LzShape<2> {
+    static constexpr int lanes = 4, unroll = 2;
+};
+template <>
+struct LzShape<3> {
+    static constexpr int lanes = 16, unroll = 1;
+};
+
+// Workgroups of a Lanczos step: enough for one pass over the rows, at most
+// kAmgMaxGrid, a multiple of 8 (the XCD-contiguous chunk order below).
+// Lanczos steps per level: the fine level's top spectrum is clustered on large
+// meshes and needs all kLanczosSteps; coarse levels settle sooner. At 200k
+// faces 8 coarse steps lose nothing and 6 under-estimate often enough to force
+// a safe-mode retry; 10 keeps a margin.
+int lanczos_steps(int level) { return level == 0 ? kLanczosSteps : kLanczosStepsCoarse; }
+
+int lanczos_grid(int n, int bs)
+{
+    const int rows = bs == 2 ? (kBlock / LzShape<2>::lanes) * LzShape<2>::unroll
+                             : (kBlock / LzShape<3>::lanes) * LzShape<3>::unroll;
+    const int g = std::max(1, std::min((n + rows - 1) / rows, kAmgMaxGrid));
+    return (g + 7) & ~7;
+}
+
+// Workgroup b is dispatched to XCD b % 8; chunk c of the rows goes to
+// workgroup (c % (G/8)) * 8 + c / (G/8), so each XCD owns one contiguous
+// eighth of the rows and its L2 serves most neighbour gathers.
+__device__ __forceinline__ int xcd_chunk(int b, int G) { return (b & 7) * (G >> 3) + (b >> 3); }
+
+template <typename T, int BS>
+__device__ __forceinline__ void load_block(const T* __restrict__ p, double (&a)[BS * BS])
+{
+#pragma unroll
+    for (int i = 0; i < BS * BS; ++i) a[i] = (double)p[i];
+}
+template <>
+__device__ __forceinline__ void load_block<double, 2>(const double* __restrict__ p, double (&a)[4])
+{
+    const double2 x = reinterpret_cast<const double2*>(p)[0], y = reinterpret_cast<const double2*>(p)[1];
+    a[0] = x.x;
+    a[1] = x.y;
+    a[2] = y.x;
+    a[3] = y.y;
+}
+template <>
+__device__ __forceinline__ void load_block<float, 2>(const float* __restrict__ p, double (&a)[4])
+{
+    const float4 x = reinterpret_cast<const float4*>(p)[0];
+    a[0] = x.x;
+    a[1] = x.y;
+    a[2] = x.z;
+    a[3] = x.w;
+}
+
+template <int BS>
+__device__ __forceinline__ void load_vec(const double* __restrict__ p, size_t u, double (&x)[BS])
+{
+#pragma unroll
+    for (int i = 0; i < BS; ++i) x[i] = p[u * BS + i];
+}
+template <>
+__device__ __forceinline__ void load_vec<2>(const double* __restrict__ p, size_t u, double (&x)[2])
+{
+    const double2 t = reinterpret_cast<const double2*>(p)[u];
+    x[0] = t.x;
+    x[1] = t.y;
+}
+
+template <int BS>
+__device__ __forceinline__ void store_vec(double* p, size_t u, const double (&x)[BS])
+{
+#pragma unroll
+    for (int i = 0; i < BS; ++i) p[u * BS + i] = x[i];
+}
+template <>
+__device__ __forceinline__ void store_vec<2>(double* p, size_t u, const double (&x)[2])
+{
+    reinterpret_cast<double2*>(p)[u] = make_double2(x[0], x[1]);
+}
 
 __device__ __forceinline__ void lanczos_coeffs(int j, const double* __restrict__ partIn, int nPartIn,
                                                const double* __restrict__ coef, double& aprev, double& bprev,
@@ -205,77 +301,134 @@ __device__ __forceinline__ void lanczos_coeffs(int j, const double* __restrict__
     bj = (gamma > 0.0 && b2 > 1e-10 * gamma) ? sqrt(b2) : 0.0;
 }
 
-template <typename T, int BS, int LANES>
-__global__ void __launch_bounds__(kBlock) k_lanczos(int j, int n, const int* __restrict__ rowPtr,
+// Launch with gridDim.x a multiple of 8 (lanczos_grid).
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_lanczos(int j, int nsteps, int n, const int* __restrict__ rowPtr,
                                                     const int* __restrict__ colIdx, const T* __restrict__ vals,
-                                                    const T* __restrict__ Dinv, LanczosBufs buf,
+                                                    const int* __restrict__ diagPos, LanczosBufs buf,
                                                     const double* __restrict__ partIn, int nPartIn, double* partOut,
                                                     double* coef)
 {
-    if (j > 0 && coef[2 * kLanczosSteps] < (double)kLanczosSteps) return;  // stopped earlier
-    double aprev, bprev, bj;
-    lanczos_coeffs(j, partIn, nPartIn, coef, aprev, bprev, bj);
-    if (bj == 0.0) {  // the Krylov space is invariant: T so far is exact
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            if (j > 0) coef[j - 1] = aprev;
-            coef[2 * kLanczosSteps] = (double)j;
-        }
-        return;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (j > 0) coef[j - 1] = aprev;
-        coef[kLanczosSteps + j] = bj;
-    }
-    const double* __restrict__ Wp = buf.W[(j + 1) & 1];
+    constexpr int LANES = LzShape<BS>::lanes, UNROLL = LzShape<BS>::unroll, RPP = kBlock / LANES;
+    if (j > 0 && coef[2 * kLanczosSteps] < (double)nsteps) return;  // stopped earlier
+    const double* __restrict__ Wp = buf.W[j % 3];
+    const double* __restrict__ Wpp = buf.W[(j + 2) % 3];
     const double* __restrict__ v1 = buf.v[(j + 2) % 3];
     const double* __restrict__ v2 = buf.v[(j + 1) % 3];
     double* vj = buf.v[j % 3];
-    double* Wj = buf.W[j & 1];
-    const double inv = 1.0 / bj;
-    int r0, r1;
-    chunk_rows(n, r0, r1);
+    double* Wj = buf.W[(j + 1) % 3];
+    const int G = gridDim.x;
+    const int c = (n + G - 1) / G;
+    const int r0 = xcd_chunk(blockIdx.x, G) * c, r1 = min(n, r0 + c);
     const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-    constexpr int RPP = kBlock / LANES;
+    const int passes = max(1, (c + RPP * UNROLL - 1) / (RPP * UNROLL));
+    double aprev = 0.0, bprev = 0.0, inv = 1.0;
     double acc[2] = {0.0, 0.0};
-    for (int u = r0 + sub; u < r1; u += RPP) {
-        double y[BS];
+    for (int p = 0; p < passes; ++p) {
+        int u[UNROLL], kb[UNROLL], ke[UNROLL];
+        double a[UNROLL][BS * BS], g[UNROLL][BS];
+        double oW[UNROLL][BS], oWp[UNROLL][BS], o1[UNROLL][BS], o2[UNROLL][BS], D[UNROLL][BS * BS];
 #pragma unroll
-        for (int i = 0; i < BS; ++i) y[i] = 0.0;
-        const int k1 = rowPtr[u + 1];
-        for (int k = rowPtr[u] + lane; k < k1; k += LANES) {
-            const int w = colIdx[k];
-            const T* a = vals + (size_t)k * BS * BS;
+        for (int r = 0; r < UNROLL; ++r) {
+            u[r] = r0 + sub + (p * UNROLL + r) * RPP;
+            const bool valid = u[r] < r1;
+            kb[r] = valid ? rowPtr[u[r]] : 0;
+            ke[r] = valid ? rowPtr[u[r] + 1] : 0;
+        }
+        // first slice of every row (a row of at most LANES blocks needs no other)
 #pragma unroll
-            for (int c = 0; c < BS; ++c) {
-                const size_t q = (size_t)w * BS + c;
-                double x = Wp[q];
-                if (j >= 1) x -= aprev * v1[q];
-                if (j >= 2) x -= bprev * v2[q];
-                x *= inv;
+        for (int r = 0; r < UNROLL; ++r) {
+            const int k = kb[r] + lane;
+            const bool has = k < ke[r];
+            const int kk = has ? k : 0;
+            load_block<T, BS>(vals + (size_t)kk * BS * BS, a[r]);
+            load_vec<BS>(Wp, has ? colIdx[kk] : 0, g[r]);
+            if (!has)
 #pragma unroll
-                for (int i = 0; i < BS; ++i) y[i] += (double)a[i * BS + c] * x;
+                for (int i = 0; i < BS * BS; ++i) a[r][i] = 0.0;
+#pragma unroll
+            for (int i = 0; i < BS; ++i) oW[r][i] = oWp[r][i] = o1[r][i] = o2[r][i] = 0.0;
+#pragma unroll
+            for (int i = 0; i < BS * BS; ++i) D[r][i] = 0.0;
+            if (lane == 0 && u[r] < r1) {
+                load_vec<BS>(Wp, u[r], oW[r]);
+                if (j >= 1) {
+                    load_vec<BS>(Wpp, u[r], oWp[r]);
+                    load_vec<BS>(v1, u[r], o1[r]);
+                }
+                if (j >= 2) load_vec<BS>(v2, u[r], o2[r]);
+                load_block<T, BS>(vals + (size_t)diagPos[u[r]] * BS * BS, D[r]);
             }
         }
+        if (p == 0) {
+            double bj;
+            lanczos_coeffs(j, partIn, nPartIn, coef, aprev, bprev, bj);
+            if (bj == 0.0) {  // the Krylov space is invariant: T so far is exact
+                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                    if (j > 0) coef[j - 1] = aprev;
+                    coef[2 * kLanczosSteps] = (double)j;
+                }
+                return;
+            }
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                if (j > 0) coef[j - 1] = aprev;
+                coef[kLanczosSteps + j] = bj;
+            }
+            inv = 1.0 / bj;
+        }
 #pragma unroll
-        for (int off = LANES / 2; off > 0; off >>= 1)
-#pragma unroll
-            for (int i = 0; i < BS; ++i) y[i] += __shfl_xor(y[i], off, LANES);
-        if (lane == 0) {
-            const T* D = Dinv + (size_t)u * BS * BS;
+        for (int r = 0; r < UNROLL; ++r) {
+            double y[BS];
 #pragma unroll
             for (int i = 0; i < BS; ++i) {
-                const size_t q = (size_t)u * BS + i;
-                double x = Wp[q];
-                if (j >= 1) x -= aprev * v1[q];
-                if (j >= 2) x -= bprev * v2[q];
-                x *= inv;
-                vj[q] = x;
-                double s = 0.0;
+                y[i] = 0.0;
 #pragma unroll
-                for (int c = 0; c < BS; ++c) s += (double)D[i * BS + c] * y[c];
-                Wj[q] = s;
-                acc[0] += x * y[i];
-                acc[1] += y[i] * s;
+                for (int q = 0; q < BS; ++q) y[i] += a[r][i * BS + q] * g[r][q];
+            }
+            for (int k = kb[r] + lane + LANES; k < ke[r]; k += LANES) {
+                double ak[BS * BS], gk[BS];
+                load_block<T, BS>(vals + (size_t)k * BS * BS, ak);
+                load_vec<BS>(Wp, colIdx[k], gk);
+#pragma unroll
+                for (int i = 0; i < BS; ++i)
+#pragma unroll
+                    for (int q = 0; q < BS; ++q) y[i] += ak[i * BS + q] * gk[q];
+            }
+#pragma unroll
+            for (int off = LANES / 2; off > 0; off >>= 1)
+#pragma unroll
+                for (int i = 0; i < BS; ++i) y[i] += __shfl_xor(y[i], off, LANES);
+            if (lane == 0 && u[r] < r1) {  // y = (A W_{j-1})_u
+                double m[BS][BS], mi[BS][BS];
+#pragma unroll
+                for (int i = 0; i < BS; ++i)
+#pragma unroll
+                    for (int q = 0; q < BS; ++q) m[i][q] = D[r][i * BS + q];
+                block_inverse<BS>(m, mi);
+                double w[BS], x[BS];
+#pragma unroll
+                for (int i = 0; i < BS; ++i) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int q = 0; q < BS; ++q) t += mi[i][q] * y[q];
+                    t -= aprev * oW[r][i];
+                    if (j >= 1) t -= bprev * oWp[r][i];
+                    w[i] = t * inv;
+                    double xv = oW[r][i];
+                    if (j >= 1) xv -= aprev * o1[r][i];
+                    if (j >= 2) xv -= bprev * o2[r][i];
+                    x[i] = xv * inv;
+                }
+#pragma unroll
+                for (int i = 0; i < BS; ++i) {
+                    double z = 0.0;  // (A v_j)_i = (D W_j)_i
+#pragma unroll
+                    for (int q = 0; q < BS; ++q) z += m[i][q] * w[q];
+                    acc[0] += x[i] * z;
+                    acc[1] += w[i] * z;
+                }
+                store_vec<BS>(vj, u[r], x);
+                store_vec<BS>(Wj, u[r], w);
             }
         }
     }
@@ -296,7 +449,8 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(int j, int n, const int* __r
 template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock) k_lanczos_start(int n, const int* __restrict__ rowPtr,
                                                           const T* __restrict__ vals, const T* __restrict__ Dinv,
-                                                          double* W, double* partOut, double* gPart, double* coef)
+                                                          double* W, double* partOut, double* gPart, double* coef,
+                                                          int nsteps)
 {
     int r0, r1;
     chunk_rows(n, r0, r1);
@@ -349,7 +503,7 @@ __global__ void __launch_bounds__(kBlock) k_lanczos_start(int n, const int* __re
         for (int w = 1; w < kBlock / kWave; ++w) m = fmax(m, gsh[w]);
         gPart[blockIdx.x] = m;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) coef[2 * kLanczosSteps] = (double)kLanczosSteps;
+    if (blockIdx.x == 0 && threadIdx.x == 0) coef[2 * kLanczosSteps] = (double)nsteps;
 }
 
 // lam[1] = the Gershgorin bound; lam[0] = the largest eigenvalue of the
@@ -357,8 +511,8 @@ __global__ void __launch_bounds__(kBlock) k_lanczos_start(int n, const int* __re
 // Lanczos steps were run), the Gershgorin bound / 1.1 so that the smoother's
 // interval ends exactly at a guaranteed upper bound.
 __global__ void __launch_bounds__(kBlock) k_lanczos_finish(const double* __restrict__ partIn, int nPartIn,
-                                                           const double* __restrict__ gPart, int safe, double* coef,
-                                                           double* lam)
+                                                           const double* __restrict__ gPart, int safe, int nsteps,
+                                                           double* coef, double* lam)
 {
     __shared__ double red[kBlock];
     {
@@ -377,14 +531,14 @@ __global__ void __launch_bounds__(kBlock) k_lanczos_finish(const double* __restr
         if (safe) return;
     }
     const int m = (int)coef[2 * kLanczosSteps];
-    if (m == kLanczosSteps) {
+    if (m == nsteps) {
         double acc[2] = {0.0, 0.0};
         for (int b = threadIdx.x; b < nPartIn; b += blockDim.x) {
             acc[0] += partIn[2 * b];
             acc[1] += partIn[2 * b + 1];
         }
         block_sum<2>(acc);
-        if (threadIdx.x == 0) coef[kLanczosSteps - 1] = acc[0];
+        if (threadIdx.x == 0) coef[nsteps - 1] = acc[0];
         __syncthreads();
     }
     if (m == 0) {  // start vector in the null space of A: no information
@@ -958,28 +1112,29 @@ template <typename T>
 void Amg<T>::lanczos0_prepare(hipStream_t s)
 {
     Level& A = *L[0];
-    const int g = rows_grid(A.n, kBlock / Lanes<2>::value);
+    const int g = lanczos_grid(A.n, 2);
     double* part[2] = {dPartials, dPartials + nPartials};
     const size_t nv = (size_t)A.n * 2;
-    double* coef = A.lz + 5 * nv;
-    LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv}};
-    hipLaunchKernelGGL((k_lanczos_start<T, 2>), dim3(g), dim3(kBlock), 0, s, A.n, A.dRowPtr, A.vals, A.Dinv, lb.W[1],
-                       part[1], dPartials + 2 * (size_t)nPartials, coef);
+    double* coef = A.lz + 6 * nv;
+    LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv, A.lz + 5 * nv}};
+    hipLaunchKernelGGL((k_lanczos_start<T, 2>), dim3(g), dim3(kBlock), 0, s, A.n, A.dRowPtr, A.vals, A.Dinv, lb.W[0],
+                       part[1], dPartials + 2 * (size_t)nPartials, coef, kLanczosSteps);
     for (int j = 0; j < 2; ++j)
-        hipLaunchKernelGGL((k_lanczos<T, 2, Lanes<2>::value>), dim3(g), dim3(kBlock), 0, s, j, A.n, A.dRowPtr,
-                           A.dColIdx, A.vals, A.Dinv, lb, part[(j + 1) & 1], g, part[j & 1], coef);
+        hipLaunchKernelGGL((k_lanczos<T, 2>), dim3(g), dim3(kBlock), 0, s, j, kLanczosSteps, A.n, A.dRowPtr,
+                           A.dColIdx, A.vals, A.dDiagPos, lb, part[(j + 1) & 1], g, part[j & 1], coef);
 }
 
 template <typename T>
 void Amg<T>::lanczos0_step(hipStream_t s)
 {
-    // step 2 reads part[1] (written by step 1) and writes part[0]: repeatable
+    // step 2 reads part[1] (written by step 1), W[2], W[1], v[1], v[0] and
+    // writes part[0], v[2], W[0]: repeatable
     Level& A = *L[0];
-    const int g = rows_grid(A.n, kBlock / Lanes<2>::value);
+    const int g = lanczos_grid(A.n, 2);
     const size_t nv = (size_t)A.n * 2;
-    LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv}};
-    hipLaunchKernelGGL((k_lanczos<T, 2, Lanes<2>::value>), dim3(g), dim3(kBlock), 0, s, 2, A.n, A.dRowPtr, A.dColIdx,
-                       A.vals, A.Dinv, lb, dPartials + nPartials, g, dPartials, A.lz + 5 * nv);
+    LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv, A.lz + 5 * nv}};
+    hipLaunchKernelGGL((k_lanczos<T, 2>), dim3(g), dim3(kBlock), 0, s, 2, kLanczosSteps, A.n, A.dRowPtr, A.dColIdx,
+                       A.vals, A.dDiagPos, lb, dPartials + nPartials, g, dPartials, A.lz + 6 * nv);
 }
 
 template <typename T>
@@ -1016,7 +1171,7 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
         dalloc(A.d, (size_t)A.n * bs);
         dalloc(A.d2, (size_t)A.n * bs);
         dalloc(A.lam, 2);
-        dalloc(A.lz, (size_t)5 * A.n * bs + 2 * kLanczosSteps + 1);
+        dalloc(A.lz, (size_t)6 * A.n * bs + 2 * kLanczosSteps + 1);
         if (l + 1 < L.size()) {
             up(s, A.dAggPtr, A.aggPtr);
             up(s, A.dAggMem, A.aggMem);
@@ -1087,29 +1242,30 @@ void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double
                                A.Dinv);
         if (l + 1 == L.size()) break;
         // lambda_max(D^-1 A) by Lanczos
-        const int lanes = bs == 2 ? Lanes<2>::value : Lanes<3>::value;
-        const int g = rows_grid(A.n, kBlock / lanes);
+        const int g = lanczos_grid(A.n, bs);
         double* part[2] = {dPartials, dPartials + nPartials};
         const size_t nv = (size_t)A.n * bs;
-        double* coef = A.lz + 5 * nv;
-        LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv}};
+        double* coef = A.lz + 6 * nv;
+        LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv, A.lz + 5 * nv}};
         double* gPart = dPartials + 2 * (size_t)nPartials;
+        const int nsteps = lanczos_steps(l);
+        const int lsafe = safe || nsteps == 0;
         if (bs == 2)
             hipLaunchKernelGGL((k_lanczos_start<T, 2>), dim3(g), dim3(kBlock), 0, s, A.n, A.dRowPtr, A.vals, A.Dinv,
-                               lb.W[1], part[1], gPart, coef);
+                               lb.W[0], part[1], gPart, coef, nsteps);
         else
             hipLaunchKernelGGL((k_lanczos_start<T, 3>), dim3(g), dim3(kBlock), 0, s, A.n, A.dRowPtr, A.vals, A.Dinv,
-                               lb.W[1], part[1], gPart, coef);
-        for (int j = 0; j < (safe ? 0 : kLanczosSteps); ++j) {
+                               lb.W[0], part[1], gPart, coef, nsteps);
+        for (int j = 0; j < (lsafe ? 0 : nsteps); ++j) {
             if (bs == 2)
-                hipLaunchKernelGGL((k_lanczos<T, 2, Lanes<2>::value>), dim3(g), dim3(kBlock), 0, s, j, A.n,
-                                   A.dRowPtr, A.dColIdx, A.vals, A.Dinv, lb, part[(j + 1) & 1], g, part[j & 1], coef);
+                hipLaunchKernelGGL((k_lanczos<T, 2>), dim3(g), dim3(kBlock), 0, s, j, nsteps, A.n, A.dRowPtr,
+                                   A.dColIdx, A.vals, A.dDiagPos, lb, part[(j + 1) & 1], g, part[j & 1], coef);
             else
-                hipLaunchKernelGGL((k_lanczos<T, 3, Lanes<3>::value>), dim3(g), dim3(kBlock), 0, s, j, A.n,
-                                   A.dRowPtr, A.dColIdx, A.vals, A.Dinv, lb, part[(j + 1) & 1], g, part[j & 1], coef);
+                hipLaunchKernelGGL((k_lanczos<T, 3>), dim3(g), dim3(kBlock), 0, s, j, nsteps, A.n, A.dRowPtr,
+                                   A.dColIdx, A.vals, A.dDiagPos, lb, part[(j + 1) & 1], g, part[j & 1], coef);
         }
-        hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(kBlock), 0, s, part[(kLanczosSteps + 1) & 1], g, gPart,
-                           safe, coef, A.lam);
+        hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(kBlock), 0, s, part[(nsteps + 1) & 1], g, gPart, lsafe,
+                           nsteps, coef, A.lam);
         // tentative prolongator, smoothing, Galerkin product
         Level& N = *L[l + 1];
         const int nP = (int)A.pCol.size(), nT = (int)A.tCol.size(), nAc = (int)N.colIdx.size();
