@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <type_traits>
 
 namespace oc {
 
@@ -17,11 +18,6 @@ constexpr int kCoarseMaxDofs = 64;  // the coarsest inverse lives in LDS (64 x 6
 constexpr int kLanczosSteps = 15;       // Lanczos steps of the lambda_max estimate (fine level; array size)
 constexpr int kLanczosStepsCoarse = 10;  // on coarse levels
 constexpr int kMaxLevels = 10;
-
-template <int BS>
-struct Lanes {
-    static constexpr int value = BS == 2 ? 8 : 16;
-};
 
 __device__ __forceinline__ void chunk_rows(int n, int& r0, int& r1)
 {
@@ -59,18 +55,6 @@ __device__ __forceinline__ void row_mv(const int* __restrict__ rowPtr, const int
     for (int off = LANES / 2; off > 0; off >>= 1)
 #pragma unroll
         for (int i = 0; i < BS; ++i) y[i] += __shfl_xor(y[i], off, LANES);
-}
-
-template <typename T, int BS>
-__device__ __forceinline__ void blk_mv(const T* __restrict__ M, const T (&x)[BS], T (&y)[BS])
-{
-#pragma unroll
-    for (int i = 0; i < BS; ++i) {
-        T s = T(0);
-#pragma unroll
-        for (int j = 0; j < BS; ++j) s += M[i * BS + j] * x[j];
-        y[i] = s;
-    }
 }
 
 // Chebyshev parameters on [lmax/30, lmax], lmax = 1.1 * lambda estimate
@@ -194,88 +178,173 @@ struct LanczosBufs {
     double* W[3];  // W_j in W[(j + 1) % 3] (W_{-1} in W[0])
 };
 
-// lane-group shape of the Lanczos step: lanes per row, rows per lane group per pass
+// Lane-group shape of the row kernels (Lanczos step, Chebyshev steps,
+// residual): lanes per block row and rows per lane group per pass. 2x2 rows
+// hold about 7 blocks: 4 lanes and two rows in flight; 3x3 coarse rows hold
+// 10-30 blocks: 16 lanes, one row.
 template <int BS>
-struct LzShape;
+struct RowShape;
 template <>
-struct LzShape<2> {
+struct RowShape<2> {
     static constexpr int lanes = 4, unroll = 2;
 };
 template <>
-struct LzShape<3> {
+struct RowShape<3> {
     static constexpr int lanes = 16, unroll = 1;
 };
 
-// Workgroups of a Lanczos step: enough for one pass over the rows, at most
+// Workgroups of a row kernel: enough for one pass over the rows, at most
 // kAmgMaxGrid, a multiple of 8 (the XCD-contiguous chunk order below).
+int row_grid(int n, int bs)
+{
+    const int rows = bs == 2 ? (kBlock / RowShape<2>::lanes) * RowShape<2>::unroll
+                             : (kBlock / RowShape<3>::lanes) * RowShape<3>::unroll;
+    const int g = std::max(1, std::min((n + rows - 1) / rows, kAmgMaxGrid));
+    return (g + 7) & ~7;
+}
+
 // Lanczos steps per level: the fine level's top spectrum is clustered on large
 // meshes and needs all kLanczosSteps; coarse levels settle sooner. At 200k
 // faces 8 coarse steps lose nothing and 6 under-estimate often enough to force
 // a safe-mode retry; 10 keeps a margin.
 int lanczos_steps(int level) { return level == 0 ? kLanczosSteps : kLanczosStepsCoarse; }
 
-int lanczos_grid(int n, int bs)
-{
-    const int rows = bs == 2 ? (kBlock / LzShape<2>::lanes) * LzShape<2>::unroll
-                             : (kBlock / LzShape<3>::lanes) * LzShape<3>::unroll;
-    const int g = std::max(1, std::min((n + rows - 1) / rows, kAmgMaxGrid));
-    return (g + 7) & ~7;
-}
-
 // Workgroup b is dispatched to XCD b % 8; chunk c of the rows goes to
 // workgroup (c % (G/8)) * 8 + c / (G/8), so each XCD owns one contiguous
 // eighth of the rows and its L2 serves most neighbour gathers.
 __device__ __forceinline__ int xcd_chunk(int b, int G) { return (b & 7) * (G >> 3) + (b >> 3); }
 
-template <typename T, int BS>
-__device__ __forceinline__ void load_block(const T* __restrict__ p, double (&a)[BS * BS])
+// Block, vector-entry and store helpers: one 16-byte load per pair of doubles
+// or per 2x2 float block where the layout allows.
+template <int BS, typename T, typename R>
+__device__ __forceinline__ void ld_block(const T* __restrict__ p, R (&a)[BS * BS])
 {
+    if constexpr (BS == 2 && std::is_same<T, double>::value) {
+        const double2 x = reinterpret_cast<const double2*>(p)[0], y = reinterpret_cast<const double2*>(p)[1];
+        a[0] = (R)x.x;
+        a[1] = (R)x.y;
+        a[2] = (R)y.x;
+        a[3] = (R)y.y;
+    } else if constexpr (BS == 2 && std::is_same<T, float>::value) {
+        const float4 x = reinterpret_cast<const float4*>(p)[0];
+        a[0] = (R)x.x;
+        a[1] = (R)x.y;
+        a[2] = (R)x.z;
+        a[3] = (R)x.w;
+    } else {
 #pragma unroll
-    for (int i = 0; i < BS * BS; ++i) a[i] = (double)p[i];
-}
-template <>
-__device__ __forceinline__ void load_block<double, 2>(const double* __restrict__ p, double (&a)[4])
-{
-    const double2 x = reinterpret_cast<const double2*>(p)[0], y = reinterpret_cast<const double2*>(p)[1];
-    a[0] = x.x;
-    a[1] = x.y;
-    a[2] = y.x;
-    a[3] = y.y;
-}
-template <>
-__device__ __forceinline__ void load_block<float, 2>(const float* __restrict__ p, double (&a)[4])
-{
-    const float4 x = reinterpret_cast<const float4*>(p)[0];
-    a[0] = x.x;
-    a[1] = x.y;
-    a[2] = x.z;
-    a[3] = x.w;
+        for (int i = 0; i < BS * BS; ++i) a[i] = (R)p[i];
+    }
 }
 
-template <int BS>
-__device__ __forceinline__ void load_vec(const double* __restrict__ p, size_t u, double (&x)[BS])
+template <int BS, typename T, typename R>
+__device__ __forceinline__ void ld_vec(const T* __restrict__ p, size_t u, R (&x)[BS])
 {
+    if constexpr (BS == 2 && std::is_same<T, double>::value) {
+        const double2 t = reinterpret_cast<const double2*>(p)[u];
+        x[0] = (R)t.x;
+        x[1] = (R)t.y;
+    } else if constexpr (BS == 2 && std::is_same<T, float>::value) {
+        const float2 t = reinterpret_cast<const float2*>(p)[u];
+        x[0] = (R)t.x;
+        x[1] = (R)t.y;
+    } else {
 #pragma unroll
-    for (int i = 0; i < BS; ++i) x[i] = p[u * BS + i];
-}
-template <>
-__device__ __forceinline__ void load_vec<2>(const double* __restrict__ p, size_t u, double (&x)[2])
-{
-    const double2 t = reinterpret_cast<const double2*>(p)[u];
-    x[0] = t.x;
-    x[1] = t.y;
+        for (int i = 0; i < BS; ++i) x[i] = (R)p[u * BS + i];
+    }
 }
 
-template <int BS>
-__device__ __forceinline__ void store_vec(double* p, size_t u, const double (&x)[BS])
+template <int BS, typename T>
+__device__ __forceinline__ void st_vec(T* p, size_t u, const T (&x)[BS])
 {
+    if constexpr (BS == 2 && std::is_same<T, double>::value) {
+        reinterpret_cast<double2*>(p)[u] = make_double2(x[0], x[1]);
+    } else if constexpr (BS == 2 && std::is_same<T, float>::value) {
+        reinterpret_cast<float2*>(p)[u] = make_float2(x[0], x[1]);
+    } else {
 #pragma unroll
-    for (int i = 0; i < BS; ++i) p[u * BS + i] = x[i];
+        for (int i = 0; i < BS; ++i) p[u * BS + i] = x[i];
+    }
 }
-template <>
-__device__ __forceinline__ void store_vec<2>(double* p, size_t u, const double (&x)[2])
+
+// One pass of a row kernel: UNROLL block rows per LANES-lane group. load()
+// issues the row pointers, the first block of every lane (a row of at most
+// LANES blocks needs no other) and its gathered x entries, so the caller can
+// issue its own-row loads before any of them is consumed; mv() finishes
+// y = (A x)_u (remaining blocks, lane-group reduction; y in every lane).
+template <typename T, int BS, int UNROLL>
+struct RowPass {
+    static constexpr int LANES = RowShape<BS>::lanes;
+    int u[UNROLL], kb[UNROLL], ke[UNROLL];
+    T a[UNROLL][BS * BS], g[UNROLL][BS];
+
+    template <typename X>
+    __device__ __forceinline__ void load(int first, int stride, int r1, const int* __restrict__ rowPtr,
+                                         const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                         const X* __restrict__ x, int lane)
+    {
+#pragma unroll
+        for (int r = 0; r < UNROLL; ++r) {
+            u[r] = first + r * stride;
+            const bool valid = u[r] < r1;
+            kb[r] = valid ? rowPtr[u[r]] : 0;
+            ke[r] = valid ? rowPtr[u[r] + 1] : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < UNROLL; ++r) {
+            const int k = kb[r] + lane;
+            const bool has = k < ke[r];
+            const int kk = has ? k : 0;
+            ld_block<BS>(vals + (size_t)kk * BS * BS, a[r]);
+            ld_vec<BS>(x, has ? colIdx[kk] : 0, g[r]);
+            if (!has)
+#pragma unroll
+                for (int i = 0; i < BS * BS; ++i) a[r][i] = T(0);
+        }
+    }
+
+    template <typename X, typename Y>
+    __device__ __forceinline__ void mv(int r, const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                       const X* __restrict__ x, int lane, Y (&y)[BS]) const
+    {
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            y[i] = Y(0);
+#pragma unroll
+            for (int q = 0; q < BS; ++q) y[i] += (Y)a[r][i * BS + q] * (Y)g[r][q];
+        }
+        for (int k = kb[r] + lane + LANES; k < ke[r]; k += LANES) {
+            T ak[BS * BS];
+            Y gk[BS];
+            ld_block<BS>(vals + (size_t)k * BS * BS, ak);
+            ld_vec<BS>(x, colIdx[k], gk);
+#pragma unroll
+            for (int i = 0; i < BS; ++i)
+#pragma unroll
+                for (int q = 0; q < BS; ++q) y[i] += (Y)ak[i * BS + q] * gk[q];
+        }
+#pragma unroll
+        for (int off = LANES / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) y[i] += __shfl_xor(y[i], off, LANES);
+    }
+};
+
+// Rows of this workgroup's chunk and the pass count (block-uniform).
+struct RowChunk {
+    int r0, r1, passes;
+};
+template <int BS>
+__device__ __forceinline__ RowChunk row_chunk(int n)
 {
-    reinterpret_cast<double2*>(p)[u] = make_double2(x[0], x[1]);
+    constexpr int RPP = kBlock / RowShape<BS>::lanes, UNROLL = RowShape<BS>::unroll;
+    const int G = gridDim.x;
+    const int c = (n + G - 1) / G;
+    RowChunk rc;
+    rc.r0 = xcd_chunk(blockIdx.x, G) * c;
+    rc.r1 = min(n, rc.r0 + c);
+    rc.passes = max(1, (c + RPP * UNROLL - 1) / (RPP * UNROLL));
+    return rc;
 }
 
 __device__ __forceinline__ void lanczos_coeffs(int j, const double* __restrict__ partIn, int nPartIn,
@@ -301,7 +370,7 @@ __device__ __forceinline__ void lanczos_coeffs(int j, const double* __restrict__
     bj = (gamma > 0.0 && b2 > 1e-10 * gamma) ? sqrt(b2) : 0.0;
 }
 
-// Launch with gridDim.x a multiple of 8 (lanczos_grid).
+// Launch with gridDim.x a multiple of 8 (row_grid).
 template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock) k_lanczos(int j, int nsteps, int n, const int* __restrict__ rowPtr,
                                                     const int* __restrict__ colIdx, const T* __restrict__ vals,
@@ -309,7 +378,7 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(int j, int nsteps, int n, co
                                                     const double* __restrict__ partIn, int nPartIn, double* partOut,
                                                     double* coef)
 {
-    constexpr int LANES = LzShape<BS>::lanes, UNROLL = LzShape<BS>::unroll, RPP = kBlock / LANES;
+    constexpr int LANES = RowShape<BS>::lanes, UNROLL = RowShape<BS>::unroll, RPP = kBlock / LANES;
     if (j > 0 && coef[2 * kLanczosSteps] < (double)nsteps) return;  // stopped earlier
     const double* __restrict__ Wp = buf.W[j % 3];
     const double* __restrict__ Wpp = buf.W[(j + 2) % 3];
@@ -341,8 +410,8 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(int j, int nsteps, int n, co
             const int k = kb[r] + lane;
             const bool has = k < ke[r];
             const int kk = has ? k : 0;
-            load_block<T, BS>(vals + (size_t)kk * BS * BS, a[r]);
-            load_vec<BS>(Wp, has ? colIdx[kk] : 0, g[r]);
+            ld_block<BS>(vals + (size_t)kk * BS * BS, a[r]);
+            ld_vec<BS>(Wp, has ? colIdx[kk] : 0, g[r]);
             if (!has)
 #pragma unroll
                 for (int i = 0; i < BS * BS; ++i) a[r][i] = 0.0;
@@ -351,13 +420,13 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(int j, int nsteps, int n, co
 #pragma unroll
             for (int i = 0; i < BS * BS; ++i) D[r][i] = 0.0;
             if (lane == 0 && u[r] < r1) {
-                load_vec<BS>(Wp, u[r], oW[r]);
+                ld_vec<BS>(Wp, u[r], oW[r]);
                 if (j >= 1) {
-                    load_vec<BS>(Wpp, u[r], oWp[r]);
-                    load_vec<BS>(v1, u[r], o1[r]);
+                    ld_vec<BS>(Wpp, u[r], oWp[r]);
+                    ld_vec<BS>(v1, u[r], o1[r]);
                 }
-                if (j >= 2) load_vec<BS>(v2, u[r], o2[r]);
-                load_block<T, BS>(vals + (size_t)diagPos[u[r]] * BS * BS, D[r]);
+                if (j >= 2) ld_vec<BS>(v2, u[r], o2[r]);
+                ld_block<BS>(vals + (size_t)diagPos[u[r]] * BS * BS, D[r]);
             }
         }
         if (p == 0) {
@@ -387,8 +456,8 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(int j, int nsteps, int n, co
             }
             for (int k = kb[r] + lane + LANES; k < ke[r]; k += LANES) {
                 double ak[BS * BS], gk[BS];
-                load_block<T, BS>(vals + (size_t)k * BS * BS, ak);
-                load_vec<BS>(Wp, colIdx[k], gk);
+                ld_block<BS>(vals + (size_t)k * BS * BS, ak);
+                ld_vec<BS>(Wp, colIdx[k], gk);
 #pragma unroll
                 for (int i = 0; i < BS; ++i)
 #pragma unroll
@@ -427,8 +496,8 @@ __global__ void __launch_bounds__(kBlock) k_lanczos(int j, int nsteps, int n, co
                     acc[0] += x[i] * z;
                     acc[1] += w[i] * z;
                 }
-                store_vec<BS>(vj, u[r], x);
-                store_vec<BS>(Wj, u[r], w);
+                st_vec<BS>(vj, u[r], x);
+                st_vec<BS>(Wj, u[r], w);
             }
         }
     }
@@ -821,6 +890,10 @@ __global__ void __launch_bounds__(256) k_dense_inverse(int nn, int bs, const int
 }
 
 // ---------------------------------------------------------------------------- V-cycle kernels
+// V-cycle kernels store vectors and matrices in T and compute in FP64: they are
+// bound by memory, and FP64 arithmetic keeps the FP32-stored V-cycle as close
+// to a fixed symmetric operator as its storage allows.
+//
 // pre-smoothing, first Chebyshev step from x = 0: r = D^-1 b, d = r/theta, x = d
 template <typename T, int BS, typename BIN>
 __global__ void __launch_bounds__(kBlock) k_cheb_pre1(const int* __restrict__ stop, int n, const BIN* __restrict__ bin, T* b,
@@ -832,27 +905,35 @@ __global__ void __launch_bounds__(kBlock) k_cheb_pre1(const int* __restrict__ st
     int r0, r1;
     chunk_rows(n, r0, r1);
     for (int u = r0 + threadIdx.x; u < r1; u += blockDim.x) {
-        T bu[BS], ru[BS];
+        double bu[BS], Dm[BS * BS];
+        ld_vec<BS>(bin, u, bu);
+        ld_block<BS>(Dinv + (size_t)u * BS * BS, Dm);
+        if ((const void*)bin != (const void*)b) {
+            T bt[BS];
 #pragma unroll
-        for (int i = 0; i < BS; ++i) bu[i] = (T)bin[(size_t)u * BS + i];
-        if ((const void*)bin != (const void*)b)
-#pragma unroll
-            for (int i = 0; i < BS; ++i) b[(size_t)u * BS + i] = bu[i];
-        blk_mv<T, BS>(Dinv + (size_t)u * BS * BS, bu, ru);
+            for (int i = 0; i < BS; ++i) bt[i] = (T)bu[i];
+            st_vec<BS>(b, u, bt);
+        }
+        T rt[BS], dt[BS];
 #pragma unroll
         for (int i = 0; i < BS; ++i) {
-            const T di = ru[i] / (T)cp.theta;
-            r[(size_t)u * BS + i] = ru[i];
-            d[(size_t)u * BS + i] = di;
-            x[(size_t)u * BS + i] = di;
+            double t = 0.0;
+#pragma unroll
+            for (int j = 0; j < BS; ++j) t += Dm[i * BS + j] * bu[j];
+            rt[i] = (T)t;
+            dt[i] = (T)(t / cp.theta);
         }
+        st_vec<BS>(r, u, rt);
+        st_vec<BS>(d, u, dt);
+        st_vec<BS>(x, u, dt);
     }
 }
 
 // Chebyshev step k (k = 0 after pre1/post1): w = A d; r -= D^-1 w;
 // d' = rho1 rho0 d + 2 rho1/delta r; x += d'. d is read for the neighbours,
-// so the new d goes to dout.
-template <typename T, int BS, int LANES, bool FINAL>
+// so the new d goes to dout. FINAL: z = x (FP64) and partial records of
+// rFinal.z per workgroup. Launch with row_grid workgroups.
+template <typename T, int BS, bool FINAL>
 __global__ void __launch_bounds__(kBlock) k_cheb_step(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
                                                       const int* __restrict__ colIdx, const T* __restrict__ vals,
                                                       const T* __restrict__ Dinv, const double* __restrict__ lam,
@@ -861,34 +942,53 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(const int* __restrict__ st
                                                       double* rzPart)
 {
     if (stop && *stop) return;
+    constexpr int LANES = RowShape<BS>::lanes, UNROLL = RowShape<BS>::unroll, RPP = kBlock / LANES;
     const ChebP cp = cheb_params(*lam);
     const double rho0 = 1.0 / cp.sigma;
     const double rho1 = 1.0 / (2.0 * cp.sigma - rho0);
-    const T c1 = (T)(rho1 * rho0), c2 = (T)(2.0 * rho1 / cp.delta);
-    int r0, r1;
-    chunk_rows(n, r0, r1);
+    const double c1 = rho1 * rho0, c2 = 2.0 * rho1 / cp.delta;
+    const RowChunk rc = row_chunk<BS>(n);
     const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-    constexpr int RPP = kBlock / LANES;
     double acc[1] = {0.0};
-    for (int u = r0 + sub; u < r1; u += RPP) {
-        T w[BS];
-        row_mv<T, BS, LANES>(rowPtr, colIdx, vals, din, u, lane, w);
-        if (lane == 0) {
-            T dw[BS];
-            blk_mv<T, BS>(Dinv + (size_t)u * BS * BS, w, dw);
+    for (int p = 0; p < rc.passes; ++p) {
+        RowPass<T, BS, UNROLL> rp;
+        rp.load(rc.r0 + sub + p * UNROLL * RPP, RPP, rc.r1, rowPtr, colIdx, vals, din, lane);
+        double Dm[UNROLL][BS * BS], ro[UNROLL][BS], dd[UNROLL][BS], xo[UNROLL][BS], rf[UNROLL][BS];
 #pragma unroll
-            for (int i = 0; i < BS; ++i) {
-                const size_t o = (size_t)u * BS + i;
-                const T rn = r[o] - dw[i];
-                const T dn = c1 * din[o] + c2 * rn;
-                const T xn = x[o] + dn;
-                r[o] = rn;
-                dout[o] = dn;
-                x[o] = xn;
+        for (int q = 0; q < UNROLL; ++q)
+            if (lane == 0 && rp.u[q] < rc.r1) {
+                ld_block<BS>(Dinv + (size_t)rp.u[q] * BS * BS, Dm[q]);
+                ld_vec<BS>(r, rp.u[q], ro[q]);
+                ld_vec<BS>(din, rp.u[q], dd[q]);
+                ld_vec<BS>(x, rp.u[q], xo[q]);
+                if (FINAL) ld_vec<BS>(rFinal, rp.u[q], rf[q]);
+            }
+#pragma unroll
+        for (int q = 0; q < UNROLL; ++q) {
+            double w[BS];
+            rp.mv(q, colIdx, vals, din, lane, w);
+            if (lane == 0 && rp.u[q] < rc.r1) {
+                double xn[BS];
+                T rt[BS], dt[BS], xt[BS];
+#pragma unroll
+                for (int i = 0; i < BS; ++i) {
+                    double dw = 0.0;
+#pragma unroll
+                    for (int j = 0; j < BS; ++j) dw += Dm[q][i * BS + j] * w[j];
+                    const double rn = ro[q][i] - dw;
+                    const double dn = c1 * dd[q][i] + c2 * rn;
+                    xn[i] = xo[q][i] + dn;
+                    rt[i] = (T)rn;
+                    dt[i] = (T)dn;
+                    xt[i] = (T)xn[i];
+                }
+                st_vec<BS>(r, rp.u[q], rt);
+                st_vec<BS>(dout, rp.u[q], dt);
+                st_vec<BS>(x, rp.u[q], xt);
                 if (FINAL) {
-                    const double z = (double)xn;
-                    zFinal[o] = z;
-                    acc[0] += rFinal[o] * z;
+#pragma unroll
+                    for (int i = 0; i < BS; ++i) acc[0] += rf[q][i] * xn[i];
+                    st_vec<BS>(zFinal, rp.u[q], xn);
                 }
             }
         }
@@ -899,28 +999,40 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(const int* __restrict__ st
     }
 }
 
-// residual res = b - A x
-template <typename T, int BS, int LANES>
+// residual res = b - A x (row_grid workgroups)
+template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock) k_residual(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
                                                      const int* __restrict__ colIdx, const T* __restrict__ vals,
                                                      const T* __restrict__ b, const T* __restrict__ x, T* res)
 {
     if (stop && *stop) return;
-    int r0, r1;
-    chunk_rows(n, r0, r1);
+    constexpr int LANES = RowShape<BS>::lanes, UNROLL = RowShape<BS>::unroll, RPP = kBlock / LANES;
+    const RowChunk rc = row_chunk<BS>(n);
     const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-    constexpr int RPP = kBlock / LANES;
-    for (int u = r0 + sub; u < r1; u += RPP) {
-        T y[BS];
-        row_mv<T, BS, LANES>(rowPtr, colIdx, vals, x, u, lane, y);
-        if (lane == 0)
+    for (int p = 0; p < rc.passes; ++p) {
+        RowPass<T, BS, UNROLL> rp;
+        rp.load(rc.r0 + sub + p * UNROLL * RPP, RPP, rc.r1, rowPtr, colIdx, vals, x, lane);
+        double bo[UNROLL][BS];
 #pragma unroll
-            for (int i = 0; i < BS; ++i) res[(size_t)u * BS + i] = b[(size_t)u * BS + i] - y[i];
+        for (int q = 0; q < UNROLL; ++q)
+            if (lane == 0 && rp.u[q] < rc.r1) ld_vec<BS>(b, rp.u[q], bo[q]);
+#pragma unroll
+        for (int q = 0; q < UNROLL; ++q) {
+            double y[BS];
+            rp.mv(q, colIdx, vals, x, lane, y);
+            if (lane == 0 && rp.u[q] < rc.r1) {
+                T o[BS];
+#pragma unroll
+                for (int i = 0; i < BS; ++i) o[i] = (T)(bo[q][i] - y[i]);
+                st_vec<BS>(res, rp.u[q], o);
+            }
+        }
     }
 }
 
-// post-smoothing first step from the prolongated x: r = D^-1 (b - A x); d = r/theta; xout = x + d
-template <typename T, int BS, int LANES>
+// post-smoothing first step from the prolongated x: r = D^-1 (b - A x); d = r/theta;
+// xout = x + d (row_grid workgroups)
+template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock) k_cheb_post1(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
                                                        const int* __restrict__ colIdx, const T* __restrict__ vals,
                                                        const T* __restrict__ Dinv, const double* __restrict__ lam,
@@ -928,26 +1040,43 @@ __global__ void __launch_bounds__(kBlock) k_cheb_post1(const int* __restrict__ s
                                                        T* xout, T* r, T* d)
 {
     if (stop && *stop) return;
+    constexpr int LANES = RowShape<BS>::lanes, UNROLL = RowShape<BS>::unroll, RPP = kBlock / LANES;
     const ChebP cp = cheb_params(*lam);
-    int r0, r1;
-    chunk_rows(n, r0, r1);
+    const RowChunk rc = row_chunk<BS>(n);
     const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-    constexpr int RPP = kBlock / LANES;
-    for (int u = r0 + sub; u < r1; u += RPP) {
-        T y[BS];
-        row_mv<T, BS, LANES>(rowPtr, colIdx, vals, xin, u, lane, y);
-        if (lane == 0) {
-            T res[BS], ru[BS];
+    for (int p = 0; p < rc.passes; ++p) {
+        RowPass<T, BS, UNROLL> rp;
+        rp.load(rc.r0 + sub + p * UNROLL * RPP, RPP, rc.r1, rowPtr, colIdx, vals, xin, lane);
+        double Dm[UNROLL][BS * BS], bo[UNROLL][BS], xo[UNROLL][BS];
 #pragma unroll
-            for (int i = 0; i < BS; ++i) res[i] = b[(size_t)u * BS + i] - y[i];
-            blk_mv<T, BS>(Dinv + (size_t)u * BS * BS, res, ru);
+        for (int q = 0; q < UNROLL; ++q)
+            if (lane == 0 && rp.u[q] < rc.r1) {
+                ld_block<BS>(Dinv + (size_t)rp.u[q] * BS * BS, Dm[q]);
+                ld_vec<BS>(b, rp.u[q], bo[q]);
+                ld_vec<BS>(xin, rp.u[q], xo[q]);
+            }
 #pragma unroll
-            for (int i = 0; i < BS; ++i) {
-                const size_t o = (size_t)u * BS + i;
-                const T di = ru[i] / (T)cp.theta;
-                r[o] = ru[i];
-                d[o] = di;
-                xout[o] = xin[o] + di;
+        for (int q = 0; q < UNROLL; ++q) {
+            double y[BS];
+            rp.mv(q, colIdx, vals, xin, lane, y);
+            if (lane == 0 && rp.u[q] < rc.r1) {
+                double res[BS];
+                T ru[BS], dv[BS], xv[BS];
+#pragma unroll
+                for (int i = 0; i < BS; ++i) res[i] = bo[q][i] - y[i];
+#pragma unroll
+                for (int i = 0; i < BS; ++i) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int j = 0; j < BS; ++j) t += Dm[q][i * BS + j] * res[j];
+                    const double dvi = t / cp.theta;
+                    ru[i] = (T)t;
+                    dv[i] = (T)dvi;
+                    xv[i] = (T)(xo[q][i] + dvi);
+                }
+                st_vec<BS>(r, rp.u[q], ru);
+                st_vec<BS>(d, rp.u[q], dv);
+                st_vec<BS>(xout, rp.u[q], xv);
             }
         }
     }
@@ -966,7 +1095,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ sto
     constexpr int LN = kRestrictLanes, RPP = kBlock / kRestrictLanes;
     const int sub = threadIdx.x / LN, lane = threadIdx.x % LN;
     for (int c = c0 + sub; c < c1; c += RPP) {
-        T s[NS] = {};
+        double s[NS] = {};
         for (int q = rPtr[c] + lane; q < rPtr[c + 1]; q += LN) {
             const int j = rP[q];
             const T* p = P + (size_t)j * BS * NS;
@@ -974,7 +1103,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ sto
 #pragma unroll
             for (int l = 0; l < BS; ++l)
 #pragma unroll
-                for (int i = 0; i < NS; ++i) s[i] += p[l * NS + i] * ru[l];
+                for (int i = 0; i < NS; ++i) s[i] += (double)p[l * NS + i] * (double)ru[l];
         }
 #pragma unroll
         for (int off = LN / 2; off > 0; off >>= 1)
@@ -982,7 +1111,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ sto
             for (int i = 0; i < NS; ++i) s[i] += __shfl_xor(s[i], off, LN);
         if (lane == 0)
 #pragma unroll
-            for (int i = 0; i < NS; ++i) bc[(size_t)c * NS + i] = s[i];
+            for (int i = 0; i < NS; ++i) bc[(size_t)c * NS + i] = (T)s[i];
     }
 }
 
@@ -994,17 +1123,17 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const int* __restrict__ stop
     if (stop && *stop) return;
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
-    T s[BS] = {};
+    double s[BS] = {};
     for (int j = pPtr[u]; j < pPtr[u + 1]; ++j) {
         const T* p = P + (size_t)j * BS * NS;
         const T* xcv = xc + (size_t)pCol[j] * NS;
 #pragma unroll
         for (int i = 0; i < BS; ++i)
 #pragma unroll
-            for (int c = 0; c < NS; ++c) s[i] += p[i * NS + c] * xcv[c];
+            for (int c = 0; c < NS; ++c) s[i] += (double)p[i * NS + c] * (double)xcv[c];
     }
 #pragma unroll
-    for (int i = 0; i < BS; ++i) x[(size_t)u * BS + i] += s[i];
+    for (int i = 0; i < BS; ++i) x[(size_t)u * BS + i] = (T)((double)x[(size_t)u * BS + i] + s[i]);
 }
 
 template <typename T>
@@ -1099,7 +1228,7 @@ void Amg<T>::release()
 template <typename T>
 int Amg<T>::rz_records() const
 {
-    return rows_grid(L.empty() ? 1 : L[0]->n, kBlock / Lanes<2>::value);
+    return row_grid(L.empty() ? 1 : L[0]->n, 2);
 }
 
 template <typename T>
@@ -1112,7 +1241,7 @@ template <typename T>
 void Amg<T>::lanczos0_prepare(hipStream_t s)
 {
     Level& A = *L[0];
-    const int g = lanczos_grid(A.n, 2);
+    const int g = row_grid(A.n, 2);
     double* part[2] = {dPartials, dPartials + nPartials};
     const size_t nv = (size_t)A.n * 2;
     double* coef = A.lz + 6 * nv;
@@ -1130,7 +1259,7 @@ void Amg<T>::lanczos0_step(hipStream_t s)
     // step 2 reads part[1] (written by step 1), W[2], W[1], v[1], v[0] and
     // writes part[0], v[2], W[0]: repeatable
     Level& A = *L[0];
-    const int g = lanczos_grid(A.n, 2);
+    const int g = row_grid(A.n, 2);
     const size_t nv = (size_t)A.n * 2;
     LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv, A.lz + 5 * nv}};
     hipLaunchKernelGGL((k_lanczos<T, 2>), dim3(g), dim3(kBlock), 0, s, 2, kLanczosSteps, A.n, A.dRowPtr, A.dColIdx,
@@ -1242,7 +1371,7 @@ void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double
                                A.Dinv);
         if (l + 1 == L.size()) break;
         // lambda_max(D^-1 A) by Lanczos
-        const int g = lanczos_grid(A.n, bs);
+        const int g = row_grid(A.n, bs);
         double* part[2] = {dPartials, dPartials + nPartials};
         const size_t nv = (size_t)A.n * bs;
         double* coef = A.lz + 6 * nv;
@@ -1337,39 +1466,37 @@ void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
     Level& N = *L[l + 1];
     const int gUp = rows_grid(A.n, kBlock);
     if (A.bs == 2) {
-        constexpr int LN = Lanes<2>::value;
-        const int g = rows_grid(A.n, kBlock / LN);
+        const int g = row_grid(A.n, 2);
         if (l > 0)
             hipLaunchKernelGGL((k_cheb_pre1<T, 2, T>), dim3(gUp), dim3(kBlock), 0, s, stop, A.n, A.b, A.b, A.Dinv, A.lam,
                                A.x, A.r, A.d);
-        hipLaunchKernelGGL((k_cheb_step<T, 2, LN, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
+        hipLaunchKernelGGL((k_cheb_step<T, 2, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
                            A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x, nullptr, nullptr, nullptr);
-        hipLaunchKernelGGL((k_residual<T, 2, LN>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+        hipLaunchKernelGGL((k_residual<T, 2>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
                            A.b, A.x, A.r);
         hipLaunchKernelGGL((k_restrict<T, 2>), dim3(rows_grid(A.nc, kBlock / kRestrictLanes)), dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP,
                            A.dPRow, A.P, A.r, N.b);
         vcycle(s, l + 1, stop);
         hipLaunchKernelGGL((k_prolong<T, 2>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
                            N.x, A.x);
-        hipLaunchKernelGGL((k_cheb_post1<T, 2, LN>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+        hipLaunchKernelGGL((k_cheb_post1<T, 2>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
                            A.Dinv, A.lam, A.b, A.x, A.x2, A.r, A.d);
     } else {
-        constexpr int LN = Lanes<3>::value;
-        const int g = rows_grid(A.n, kBlock / LN);
+        const int g = row_grid(A.n, 3);
         hipLaunchKernelGGL((k_cheb_pre1<T, 3, T>), dim3(gUp), dim3(kBlock), 0, s, stop, A.n, A.b, A.b, A.Dinv, A.lam, A.x,
                            A.r, A.d);
-        hipLaunchKernelGGL((k_cheb_step<T, 3, LN, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
+        hipLaunchKernelGGL((k_cheb_step<T, 3, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
                            A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x, nullptr, nullptr, nullptr);
-        hipLaunchKernelGGL((k_residual<T, 3, LN>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+        hipLaunchKernelGGL((k_residual<T, 3>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
                            A.b, A.x, A.r);
         hipLaunchKernelGGL((k_restrict<T, 3>), dim3(rows_grid(A.nc, kBlock / kRestrictLanes)), dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP,
                            A.dPRow, A.P, A.r, N.b);
         vcycle(s, l + 1, stop);
         hipLaunchKernelGGL((k_prolong<T, 3>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
                            N.x, A.x);
-        hipLaunchKernelGGL((k_cheb_post1<T, 3, LN>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+        hipLaunchKernelGGL((k_cheb_post1<T, 3>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
                            A.Dinv, A.lam, A.b, A.x, A.x2, A.r, A.d);
-        hipLaunchKernelGGL((k_cheb_step<T, 3, LN, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
+        hipLaunchKernelGGL((k_cheb_step<T, 3, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
                            A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x2, nullptr, nullptr, nullptr);
         std::swap(A.x, A.x2);
     }
@@ -1379,8 +1506,7 @@ template <typename T>
 void Amg<T>::apply(hipStream_t s, const double* r, double* z, double* rzPart, const int* stop)
 {
     Level& A = *L[0];
-    constexpr int LN = Lanes<2>::value;
-    const int g = rows_grid(A.n, kBlock / LN);
+    const int g = row_grid(A.n, 2);
     if (L.size() == 1) {
         // single level: the dense inverse is the whole preconditioner
         hipLaunchKernelGGL((k_cvt<double, T>), dim3(grid_for((long)A.n * 2)), dim3(kBlock), 0, s, (long)A.n * 2, r, A.b);
@@ -1393,7 +1519,7 @@ void Amg<T>::apply(hipStream_t s, const double* r, double* z, double* rzPart, co
                        A.Dinv, A.lam, A.x, A.r, A.d);
     vcycle(s, 0, stop);
     // final post-smoothing step writes z (FP64) and the r.z partial records
-    hipLaunchKernelGGL((k_cheb_step<T, 2, LN, true>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
+    hipLaunchKernelGGL((k_cheb_step<T, 2, true>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
                        A.Dinv, A.lam, A.d, A.d2, A.r, A.x2, r, z, rzPart);
 }
 
