@@ -639,10 +639,13 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init(int nV, const double2* __re
 }
 
 int pcg_grid_update(int nV) { return std::max(1, std::min((nV + kBlock - 1) / kBlock, kPcgMaxGrid)); }
+// Workgroups of the PCG product: one pass of kPcgRows rows per kPcgLanes-lane
+// group, at most kPcgMaxGrid, a multiple of 8 (XCD-contiguous chunk order).
 int pcg_grid_spmv(int nV)
 {
-    constexpr int rowsPerPass = kBlock / kSpmvLanes;
-    return std::max(1, std::min((nV + rowsPerPass - 1) / rowsPerPass, kPcgMaxGrid));
+    constexpr int rows = (kBlock / kPcgLanes) * kPcgRows;
+    const int g = std::max(1, std::min((nV + rows - 1) / rows, kPcgMaxGrid));
+    return (g + 7) & ~7;
 }
 
 void launch_pcg_init(hipStream_t s, int nV, const double* b, const double* Minv, double* x, double* r, double* z,
@@ -688,6 +691,19 @@ __device__ __forceinline__ double2 row_apply(const int* __restrict__ rowPtr, con
     return make_double2(yx, yy);
 }
 
+__device__ __forceinline__ double4 to_double4(const double4& v) { return v; }
+__device__ __forceinline__ double4 to_double4(const float4& v)
+{
+    return make_double4((double)v.x, (double)v.y, (double)v.z, (double)v.w);
+}
+
+// PCG direction and product: p_new = z + beta p_old, q = A p_new, partial
+// records of p_new.q. beta needs the previous update's reductions, so the
+// row loads (blocks, gathered z and p_old) are issued first and the records
+// reduced while they are in flight; p_w = z_w + beta p_old_w is formed per
+// gathered entry. kPcgRows rows per kPcgLanes-lane group; workgroup b takes
+// row chunk (b % 8) * (G / 8) + b / 8 so each XCD owns a contiguous eighth of
+// the rows (G = pcg_grid_spmv, a multiple of 8).
 template <typename V4>
 __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* __restrict__ rowPtr,
                                                      const int* __restrict__ colIdx,
@@ -699,40 +715,91 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* 
                                                      double* partS, PcgCtrl* ctrl)
 {
     if (ctrl->done) return;
-    const double rz = reduce_strided(partRz, nRz, sRz);
-    const double rr = reduce_strided(partRr, nRr, sRr);
-    const double bb = (it == 0) ? rr : ctrl->bb;
-    const bool conv = (bb == 0.0) || (rr <= ctrl->tol2 * bb);
-    const bool capped = it >= ctrl->maxit;
-    const bool indef = !conv && !(rz > 0.0);  // preconditioner not positive definite on r
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (it == 0) ctrl->bb = bb;
-        ctrl->rzSlot[it & 1] = rz;
-        ctrl->rr = rr;
-        ctrl->iter = it;
-        if (conv) ctrl->done = 1;
-        else if (indef) ctrl->done = 3;
-        else if (capped) ctrl->done = 2;
-    }
-    if (conv || capped || indef) return;
-    const double beta = (it == 0) ? 0.0 : rz / ctrl->rzSlot[(it + 1) & 1];
-    int r0, r1;
-    row_chunk(nV, r0, r1);
-    const int sub = threadIdx.x / kSpmvLanes, lane = threadIdx.x % kSpmvLanes;
-    constexpr int rowsPerPass = kBlock / kSpmvLanes;
+    constexpr int LANES = kPcgLanes, UNROLL = kPcgRows, RPP = kBlock / LANES;
+    const int G = gridDim.x;
+    const int c = (nV + G - 1) / G;
+    const int r0 = ((int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3)) * c, r1 = min(nV, r0 + c);
+    const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    const int passes = max(1, (c + RPP * UNROLL - 1) / (RPP * UNROLL));
+    const bool first = it == 0;
+    double beta = 0.0;
     double acc[1] = {0.0};
-    for (int v = r0 + sub; v < r1; v += rowsPerPass) {
-        const double2 y = row_apply(rowPtr, colIdx, vals, z, it == 0 ? nullptr : pOld, beta, v, lane);
-        if (lane == 0) {
-            const double2 zv = z[v];
-            double2 pn = zv;
-            if (it != 0) {
-                const double2 pv = pOld[v];
-                pn = make_double2(zv.x + beta * pv.x, zv.y + beta * pv.y);
+    for (int pass = 0; pass < passes; ++pass) {
+        int u[UNROLL], kb[UNROLL], ke[UNROLL];
+        double4 a[UNROLL];
+        double2 gz[UNROLL], gp[UNROLL], oz[UNROLL], op[UNROLL];
+#pragma unroll
+        for (int r = 0; r < UNROLL; ++r) {
+            u[r] = r0 + sub + (pass * UNROLL + r) * RPP;
+            const bool valid = u[r] < r1;
+            kb[r] = valid ? rowPtr[u[r]] : 0;
+            ke[r] = valid ? rowPtr[u[r] + 1] : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < UNROLL; ++r) {
+            const int k = kb[r] + lane;
+            const bool has = k < ke[r];
+            const int kk = has ? k : 0;
+            const double4 av = to_double4(vals[kk]);
+            a[r] = has ? av : make_double4(0.0, 0.0, 0.0, 0.0);
+            const int w = has ? colIdx[kk] : 0;
+            gz[r] = z[w];
+            gp[r] = first ? make_double2(0.0, 0.0) : pOld[w];
+            oz[r] = op[r] = make_double2(0.0, 0.0);
+            if (lane == 0 && u[r] < r1) {
+                oz[r] = z[u[r]];
+                if (!first) op[r] = pOld[u[r]];
             }
-            pNew[v] = pn;
-            q[v] = y;
-            acc[0] += pn.x * y.x + pn.y * y.y;
+        }
+        if (pass == 0) {
+            const double rz = reduce_strided(partRz, nRz, sRz);
+            const double rr = reduce_strided(partRr, nRr, sRr);
+            const double bb = first ? rr : ctrl->bb;
+            const bool conv = (bb == 0.0) || (rr <= ctrl->tol2 * bb);
+            const bool capped = it >= ctrl->maxit;
+            const bool indef = !conv && !(rz > 0.0);  // preconditioner not positive definite on r
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                if (first) ctrl->bb = bb;
+                ctrl->rzSlot[it & 1] = rz;
+                ctrl->rr = rr;
+                ctrl->iter = it;
+                if (conv) ctrl->done = 1;
+                else if (indef) ctrl->done = 3;
+                else if (capped) ctrl->done = 2;
+            }
+            if (conv || capped || indef) return;
+            beta = first ? 0.0 : rz / ctrl->rzSlot[(it + 1) & 1];
+        }
+#pragma unroll
+        for (int r = 0; r < UNROLL; ++r) {
+            double px = gz[r].x + beta * gp[r].x, py = gz[r].y + beta * gp[r].y;
+            double yx = a[r].x * px + a[r].y * py;
+            double yy = a[r].z * px + a[r].w * py;
+            for (int k = kb[r] + lane + LANES; k < ke[r]; k += LANES) {
+                const int w = colIdx[k];
+                const double4 ak = to_double4(vals[k]);
+                const double2 zw = z[w];
+                px = zw.x;
+                py = zw.y;
+                if (!first) {
+                    const double2 pw = pOld[w];
+                    px += beta * pw.x;
+                    py += beta * pw.y;
+                }
+                yx += ak.x * px + ak.y * py;
+                yy += ak.z * px + ak.w * py;
+            }
+#pragma unroll
+            for (int off = LANES / 2; off > 0; off >>= 1) {
+                yx += __shfl_xor(yx, off, LANES);
+                yy += __shfl_xor(yy, off, LANES);
+            }
+            if (lane == 0 && u[r] < r1) {
+                const double2 pn = make_double2(oz[r].x + beta * op[r].x, oz[r].y + beta * op[r].y);
+                pNew[u[r]] = pn;
+                q[u[r]] = make_double2(yx, yy);
+                acc[0] += pn.x * yx + pn.y * yy;
+            }
         }
     }
     block_sum<1>(acc);

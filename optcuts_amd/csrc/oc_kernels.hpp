@@ -81,6 +81,8 @@ void launch_axpy_commit(hipStream_t s, long n, double alpha, const double* d, do
 
 // ---- PCG on the 2x2-block CSR ------------------------------------------------
 constexpr int kSpmvLanes = 8;     // lanes per block-row in the SpMV
+constexpr int kPcgLanes = 4;      // lanes per block-row in the PCG product
+constexpr int kPcgRows = 2;       // block rows per lane group and pass in the PCG product
 constexpr int kPcgMaxGrid = 1024;  // workgroups of the PCG kernels (bounds the partial records)
 
 struct PcgCtrl {
