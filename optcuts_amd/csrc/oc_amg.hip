@@ -616,73 +616,88 @@ __global__ void __launch_bounds__(kBlock) k_lanczos_finish(const double* __restr
     }
     // T: al[0..m), be[i] couples rows i-1 and i (i >= 1)
     __shared__ double al[kLanczosSteps], be[kLanczosSteps];
-    __shared__ double lohi[2];
-    __shared__ int hit[kWave];
     if ((int)threadIdx.x < m) {
         al[threadIdx.x] = coef[threadIdx.x];
         be[threadIdx.x] = coef[kLanczosSteps + threadIdx.x];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double lo = al[0], hi = al[0];
-        for (int i = 0; i < m; ++i) {
-            const double r = (i > 0 ? fabs(be[i]) : 0.0) + (i + 1 < m ? fabs(be[i + 1]) : 0.0);
-            lo = fmin(lo, al[i] - r);
-            hi = fmax(hi, al[i] + r);
-        }
-        lohi[0] = lo;
-        lohi[1] = hi;
+    if (threadIdx.x >= kWave) return;  // the rest runs in wave 0
+    const int l = threadIdx.x;
+    // Gershgorin interval of T
+    double lo = 1e300, hi = -1e300;
+    if (l < m) {
+        const double r = (l > 0 ? fabs(be[l]) : 0.0) + (l + 1 < m ? fabs(be[l + 1]) : 0.0);
+        lo = al[l] - r;
+        hi = al[l] + r;
     }
-    __syncthreads();
+    lo = wave_min(lo);
+    hi = wave_max(hi);
     // Multisection on the Sturm count (number of eigenvalues of T below x):
-    // 64 points per round shrink [lo, hi] around lambda_max by 65x.
+    // 64 points per round, one per lane, shrink [lo, hi] around lambda_max by
+    // 65x; counts grow with x, so the first point below which all m eigenvalues
+    // lie brackets lambda_max.
     for (int round = 0; round < 6; ++round) {
-        const double lo = lohi[0], h = (lohi[1] - lohi[0]) / (kWave + 1);
-        if ((int)threadIdx.x < kWave) {
-            const double x = lo + (threadIdx.x + 1) * h;
-            int cnt = 0;
-            double d = 1.0;
-            for (int i = 0; i < m; ++i) {
-                d = al[i] - x - (i > 0 ? be[i] * be[i] / d : 0.0);
-                if (d == 0.0) d = -1e-300;
-                if (d < 0.0) ++cnt;
-            }
-            hit[threadIdx.x] = cnt == m;
+        const double h = (hi - lo) / (kWave + 1);
+        const double x = lo + (l + 1) * h;
+        int cnt = 0;
+        double d = 1.0;
+        for (int i = 0; i < m; ++i) {
+            d = al[i] - x - (i > 0 ? be[i] * be[i] / d : 0.0);
+            if (d == 0.0) d = -1e-300;
+            if (d < 0.0) ++cnt;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int k = 0;
-            while (k < kWave && !hit[k]) ++k;  // counts grow with x: the first hit brackets lambda_max
-            lohi[0] = lo + k * h;
-            if (k < kWave) lohi[1] = lo + (k + 1) * h;
-        }
-        __syncthreads();
+        const unsigned long long hits = __ballot(cnt == m);
+        const int k = hits ? __ffsll((long long)hits) - 1 : kWave;
+        const double base = lo;
+        lo = base + k * h;
+        if (k < kWave) hi = base + (k + 1) * h;
     }
-    if (threadIdx.x == 0) lam[0] = lohi[1];
+    if (l == 0) lam[0] = hi;
 }
+
+// Lanes per output of the set-up kernels: the contribution lists of one
+// output are split over a lane group and reduced by shuffles, so their
+// scattered block loads are in flight together.
+constexpr int kSetupLanes = 8;
+
+template <int LANES, int NV>
+__device__ __forceinline__ void group_sum(double (&v)[NV])
+{
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] += __shfl_xor(v[i], off, LANES);
+}
+
+int setup_grid(long nOut, int lanes) { return (int)std::max<long>(1, (nOut * lanes + kBlock - 1) / kBlock); }
 
 // Tentative prolongator: per aggregate, B (rows of its members) = Q R by
 // Cholesky of the 3x3 Gram matrix (columns whose remaining norm vanishes are
 // dropped: zero Q column, zero R row). Q rows go to Pt (per member node),
-// R to Bout (the coarse node's near-null-space rows).
+// R to Bout (the coarse node's near-null-space rows). kSetupLanes lanes per
+// aggregate share its members.
 template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock) k_tentative(int nc, const int* __restrict__ aggPtr,
                                                       const int* __restrict__ aggMem,
                                                       const double2* __restrict__ UV,
                                                       const double* __restrict__ Bin, T* Pt, double* Bout)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
+    constexpr int LANES = kSetupLanes;
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) / LANES, lane = threadIdx.x % LANES;
+    const bool active = c < nc;  // whole lane groups are active or not
+    if (!active) return;
     const int m0 = aggPtr[c], m1 = aggPtr[c + 1];
     double cu = 0.0, cv = 0.0;
     if (BS == 2) {
-        for (int m = m0; m < m1; ++m) {
+        double cs[2] = {0.0, 0.0};
+        for (int m = m0 + lane; m < m1; m += LANES) {
             const double2 p = UV[aggMem[m]];
-            cu += p.x;
-            cv += p.y;
+            cs[0] += p.x;
+            cs[1] += p.y;
         }
-        cu /= (double)(m1 - m0);
-        cv /= (double)(m1 - m0);
+        group_sum<LANES, 2>(cs);
+        cu = cs[0] / (double)(m1 - m0);
+        cv = cs[1] / (double)(m1 - m0);
     }
     auto brows = [&](int u, double (&B)[BS][NS]) {
         if (BS == 2) {
@@ -696,8 +711,8 @@ __global__ void __launch_bounds__(kBlock) k_tentative(int nc, const int* __restr
                 for (int j = 0; j < NS; ++j) B[i][j] = Bin[(size_t)u * BS * NS + i * NS + j];
         }
     };
-    double G[NS][NS] = {};
-    for (int m = m0; m < m1; ++m) {
+    double Gv[NS * NS] = {};
+    for (int m = m0 + lane; m < m1; m += LANES) {
         double B[BS][NS];
         brows(aggMem[m], B);
 #pragma unroll
@@ -705,28 +720,29 @@ __global__ void __launch_bounds__(kBlock) k_tentative(int nc, const int* __restr
 #pragma unroll
             for (int a = 0; a < NS; ++a)
 #pragma unroll
-                for (int b = 0; b < NS; ++b) G[a][b] += B[i][a] * B[i][b];
+                for (int b = 0; b < NS; ++b) Gv[a * NS + b] += B[i][a] * B[i][b];
     }
+    group_sum<LANES, NS * NS>(Gv);
     double R[NS][NS] = {};
     bool keep[NS];
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-        double d = G[j][j];
+        double d = Gv[j * NS + j];
 #pragma unroll
         for (int k = 0; k < j; ++k) d -= R[k][j] * R[k][j];
-        keep[j] = G[j][j] > 0.0 && d > 1e-12 * G[j][j];
+        keep[j] = Gv[j * NS + j] > 0.0 && d > 1e-12 * Gv[j * NS + j];
         if (keep[j]) {
             R[j][j] = sqrt(d);
 #pragma unroll
             for (int k = j + 1; k < NS; ++k) {
-                double s = G[j][k];
+                double s = Gv[j * NS + k];
 #pragma unroll
                 for (int i = 0; i < j; ++i) s -= R[i][j] * R[i][k];
                 R[j][k] = s / R[j][j];
             }
         }
     }
-    for (int m = m0; m < m1; ++m) {
+    for (int m = m0 + lane; m < m1; m += LANES) {
         const int u = aggMem[m];
         double B[BS][NS];
         brows(u, B);
@@ -750,16 +766,18 @@ __global__ void __launch_bounds__(kBlock) k_tentative(int nc, const int* __restr
     // rotation about one common point (the UV origin), so R becomes R C with
     // C = [[1, 0, -c_v], [0, 1, c_u], [0, 0, 1]] (rotation about the origin =
     // rotation about c + translation (-c_v, c_u)).
+    if (lane == 0)
 #pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        double* o = Bout + (size_t)c * NS * NS + i * NS;
-        o[0] = R[i][0];
-        o[1] = R[i][1];
-        o[2] = BS == 2 ? (-cv * R[i][0] + cu * R[i][1] + R[i][2]) : R[i][2];
-    }
+        for (int i = 0; i < NS; ++i) {
+            double* o = Bout + (size_t)c * NS * NS + i * NS;
+            o[0] = R[i][0];
+            o[1] = R[i][1];
+            o[2] = BS == 2 ? (-cv * R[i][0] + cu * R[i][1] + R[i][2]) : R[i][2];
+        }
 }
 
-// P(u,c) = [c == agg(u)] Pt_u - omega Dinv_u sum_k A_k Pt_{col(k)}, omega = 4/(3 lambda)
+// P(u,c) = [c == agg(u)] Pt_u - omega Dinv_u sum_k A_k Pt_{col(k)}, omega = 4/(3 lambda);
+// kSetupLanes lanes per P block
 template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock) k_smooth_p(int nP, const int* __restrict__ pRow,
                                                      const unsigned char* __restrict__ pIsTent,
@@ -768,11 +786,12 @@ __global__ void __launch_bounds__(kBlock) k_smooth_p(int nP, const int* __restri
                                                      const T* __restrict__ Dinv, const T* __restrict__ Pt,
                                                      const double* __restrict__ lam, T* P)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int LANES = kSetupLanes;
+    const int j = (blockIdx.x * blockDim.x + threadIdx.x) / LANES, lane = threadIdx.x % LANES;
     if (j >= nP) return;
     const int u = pRow[j];
-    T S[BS][NS] = {};
-    for (int q = psPtr[j]; q < psPtr[j + 1]; ++q) {
+    double S[BS * NS] = {};
+    for (int q = psPtr[j] + lane; q < psPtr[j + 1]; q += LANES) {
         const int k = psA[q];
         const T* a = vals + (size_t)k * BS * BS;
         const T* pt = Pt + (size_t)colIdx[k] * BS * NS;
@@ -781,9 +800,11 @@ __global__ void __launch_bounds__(kBlock) k_smooth_p(int nP, const int* __restri
 #pragma unroll
             for (int l = 0; l < BS; ++l)
 #pragma unroll
-                for (int c = 0; c < NS; ++c) S[i][c] += a[i * BS + l] * pt[l * NS + c];
+                for (int c = 0; c < NS; ++c) S[i * NS + c] += (double)a[i * BS + l] * (double)pt[l * NS + c];
     }
-    const T omega = (T)(4.0 / (3.0 * *lam));
+    group_sum<LANES, BS * NS>(S);
+    if (lane != 0) return;
+    const double omega = 4.0 / (3.0 * *lam);
     const T* D = Dinv + (size_t)u * BS * BS;
     const T* ptu = Pt + (size_t)u * BS * NS;
     const bool tent = pIsTent[j] != 0;
@@ -792,23 +813,24 @@ __global__ void __launch_bounds__(kBlock) k_smooth_p(int nP, const int* __restri
     for (int i = 0; i < BS; ++i)
 #pragma unroll
         for (int c = 0; c < NS; ++c) {
-            T s = T(0);
+            double s = 0.0;
 #pragma unroll
-            for (int l = 0; l < BS; ++l) s += D[i * BS + l] * S[l][c];
-            o[i * NS + c] = (tent ? ptu[i * NS + c] : T(0)) - omega * s;
+            for (int l = 0; l < BS; ++l) s += (double)D[i * BS + l] * S[l * NS + c];
+            o[i * NS + c] = (T)((tent ? (double)ptu[i * NS + c] : 0.0) - omega * s);
         }
 }
 
-// T block = sum over (A block, P block) pairs of A_k P_j (BS x NS)
+// T block = sum over (A block, P block) pairs of A_k P_j (BS x NS); kSetupLanes lanes per block
 template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock) k_ap(int nT, const int* __restrict__ cPtr, const int* __restrict__ cA,
                                                const int* __restrict__ cP, const T* __restrict__ vals,
                                                const T* __restrict__ P, T* Tv)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int LANES = kSetupLanes;
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) / LANES, lane = threadIdx.x % LANES;
     if (t >= nT) return;
-    T S[BS][NS] = {};
-    for (int q = cPtr[t]; q < cPtr[t + 1]; ++q) {
+    double S[BS * NS] = {};
+    for (int q = cPtr[t] + lane; q < cPtr[t + 1]; q += LANES) {
         const T* a = vals + (size_t)cA[q] * BS * BS;
         const T* p = P + (size_t)cP[q] * BS * NS;
 #pragma unroll
@@ -816,25 +838,26 @@ __global__ void __launch_bounds__(kBlock) k_ap(int nT, const int* __restrict__ c
 #pragma unroll
             for (int l = 0; l < BS; ++l)
 #pragma unroll
-                for (int c = 0; c < NS; ++c) S[i][c] += a[i * BS + l] * p[l * NS + c];
+                for (int c = 0; c < NS; ++c) S[i * NS + c] += (double)a[i * BS + l] * (double)p[l * NS + c];
     }
+    group_sum<LANES, BS * NS>(S);
+    if (lane != 0) return;
     T* o = Tv + (size_t)t * BS * NS;
 #pragma unroll
-    for (int i = 0; i < BS; ++i)
-#pragma unroll
-        for (int c = 0; c < NS; ++c) o[i * NS + c] = S[i][c];
+    for (int i = 0; i < BS * NS; ++i) o[i] = (T)S[i];
 }
 
-// coarse block = sum over (P block, T block) pairs of P_j^T T_t (NS x NS)
+// coarse block = sum over (P block, T block) pairs of P_j^T T_t (NS x NS); kSetupLanes lanes per block
 template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock) k_ptap(int nAc, const int* __restrict__ cPtr, const int* __restrict__ cP,
                                                  const int* __restrict__ cT, const T* __restrict__ P,
                                                  const T* __restrict__ Tv, T* Ac)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int LANES = kSetupLanes;
+    const int a = (blockIdx.x * blockDim.x + threadIdx.x) / LANES, lane = threadIdx.x % LANES;
     if (a >= nAc) return;
-    T S[NS][NS] = {};
-    for (int q = cPtr[a]; q < cPtr[a + 1]; ++q) {
+    double S[NS * NS] = {};
+    for (int q = cPtr[a] + lane; q < cPtr[a + 1]; q += LANES) {
         const T* p = P + (size_t)cP[q] * BS * NS;
         const T* t = Tv + (size_t)cT[q] * BS * NS;
 #pragma unroll
@@ -842,13 +865,13 @@ __global__ void __launch_bounds__(kBlock) k_ptap(int nAc, const int* __restrict_
 #pragma unroll
             for (int i = 0; i < NS; ++i)
 #pragma unroll
-                for (int j = 0; j < NS; ++j) S[i][j] += p[l * NS + i] * t[l * NS + j];
+                for (int j = 0; j < NS; ++j) S[i * NS + j] += (double)p[l * NS + i] * (double)t[l * NS + j];
     }
+    group_sum<LANES, NS * NS>(S);
+    if (lane != 0) return;
     T* o = Ac + (size_t)a * NS * NS;
 #pragma unroll
-    for (int i = 0; i < NS; ++i)
-#pragma unroll
-        for (int j = 0; j < NS; ++j) o[i * NS + j] = S[i][j];
+    for (int i = 0; i < NS * NS; ++i) o[i] = (T)S[i];
 }
 
 // Dense coarsest matrix (double) and its inverse by in-place Gauss-Jordan in
@@ -1400,22 +1423,22 @@ void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double
         const int nP = (int)A.pCol.size(), nT = (int)A.tCol.size(), nAc = (int)N.colIdx.size();
         const double* Bin = l == 0 ? nullptr : L[l - 1]->Bnext;
         if (bs == 2) {
-            hipLaunchKernelGGL((k_tentative<T, 2>), dim3(grid_for(A.nc)), dim3(kBlock), 0, s, A.nc, A.dAggPtr,
+            hipLaunchKernelGGL((k_tentative<T, 2>), dim3(setup_grid(A.nc, kSetupLanes)), dim3(kBlock), 0, s, A.nc, A.dAggPtr,
                                A.dAggMem, reinterpret_cast<const double2*>(UV), Bin, A.Pt, A.Bnext);
-            hipLaunchKernelGGL((k_smooth_p<T, 2>), dim3(grid_for(nP)), dim3(kBlock), 0, s, nP, A.dPRow, A.dPIsTent,
+            hipLaunchKernelGGL((k_smooth_p<T, 2>), dim3(setup_grid(nP, kSetupLanes)), dim3(kBlock), 0, s, nP, A.dPRow, A.dPIsTent,
                                A.dPsPtr, A.dPsA, A.dColIdx, A.vals, A.Dinv, A.Pt, A.lam, A.P);
-            hipLaunchKernelGGL((k_ap<T, 2>), dim3(grid_for(nT)), dim3(kBlock), 0, s, nT, A.dTcPtr, A.dTcA, A.dTcP,
+            hipLaunchKernelGGL((k_ap<T, 2>), dim3(setup_grid(nT, kSetupLanes)), dim3(kBlock), 0, s, nT, A.dTcPtr, A.dTcA, A.dTcP,
                                A.vals, A.P, A.Tv);
-            hipLaunchKernelGGL((k_ptap<T, 2>), dim3(grid_for(nAc)), dim3(kBlock), 0, s, nAc, A.dAcPtr, A.dAcP,
+            hipLaunchKernelGGL((k_ptap<T, 2>), dim3(setup_grid(nAc, kSetupLanes)), dim3(kBlock), 0, s, nAc, A.dAcPtr, A.dAcP,
                                A.dAcT, A.P, A.Tv, N.vals);
         } else {
-            hipLaunchKernelGGL((k_tentative<T, 3>), dim3(grid_for(A.nc)), dim3(kBlock), 0, s, A.nc, A.dAggPtr,
+            hipLaunchKernelGGL((k_tentative<T, 3>), dim3(setup_grid(A.nc, kSetupLanes)), dim3(kBlock), 0, s, A.nc, A.dAggPtr,
                                A.dAggMem, reinterpret_cast<const double2*>(UV), Bin, A.Pt, A.Bnext);
-            hipLaunchKernelGGL((k_smooth_p<T, 3>), dim3(grid_for(nP)), dim3(kBlock), 0, s, nP, A.dPRow, A.dPIsTent,
+            hipLaunchKernelGGL((k_smooth_p<T, 3>), dim3(setup_grid(nP, kSetupLanes)), dim3(kBlock), 0, s, nP, A.dPRow, A.dPIsTent,
                                A.dPsPtr, A.dPsA, A.dColIdx, A.vals, A.Dinv, A.Pt, A.lam, A.P);
-            hipLaunchKernelGGL((k_ap<T, 3>), dim3(grid_for(nT)), dim3(kBlock), 0, s, nT, A.dTcPtr, A.dTcA, A.dTcP,
+            hipLaunchKernelGGL((k_ap<T, 3>), dim3(setup_grid(nT, kSetupLanes)), dim3(kBlock), 0, s, nT, A.dTcPtr, A.dTcA, A.dTcP,
                                A.vals, A.P, A.Tv);
-            hipLaunchKernelGGL((k_ptap<T, 3>), dim3(grid_for(nAc)), dim3(kBlock), 0, s, nAc, A.dAcPtr, A.dAcP,
+            hipLaunchKernelGGL((k_ptap<T, 3>), dim3(setup_grid(nAc, kSetupLanes)), dim3(kBlock), 0, s, nAc, A.dAcPtr, A.dAcP,
                                A.dAcT, A.P, A.Tv, N.vals);
         }
     }
