@@ -194,6 +194,34 @@ enum { OC_TUTTE_COMPONENTS = 0, OC_TUTTE_ALL_LOOPS = 1 };
 int oc_tutte(oc_ctx* ctx, int layout, int precond, double relTol, int maxIter, double* UVout, int* nLoops,
              int* nBnd, int* iters);
 
+/* ---- files: OBJ meshes and per-iteration traces ----------------------------------------------------
+ * Host-side I/O in the formats OptCuts reads and writes. No GPU is needed for the OBJ calls.
+ * oc_obj_read_sizes / oc_obj_read replace igl::readOBJ(path, V, UV, N, F, FUV, FN) as main.cpp:1153 calls it:
+ * V nV x 3, UV nUV x 2 (texture coordinates), F nF x 3 and FUV nF x 3 (0-based; *hasFUV = 0 when faces carry
+ * no texture indices). Triangles with 3-coordinate vertices only. Errors: OC_ERR_ARG with
+ * oc_io_last_error(). */
+const char* oc_io_last_error(void);
+int oc_obj_read_sizes(const char* path, int* nV, int* nUV, int* nF, int* hasFUV);
+int oc_obj_read(const char* path, double* V, double* UV, int* F, int* FUV);
+/* TriMesh::save(path, V, F, UV, FUV) (TriMesh.cpp:1758-1798): "v"/"vt" lines in %e, "f v/t v/t v/t";
+ * FUV null writes F for both indices. */
+int oc_obj_write(const char* path, int nV, const double* V, int nF, const int* F, int nUV, const double* UV,
+                 const int* FUV);
+/* TriMesh::saveAsMesh(path, F0, scaleUV) (TriMesh.cpp:1939-1969, written by main.cpp:341-342): the cut mesh
+ * (nV rest positions Vrest, UV, triangles F) is written on the input mesh's vertices F0 (same triangle
+ * order) with the cut mesh's UV as texture coordinates, normalised to the unit square when scaleUV. */
+int oc_obj_write_mesh(const char* path, int nV, const double* Vrest, const double* UV, int nF, const int* F,
+                      const int* F0, int scaleUV);
+/* Per-iteration traces of the device Newton loop in the files Optimizer writes (Optimizer.cpp:61-62,
+ * 706-739): folder/energyValPerIter.txt gets "E_total E_SD seam a0" at oc_newton_init and after every step
+ * that did not stop; folder/gradientPerIter.txt gets "|g|^2 |g_SD|^2" at the start of every step. The seam
+ * column is the caller's value (TriMesh::computeSeamSparsity / virtualRadius, Optimizer.cpp:708-710; 0 until
+ * set). Single-mesh contexts only (not with oc_set_charts). oc_trace_close (or oc_ctx_destroy) closes the
+ * files. */
+int oc_trace_open(oc_ctx* ctx, const char* folder);
+int oc_trace_set_seam(oc_ctx* ctx, double seamTerm);
+int oc_trace_close(oc_ctx* ctx);
+
 /* ---- batched charts ----------------------------------------------------------------------------------
  * Many independent charts in one context: after oc_set_mesh with the union of the charts (chart k owns the
  * vertices [vertexStart[k], vertexStart[k+1]) and the triangles [faceStart[k], faceStart[k+1]), triangles

@@ -105,6 +105,14 @@ _SIGS = {
     "oc_newton_step": (C.c_int, [_ctx, C.POINTER(IterStats), C.POINTER(C.c_int)]),
     "oc_time_kernel": (C.c_int, [_ctx, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "oc_system_sizes": (C.c_int, [_ctx] + [C.POINTER(C.c_longlong)] * 4),
+    "oc_io_last_error": (C.c_char_p, []),
+    "oc_obj_read_sizes": (C.c_int, [C.c_char_p] + [C.POINTER(C.c_int)] * 4),
+    "oc_obj_read": (C.c_int, [C.c_char_p, _vp, _vp, _vp, _vp]),
+    "oc_obj_write": (C.c_int, [C.c_char_p, C.c_int, _dp, C.c_int, _ip, C.c_int, _vp, _vp]),
+    "oc_obj_write_mesh": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, C.c_int, _ip, _ip, C.c_int]),
+    "oc_trace_open": (C.c_int, [_ctx, C.c_char_p]),
+    "oc_trace_set_seam": (C.c_int, [_ctx, C.c_double]),
+    "oc_trace_close": (C.c_int, [_ctx]),
 }
 
 _lib = None
@@ -138,6 +146,44 @@ def _i32(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _io_chk(st):
+    if st != OC_OK:
+        raise OCError(st, load().oc_io_last_error().decode())
+
+
+def obj_read(path):
+    """Read an OBJ mesh as igl::readOBJ does (main.cpp:1153). Returns (V (nV,3), UV (nUV,2), F (nF,3),
+    FUV (nF,3) or None), indices 0-based. Host only: no GPU is used."""
+    L = load()
+    p = os.fsencode(path)
+    nV, nUV, nF, has = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _io_chk(L.oc_obj_read_sizes(p, C.byref(nV), C.byref(nUV), C.byref(nF), C.byref(has)))
+    V = np.zeros((nV.value, 3))
+    UV = np.zeros((nUV.value, 2))
+    F = np.zeros((nF.value, 3), np.int32)
+    FUV = np.zeros((nF.value, 3), np.int32) if has.value else None
+    _io_chk(L.oc_obj_read(p, V.ctypes.data, UV.ctypes.data, F.ctypes.data,
+                          FUV.ctypes.data if FUV is not None else None))
+    return V, UV, F, FUV
+
+
+def obj_write(path, V, F, UV=None, FUV=None):
+    """Write an OBJ mesh as TriMesh::save does (TriMesh.cpp:1758-1798)."""
+    V, F = _f64(V), _i32(F)
+    UV = _f64(UV) if UV is not None else np.zeros((0, 2))
+    FUV = _i32(FUV) if FUV is not None else None
+    _io_chk(load().oc_obj_write(os.fsencode(path), V.shape[0], V, F.shape[0], F, UV.shape[0], UV.ctypes.data,
+                                FUV.ctypes.data if FUV is not None else None))
+
+
+def obj_write_mesh(path, Vrest, UV, F, F0, scale_uv=False):
+    """Write the cut mesh on the input mesh's vertices as TriMesh::saveAsMesh(path, F0, scaleUV) does
+    (TriMesh.cpp:1939-1969)."""
+    Vrest, UV, F, F0 = _f64(Vrest), _f64(UV), _i32(F), _i32(F0)
+    _io_chk(load().oc_obj_write_mesh(os.fsencode(path), Vrest.shape[0], Vrest, UV, F.shape[0], F, F0,
+                                     int(bool(scale_uv))))
 
 
 class Context:
@@ -290,6 +336,16 @@ class Context:
         self._chk(self.L.oc_tutte(self.h, int(layout), int(precond), float(rel_tol), int(max_iter),
                                   out.ctypes.data, C.byref(nl), C.byref(nb), C.byref(it)))
         return out, nl.value, nb.value, it.value
+
+    def trace_open(self, folder):
+        """Append per-iteration lines to folder/energyValPerIter.txt and folder/gradientPerIter.txt."""
+        self._chk(self.L.oc_trace_open(self.h, os.fsencode(folder)))
+
+    def trace_set_seam(self, seam):
+        self._chk(self.L.oc_trace_set_seam(self.h, float(seam)))
+
+    def trace_close(self):
+        self._chk(self.L.oc_trace_close(self.h))
 
     def set_charts(self, vertex_start, face_start):
         """Batched charts: contiguous vertex / triangle ranges of the mesh set by set_mesh."""

@@ -6,6 +6,7 @@
 #include "oc_amg_structure.hpp"
 #include "oc_common.hpp"
 #include "oc_gnn.hpp"
+#include "oc_io.hpp"
 #include "oc_kernels.hpp"
 
 #include <algorithm>
@@ -72,6 +73,11 @@ struct oc_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     char arch[64] = {0};
+
+    // ---- per-iteration traces (energyValPerIter.txt, gradientPerIter.txt) ----
+    FILE* trE = nullptr;
+    FILE* trG = nullptr;
+    double trSeam = 0.0;
 
     // ---- mesh block (block 0) -----------------------------------------------
     int nV = 0, nF = 0;
@@ -691,6 +697,7 @@ int oc_ctx_create(int device, oc_ctx** out)
 int oc_ctx_destroy(oc_ctx* c)
 {
     if (!c) return OC_ERR_ARG;
+    (void)oc_trace_close(c);
     (void)hipSetDevice(c->device);
     if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
@@ -701,6 +708,46 @@ int oc_ctx_destroy(oc_ctx* c)
 }
 
 const char* oc_last_error(const oc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int oc_trace_close(oc_ctx* c)
+{
+    if (!c) return OC_ERR_ARG;
+    int bad = 0;
+    for (FILE** f : {&c->trE, &c->trG})
+        if (*f) {
+            bad |= std::fclose(*f);
+            *f = nullptr;
+        }
+    if (bad) {
+        c->err = "oc_trace_close: write error";
+        return OC_ERR_STATE;
+    }
+    return OC_OK;
+}
+
+int oc_trace_open(oc_ctx* c, const char* folder)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, folder != nullptr, "oc_trace_open: null folder");
+    (void)oc_trace_close(c);
+    std::string base(folder);
+    if (!base.empty() && base.back() != '/') base += '/';
+    c->trE = std::fopen((base + "energyValPerIter.txt").c_str(), "w");
+    c->trG = std::fopen((base + "gradientPerIter.txt").c_str(), "w");
+    if (!c->trE || !c->trG) {
+        (void)oc_trace_close(c);
+        c->err = "oc_trace_open: cannot create the trace files in " + base;
+        return OC_ERR_ARG;
+    }
+    return OC_OK;
+}
+
+int oc_trace_set_seam(oc_ctx* c, double seamTerm)
+{
+    if (!c) return OC_ERR_ARG;
+    c->trSeam = seamTerm;
+    return OC_OK;
+}
 
 int oc_device_arch(const oc_ctx* c, char* buf, int buflen)
 {
@@ -1668,6 +1715,7 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
         c->lastE = e;
     }
     if (E0) *E0 = c->lastE;
+    if (c->trE && c->nCharts == 0) oc::io::trace_energy_line(c->trE, c->lastE, c->lastESD, c->trSeam, c->a0);
     return OC_OK;
 }
 
@@ -1924,6 +1972,22 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     S->E_total = c->lastE;
     S->E_SD = c->lastESD;
     S->E_scaf = c->lastEScaf;
+    if (c->trG) {  // Optimizer.cpp:211-214: |g|^2 and the SD term's own |g_SD|^2
+        double gsd = gsq / (c->a0 * c->a0);
+        if (c->nFair) {
+            if (ensure_scratch(c, c->nV)) return OC_ERR_HIP;
+            OC_HIP(c, c->dTmpVec.ensure(sizeof(double) * 2 * (size_t)nVt));
+            oc::launch_element(s, mesh_set(c, 0), c->dUV.as<double>(), nullptr, c->dElemG.as<double>(), nullptr, 0,
+                               nullptr, nullptr);
+            oc::launch_gather_grad(s, c->nV, c->dMvcPtr.as<int>(), c->dMvcSrc.as<int>(),
+                                   c->dFixedMaskMesh.as<unsigned char>(), c->dElemG.as<double>(),
+                                   c->dTmpVec.as<double>(), c->dPartials.as<double>(), scalar(c, 8));
+            OC_HIP(c, hipGetLastError());
+            OC_HIP(c, hipMemcpyAsync(&gsd, scalar(c, 8), sizeof(double), hipMemcpyDeviceToHost, s));
+            OC_HIP(c, hipStreamSynchronize(s));
+        }
+        oc::io::trace_grad_line(c->trG, gsq, gsd);
+    }
     if (gsq < c->targetGRes) {  // Optimizer.cpp:215-221
         S->converged = 1;
         *done = 1;
@@ -2017,6 +2081,8 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     S->halvings = halvings;
     S->stopped = stopped ? 1 : 0;
     *done = stopped ? 1 : 0;
+    if (c->trE && !stopped)  // Optimizer.cpp:645-648
+        oc::io::trace_energy_line(c->trE, testE, testESD, c->trSeam, c->a0);
     return OC_OK;
 }
 

@@ -105,6 +105,10 @@ def ref():
         R.ref_eigvals6.argtypes = [_dp, _dp]
         R.ref_tutte.restype = C.c_int
         R.ref_tutte.argtypes = [C.c_int, _dp, C.c_int, _ip, C.c_int, _dp, _ip, _ip]
+        R.ref_read_obj.restype = C.c_int
+        R.ref_read_obj.argtypes = [C.c_char_p, _ip, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        R.ref_stream_double.restype = C.c_int
+        R.ref_stream_double.argtypes = [C.c_double, C.c_int, C.c_char_p, C.c_int]
         _ref = R
     return _ref
 
@@ -511,3 +515,26 @@ def tutte_uv(V, F, layout=0):
         rhs = -(Q[I][:, bnd] @ bc)
         W[I] = spla.spsolve(Q[I][:, I].tocsc(), rhs).reshape(-1, 2)
     return W, loops
+
+
+def ref_read_obj(path):
+    """igl::readOBJ(path, V, TC, N, F, FTC, FN) as main.cpp:1153 calls it (vendored libigl).
+    Returns (V, TC, F, FTC) or None when the reader fails."""
+    R = ref()
+    sizes = np.zeros(7, np.int32)
+    p = os.fsencode(path)
+    if not R.ref_read_obj(p, sizes, None, None, None, None):
+        return None
+    V = np.zeros((sizes[0], sizes[1]))
+    TC = np.zeros((sizes[2], sizes[3]))
+    F = np.zeros((sizes[4], sizes[5]), np.int32)
+    FTC = np.zeros((sizes[6], sizes[5]), np.int32)
+    R.ref_read_obj(p, sizes, V.ctypes.data, TC.ctypes.data, F.ctypes.data, FTC.ctypes.data)
+    return V, TC, F, FTC
+
+
+def ref_stream_double(x, scientific):
+    """x as std::ostream writes it, default flags or std::scientific."""
+    buf = C.create_string_buffer(64)
+    ref().ref_stream_double(float(x), int(scientific), buf, 64)
+    return buf.value.decode()

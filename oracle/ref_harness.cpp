@@ -10,7 +10,10 @@
 //   - the vendored libigl's header-only igl::facet_components,
 //     igl::boundary_loop, igl::map_vertices_to_circle and igl::harmonic, called
 //     as the reference's Tutte initialisation calls them (main.cpp:1313-1364,
-//     1380-1395, 1486-1532). Its Laplacian
+//     1380-1395, 1486-1532);
+//   - igl::readOBJ as main.cpp:1153 calls it, and std::ostream formatting of
+//     doubles with the flags TriMesh::save (std::scientific) and the
+//     Optimizer trace writers (defaults) use. Its Laplacian
 //     comes from IglUtils::computeUniformLaplacian (IglUtils.cpp:39-60), a
 //     translation unit that needs TBB; ref_tutte assembles the same triplets
 //     (1 per directed triangle edge, -2 per corner) itself.
@@ -25,9 +28,12 @@
 #include <igl/facet_components.h>
 #include <igl/harmonic.h>
 #include <igl/map_vertices_to_circle.h>
+#include <igl/readOBJ.h>
 
 #include <cstring>
+#include <iomanip>
 #include <set>
+#include <sstream>
 #include <vector>
 
 using Solver = OptCuts::EigenLibSolver<Eigen::VectorXi, Eigen::VectorXd>;
@@ -206,6 +212,48 @@ int ref_tutte(int nV, const double* Vrest, int nF, const int* F, int layout, dou
     if (bndOut)
         for (int i = 0; i < nb; ++i) bndOut[i] = bnd[i];
     return (int)loops.size();
+}
+
+// igl::readOBJ(path, V, TC, N, F, FTC, FN) into caller arrays: the sizes are
+// always returned (sizes[0..6] = V rows, V cols, TC rows, TC cols, F rows,
+// F cols, FTC rows); arrays are filled when non-null. Returns 1 on success.
+int ref_read_obj(const char* path, int* sizes, double* V, double* TC, int* F, int* FTC)
+{
+    Eigen::MatrixXd Vm, TCm, Nm;
+    Eigen::MatrixXi Fm, FTCm, FNm;
+    if (!igl::readOBJ(path, Vm, TCm, Nm, Fm, FTCm, FNm)) return 0;
+    sizes[0] = (int)Vm.rows();
+    sizes[1] = (int)Vm.cols();
+    sizes[2] = (int)TCm.rows();
+    sizes[3] = (int)TCm.cols();
+    sizes[4] = (int)Fm.rows();
+    sizes[5] = (int)Fm.cols();
+    sizes[6] = (int)FTCm.rows();
+    if (V)
+        for (int i = 0; i < Vm.rows(); ++i)
+            for (int j = 0; j < Vm.cols(); ++j) V[i * Vm.cols() + j] = Vm(i, j);
+    if (TC)
+        for (int i = 0; i < TCm.rows(); ++i)
+            for (int j = 0; j < TCm.cols(); ++j) TC[i * TCm.cols() + j] = TCm(i, j);
+    if (F)
+        for (int i = 0; i < Fm.rows(); ++i)
+            for (int j = 0; j < Fm.cols(); ++j) F[i * Fm.cols() + j] = Fm(i, j);
+    if (FTC)
+        for (int i = 0; i < FTCm.rows(); ++i)
+            for (int j = 0; j < FTCm.cols(); ++j) FTC[i * FTCm.cols() + j] = FTCm(i, j);
+    return 1;
+}
+
+// A double as an std::ostream writes it with default flags (scientific = 0)
+// or after std::scientific (scientific = 1); returns the string length.
+int ref_stream_double(double x, int scientific, char* out, int cap)
+{
+    std::ostringstream os;
+    if (scientific) os << std::scientific;
+    os << x;
+    const std::string t = os.str();
+    std::snprintf(out, cap, "%s", t.c_str());
+    return (int)t.size();
 }
 
 }  // extern "C"

@@ -9,6 +9,7 @@ reference code).
 
     python tests/golden/make_golden.py
 """
+import json
 import os
 import sys
 
@@ -20,6 +21,59 @@ sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
 from oracle import oracle as O  # noqa: E402
 from optcuts_amd.meshgen import height_field_disc  # noqa: E402
 
+
+# Hand-written OBJ fixtures: TriMesh::save style; mixed syntax (groups,
+# smoothing, materials, normals, i/t/n and negative indices, trailing
+# comments, tabs, 3-value texture coordinates); positions and faces only.
+OBJ_TEXTS = {
+    "basic": """# two triangles, texture indices
+v 0.000000e+00 0.000000e+00 0.000000e+00
+v 1.000000e+00 0.000000e+00 0.000000e+00
+v 0.000000e+00 1.000000e+00 2.500000e-01
+
+v 1.000000e+00 1.000000e+00 -3.125000e-02
+vt 0.000000e+00 0.000000e+00
+vt 1.000000e+00 0.000000e+00
+vt 0.000000e+00 1.000000e+00
+vt 1.000000e+00 1.000000e+00
+f 1/1 2/2 3/3
+f 2/2 4/4 3/3
+""",
+    "mixed_syntax": """####
+# exported by a tool
+####
+mtllib scene.mtl
+o patch
+g patch_group
+v 0.1 0.2 0.3 # first
+v\t1.5 -2.25   3e-3
+v 4 5 6
+v -1e2 2.5e+1 7
+vn 0 0 1
+vn 0 0 1
+vn 0 1 0
+vt 0.25 0.5 0
+vt 0.75 0.5 0
+vt 0.5 1.0 0
+vt 0.0 0.125 0
+usemtl skin
+s off
+f 1/1/1 2/2/2 3/3/3
+f -3/-3/-2 -1/-1/-1 -2/-2/-3
+s 1
+f 4/4/1 1/1/2 3/3/3
+""",
+    "no_uv": """v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+v 0.5 0.5 1
+f 1 2 5
+f 2//1 3//1 5//1
+f 3 4 5
+f 4 1 5
+""",
+}
 
 def distorted_uv(V, F, UV, seed, amp):
     """A random smooth distortion that keeps every triangle CCW."""
@@ -96,6 +150,35 @@ def main():
             key = "%s_l%d" % (name, layout)
             out.update({key + "_UV": UVt, key + "_bnd": np.concatenate(loops),
                         key + "_loops": np.array([len(l) for l in loops], np.int32)})
+    # --- OBJ files (main.cpp:1153 reads them with igl::readOBJ): fixtures written
+    # here, parsed by the vendored libigl reader
+    obj_dir = os.path.join(HERE, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    texts = dict(OBJ_TEXTS)
+    Vd, Fd, UVd = height_field_disc(7, seed=3)
+    lines = ["v %e %e %e" % tuple(v) for v in Vd] + ["vt %e %e" % tuple(t) for t in UVd]
+    lines += ["f %d/%d %d/%d %d/%d" % (a, a, b, b, c, c) for a, b, c in Fd + 1]
+    texts["disc7"] = "\n".join(lines) + "\n"
+    for name, text in sorted(texts.items()):
+        path = os.path.join(obj_dir, name + ".obj")
+        with open(path, "w") as f:
+            f.write(text)
+        V, TC, F, FTC = O.ref_read_obj(path)
+        out.update({"obj_%s_V" % name: V, "obj_%s_TC" % name: TC, "obj_%s_F" % name: F, "obj_%s_FTC" % name: FTC})
+    # --- std::ostream formatting of doubles (TriMesh::save: std::scientific; the
+    # Optimizer traces: default flags)
+    rng = np.random.default_rng(21)
+    vals = np.concatenate([
+        rng.standard_normal(60) * 10.0 ** rng.integers(-12, 12, 60),
+        rng.uniform(-1, 1, 40),
+        np.array([0.0, -0.0, 1.0, -1.0, 0.1, 1e-300, 5e-324, 1.7976931348623157e308, 123456789.0, 1e6, 999999.5,
+                  1e-5, 0.0001, 4.812089839905548, 2.5e-7, 100000.0, 1234567.0, 9.9999995, 0.99999995]),
+    ])
+    fmt = {"values_hex": [float(v).hex() for v in vals],
+           "scientific": [O.ref_stream_double(v, 1) for v in vals],
+           "default": [O.ref_stream_double(v, 0) for v in vals]}
+    with open(os.path.join(HERE, "stream_format.json"), "w") as f:
+        json.dump(fmt, f, indent=0)
     path = os.path.join(HERE, "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
