@@ -42,13 +42,14 @@ def lanczos0_bytes(nnzb, nV):
 
 
 # HBM traffic per launch of the roofline kernels at the 200k-face workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), FETCH_SIZE
-# scaled by 2.0 and WRITE_SIZE by 1.0: the factors that make the same counters
-# read the known size of a plain 512 MiB device copy. Summaries with commands:
-# profiles/pmc_traffic_chosen_200k.md.
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch, mean over
+# full dispatches: those queued past PCG convergence return at once and are
+# left out), FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0: the factors that
+# make the same counters read the known size of a plain 512 MiB device copy.
+# Summaries with commands: profiles/pmc_traffic_200k.md.
 TRAFFIC_200K = {
-    "k_lanczos_fine": (2.0 * 21601.4 + 3317.9) * 1024,
-    "k_pcg_spmv": (2.0 * 13153.8 + 2428.7) * 1024,
+    "k_lanczos_fine": (2.0 * 17083.3 + 3188.7) * 1024,
+    "k_pcg_spmv": (2.0 * 14334.1 + 3207.3) * 1024,
 }
 
 
