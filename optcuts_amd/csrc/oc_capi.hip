@@ -1783,6 +1783,17 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S)
             st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, std::max(c->pcgMaxIter, 200000), it2, rr,
                          nullptr);
             it += it2;
+            if (st == OC_ERR_BREAKDOWN) {
+                // Negative curvature along p (the assembled matrix is not positive
+                // definite): truncated Newton. The iterate before the breakdown
+                // minimises the quadratic model over the Krylov space so far and is
+                // a descent direction; with no iterate, steepest descent.
+                if (it2 == 0)
+                    OC_HIP(c, hipMemcpyAsync(c->dX.p, c->dB.p, sizeof(double) * 2 * (size_t)c->sys.nV,
+                                             hipMemcpyDeviceToDevice, s));
+                S->pcg_fallback = 3;
+                st = OC_OK;
+            }
         }
     }
     S->pcg_iters = it;
