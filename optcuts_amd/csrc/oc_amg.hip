@@ -1105,12 +1105,17 @@ __global__ void __launch_bounds__(kBlock) k_cheb_post1(const int* __restrict__ s
     }
 }
 
-// bc_c = sum over P blocks (u,c) of P^T res_u, kRestrictLanes lanes per coarse node
+// bc_c = sum over P blocks (u,c) of P^T res_u, kRestrictLanes lanes per coarse node.
+// PRE1: the next level's first pre-smoothing step (k_cheb_pre1 from x = 0:
+// r = D^-1 b, d = r/theta, x = d) is row-local, so it runs here on each
+// restricted row instead of in a launch of its own.
 constexpr int kRestrictLanes = 16;
-template <typename T, int BS>
+template <typename T, int BS, bool PRE1>
 __global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ stop, int nc, const int* __restrict__ rPtr,
                                                      const int* __restrict__ rP, const int* __restrict__ pRow,
-                                                     const T* __restrict__ P, const T* __restrict__ res, T* bc)
+                                                     const T* __restrict__ P, const T* __restrict__ res, T* bc,
+                                                     const T* __restrict__ DinvN, const double* __restrict__ lamN,
+                                                     T* xN, T* rN, T* dN)
 {
     if (stop && *stop) return;
     int c0, c1;
@@ -1132,9 +1137,29 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ sto
         for (int off = LN / 2; off > 0; off >>= 1)
 #pragma unroll
             for (int i = 0; i < NS; ++i) s[i] += __shfl_xor(s[i], off, LN);
-        if (lane == 0)
+        if (lane == 0) {
+            T bt[NS];
 #pragma unroll
-            for (int i = 0; i < NS; ++i) bc[(size_t)c * NS + i] = (T)s[i];
+            for (int i = 0; i < NS; ++i) bt[i] = (T)s[i];
+            st_vec<NS>(bc, c, bt);
+            if (PRE1) {
+                const double theta = cheb_params(*lamN).theta;
+                double Dm[NS * NS];
+                ld_block<NS>(DinvN + (size_t)c * NS * NS, Dm);
+                T rt[NS], dt[NS];
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) t += Dm[i * NS + j] * (double)bt[j];
+                    rt[i] = (T)t;
+                    dt[i] = (T)(t / theta);
+                }
+                st_vec<NS>(rN, c, rt);
+                st_vec<NS>(dN, c, dt);
+                st_vec<NS>(xN, c, dt);
+            }
+        }
     }
 }
 
@@ -1478,6 +1503,33 @@ void Amg<T>::coarse_solve(hipStream_t s, const int* stop)
         hipLaunchKernelGGL((k_coarse_diag<T, 3>), dim3(grid_for(C.n)), dim3(kBlock), 0, s, stop, C.n, C.Dinv, C.b, C.x);
 }
 
+// Restriction of level l's residual to level l + 1's right-hand side, with
+// level l + 1's first pre-smoothing step fused in when that level is smoothed
+// (not the coarsest).
+template <typename T>
+void Amg<T>::restrict_to(hipStream_t s, int l, const int* stop)
+{
+    Level& A = *L[l];
+    Level& N = *L[l + 1];
+    const bool fuse = l + 2 < (int)L.size();
+    const dim3 g(rows_grid(A.nc, kBlock / kRestrictLanes));
+    if (A.bs == 2) {
+        if (fuse)
+            hipLaunchKernelGGL((k_restrict<T, 2, true>), g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P,
+                               A.r, N.b, N.Dinv, N.lam, N.x, N.r, N.d);
+        else
+            hipLaunchKernelGGL((k_restrict<T, 2, false>), g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P,
+                               A.r, N.b, nullptr, nullptr, nullptr, nullptr, nullptr);
+    } else {
+        if (fuse)
+            hipLaunchKernelGGL((k_restrict<T, 3, true>), g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P,
+                               A.r, N.b, N.Dinv, N.lam, N.x, N.r, N.d);
+        else
+            hipLaunchKernelGGL((k_restrict<T, 3, false>), g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P,
+                               A.r, N.b, nullptr, nullptr, nullptr, nullptr, nullptr);
+    }
+}
+
 template <typename T>
 void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
 {
@@ -1490,15 +1542,12 @@ void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
     const int gUp = rows_grid(A.n, kBlock);
     if (A.bs == 2) {
         const int g = row_grid(A.n, 2);
-        if (l > 0)
-            hipLaunchKernelGGL((k_cheb_pre1<T, 2, T>), dim3(gUp), dim3(kBlock), 0, s, stop, A.n, A.b, A.b, A.Dinv, A.lam,
-                               A.x, A.r, A.d);
+        (void)gUp;  // level 0 starts from k_cheb_pre1 in apply(); lower levels from the fused restriction
         hipLaunchKernelGGL((k_cheb_step<T, 2, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
                            A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x, nullptr, nullptr, nullptr);
         hipLaunchKernelGGL((k_residual<T, 2>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
                            A.b, A.x, A.r);
-        hipLaunchKernelGGL((k_restrict<T, 2>), dim3(rows_grid(A.nc, kBlock / kRestrictLanes)), dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP,
-                           A.dPRow, A.P, A.r, N.b);
+        restrict_to(s, l, stop);
         vcycle(s, l + 1, stop);
         hipLaunchKernelGGL((k_prolong<T, 2>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
                            N.x, A.x);
@@ -1506,14 +1555,14 @@ void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
                            A.Dinv, A.lam, A.b, A.x, A.x2, A.r, A.d);
     } else {
         const int g = row_grid(A.n, 3);
-        hipLaunchKernelGGL((k_cheb_pre1<T, 3, T>), dim3(gUp), dim3(kBlock), 0, s, stop, A.n, A.b, A.b, A.Dinv, A.lam, A.x,
-                           A.r, A.d);
+        if (l == 0)  // levels below the finest start from the restriction's fused first step
+            hipLaunchKernelGGL((k_cheb_pre1<T, 3, T>), dim3(gUp), dim3(kBlock), 0, s, stop, A.n, A.b, A.b, A.Dinv, A.lam,
+                               A.x, A.r, A.d);
         hipLaunchKernelGGL((k_cheb_step<T, 3, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
                            A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x, nullptr, nullptr, nullptr);
         hipLaunchKernelGGL((k_residual<T, 3>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
                            A.b, A.x, A.r);
-        hipLaunchKernelGGL((k_restrict<T, 3>), dim3(rows_grid(A.nc, kBlock / kRestrictLanes)), dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP,
-                           A.dPRow, A.P, A.r, N.b);
+        restrict_to(s, l, stop);
         vcycle(s, l + 1, stop);
         hipLaunchKernelGGL((k_prolong<T, 3>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
                            N.x, A.x);
