@@ -478,6 +478,128 @@ int step_bound(oc_ctx* c, const TriSet& T, const double* dDir, double cap, doubl
 }
 
 // ---- Newton system pattern ------------------------------------------------------
+// Block pattern of the Newton matrix of triangles Ft over nVt vertices (one
+// row per vertex from its triangles: fixed rows keep only the diagonal, free
+// rows drop fixed columns), its upper blocks and, per upper block, the element
+// blocks that assemble it in triangle order. vcPtr/vcSrc: vertex -> (triangle*3
+// + corner) lists of Ft. Row chunks run on host threads and are concatenated in
+// order, so the output does not depend on the thread count.
+void build_pattern_maps(int nVt, int nFt, const std::vector<int>& Ft, const unsigned char* fixed,
+                        const std::vector<int>& vcPtr, const std::vector<int>& vcSrc, BlockPattern& Pout,
+                        std::vector<int>& ubRow, std::vector<int>& ubCol, std::vector<int>& ubPosU,
+                        std::vector<int>& ubPosL, std::vector<int>& ubCPtr, std::vector<int>& ubCSrc)
+{
+    const int nth = oc::setup_threads(3L * nFt);
+    // Block pattern, one row per vertex from its triangles: fixed rows keep
+    // only the diagonal, free rows drop fixed columns. Row chunks in parallel,
+    // concatenated in order.
+    {
+        std::vector<std::vector<int>> part(nth);
+        std::vector<int> len(nVt);
+        oc::parallel_row_chunks(nVt, nth, [&](int ch, int r0, int r1) {
+            std::vector<int> loc;
+            auto& out = part[ch];
+            for (int v = r0; v < r1; ++v) {
+                const size_t n0 = out.size();
+                if (fixed[v]) {
+                    out.push_back(v);
+                } else {
+                    loc.assign(1, v);
+                    for (int q = vcPtr[v]; q < vcPtr[v + 1]; ++q) {
+                        const int t = vcSrc[q] / 3;
+                        for (int k = 0; k < 3; ++k) loc.push_back(Ft[3 * (size_t)t + k]);
+                    }
+                    std::sort(loc.begin(), loc.end());
+                    loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+                    for (int w : loc)
+                        if (w == v || !fixed[w]) out.push_back(w);
+                }
+                len[v] = (int)(out.size() - n0);
+            }
+        });
+        BlockPattern& P = Pout;
+        P.nV = nVt;
+        P.rowPtr.assign(nVt + 1, 0);
+        for (int v = 0; v < nVt; ++v) P.rowPtr[v + 1] = P.rowPtr[v] + len[v];
+        P.colIdx.clear();
+        P.colIdx.reserve(P.rowPtr[nVt]);
+        for (auto& x : part) P.colIdx.insert(P.colIdx.end(), x.begin(), x.end());
+        P.diagPos.assign(nVt, -1);
+        for (int v = 0; v < nVt; ++v)
+            for (int k = P.rowPtr[v]; k < P.rowPtr[v + 1]; ++k)
+                if (P.colIdx[k] == v) P.diagPos[v] = k;
+    }
+    const BlockPattern& P = Pout;
+    // upper blocks (w >= u) in row order, and each position's upper block
+    std::vector<int> ubStart(nVt + 1, 0), posToUB(P.nnzb(), -1);
+    for (int u = 0; u < nVt; ++u) {
+        int n = 0;
+        for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k) n += P.colIdx[k] >= u;
+        ubStart[u + 1] = ubStart[u] + n;
+    }
+    const int nUB = ubStart[nVt];
+    ubRow.assign(nUB, 0);
+    ubCol.assign(nUB, 0);
+    ubPosU.assign(nUB, 0);
+    ubPosL.assign(nUB, 0);
+    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
+        for (int u = r0; u < r1; ++u) {
+            int b = ubStart[u];
+            for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k) {
+                const int w = P.colIdx[k];
+                if (w < u) continue;
+                posToUB[k] = b;
+                ubRow[b] = u;
+                ubCol[b] = w;
+                ubPosU[b] = k;
+                ubPosL[b] = w == u ? -1 : P.find(w, u);
+                ++b;
+            }
+        }
+    });
+    // Contributions per upper block in triangle order. Block (u, w), u <= w,
+    // receives the element block (a, b) of every triangle t with f[a] = u and
+    // f[b] = w (for u == w only a == b); walking u's triangles in order gives
+    // each of its blocks its contributions in triangle order.
+    ubCPtr.assign(nUB + 1, 0);
+    auto row_contribs = [&](int u, auto&& fn) {
+        if (fixed[u]) return;
+        for (int q = vcPtr[u]; q < vcPtr[u + 1]; ++q) {
+            const int t = vcSrc[q] / 3, a = vcSrc[q] % 3;
+            const int* f = &Ft[3 * (size_t)t];
+            for (int b = 0; b < 3; ++b) {
+                const int w = f[b];
+                if (fixed[w] || w < u || (w == u && a != b)) continue;
+                const int lo = a < b ? a : b, hi = a < b ? b : a;
+                const int id = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+                const unsigned int code = (unsigned int)(6 * t + id) | (a > b ? 0x80000000u : 0u);
+                fn(posToUB[P.find(u, w)], code);
+            }
+        }
+    };
+    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
+        for (int u = r0; u < r1; ++u) row_contribs(u, [&](int ub, unsigned int) { ubCPtr[ub + 1]++; });
+    });
+    for (int b = 0; b < nUB; ++b) ubCPtr[b + 1] += ubCPtr[b];
+    ubCSrc.assign(ubCPtr[nUB], 0);
+    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
+        std::vector<int> fill;
+        for (int u = r0; u < r1; ++u) {
+            const int b0 = ubStart[u], b1 = ubStart[u + 1];
+            fill.assign(ubCPtr.begin() + b0, ubCPtr.begin() + b1);
+            row_contribs(u, [&](int ub, unsigned int code) { ubCSrc[fill[ub - b0]++] = (int)code; });
+        }
+    });
+
+}
+
+// Numeric multigrid hierarchy for the current Newton matrix (safe: the
+// guaranteed smoother interval).
+void amg_numeric(oc_ctx* c, int safe)
+{
+    c->amg->numeric(c->stream, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>(), safe);
+}
+
 int build_system(oc_ctx* c)
 {
     const int nVt = c->nV + (c->nFair ? c->nVair - c->nBnd : 0);
@@ -499,109 +621,8 @@ int build_system(oc_ctx* c)
     // vertex -> (triangle*3 + corner) lists in triangle order (the gradient
     // gathers use them, and so does everything below)
     build_vertex_lists(nVt, nFt, Ft.data(), c->vcPtr, c->vcSrc);
-    const unsigned char* fixed = c->fixedMaskSys.data();
-    const int nth = oc::setup_threads(3L * nFt);
-    // Block pattern, one row per vertex from its triangles: fixed rows keep
-    // only the diagonal, free rows drop fixed columns. Row chunks in parallel,
-    // concatenated in order.
-    {
-        std::vector<std::vector<int>> part(nth);
-        std::vector<int> len(nVt);
-        oc::parallel_row_chunks(nVt, nth, [&](int ch, int r0, int r1) {
-            std::vector<int> loc;
-            auto& out = part[ch];
-            for (int v = r0; v < r1; ++v) {
-                const size_t n0 = out.size();
-                if (fixed[v]) {
-                    out.push_back(v);
-                } else {
-                    loc.assign(1, v);
-                    for (int q = c->vcPtr[v]; q < c->vcPtr[v + 1]; ++q) {
-                        const int t = c->vcSrc[q] / 3;
-                        for (int k = 0; k < 3; ++k) loc.push_back(Ft[3 * (size_t)t + k]);
-                    }
-                    std::sort(loc.begin(), loc.end());
-                    loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
-                    for (int w : loc)
-                        if (w == v || !fixed[w]) out.push_back(w);
-                }
-                len[v] = (int)(out.size() - n0);
-            }
-        });
-        BlockPattern& P = c->sys;
-        P.nV = nVt;
-        P.rowPtr.assign(nVt + 1, 0);
-        for (int v = 0; v < nVt; ++v) P.rowPtr[v + 1] = P.rowPtr[v] + len[v];
-        P.colIdx.clear();
-        P.colIdx.reserve(P.rowPtr[nVt]);
-        for (auto& x : part) P.colIdx.insert(P.colIdx.end(), x.begin(), x.end());
-        P.diagPos.assign(nVt, -1);
-        for (int v = 0; v < nVt; ++v)
-            for (int k = P.rowPtr[v]; k < P.rowPtr[v + 1]; ++k)
-                if (P.colIdx[k] == v) P.diagPos[v] = k;
-    }
-    const BlockPattern& P = c->sys;
-    // upper blocks (w >= u) in row order, and each position's upper block
-    std::vector<int> ubStart(nVt + 1, 0), posToUB(P.nnzb(), -1);
-    for (int u = 0; u < nVt; ++u) {
-        int n = 0;
-        for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k) n += P.colIdx[k] >= u;
-        ubStart[u + 1] = ubStart[u] + n;
-    }
-    const int nUB = ubStart[nVt];
-    c->ubRow.assign(nUB, 0);
-    c->ubCol.assign(nUB, 0);
-    c->ubPosU.assign(nUB, 0);
-    c->ubPosL.assign(nUB, 0);
-    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
-        for (int u = r0; u < r1; ++u) {
-            int b = ubStart[u];
-            for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k) {
-                const int w = P.colIdx[k];
-                if (w < u) continue;
-                posToUB[k] = b;
-                c->ubRow[b] = u;
-                c->ubCol[b] = w;
-                c->ubPosU[b] = k;
-                c->ubPosL[b] = w == u ? -1 : P.find(w, u);
-                ++b;
-            }
-        }
-    });
-    // Contributions per upper block in triangle order. Block (u, w), u <= w,
-    // receives the element block (a, b) of every triangle t with f[a] = u and
-    // f[b] = w (for u == w only a == b); walking u's triangles in order gives
-    // each of its blocks its contributions in triangle order.
-    c->ubCPtr.assign(nUB + 1, 0);
-    auto row_contribs = [&](int u, auto&& fn) {
-        if (fixed[u]) return;
-        for (int q = c->vcPtr[u]; q < c->vcPtr[u + 1]; ++q) {
-            const int t = c->vcSrc[q] / 3, a = c->vcSrc[q] % 3;
-            const int* f = &Ft[3 * (size_t)t];
-            for (int b = 0; b < 3; ++b) {
-                const int w = f[b];
-                if (fixed[w] || w < u || (w == u && a != b)) continue;
-                const int lo = a < b ? a : b, hi = a < b ? b : a;
-                const int id = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
-                const unsigned int code = (unsigned int)(6 * t + id) | (a > b ? 0x80000000u : 0u);
-                fn(posToUB[P.find(u, w)], code);
-            }
-        }
-    };
-    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
-        for (int u = r0; u < r1; ++u) row_contribs(u, [&](int ub, unsigned int) { c->ubCPtr[ub + 1]++; });
-    });
-    for (int b = 0; b < nUB; ++b) c->ubCPtr[b + 1] += c->ubCPtr[b];
-    c->ubCSrc.assign(c->ubCPtr[nUB], 0);
-    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
-        std::vector<int> fill;
-        for (int u = r0; u < r1; ++u) {
-            const int b0 = ubStart[u], b1 = ubStart[u + 1];
-            fill.assign(c->ubCPtr.begin() + b0, c->ubCPtr.begin() + b1);
-            row_contribs(u, [&](int ub, unsigned int code) { c->ubCSrc[fill[ub - b0]++] = (int)code; });
-        }
-    });
-
+    build_pattern_maps(nVt, nFt, Ft, c->fixedMaskSys.data(), c->vcPtr, c->vcSrc, c->sys, c->ubRow, c->ubCol,
+                       c->ubPosU, c->ubPosL, c->ubCPtr, c->ubCSrc);
     if (upload_pattern(c, c->sys)) return OC_ERR_HIP;
     // The PCG matrix stays FP64 in both precisions: rounding the assembled
     // Hessian to FP32 can make it indefinite along its near-null directions
@@ -617,7 +638,7 @@ int build_system(oc_ctx* c)
     OC_HIP(c, c->dElemH.ensure(sizeof(double) * 24 * (size_t)std::max(1, nFt)));
     OC_HIP(c, c->dG.ensure(sizeof(double) * 2 * (size_t)std::max(1, nVt)));
     if (ensure_pcg(c, nVt)) return OC_ERR_HIP;
-    if (ensure_scratch(c, std::max(nFt, nUB))) return OC_ERR_HIP;
+    if (ensure_scratch(c, std::max(nFt, (int)c->ubRow.size()))) return OC_ERR_HIP;
     c->amgValid = false;
     c->amg = nullptr;
     c->ext = nullptr;
@@ -646,12 +667,12 @@ int build_system(oc_ctx* c)
             c->err = e;
             return OC_ERR_HIP;
         }
-        if (c->amg->rz_records() > kRzRecords) {
+        c->ext = c->amg;
+        if (c->ext->rz_records() > kRzRecords) {
             c->err = "AMG: too many partial records";
             return OC_ERR_STATE;
         }
         c->amgValid = true;
-        c->ext = c->amg;
     }
     OC_HIP(c, hipStreamSynchronize(c->stream));
     c->sysReady = true;
@@ -1756,7 +1777,7 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S)
     // faces, early Newton steps change the matrix enough to multiply the PCG
     // iterations (12 -> 733), which costs more than the set-up saved.
     if (c->amgValid) {
-        c->amg->numeric(s, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>());
+        amg_numeric(c, 0);
         ++S->precond_setups;
     }
     if (c->nPrecond == OC_PRECOND_LEARNED) c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
@@ -1771,7 +1792,7 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S)
         S->pcg_iters_first = it;
         S->pcg_fallback = 2;
         if (c->amgValid) {
-            c->amg->numeric(s, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>(), 1);
+            amg_numeric(c, 1);
             ++S->precond_setups;
             int it2 = 0;
             st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, c->ext);
@@ -2165,8 +2186,7 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
             c->ext->apply(s, c->dR.as<double>(), c->dTmpVec.as<double>(), c->dPartials.as<double>(), nullptr);
             break;
         case 8:
-            if (c->amgValid)
-                c->amg->numeric(s, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>());
+            if (c->amgValid) amg_numeric(c, 0);
             else
                 c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
             break;

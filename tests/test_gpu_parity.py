@@ -195,6 +195,73 @@ def test_newton_step_with_scaffold_matches_oracle(ctx):
     np.testing.assert_allclose(ctx.get_uv(), X[: V.shape[0]], rtol=0, atol=1e-9)
 
 
+def test_newton_step_with_scaffold_amg_matches_oracle(ctx):
+    """Mesh + air mesh with the multigrid preconditioner at a tight PCG tolerance: the
+    oracle's trajectory."""
+    m = 11
+    V, F, UV = height_field_disc(m, seed=3)
+    airUV, Fair, bnd = grid_air_mesh(m, rings=2)
+    a0, w_scaf = 1.0, 0.01
+    area_thres = np.sqrt(3.0) / 4.0 * (0.5 * O.RestFeatures(V, F).avgEdgeLen) ** 2
+    so = ScaffoldOracle(V, F, airUV, Fair, bnd, area_thres, a0, w_scaf)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    ctx.set_air_mesh(airUV, Fair, bnd, area_thres, w_scaf)
+    E0 = ctx.newton_init(a0=a0, rel_gl2_tol=1e-30, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                         pcg_rel_tol=1e-13, pcg_max_iter=2000)
+    X = np.concatenate([UV, airUV[len(bnd):]])
+    assert E0 == pytest.approx(so.energy(X)[0], rel=1e-13)
+    for k in range(3):
+        X, Eo_new, step, halv, gsq = so.newton_step(X, None)
+        s, done = ctx.newton_step()
+        print(k, s, flush=True)
+        assert s["pcg_fallback"] == 0
+        assert s["E_total"] == pytest.approx(Eo_new, rel=1e-10), k
+        assert s["step"] == pytest.approx(step, rel=1e-7), k
+
+
+def test_scaffold_amg_200k_faces_new_air_meshes(ctx):
+    """200k faces with a 5k-triangle air band: the multigrid step agrees with Jacobi's,
+    converges in a bounded number of iterations, and a replaced air mesh (new pattern,
+    new hierarchy) solves correctly."""
+    m = 317
+    V, F, UV = height_field_disc(m, seed=0)
+    airUV, Fair, bnd = grid_air_mesh(m, rings=2)
+    area_thres = np.sqrt(3.0) / 4.0 * (0.5 * O.RestFeatures(V, F).avgEdgeLen) ** 2
+    out = {}
+    for prec in (capi.PRECOND_AMG, capi.PRECOND_JACOBI):
+        ctx.set_mesh(V, F, fixed=(0,))
+        ctx.set_uv(UV)
+        ctx.set_air_mesh(airUV, Fair, bnd, area_thres, 0.01)
+        ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=prec, pcg_rel_tol=1e-10,
+                        pcg_max_iter=200000)
+        s, _ = ctx.newton_step()
+        print(prec, s, flush=True)
+        out[prec] = (s, ctx.get_uv())
+    s_amg, uv_amg = out[capi.PRECOND_AMG]
+    s_jac, uv_jac = out[capi.PRECOND_JACOBI]
+    assert s_amg["pcg_fallback"] == 0 and s_amg["pcg_iters"] < 150
+    assert s_amg["E_total"] == pytest.approx(s_jac["E_total"], rel=1e-8)
+    assert np.abs(uv_amg - uv_jac).max() < 1e-6
+    # a second air mesh (two rings wider) replacing the first before the solve:
+    # re-initialise and step
+    airUV2, Fair2, bnd2 = grid_air_mesh(m, rings=4)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    ctx.set_air_mesh(airUV, Fair, bnd, area_thres, 0.01)
+    ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                    pcg_rel_tol=1e-6, pcg_max_iter=1000)
+    ctx.set_air_mesh(airUV2, Fair2, bnd2, area_thres, 0.01)
+    E_prev = ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                             pcg_rel_tol=1e-6, pcg_max_iter=1000)
+    for k in range(3):
+        s, _ = ctx.newton_step()
+        print("air swap", k, s, flush=True)
+        assert s["pcg_fallback"] == 0 and s["E_total"] < E_prev
+        E_prev = s["E_total"]
+    assert O.check_inversion(F, ctx.get_uv())
+
+
 def test_newton_200k_faces_properties(ctx):
     """Full-size workload (317x317 grid, 199,712 faces): size-independent properties."""
     V, F, UV = height_field_disc(317, seed=0)
