@@ -82,3 +82,37 @@ def test_traces_with_scaffold_report_the_mesh_gradient(ctx, tmp_path):
     # deformed it, the total and the mesh term differ
     assert all(abs(a - b) > 1e-6 * a for a, b in zip(gtot[1:], gsd[1:])), (gtot, gsd)
     assert len(_read(tmp_path / "energyValPerIter.txt")) == 4
+
+
+def test_obj_to_tutte_to_newton_to_obj(ctx, tmp_path):
+    """The geometry step end to end from files: a disc read from OBJ, Tutte's initial map,
+    Newton with the multigrid preconditioner to the gradient tolerance, the result written
+    as TriMesh::saveAsMesh writes it and read back."""
+    import os
+    src = os.path.join(os.path.dirname(__file__), "golden", "obj", "disc7.obj")
+    V, UV0, F, FUV = capi.obj_read(src)
+    assert FUV is not None and np.array_equal(FUV, F)
+    ctx.set_mesh(V, F, fixed=(0,))
+    UV, nl, nb, _ = ctx.tutte(precond=capi.PRECOND_AMG, rel_tol=1e-12, max_iter=1000)
+    assert nl == 1 and nb == 24 and O.check_inversion(F, UV)
+    ctx.trace_open(str(tmp_path))
+    ctx.newton_init(a0=1.0, rel_gl2_tol=1e-12, allow_edec_rel_tol=True, precond=capi.PRECOND_AMG,
+                    pcg_rel_tol=1e-8, pcg_max_iter=1000)
+    for _ in range(50):
+        s, done = ctx.newton_step()
+        if done:
+            break
+    assert done
+    ctx.trace_close()
+    UVf = ctx.get_uv()
+    feat = O.RestFeatures(V, F)
+    assert O.check_inversion(F, UVf)
+    assert O.energy(F, UVf, feat) < O.energy(F, UV, feat)
+    out = str(tmp_path / "result_mesh.obj")
+    capi.obj_write_mesh(out, V, UVf, F, F, scale_uv=True)
+    V2, UV2, F2, FUV2 = capi.obj_read(out)
+    np.testing.assert_allclose(V2, V, rtol=6e-7, atol=1e-12)
+    np.testing.assert_array_equal(F2, F)
+    span = (UVf.max(0) - UVf.min(0)).max()
+    np.testing.assert_allclose(UV2, (UVf - UVf.min(0)) / span, rtol=0, atol=1e-6)
+    assert len(_read(tmp_path / "gradientPerIter.txt")) >= 2
