@@ -1217,16 +1217,16 @@ __global__ void __launch_bounds__(kBlock) k_finish_z(const int* __restrict__ sto
     if (threadIdx.x == 0) rzPart[blockIdx.x] = acc[0];
 }
 
-template <typename T>
-void dalloc(T*& p, size_t n)
+template <typename P, typename T>
+void dalloc(P& pool, T*& p, size_t n)
 {
-    if (hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) p = nullptr;
+    p = static_cast<T*>(pool.get(sizeof(T) * std::max<size_t>(n, 1)));
 }
 
-template <typename V>
-void up(hipStream_t s, int*& d, const std::vector<V>& h)
+template <typename P, typename V>
+void up(P& pool, hipStream_t s, int*& d, const std::vector<V>& h)
 {
-    dalloc(d, h.size());
+    dalloc(pool, d, h.size());
     if (d && !h.empty()) (void)hipMemcpyAsync(d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice, s);
 }
 
@@ -1243,31 +1243,76 @@ struct Amg<T>::Level : AmgHostLevel {
     unsigned char* dPIsTent = nullptr;
     T *vals = nullptr, *Dinv = nullptr, *Pt = nullptr, *P = nullptr, *Tv = nullptr;
     T *b = nullptr, *x = nullptr, *x2 = nullptr, *r = nullptr, *d = nullptr, *d2 = nullptr;
-    double *Bnext = nullptr, *lz = nullptr, *lam = nullptr;  // lz: 5 Lanczos vectors, then alpha/beta/count
-    ~Level()
+    double *Bnext = nullptr, *lz = nullptr, *lam = nullptr;  // lz: 6 Lanczos vectors, then alpha/beta/count
+    // hand every buffer back to the owner's pool
+    void give_back(Pool& pool)
     {
         for (void* p : {(void*)dRowPtr, (void*)dColIdx, (void*)dDiagPos, (void*)dAggPtr, (void*)dAggMem, (void*)dPPtr,
                         (void*)dPCol, (void*)dPRow, (void*)dPsPtr, (void*)dPsA, (void*)dRPtr, (void*)dRP,
                         (void*)dTcPtr, (void*)dTcA, (void*)dTcP, (void*)dAcPtr, (void*)dAcP, (void*)dAcT,
                         (void*)dPIsTent, (void*)vals, (void*)Dinv, (void*)Pt, (void*)P, (void*)Tv, (void*)b,
                         (void*)x, (void*)x2, (void*)r, (void*)d, (void*)d2, (void*)Bnext, (void*)lz, (void*)lam})
-            if (p) (void)hipFree(p);
+            if (p) pool.put(p);
     }
 };
+
+template <typename T>
+void* Amg<T>::Pool::get(size_t bytes)
+{
+    // smallest free buffer that fits and wastes at most half of itself
+    auto it = free.lower_bound(bytes);
+    if (it != free.end() && it->first <= 2 * bytes) {
+        void* p = it->second;
+        free.erase(it);
+        return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    size[p] = bytes;
+    return p;
+}
+
+template <typename T>
+void Amg<T>::Pool::put(void* p)
+{
+    free.emplace(size.at(p), p);
+}
+
+template <typename T>
+void Amg<T>::Pool::trim()
+{
+    for (auto& kv : free) {
+        (void)hipFree(kv.second);
+        size.erase(kv.second);
+    }
+    free.clear();
+}
+
+template <typename T>
+void Amg<T>::Pool::clear()
+{
+    for (auto& kv : size) (void)hipFree(kv.first);
+    free.clear();
+    size.clear();
+}
 
 template <typename T>
 Amg<T>::~Amg()
 {
     release();
+    pool_.clear();
 }
 
 template <typename T>
 void Amg<T>::release()
 {
-    for (Level* l : L) delete l;
+    for (Level* l : L) {
+        l->give_back(pool_);
+        delete l;
+    }
     L.clear();
-    if (dCoarseInv) (void)hipFree(dCoarseInv);
-    if (dPartials) (void)hipFree(dPartials);
+    if (dCoarseInv) pool_.put(dCoarseInv);
+    if (dPartials) pool_.put(dPartials);
     dCoarseInv = nullptr;
     dPartials = nullptr;
     ready_ = false;
@@ -1336,43 +1381,43 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
     for (size_t l = 0; l < L.size(); ++l) {
         Level& A = *L[l];
         const int bs = A.bs;
-        up(s, A.dRowPtr, A.rowPtr);
-        up(s, A.dColIdx, A.colIdx);
-        up(s, A.dDiagPos, A.diagPos);
-        dalloc(A.vals, A.colIdx.size() * bs * bs);
-        dalloc(A.Dinv, (size_t)A.n * bs * bs);
-        dalloc(A.b, (size_t)A.n * bs);
-        dalloc(A.x, (size_t)A.n * bs);
-        dalloc(A.x2, (size_t)A.n * bs);
-        dalloc(A.r, (size_t)A.n * bs);
-        dalloc(A.d, (size_t)A.n * bs);
-        dalloc(A.d2, (size_t)A.n * bs);
-        dalloc(A.lam, 2);
-        dalloc(A.lz, (size_t)6 * A.n * bs + 2 * kLanczosSteps + 1);
+        up(pool_, s, A.dRowPtr, A.rowPtr);
+        up(pool_, s, A.dColIdx, A.colIdx);
+        up(pool_, s, A.dDiagPos, A.diagPos);
+        dalloc(pool_, A.vals, A.colIdx.size() * bs * bs);
+        dalloc(pool_, A.Dinv, (size_t)A.n * bs * bs);
+        dalloc(pool_, A.b, (size_t)A.n * bs);
+        dalloc(pool_, A.x, (size_t)A.n * bs);
+        dalloc(pool_, A.x2, (size_t)A.n * bs);
+        dalloc(pool_, A.r, (size_t)A.n * bs);
+        dalloc(pool_, A.d, (size_t)A.n * bs);
+        dalloc(pool_, A.d2, (size_t)A.n * bs);
+        dalloc(pool_, A.lam, 2);
+        dalloc(pool_, A.lz, (size_t)6 * A.n * bs + 2 * kLanczosSteps + 1);
         if (l + 1 < L.size()) {
-            up(s, A.dAggPtr, A.aggPtr);
-            up(s, A.dAggMem, A.aggMem);
-            up(s, A.dPPtr, A.pPtr);
-            up(s, A.dPCol, A.pCol);
-            up(s, A.dPRow, A.pRow);
-            up(s, A.dPsPtr, A.psPtr);
-            up(s, A.dPsA, A.psA);
-            up(s, A.dRPtr, A.rPtr);
-            up(s, A.dRP, A.rP);
-            up(s, A.dTcPtr, A.tcPtr);
-            up(s, A.dTcA, A.tcA);
-            up(s, A.dTcP, A.tcP);
-            up(s, A.dAcPtr, A.acPtr);
-            up(s, A.dAcP, A.acP);
-            up(s, A.dAcT, A.acT);
-            dalloc(A.dPIsTent, A.pIsTent.size());
+            up(pool_, s, A.dAggPtr, A.aggPtr);
+            up(pool_, s, A.dAggMem, A.aggMem);
+            up(pool_, s, A.dPPtr, A.pPtr);
+            up(pool_, s, A.dPCol, A.pCol);
+            up(pool_, s, A.dPRow, A.pRow);
+            up(pool_, s, A.dPsPtr, A.psPtr);
+            up(pool_, s, A.dPsA, A.psA);
+            up(pool_, s, A.dRPtr, A.rPtr);
+            up(pool_, s, A.dRP, A.rP);
+            up(pool_, s, A.dTcPtr, A.tcPtr);
+            up(pool_, s, A.dTcA, A.tcA);
+            up(pool_, s, A.dTcP, A.tcP);
+            up(pool_, s, A.dAcPtr, A.acPtr);
+            up(pool_, s, A.dAcP, A.acP);
+            up(pool_, s, A.dAcT, A.acT);
+            dalloc(pool_, A.dPIsTent, A.pIsTent.size());
             if (A.dPIsTent && !A.pIsTent.empty())
                 (void)hipMemcpyAsync(A.dPIsTent, A.pIsTent.data(), A.pIsTent.size(), hipMemcpyHostToDevice, s);
-            dalloc(A.Pt, (size_t)A.n * bs * NS);
+            dalloc(pool_, A.Pt, (size_t)A.n * bs * NS);
             (void)hipMemsetAsync(A.Pt, 0, sizeof(T) * (size_t)A.n * bs * NS, s);
-            dalloc(A.P, A.pCol.size() * bs * NS);
-            dalloc(A.Tv, A.tCol.size() * bs * NS);
-            dalloc(A.Bnext, (size_t)A.nc * NS * NS);
+            dalloc(pool_, A.P, A.pCol.size() * bs * NS);
+            dalloc(pool_, A.Tv, A.tCol.size() * bs * NS);
+            dalloc(pool_, A.Bnext, (size_t)A.nc * NS * NS);
         }
     }
     const Level& C = *L.back();
@@ -1384,14 +1429,15 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
         release();
         return -1;
     }
-    if (!coarseDiag) dalloc(dCoarseInv, (size_t)nCoarse * nCoarse);
+    if (!coarseDiag) dalloc(pool_, dCoarseInv, (size_t)nCoarse * nCoarse);
     nPartials = 2 * kAmgMaxGrid;  // two doubles per workgroup record
-    dalloc(dPartials, (size_t)nPartials * 2 + kAmgMaxGrid);  // two record buffers + Gershgorin records
+    dalloc(pool_, dPartials, (size_t)nPartials * 2 + kAmgMaxGrid);  // two record buffers + Gershgorin records
     if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) {
         err = "AMG: device allocation or upload failed";
         release();
         return -1;
     }
+    pool_.trim();  // buffers of the previous hierarchy this one did not reuse
     ready_ = true;
     return 0;
 }

@@ -26,6 +26,7 @@
 #include "oc_precond.hpp"
 
 #include <string>
+#include <map>
 #include <vector>
 
 namespace oc {
@@ -94,6 +95,17 @@ private:
     double* dCoarseInv = nullptr;  // nCoarse x nCoarse, double
     double* dPartials = nullptr;
     int nPartials = 0;
+    // Device buffers of released hierarchies, reused by the next set-up (a new
+    // pattern of similar size, e.g. after the caller replaces the air mesh,
+    // then needs no hipFree/hipMalloc).
+    struct Pool {
+        std::multimap<size_t, void*> free;
+        std::map<void*, size_t> size;
+        void* get(size_t bytes);
+        void put(void* p);
+        void trim();   // free the buffers nobody took back
+        void clear();  // free everything
+    } pool_;
     bool ready_ = false;
     void release();
     void vcycle(hipStream_t s, int l, const int* stop);
