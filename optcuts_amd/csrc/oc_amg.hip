@@ -1369,9 +1369,13 @@ template <typename T>
 int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
                   const std::vector<unsigned char>& fixedMask, std::string& err)
 {
-    release();
+    // The previous hierarchy's finest rows are reused where they come out
+    // the same (an air-mesh change leaves most of the mesh rows as they were).
     std::vector<AmgHostLevel> H;
-    if (amg_build_structure(nV, rowPtr, colIdx, fixedMask, kCoarseMaxDofs, kMaxLevels, H, err) != 0) return -1;
+    const AmgPrevious prev{L.size() > 1 ? L[0] : nullptr, L.size() > 1 ? L[1] : nullptr};
+    const int st = amg_build_structure(nV, rowPtr, colIdx, fixedMask, kCoarseMaxDofs, kMaxLevels, H, err, &prev);
+    release();
+    if (st != 0) return -1;
     for (auto& h : H) {
         Level* lv = new Level();
         static_cast<AmgHostLevel&>(*lv) = std::move(h);

@@ -4,6 +4,9 @@
 #pragma once
 
 #include <algorithm>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <numeric>
 #include <string>
@@ -12,18 +15,124 @@
 
 namespace oc {
 
+// Persistent host workers for the row-chunk loops of the symbolic set-up
+// (pattern, slot maps, multigrid structure): a set-up runs a dozen parallel
+// regions, and starting threads per region would cost more than some of them.
+// One region at a time; a caller that finds the pool busy (another context's
+// set-up on another host thread) starts its own threads instead.
+class RowPool {
+public:
+    static RowPool& instance()
+    {
+        static RowPool pool;
+        return pool;
+    }
+    int workers() const { return (int)workers_.size(); }
+    // Runs task(c) for c in [0, n) on the workers and the calling thread;
+    // false (nothing run) when the pool is busy.
+    bool run(int n, const std::function<void(int)>& task)
+    {
+        std::unique_lock<std::mutex> busy(busy_, std::try_to_lock);
+        if (!busy.owns_lock()) return false;
+        {
+            std::lock_guard<std::mutex> g(m_);
+            task_ = &task;
+            n_ = n;
+            next_ = 0;
+            pending_ = n;
+            ++gen_;
+        }
+        cv_.notify_all();
+        work(&task, n);
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [&] { return pending_ == 0 && active_ == 0; });
+        task_ = nullptr;
+        return true;
+    }
+
+private:
+    RowPool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int nw = (int)std::min<unsigned>(hw ? hw : 1, 16) - 1;
+        for (int i = 0; i < nw; ++i) workers_.emplace_back([this] { loop(); });
+    }
+    ~RowPool()
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    // Claims chunks until none is left. The region's task and size are read
+    // under the lock by the worker before it enters, and run() returns only
+    // after every worker that entered has left, so a late worker never sees
+    // the next region's counter with this region's task.
+    void work(const std::function<void(int)>* task, int n)
+    {
+        for (;;) {
+            int c;
+            {
+                std::lock_guard<std::mutex> g(m_);
+                c = next_ < n ? next_++ : -1;
+            }
+            if (c < 0) return;
+            (*task)(c);
+            std::lock_guard<std::mutex> g(m_);
+            if (--pending_ == 0) done_.notify_all();
+        }
+    }
+    void loop()
+    {
+        unsigned long seen = 0;
+        for (;;) {
+            const std::function<void(int)>* task;
+            int n;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return stop_ || (gen_ != seen && task_); });
+                if (stop_) return;
+                seen = gen_;
+                task = task_;
+                n = n_;
+                ++active_;
+            }
+            work(task, n);
+            std::lock_guard<std::mutex> g(m_);
+            if (--active_ == 0) done_.notify_all();
+        }
+    }
+
+    std::mutex busy_, m_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> workers_;
+    const std::function<void(int)>* task_ = nullptr;
+    int n_ = 0, next_ = 0, pending_ = 0, active_ = 0;
+    unsigned long gen_ = 0;
+    bool stop_ = false;
+};
+
 // Row-block parallel loop for the symbolic set-up: fn(chunk, r0, r1) on
-// nChunks contiguous row ranges, each on its own host thread. Callers write
-// chunk-local outputs and concatenate them in chunk order, so the result does
-// not depend on the thread count.
+// nChunks contiguous row ranges, in parallel. Callers write chunk-local
+// outputs or disjoint row ranges, so the result does not depend on the number
+// or order of threads.
 template <typename Fn>
 inline void parallel_row_chunks(int n, int nChunks, Fn&& fn)
 {
-    std::vector<std::thread> th;
-    for (int c = 0; c < nChunks; ++c) {
+    const auto range = [&](int c) {
         const int r0 = (int)((long)n * c / nChunks), r1 = (int)((long)n * (c + 1) / nChunks);
-        th.emplace_back([&fn, c, r0, r1] { fn(c, r0, r1); });
+        fn(c, r0, r1);
+    };
+    if (nChunks <= 1) {
+        if (nChunks == 1) range(0);
+        return;
     }
+    const std::function<void(int)> task(range);
+    if (RowPool::instance().run(nChunks, task)) return;
+    std::vector<std::thread> th;
+    for (int c = 0; c < nChunks; ++c) th.emplace_back([&task, c] { task(c); });
     for (auto& t : th) t.join();
 }
 
@@ -45,8 +154,8 @@ struct SymRows {
     std::vector<int> a, b;    // pairs
 };
 
-template <typename Gen>
-inline void symbolic_rows(int r0, int r1, int ncols, Gen&& gen, SymRows& out)
+template <typename Gen, typename Skip>
+inline void symbolic_rows(int r0, int r1, int ncols, Gen&& gen, Skip&& skip, SymRows& out)
 {
     struct Item {
         int c, x, y;
@@ -54,6 +163,7 @@ inline void symbolic_rows(int r0, int r1, int ncols, Gen&& gen, SymRows& out)
     std::vector<Item> buf;
     std::vector<int> slot(ncols, -1), cols, cnt, fill;
     for (int r = r0; r < r1; ++r) {
+        if (skip(r)) continue;
         buf.clear();
         gen(r, [&](int c, int x, int y) { buf.push_back(Item{c, x, y}); });
         cols.clear();
@@ -86,44 +196,87 @@ inline void symbolic_rows(int r0, int r1, int ncols, Gen&& gen, SymRows& out)
     }
 }
 
-// Concatenates chunk results into CSR row pointers, columns, pair-group
-// pointers and pairs.
-inline void concat_rows(int n, std::vector<SymRows>& parts, std::vector<int>& rowPtr, std::vector<int>& col,
-                        std::vector<int>& grpPtr, std::vector<int>& pa, std::vector<int>& pb)
+// A symbolic product of an earlier set-up (CSR rows, pair-group ends, pairs),
+// whose rows are reused where they come out the same.
+struct ProductRef {
+    const std::vector<int>* rowPtr = nullptr;
+    const std::vector<int>* col = nullptr;
+    const std::vector<int>* grp = nullptr;
+    const std::vector<int>* pa = nullptr;
+    const std::vector<int>* pb = nullptr;
+};
+
+// Rows [0, n) of a symbolic product: CSR row pointers and columns, per column
+// entry the end of its pair group (grpPtr[0] = 0) and the pairs pa/pb (pb may
+// be null). Row r comes from gen unless keep(r): then it is row r of the
+// earlier product `old` with its pairs renumbered by mapA(r, a) and
+// mapB(r, a, b) (the blocks they name have moved). Generated rows run in row
+// chunks on host threads and every row is written at its final offset, so
+// the result does not depend on the thread count.
+template <typename Gen, typename Keep, typename MapA, typename MapB>
+inline void product_rows(int n, int ncols, int nt, Gen&& gen, Keep&& keep, const ProductRef& old, MapA&& mapA,
+                         MapB&& mapB, std::vector<int>& rowPtr, std::vector<int>& col, std::vector<int>& grpPtr,
+                         std::vector<int>& pa, std::vector<int>* pb)
 {
-    rowPtr.assign(n + 1, 0);
-    col.clear();
-    grpPtr.assign(1, 0);
-    pa.clear();
-    pb.clear();
-    size_t nc = 0, np = 0;
-    for (auto& P : parts) {
-        nc += P.cols.size();
-        np += P.a.size();
-    }
-    col.reserve(nc);
-    grpPtr.reserve(nc + 1);
-    pa.reserve(np);
-    pb.reserve(np);
-    int r = 0;
-    for (auto& P : parts) {
-        size_t ci = 0, pbase = 0;
-        for (int k : P.nCols) {
-            // pairs of this row start at pbase in P.a; group ends are row-local
-            int rowPairs = 0;
-            for (int q = 0; q < k; ++q) {
-                col.push_back(P.cols[ci + q]);
-                grpPtr.push_back((int)(pa.size() + P.grpEnd[ci + q]));
-                rowPairs = P.grpEnd[ci + q];
+    std::vector<SymRows> parts(nt);
+    std::vector<int> nCol(n), nPair(n);
+    parallel_row_chunks(n, nt, [&](int ch, int r0, int r1) {
+        SymRows& S = parts[ch];
+        symbolic_rows(r0, r1, ncols, gen, keep, S);
+        size_t ci = 0, ri = 0;
+        for (int r = r0; r < r1; ++r) {
+            if (keep(r)) {
+                const int e0 = (*old.rowPtr)[r], e1 = (*old.rowPtr)[r + 1];
+                nCol[r] = e1 - e0;
+                nPair[r] = (*old.grp)[e1] - (*old.grp)[e0];
+            } else {
+                const int k = S.nCols[ri++];
+                nCol[r] = k;
+                nPair[r] = k ? S.grpEnd[ci + k - 1] : 0;
+                ci += k;
             }
-            pa.insert(pa.end(), P.a.begin() + pbase, P.a.begin() + pbase + rowPairs);
-            pb.insert(pb.end(), P.b.begin() + pbase, P.b.begin() + pbase + rowPairs);
-            pbase += rowPairs;
-            ci += k;
-            rowPtr[r + 1] = rowPtr[r] + k;
-            ++r;
         }
+    });
+    std::vector<int> pairStart(n + 1, 0);
+    rowPtr.assign(n + 1, 0);
+    for (int r = 0; r < n; ++r) {
+        rowPtr[r + 1] = rowPtr[r] + nCol[r];
+        pairStart[r + 1] = pairStart[r] + nPair[r];
     }
+    col.resize(rowPtr[n]);
+    grpPtr.resize((size_t)rowPtr[n] + 1);
+    grpPtr[0] = 0;
+    pa.resize(pairStart[n]);
+    if (pb) pb->resize(pairStart[n]);
+    parallel_row_chunks(n, nt, [&](int ch, int r0, int r1) {
+        const SymRows& S = parts[ch];
+        size_t ci = 0, ri = 0, pbase = 0;
+        for (int r = r0; r < r1; ++r) {
+            const int e = rowPtr[r], p = pairStart[r];
+            if (keep(r)) {
+                const int e0 = (*old.rowPtr)[r], p0 = (*old.grp)[e0];
+                for (int q = 0; q < nCol[r]; ++q) {
+                    col[e + q] = (*old.col)[e0 + q];
+                    grpPtr[e + q + 1] = (*old.grp)[e0 + q + 1] - p0 + p;
+                }
+                for (int q = 0; q < nPair[r]; ++q) {
+                    const int a = (*old.pa)[p0 + q];
+                    pa[p + q] = mapA(r, a);
+                    if (pb) (*pb)[p + q] = mapB(r, a, (*old.pb)[p0 + q]);
+                }
+            } else {
+                const int k = S.nCols[ri++];
+                for (int q = 0; q < k; ++q) {
+                    col[e + q] = S.cols[ci + q];
+                    grpPtr[e + q + 1] = p + S.grpEnd[ci + q];
+                }
+                std::copy(S.a.begin() + pbase, S.a.begin() + pbase + nPair[r], pa.begin() + p);
+                if (pb) std::copy(S.b.begin() + pbase, S.b.begin() + pbase + nPair[r], pb->begin() + p);
+                ci += k;
+                pbase += nPair[r];
+            }
+        }
+    });
 }
 
 constexpr int kAmgNullDim = 3;  // two translations and a rotation of the UV plane
@@ -142,6 +295,7 @@ struct AmgHostLevel {
     std::vector<int> tPtr, tCol;             // T = A P pattern
     std::vector<int> tcPtr, tcA, tcP;        // per T block: (A block, P block) pairs
     std::vector<int> acPtr, acP, acT;        // per next-level block: (P block, T block) pairs
+    std::vector<int> agg;                    // aggregate of each node (-1: excluded)
 };
 
 // Greedy aggregation (phase 1: roots whose neighbours are all free take
@@ -184,11 +338,20 @@ inline int aggregate_nodes(int n, const std::vector<int>& rowPtr, const std::vec
 }
 
 
+// Levels 0 and 1 of an earlier structure build: rows of the new finest level
+// that come out the same (same matrix row, same aggregates around it) are
+// copied from it instead of regenerated; the result is identical to a build
+// without it.
+struct AmgPrevious {
+    const AmgHostLevel* fine = nullptr;
+    const AmgHostLevel* coarse = nullptr;
+};
+
 // Builds every level down to at most coarseMaxDofs unknowns (or maxLevels
 // levels, or until coarsening stalls). Returns 0, or -1 with a message.
 inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
                                const std::vector<unsigned char>& fixedMask, long coarseMaxDofs, int maxLevels,
-                               std::vector<AmgHostLevel>& L, std::string& err)
+                               std::vector<AmgHostLevel>& L, std::string& err, const AmgPrevious* prev = nullptr)
 {
     L.clear();
     L.emplace_back();
@@ -199,6 +362,7 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
     L[0].excluded = fixedMask;
     for (;;) {
         AmgHostLevel& A = L.back();
+        const bool fine = L.size() == 1;
         A.diagPos.assign(A.n, -1);
         for (int u = 0; u < A.n; ++u)
             for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k)
@@ -221,10 +385,54 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
             nIso += A.rowPtr[u + 1] - A.rowPtr[u] == 1;
         }
         if (nIso == nIn) break;
-        std::vector<int> agg;
+        std::vector<int>& agg = A.agg;
         const int nc = aggregate_nodes(A.n, A.rowPtr, A.colIdx, A.excluded, agg);
         if (nc == 0 || (long)(nc - nIso) * 10 > (long)(nIn - nIso) * 9) break;  // coupled part stopped coarsening
         A.nc = nc;
+        // Rows of an earlier finest level that come out the same: matrix row
+        // (cleanA), prolongator row (cleanP: same aggregates in its closed
+        // neighbourhood), A P row (cleanT: cleanA and every neighbour cleanP);
+        // dA/dP/dT: how far each such row moved in its CSR arrays.
+        const AmgHostLevel* O =
+            (fine && prev && prev->fine && prev->fine->nc > 0 && prev->coarse) ? prev->fine : nullptr;
+        std::vector<unsigned char> cleanA, cleanP, cleanT;
+        std::vector<int> dA, dP, dT;
+        const int nt0 = setup_threads((long)A.colIdx.size());
+        if (O) {
+            cleanA.assign(A.n, 0);
+            cleanP.assign(A.n, 0);
+            cleanT.assign(A.n, 0);
+            dA.assign(A.n, 0);
+            dP.assign(A.n, 0);
+            dT.assign(A.n, 0);
+            parallel_row_chunks(A.n, nt0, [&](int, int r0, int r1) {
+                for (int u = r0; u < r1; ++u) {
+                    if (u >= O->n || A.excluded[u] != O->excluded[u]) continue;
+                    const int k0 = A.rowPtr[u], k1 = A.rowPtr[u + 1], o0 = O->rowPtr[u];
+                    if (k1 - k0 != O->rowPtr[u + 1] - o0 ||
+                        !std::equal(A.colIdx.begin() + k0, A.colIdx.begin() + k1, O->colIdx.begin() + o0))
+                        continue;
+                    cleanA[u] = 1;
+                    dA[u] = k0 - o0;
+                }
+            });
+            parallel_row_chunks(A.n, nt0, [&](int, int r0, int r1) {
+                for (int u = r0; u < r1; ++u) {
+                    if (!cleanA[u]) continue;
+                    bool same = true;
+                    for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1] && same; ++k) same = agg[A.colIdx[k]] == O->agg[A.colIdx[k]];
+                    cleanP[u] = same;
+                }
+            });
+            parallel_row_chunks(A.n, nt0, [&](int, int r0, int r1) {
+                for (int u = r0; u < r1; ++u) {
+                    if (!cleanA[u]) continue;
+                    bool same = true;
+                    for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1] && same; ++k) same = cleanP[A.colIdx[k]];
+                    cleanT[u] = same;
+                }
+            });
+        }
         // aggregate members
         A.aggPtr.assign(nc + 1, 0);
         for (int u = 0; u < A.n; ++u)
@@ -236,25 +444,27 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
             for (int u = 0; u < A.n; ++u)
                 if (agg[u] >= 0) A.aggMem[f[agg[u]]++] = u;
         }
+        const auto none = [](int) { return false; };
         // P pattern: row u -> aggregates of its (non-excluded) closed
         // neighbourhood; with it the smoothing contributions (the A blocks of
         // row u grouped by the aggregate of their column, k ascending).
         {
-            const int nt = setup_threads((long)A.colIdx.size());
-            std::vector<SymRows> parts(nt);
-            parallel_row_chunks(A.n, nt, [&](int ch, int r0, int r1) {
-                symbolic_rows(r0, r1, nc,
-                              [&](int u, auto&& emit) {
-                                  if (A.excluded[u]) return;
-                                  for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
-                                      const int w = A.colIdx[k];
-                                      if (!A.excluded[w]) emit(agg[w], k, 0);
-                                  }
-                              },
-                              parts[ch]);
-            });
-            std::vector<int> unused;
-            concat_rows(A.n, parts, A.pPtr, A.pCol, A.psPtr, A.psA, unused);
+            const auto gen = [&](int u, auto&& emit) {
+                if (A.excluded[u]) return;
+                for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
+                    const int w = A.colIdx[k];
+                    if (!A.excluded[w]) emit(agg[w], k, 0);
+                }
+            };
+            const auto mapA = [&](int u, int k) { return k + dA[u]; };
+            const auto mapB = [](int, int, int b) { return b; };
+            if (O)
+                product_rows(A.n, nc, nt0, gen, [&](int u) { return cleanP[u] != 0; },
+                             ProductRef{&O->pPtr, &O->pCol, &O->psPtr, &O->psA, nullptr}, mapA, mapB, A.pPtr,
+                             A.pCol, A.psPtr, A.psA, nullptr);
+            else
+                product_rows(A.n, nc, nt0, gen, none, ProductRef{}, mapA, mapB, A.pPtr, A.pCol, A.psPtr, A.psA,
+                             nullptr);
         }
         const int nP = (int)A.pCol.size();
         A.pRow.assign(nP, 0);
@@ -264,6 +474,9 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
                 A.pRow[j] = u;
                 A.pIsTent[j] = A.pCol[j] == agg[u] ? 1 : 0;
             }
+        if (O)
+            for (int u = 0; u < A.n; ++u)
+                if (cleanP[u]) dP[u] = A.pPtr[u] - O->pPtr[u];
         // restriction: P blocks by column
         A.rPtr.assign(nc + 1, 0);
         for (int j = 0; j < nP; ++j) A.rPtr[A.pCol[j] + 1]++;
@@ -277,22 +490,29 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
         // coarse column in generation order (k ascending, then j).
         {
             const int nt = setup_threads((long)A.pCol.size() * 8);
-            std::vector<SymRows> parts(nt);
-            parallel_row_chunks(A.n, nt, [&](int ch, int r0, int r1) {
-                symbolic_rows(r0, r1, nc,
-                              [&](int u, auto&& emit) {
-                                  for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
-                                      const int w = A.colIdx[k];
-                                      for (int j = A.pPtr[w]; j < A.pPtr[w + 1]; ++j) emit(A.pCol[j], k, j);
-                                  }
-                              },
-                              parts[ch]);
-            });
-            concat_rows(A.n, parts, A.tPtr, A.tCol, A.tcPtr, A.tcA, A.tcP);
+            const auto gen = [&](int u, auto&& emit) {
+                for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
+                    const int w = A.colIdx[k];
+                    for (int j = A.pPtr[w]; j < A.pPtr[w + 1]; ++j) emit(A.pCol[j], k, j);
+                }
+            };
+            const auto mapA = [&](int u, int k) { return k + dA[u]; };
+            const auto mapB = [&](int, int, int j) { return j + dP[O->pRow[j]]; };
+            if (O)
+                product_rows(A.n, nc, nt, gen, [&](int u) { return cleanT[u] != 0; },
+                             ProductRef{&O->tPtr, &O->tCol, &O->tcPtr, &O->tcA, &O->tcP}, mapA, mapB, A.tPtr, A.tCol,
+                             A.tcPtr, A.tcA, &A.tcP);
+            else
+                product_rows(A.n, nc, nt, gen, none, ProductRef{}, mapA, mapB, A.tPtr, A.tCol, A.tcPtr, A.tcA,
+                             &A.tcP);
         }
+        if (O)
+            for (int u = 0; u < A.n; ++u)
+                if (cleanT[u]) dT[u] = A.tPtr[u] - O->tPtr[u];
         // coarse A = P^T T: pattern and (P block, T block) pairs, rows by coarse
         // node: over the P blocks (u, c1) in increasing block order (the
-        // restriction lists), the T entries of row u.
+        // restriction lists), the T entries of row u. A coarse row is reused
+        // when its restriction list has the same rows, each cleanP and cleanT.
         AmgHostLevel nxl;
         AmgHostLevel* nx = &nxl;
         nx->n = nc;
@@ -300,19 +520,38 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
         nx->excluded.assign(nc, 0);
         {
             const int nt = setup_threads((long)A.tCol.size() * 4);
-            std::vector<SymRows> parts(nt);
-            parallel_row_chunks(nc, nt, [&](int ch, int r0, int r1) {
-                symbolic_rows(r0, r1, nc,
-                              [&](int c1, auto&& emit) {
-                                  for (int q = A.rPtr[c1]; q < A.rPtr[c1 + 1]; ++q) {
-                                      const int j = A.rP[q];
-                                      const int u = A.pRow[j];
-                                      for (int t = A.tPtr[u]; t < A.tPtr[u + 1]; ++t) emit(A.tCol[t], j, t);
-                                  }
-                              },
-                              parts[ch]);
-            });
-            concat_rows(nc, parts, nx->rowPtr, nx->colIdx, A.acPtr, A.acP, A.acT);
+            std::vector<unsigned char> cleanC;
+            if (O) {
+                cleanC.assign(nc, 0);
+                parallel_row_chunks(nc, nt, [&](int, int r0, int r1) {
+                    for (int c1 = r0; c1 < r1 && c1 < O->nc; ++c1) {
+                        const int q0 = A.rPtr[c1], q1 = A.rPtr[c1 + 1], o0 = O->rPtr[c1];
+                        if (q1 - q0 != O->rPtr[c1 + 1] - o0) continue;
+                        bool same = true;
+                        for (int q = 0; q < q1 - q0 && same; ++q) {
+                            const int u = A.pRow[A.rP[q0 + q]];
+                            same = u == O->pRow[O->rP[o0 + q]] && cleanP[u] && cleanT[u];
+                        }
+                        cleanC[c1] = same;
+                    }
+                });
+            }
+            const auto gen = [&](int c1, auto&& emit) {
+                for (int q = A.rPtr[c1]; q < A.rPtr[c1 + 1]; ++q) {
+                    const int j = A.rP[q];
+                    const int u = A.pRow[j];
+                    for (int t = A.tPtr[u]; t < A.tPtr[u + 1]; ++t) emit(A.tCol[t], j, t);
+                }
+            };
+            const auto mapA = [&](int, int j) { return j + dP[O->pRow[j]]; };
+            const auto mapB = [&](int, int j, int t) { return t + dT[O->pRow[j]]; };
+            if (O)
+                product_rows(nc, nc, nt, gen, [&](int c1) { return cleanC[c1] != 0; },
+                             ProductRef{&prev->coarse->rowPtr, &prev->coarse->colIdx, &O->acPtr, &O->acP, &O->acT},
+                             mapA, mapB, nx->rowPtr, nx->colIdx, A.acPtr, A.acP, &A.acT);
+            else
+                product_rows(nc, nc, nt, gen, none, ProductRef{}, mapA, mapB, nx->rowPtr, nx->colIdx, A.acPtr,
+                             A.acP, &A.acT);
         }
         L.push_back(std::move(nxl));
     }

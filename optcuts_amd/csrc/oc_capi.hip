@@ -8,6 +8,7 @@
 #include "oc_gnn.hpp"
 #include "oc_io.hpp"
 #include "oc_kernels.hpp"
+#include "oc_system_pattern.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -51,19 +52,9 @@ struct DevBuf {
 };
 
 // Full symmetric 2x2-block CSR over nV vertices.
-struct BlockPattern {
-    int nV = 0;
-    std::vector<int> rowPtr, colIdx, diagPos;
+struct BlockPattern : oc::HostPattern {
     DevBuf dRowPtr, dColIdx, dDiagPos, dVals, dMinv;
     int vals32 = 0;  // dVals holds float blocks (room for double blocks is always allocated)
-    long nnzb() const { return colIdx.size(); }
-    int find(int u, int w) const
-    {
-        const int* b = colIdx.data() + rowPtr[u];
-        const int* e = colIdx.data() + rowPtr[u + 1];
-        const int* p = std::lower_bound(b, e, w);
-        return (p != e && *p == w) ? int(p - colIdx.data()) : -1;
-    }
 };
 
 }  // namespace
@@ -115,7 +106,14 @@ struct oc_ctx {
     bool sysReady = false;
     int nVt = 0, nFt = 0;
     BlockPattern sys;
-    std::vector<int> ubRow, ubCol, ubPosU, ubPosL, ubCPtr, ubCSrc, vcPtr, vcSrc;
+    oc::SystemMaps sm;  // upper blocks of sys and their assembly map
+    std::vector<int> vcPtr, vcSrc;
+    // what sys and sm were built from: the mesh (meshGen of oc_set_mesh) and
+    // the mesh vertices in air triangles; rows of the other mesh vertices are
+    // reused when only the air mesh changed
+    unsigned long meshGen = 0, sysMeshGen = 0;
+    bool sysBuilt = false;
+    std::vector<unsigned char> sysAirTouched;
     std::vector<unsigned char> fixedMaskSys;
     std::vector<double> fixedDiag;
     DevBuf dUbRow, dUbCol, dUbPosU, dUbPosL, dUbCPtr, dUbCSrc, dVcPtr, dVcSrc, dFixedMaskSys, dFixedDiag;
@@ -252,18 +250,7 @@ TriSet system_set(oc_ctx* c)
     return T;
 }
 
-// vertex -> (triangle*3 + corner) lists, triangles ascending
-void build_vertex_lists(int nV, int nF, const int* F, std::vector<int>& ptr, std::vector<int>& src)
-{
-    ptr.assign(nV + 1, 0);
-    for (int t = 0; t < nF; ++t)
-        for (int k = 0; k < 3; ++k) ptr[F[3 * t + k] + 1]++;
-    for (int v = 0; v < nV; ++v) ptr[v + 1] += ptr[v];
-    src.resize(ptr[nV]);
-    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-    for (int t = 0; t < nF; ++t)
-        for (int k = 0; k < 3; ++k) src[fill[F[3 * t + k]]++] = 3 * t + k;
-}
+using oc::build_vertex_lists;
 
 template <typename T>
 int upload(oc_ctx* c, DevBuf& d, const std::vector<T>& h)
@@ -477,122 +464,6 @@ int step_bound(oc_ctx* c, const TriSet& T, const double* dDir, double cap, doubl
     return OC_OK;
 }
 
-// ---- Newton system pattern ------------------------------------------------------
-// Block pattern of the Newton matrix of triangles Ft over nVt vertices (one
-// row per vertex from its triangles: fixed rows keep only the diagonal, free
-// rows drop fixed columns), its upper blocks and, per upper block, the element
-// blocks that assemble it in triangle order. vcPtr/vcSrc: vertex -> (triangle*3
-// + corner) lists of Ft. Row chunks run on host threads and are concatenated in
-// order, so the output does not depend on the thread count.
-void build_pattern_maps(int nVt, int nFt, const std::vector<int>& Ft, const unsigned char* fixed,
-                        const std::vector<int>& vcPtr, const std::vector<int>& vcSrc, BlockPattern& Pout,
-                        std::vector<int>& ubRow, std::vector<int>& ubCol, std::vector<int>& ubPosU,
-                        std::vector<int>& ubPosL, std::vector<int>& ubCPtr, std::vector<int>& ubCSrc)
-{
-    const int nth = oc::setup_threads(3L * nFt);
-    // Block pattern, one row per vertex from its triangles: fixed rows keep
-    // only the diagonal, free rows drop fixed columns. Row chunks in parallel,
-    // concatenated in order.
-    {
-        std::vector<std::vector<int>> part(nth);
-        std::vector<int> len(nVt);
-        oc::parallel_row_chunks(nVt, nth, [&](int ch, int r0, int r1) {
-            std::vector<int> loc;
-            auto& out = part[ch];
-            for (int v = r0; v < r1; ++v) {
-                const size_t n0 = out.size();
-                if (fixed[v]) {
-                    out.push_back(v);
-                } else {
-                    loc.assign(1, v);
-                    for (int q = vcPtr[v]; q < vcPtr[v + 1]; ++q) {
-                        const int t = vcSrc[q] / 3;
-                        for (int k = 0; k < 3; ++k) loc.push_back(Ft[3 * (size_t)t + k]);
-                    }
-                    std::sort(loc.begin(), loc.end());
-                    loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
-                    for (int w : loc)
-                        if (w == v || !fixed[w]) out.push_back(w);
-                }
-                len[v] = (int)(out.size() - n0);
-            }
-        });
-        BlockPattern& P = Pout;
-        P.nV = nVt;
-        P.rowPtr.assign(nVt + 1, 0);
-        for (int v = 0; v < nVt; ++v) P.rowPtr[v + 1] = P.rowPtr[v] + len[v];
-        P.colIdx.clear();
-        P.colIdx.reserve(P.rowPtr[nVt]);
-        for (auto& x : part) P.colIdx.insert(P.colIdx.end(), x.begin(), x.end());
-        P.diagPos.assign(nVt, -1);
-        for (int v = 0; v < nVt; ++v)
-            for (int k = P.rowPtr[v]; k < P.rowPtr[v + 1]; ++k)
-                if (P.colIdx[k] == v) P.diagPos[v] = k;
-    }
-    const BlockPattern& P = Pout;
-    // upper blocks (w >= u) in row order, and each position's upper block
-    std::vector<int> ubStart(nVt + 1, 0), posToUB(P.nnzb(), -1);
-    for (int u = 0; u < nVt; ++u) {
-        int n = 0;
-        for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k) n += P.colIdx[k] >= u;
-        ubStart[u + 1] = ubStart[u] + n;
-    }
-    const int nUB = ubStart[nVt];
-    ubRow.assign(nUB, 0);
-    ubCol.assign(nUB, 0);
-    ubPosU.assign(nUB, 0);
-    ubPosL.assign(nUB, 0);
-    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
-        for (int u = r0; u < r1; ++u) {
-            int b = ubStart[u];
-            for (int k = P.rowPtr[u]; k < P.rowPtr[u + 1]; ++k) {
-                const int w = P.colIdx[k];
-                if (w < u) continue;
-                posToUB[k] = b;
-                ubRow[b] = u;
-                ubCol[b] = w;
-                ubPosU[b] = k;
-                ubPosL[b] = w == u ? -1 : P.find(w, u);
-                ++b;
-            }
-        }
-    });
-    // Contributions per upper block in triangle order. Block (u, w), u <= w,
-    // receives the element block (a, b) of every triangle t with f[a] = u and
-    // f[b] = w (for u == w only a == b); walking u's triangles in order gives
-    // each of its blocks its contributions in triangle order.
-    ubCPtr.assign(nUB + 1, 0);
-    auto row_contribs = [&](int u, auto&& fn) {
-        if (fixed[u]) return;
-        for (int q = vcPtr[u]; q < vcPtr[u + 1]; ++q) {
-            const int t = vcSrc[q] / 3, a = vcSrc[q] % 3;
-            const int* f = &Ft[3 * (size_t)t];
-            for (int b = 0; b < 3; ++b) {
-                const int w = f[b];
-                if (fixed[w] || w < u || (w == u && a != b)) continue;
-                const int lo = a < b ? a : b, hi = a < b ? b : a;
-                const int id = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
-                const unsigned int code = (unsigned int)(6 * t + id) | (a > b ? 0x80000000u : 0u);
-                fn(posToUB[P.find(u, w)], code);
-            }
-        }
-    };
-    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
-        for (int u = r0; u < r1; ++u) row_contribs(u, [&](int ub, unsigned int) { ubCPtr[ub + 1]++; });
-    });
-    for (int b = 0; b < nUB; ++b) ubCPtr[b + 1] += ubCPtr[b];
-    ubCSrc.assign(ubCPtr[nUB], 0);
-    oc::parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
-        std::vector<int> fill;
-        for (int u = r0; u < r1; ++u) {
-            const int b0 = ubStart[u], b1 = ubStart[u + 1];
-            fill.assign(ubCPtr.begin() + b0, ubCPtr.begin() + b1);
-            row_contribs(u, [&](int ub, unsigned int code) { ubCSrc[fill[ub - b0]++] = (int)code; });
-        }
-    });
-
-}
-
 // Numeric multigrid hierarchy for the current Newton matrix (safe: the
 // guaranteed smoother interval).
 void amg_numeric(oc_ctx* c, int safe)
@@ -621,16 +492,39 @@ int build_system(oc_ctx* c)
     // vertex -> (triangle*3 + corner) lists in triangle order (the gradient
     // gathers use them, and so does everything below)
     build_vertex_lists(nVt, nFt, Ft.data(), c->vcPtr, c->vcSrc);
-    build_pattern_maps(nVt, nFt, Ft, c->fixedMaskSys.data(), c->vcPtr, c->vcSrc, c->sys, c->ubRow, c->ubCol,
-                       c->ubPosU, c->ubPosL, c->ubCPtr, c->ubCSrc);
+    // Pattern and slot maps. After an air-mesh change on the same mesh (the
+    // scaffold is rebuilt every Newton iteration, Optimizer.cpp:251-257, and
+    // re-patterned, :522-530), the rows of mesh vertices outside the old and
+    // the new air triangles are copied from the previous build.
+    std::vector<unsigned char> touched(c->nV, 0);
+    for (int v : c->FairGlobal)
+        if (v < c->nV) touched[v] = 1;
+    std::vector<unsigned char> keep;
+    if (c->sysBuilt && c->sysMeshGen == c->meshGen && c->sysAirTouched.size() == (size_t)c->nV) {
+        keep.resize(c->nV);
+        for (int v = 0; v < c->nV; ++v) keep[v] = !touched[v] && !c->sysAirTouched[v];
+    }
+    {
+        oc::HostPattern P;
+        oc::SystemMaps M;
+        oc::build_system_maps(nVt, Ft, c->fixedMaskSys.data(), c->vcPtr, c->vcSrc, P, M,
+                              keep.empty() ? nullptr : &c->sys, keep.empty() ? nullptr : &c->sm,
+                              keep.empty() ? nullptr : keep.data(), (int)keep.size());
+        static_cast<oc::HostPattern&>(c->sys) = std::move(P);
+        c->sm = std::move(M);
+    }
+    c->sysBuilt = true;
+    c->sysMeshGen = c->meshGen;
+    c->sysAirTouched = std::move(touched);
     if (upload_pattern(c, c->sys)) return OC_ERR_HIP;
     // The PCG matrix stays FP64 in both precisions: rounding the assembled
     // Hessian to FP32 can make it indefinite along its near-null directions
     // (rigid rotation about the fixed vertex), which breaks CG. The mixed
     // form's FP32 lives in the element terms and the preconditioner.
     c->sys.vals32 = 0;
-    if (upload(c, c->dUbRow, c->ubRow) || upload(c, c->dUbCol, c->ubCol) || upload(c, c->dUbPosU, c->ubPosU) ||
-        upload(c, c->dUbPosL, c->ubPosL) || upload(c, c->dUbCPtr, c->ubCPtr) || upload(c, c->dUbCSrc, c->ubCSrc) ||
+    if (upload(c, c->dUbRow, c->sm.ubRow) || upload(c, c->dUbCol, c->sm.ubCol) ||
+        upload(c, c->dUbPosU, c->sm.ubPosU) || upload(c, c->dUbPosL, c->sm.ubPosL) ||
+        upload(c, c->dUbCPtr, c->sm.ubCPtr) || upload(c, c->dUbCSrc, c->sm.ubCSrc) ||
         upload(c, c->dVcPtr, c->vcPtr) || upload(c, c->dVcSrc, c->vcSrc) ||
         upload(c, c->dFixedMaskSys, c->fixedMaskSys) || upload(c, c->dFixedDiag, c->fixedDiag))
         return OC_ERR_HIP;
@@ -638,7 +532,7 @@ int build_system(oc_ctx* c)
     OC_HIP(c, c->dElemH.ensure(sizeof(double) * 24 * (size_t)std::max(1, nFt)));
     OC_HIP(c, c->dG.ensure(sizeof(double) * 2 * (size_t)std::max(1, nVt)));
     if (ensure_pcg(c, nVt)) return OC_ERR_HIP;
-    if (ensure_scratch(c, std::max(nFt, (int)c->ubRow.size()))) return OC_ERR_HIP;
+    if (ensure_scratch(c, std::max(nFt, (int)c->sm.ubRow.size()))) return OC_ERR_HIP;
     c->amgValid = false;
     c->amg = nullptr;
     c->ext = nullptr;
@@ -794,6 +688,8 @@ int oc_set_mesh(oc_ctx* c, int nV, const double* Vrest, int nF, const int* F, in
     c->nCharts = 0;
     c->nV = nV;
     c->nF = nF;
+    ++c->meshGen;
+    c->sysBuilt = false;
     c->Fh.assign(F, F + 3 * (size_t)nF);
     c->Vh.assign(Vrest, Vrest + 3 * (size_t)nV);
     c->fixedSorted.assign(fixedV, fixedV + nFixed);
@@ -1960,7 +1856,7 @@ int newton_step_charts(oc_ctx* c, oc_iter_stats* S, int* done)
 
 void system_assemble(oc_ctx* c)
 {
-    const int nUB = (int)c->ubRow.size();
+    const int nUB = (int)c->sm.ubRow.size();
     const int bj = c->nPrecond == OC_PRECOND_BLOCK_JACOBI;
     if (c->sysPrecision == OC_PRECISION_MIXED)
         oc::launch_assemble<float>(c->stream, nUB, c->dUbRow.as<int>(), c->dUbCol.as<int>(), c->dUbPosU.as<int>(),
@@ -2287,7 +2183,7 @@ int oc_system_sizes(oc_ctx* c, long long* nF, long long* nV, long long* nnzBlock
     if (nF) *nF = c->sysReady ? c->nFt : c->nF;
     if (nV) *nV = c->sysReady ? c->nVt : c->nV;
     if (nnzBlocks) *nnzBlocks = c->sysReady ? c->sys.nnzb() : 0;
-    if (nUpperBlocks) *nUpperBlocks = c->sysReady ? (long long)c->ubRow.size() : 0;
+    if (nUpperBlocks) *nUpperBlocks = c->sysReady ? (long long)c->sm.ubRow.size() : 0;
     return OC_OK;
 }
 

@@ -56,6 +56,20 @@ def grid_air_mesh(m, rings=2):
     return airUV, Fair, bnd
 
 
+def flip_air_quads(air, seed, frac=0.5):
+    """The same air vertices triangulated differently: grid_air_mesh emits each
+    quad as (a,b,d), (a,d,c); a random subset takes the other diagonal."""
+    airUV, Fair, bnd = air
+    rng = np.random.default_rng(seed)
+    F = Fair.copy()
+    for q in np.flatnonzero(rng.random(F.shape[0] // 2) < frac):
+        a, b, d = Fair[2 * q]
+        c = Fair[2 * q + 1, 2]
+        F[2 * q] = (a, b, c)
+        F[2 * q + 1] = (b, d, c)
+    return airUV, F, bnd
+
+
 class ScaffoldOracle:
     """Objective a0*E_SD(mesh) + (w_scaf/F_air)*sum SD_uniform(air) on the oracle."""
 

@@ -14,7 +14,7 @@ from oracle import oracle as O
 from optcuts_amd import capi
 from optcuts_amd.meshgen import height_field_disc
 
-from _helpers import ScaffoldOracle, distort, grid_air_mesh
+from _helpers import ScaffoldOracle, distort, flip_air_quads, grid_air_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -260,6 +260,40 @@ def test_scaffold_amg_200k_faces_new_air_meshes(ctx):
         assert s["pcg_fallback"] == 0 and s["E_total"] < E_prev
         E_prev = s["E_total"]
     assert O.check_inversion(F, ctx.get_uv())
+
+
+@pytest.mark.parametrize("precond", [capi.PRECOND_AMG, capi.PRECOND_JACOBI])
+def test_air_mesh_change_reuses_rows_bit_exactly(ctx, precond):
+    """A new air mesh on the same mesh copies the pattern, slot-map and multigrid rows of the mesh vertices
+    away from the air triangles from the previous build; the Newton steps that follow are bit-identical to
+    those of a context that built the same system from scratch."""
+    m = 101
+    V, F, UV = height_field_disc(m, seed=0)
+    area_thres = np.sqrt(3.0) / 4.0 * (0.5 * O.RestFeatures(V, F).avgEdgeLen) ** 2
+    air1 = grid_air_mesh(m, rings=2)
+    air2 = flip_air_quads(grid_air_mesh(m, rings=3), seed=4)
+    kw = dict(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=precond, pcg_rel_tol=1e-6,
+              pcg_max_iter=20000)
+    runs = []
+    for first in (air1, None):
+        c = ctx if first is not None else capi.Context(0)
+        c.set_mesh(V, F, fixed=(0,))
+        c.set_uv(UV)
+        if first is not None:
+            c.set_air_mesh(*first, area_thres, 0.01)
+            c.newton_init(**kw)
+        c.set_air_mesh(*air2, area_thres, 0.01)
+        E0 = c.newton_init(**kw)
+        steps = [c.newton_step()[0] for _ in range(3)]
+        runs.append((E0, steps, c.get_uv()))
+        if first is None:
+            c.close()
+    (Ea, sa, uva), (Eb, sb, uvb) = runs
+    assert Ea == Eb
+    for x, y in zip(sa, sb):
+        print(x["E_total"], x["pcg_iters"], flush=True)
+        assert x["E_total"] == y["E_total"] and x["pcg_iters"] == y["pcg_iters"] and x["step"] == y["step"]
+    assert np.array_equal(uva, uvb)
 
 
 def test_newton_200k_faces_properties(ctx):
