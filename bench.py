@@ -159,6 +159,51 @@ def charts_bench(ctx, max_iter=60):
             "max_rel_diff_final_E_SD": float(abs(e_b - e_s).max() / e_s.min())}
 
 
+def scaffold_bench(ctx, V, F, UV, forms, iters=10, w_scaf=0.01):
+    """The geometry loop with the bijectivity scaffold, as the reference runs it by default: before every
+    Newton step a new air mesh is made at the current UV (Optimizer.cpp:251-257; the reference re-meshes the
+    air region with Triangle, here a two-row band outside the boundary with fresh random diagonals stands in
+    for it), set (oc_set_air_mesh), and the system re-initialised (new pattern, slot maps and multigrid
+    structure: oc_newton_init), then one step. Per-iteration wall times of the library calls; the air-mesh
+    generation is timed apart and left out of iters_per_s."""
+    import numpy as np
+    from optcuts_amd.meshgen import band_air_mesh, boundary_loop_ccw
+    loop = boundary_loop_ccw(F)
+    e = np.linalg.norm(UV[F[:, [1, 2, 0]]] - UV[F], axis=2).mean()
+    area_thres = np.sqrt(3.0) / 4.0 * (0.5 * e) ** 2
+    out = {}
+    for name, (precond, tol) in forms.items():
+        ctx.set_mesh(V, F, fixed=(0,))
+        ctx.set_uv(UV)
+        rows = []
+        for k in range(iters):
+            t0 = time.perf_counter()
+            airUV, Fair, bnd = band_air_mesh(ctx.get_uv(), loop, rings=2, seed=k)
+            t1 = time.perf_counter()
+            ctx.set_air_mesh(airUV, Fair, bnd, area_thres, w_scaf)
+            ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=precond, pcg_rel_tol=tol,
+                            pcg_max_iter=max_iter_for(precond))
+            t2 = time.perf_counter()
+            s, done = ctx.newton_step()
+            t3 = time.perf_counter()
+            rows.append((t1 - t0, t2 - t1, t3 - t2, s["pcg_iters"], s["pcg_fallback"], s["E_SD"]))
+            if done:
+                break
+        a = np.array([r[:3] for r in rows]) * 1e3
+        out[name] = {"newton_iterations": len(rows), "pcg_rel_tol": tol,
+                     "iters_per_s": len(rows) / (a[:, 1].sum() + a[:, 2].sum()) * 1e3,
+                     "reinit_ms_median": float(np.median(a[1:, 1])) if len(rows) > 1 else float(a[0, 1]),
+                     "step_ms_median": float(np.median(a[1:, 2])) if len(rows) > 1 else float(a[0, 2]),
+                     "first_iteration_ms": float(a[0, 1] + a[0, 2]),
+                     "air_mesh_generation_ms_median": float(np.median(a[:, 0])),
+                     "pcg_iters": [int(r[3]) for r in rows], "pcg_fallback_codes": [int(r[4]) for r in rows],
+                     "E_SD_last": rows[-1][5],
+                     "air_triangles": int(Fair.shape[0]), "air_vertices": int(airUV.shape[0])}
+    if "chosen" in out and "plain" in out:
+        out["speedup_iters_per_s_chosen_vs_plain"] = out["chosen"]["iters_per_s"] / out["plain"]["iters_per_s"]
+    return out
+
+
 def cpu_baseline(V, F, UV, iters=3):
     from oracle import oracle as O
     t = time.perf_counter()
@@ -185,6 +230,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-charts", action="store_true", help="skip the 64-chart batched run")
     ap.add_argument("--no-tte", action="store_true", help="skip the time-to-E_SD run")
+    ap.add_argument("--no-scaffold", action="store_true", help="skip the scaffolded loop")
+    ap.add_argument("--scaffold-iters", type=int, default=10)
     ap.add_argument("--tte-max-iter", type=int, default=60)
     ap.add_argument("--target-iters", type=int, default=0, metavar="K",
                     help="workloads without a stored E*: the plain form's E_SD after K Newton iterations from "
@@ -273,6 +320,14 @@ def main():
                                                     args.tte_max_iter)
         res[name] = r
     charts = None if args.no_charts else charts_bench(ctx)
+    scaf = None
+    if not args.no_scaffold:
+        sf = {}
+        if "chosen" in res:
+            sf["chosen"] = (capi.PRECOND_AMG, args.chosen_tol)
+        if "plain" in res:
+            sf["plain"] = (capi.PRECOND_JACOBI, args.pcg_tol)
+        scaf = scaffold_bench(ctx, V, F, UV, sf, iters=args.scaffold_iters) if sf else None
     ctx.close()
     if net is not None:
         net.close()
@@ -324,6 +379,7 @@ def main():
         "time_to_E_SD_speedup_chosen_vs_plain": speed,
         "time_to_E_SD_speedup_vs_plain": speeds,
         "batched_charts": charts,
+        "scaffold": scaf,
         "roofline": rl,
         "roofline_plain_form": rl_plain,
         "cpu_baseline": None,

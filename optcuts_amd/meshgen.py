@@ -89,3 +89,54 @@ def union_of_charts(charts):
         fs.append(fs[-1] + F.shape[0])
     return (np.concatenate(Vs), np.concatenate(Fs).astype(np.int32), np.concatenate(UVs),
             np.array(vs, np.int32), np.array(fs, np.int32), np.array(fixed, np.int32))
+
+
+def boundary_loop_ccw(F):
+    """The boundary loop of a disc mesh with counter-clockwise triangles, in
+    the order that keeps the mesh on its left (vertex ids, int32)."""
+    F = np.asarray(F)
+    he = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    key = he[:, 0].astype(np.int64) * (int(F.max()) + 1) + he[:, 1]
+    rev = he[:, 1].astype(np.int64) * (int(F.max()) + 1) + he[:, 0]
+    bnd = he[~np.isin(key, rev)]
+    nxt = dict(zip(bnd[:, 0].tolist(), bnd[:, 1].tolist()))
+    start = int(bnd[:, 0].min())
+    loop = [start]
+    while nxt[loop[-1]] != start:
+        loop.append(nxt[loop[-1]])
+    return np.array(loop, np.int32)
+
+
+def band_air_mesh(UV, loop, rings=2, seed=0, spacing=None):
+    """Air mesh for the bijectivity scaffold: `rings` rows of quads outside the
+    boundary loop (mesh on its left) at the current UV, rows offset along the
+    outward vertex normals by multiples of the mean boundary edge length (or
+    `spacing`), each quad split along a diagonal drawn from default_rng(seed)
+    (a fresh triangulation per call, like a re-meshed air region). Returns
+    (airUV, Fair, bnd) in the layout of the scaffold interface: the first
+    len(loop) air vertices are the boundary vertices `loop`. The spacing halves
+    until every air triangle is counter-clockwise; ValueError if none works."""
+    P = np.asarray(UV, np.float64)[loop]
+    nb = len(loop)
+    t = np.roll(P, -1, 0) - np.roll(P, 1, 0)
+    nrm = np.stack([t[:, 1], -t[:, 0]], 1)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    h = spacing if spacing else float(np.linalg.norm(np.roll(P, -1, 0) - P, axis=1).mean())
+    flip = np.random.default_rng(seed).random((rings, nb)) < 0.5
+    i = np.arange(nb)
+    j = (i + 1) % nb
+    tris = []
+    for r in range(rings):
+        a, b, c, d = r * nb + i, r * nb + j, (r + 1) * nb + i, (r + 1) * nb + j
+        t1 = np.where(flip[r][:, None], np.stack([a, c, b], 1), np.stack([a, c, d], 1))
+        t2 = np.where(flip[r][:, None], np.stack([b, c, d], 1), np.stack([a, d, b], 1))
+        tris += [t1, t2]
+    Fair = np.concatenate(tris).astype(np.int32)
+    for _ in range(10):
+        airUV = np.concatenate([P] + [P + r * h * nrm for r in range(1, rings + 1)])
+        e1 = airUV[Fair[:, 1]] - airUV[Fair[:, 0]]
+        e2 = airUV[Fair[:, 2]] - airUV[Fair[:, 0]]
+        if ((e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]) > 0).all():
+            return airUV, Fair, np.asarray(loop, np.int32)
+        h *= 0.5
+    raise ValueError("band_air_mesh: no spacing gives a counter-clockwise band")
