@@ -204,6 +204,39 @@ def scaffold_bench(ctx, V, F, UV, forms, iters=10, w_scaf=0.01):
     return out
 
 
+def prescreen_bench(ctx, V, F, UV, reps=5):
+    """Topology-step pre-screen at the workload's size: oc_split_candidates (GPU keys, host selection) for
+    boundary and interior splits, median wall time of reps calls."""
+    import numpy as np
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    out = {}
+    for name, interior in (("boundary", False), ("interior", True)):
+        ctx.split_candidates(interior)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            cand, ns = ctx.split_candidates(interior)
+            ts.append(time.perf_counter() - t0)
+        out[name] = {"ms_median": float(np.median(ts) * 1e3), "candidates": int(len(cand)), "map_size": int(ns)}
+    return out
+
+
+def prescreen_cpu_baseline(V, F, UV):
+    """The same pre-screen by the oracle's restatement of computeDivGradPerVert + querySplit's filter
+    (1 thread)."""
+    from oracle import oracle as O
+    feat = O.RestFeatures(V, F)
+    out = {}
+    for name, interior in (("boundary", False), ("interior", True)):
+        t0 = time.perf_counter()
+        dg = O.div_grad_per_vert(F, UV, feat)
+        O.split_candidates(F, V.shape[0], dg, None, interior, 0.6)
+        out[name + "_ms"] = (time.perf_counter() - t0) * 1e3
+    out["cores"] = 1
+    return out
+
+
 def cpu_baseline(V, F, UV, iters=3):
     from oracle import oracle as O
     t = time.perf_counter()
@@ -231,6 +264,7 @@ def main():
     ap.add_argument("--no-charts", action="store_true", help="skip the 64-chart batched run")
     ap.add_argument("--no-tte", action="store_true", help="skip the time-to-E_SD run")
     ap.add_argument("--no-scaffold", action="store_true", help="skip the scaffolded loop")
+    ap.add_argument("--no-prescreen", action="store_true", help="skip the split-query pre-screen timing")
     ap.add_argument("--scaffold-iters", type=int, default=10)
     ap.add_argument("--tte-max-iter", type=int, default=60)
     ap.add_argument("--target-iters", type=int, default=0, metavar="K",
@@ -328,6 +362,7 @@ def main():
         if "plain" in res:
             sf["plain"] = (capi.PRECOND_JACOBI, args.pcg_tol)
         scaf = scaffold_bench(ctx, V, F, UV, sf, iters=args.scaffold_iters) if sf else None
+    pre = None if args.no_prescreen else prescreen_bench(ctx, V, F, UV)
     ctx.close()
     if net is not None:
         net.close()
@@ -380,12 +415,15 @@ def main():
         "time_to_E_SD_speedup_vs_plain": speeds,
         "batched_charts": charts,
         "scaffold": scaf,
+        "split_prescreen": pre,
         "roofline": rl,
         "roofline_plain_form": rl_plain,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(V, F, UV)
+        if pre is not None:
+            pre["cpu_oracle"] = prescreen_cpu_baseline(V, F, UV)
     print(json.dumps(out), flush=True)
 
 

@@ -101,6 +101,23 @@ int oc_init_step_size(oc_ctx* ctx, const double* dir, double* stepSize);
 /* TriMesh::checkInversion (TriMesh.cpp:1736-1756): *ok = 1 if no triangle has negative signed UV area. */
 int oc_check_inversion(oc_ctx* ctx, int* ok);
 
+/* ---- split-query pre-screen (topology step) ------------------------------ */
+/* SymDirichletEnergy::computeDivGradPerVert (SymDirichletEnergy.cpp:108-154, the STANDARD_DEVIATION_FILTERING
+ * branch it compiles) at the context's UV: per mesh vertex the sample standard deviation of its triangles'
+ * local gradient rows (computeLocalGradient, :215-256, weights triArea / surfaceArea), 0 for a vertex in one
+ * triangle. out: nV doubles. */
+int oc_div_grad_per_vert(oc_ctx* ctx, double* out);
+/* The candidate filter of TriMesh::querySplit (TriMesh.cpp:556-633) for propagate = false, which picks the
+ * vertices whose splits the topology step then evaluates (computeLocalLDec): vertices with more than two
+ * neighbours that are on the boundary (splitInterior = 0) or neither on nor next to it (splitInterior = 1),
+ * keyed by -divGrad / vertWeight (vertWeight null: all 1, TriMesh.cpp:203) in a std::map (a later vertex with
+ * an equal key replaces the earlier), the first max(2, (int)(size^e)) keys in order, e = 0.8 for boundary
+ * splits and filterExpIn (the reference's filterExp_in, main.cpp:70) for interior ones. cand receives up to
+ * capacity of the *nCand candidates in that order; *nSorted = the map's size (inSplitTotalAmt for interior
+ * splits). The keys come from the GPU; the selection runs on the host. */
+int oc_split_candidates(oc_ctx* ctx, int splitInterior, double filterExpIn, const double* vertWeight, int capacity,
+                        int* cand, int* nCand, int* nSorted);
+
 /* ---- bijectivity scaffold (Scaffold.cpp:27-313) ------------------------- */
 /* The air mesh built by the Scaffold constructor: nVair vertices with UV airUV (nVair x 2; its rest shape is
  * this UV with z = 0, Scaffold.cpp:174-177), nFair triangles Fair in air-mesh indices, the first nBnd air

@@ -68,6 +68,9 @@ _SIGS = {
     "oc_hessian_triplets": (C.c_int, [_ctx, C.c_int, C.c_longlong, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
     "oc_init_step_size": (C.c_int, [_ctx, _dp, C.POINTER(C.c_double)]),
     "oc_check_inversion": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
+    "oc_div_grad_per_vert": (C.c_int, [_ctx, _dp]),
+    "oc_split_candidates": (C.c_int, [_ctx, C.c_int, C.c_double, _vp, C.c_int, _ip, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     "oc_set_air_mesh": (C.c_int, [_ctx, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_double]),
     "oc_solver_set_pattern": (C.c_int, [_ctx, C.c_int, _ip, _ip, C.c_int, _ip, C.POINTER(C.c_int)]),
     "oc_solver_get_pattern": (C.c_int, [_ctx, _ip, _ip]),
@@ -272,6 +275,22 @@ class Context:
         ok = C.c_int()
         self._chk(self.L.oc_check_inversion(self.h, C.byref(ok)))
         return bool(ok.value)
+
+    def div_grad_per_vert(self):
+        """SymDirichletEnergy::computeDivGradPerVert at the current UV: (nV,)."""
+        out = np.zeros(self.nV)
+        self._chk(self.L.oc_div_grad_per_vert(self.h, out))
+        return out
+
+    def split_candidates(self, split_interior=False, filter_exp_in=0.6, vert_weight=None):
+        """TriMesh::querySplit's candidate filter (propagate = false): (candidates in key order, map size)."""
+        w = None if vert_weight is None else _f64(vert_weight)
+        cand = np.zeros(max(self.nV, 2), np.int32)
+        n, ns = C.c_int(), C.c_int()
+        self._chk(self.L.oc_split_candidates(self.h, int(split_interior), float(filter_exp_in),
+                                             None if w is None else w.ctypes.data, cand.size, cand, C.byref(n),
+                                             C.byref(ns)))
+        return cand[:n.value].copy(), ns.value
 
     def set_air_mesh(self, airUV, Fair, bnd, area_thres, w_scaf):
         airUV, Fair, bnd = _f64(airUV), _i32(Fair), _i32(bnd)

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -80,6 +81,12 @@ struct oc_ctx {
     // mesh-only gather lists (Energy interface)
     std::vector<int> mvcPtr, mvcSrc;
     DevBuf dMvcPtr, dMvcSrc;
+    // split-query pre-screen: per-vertex keys, and the mesh's boundary flags
+    // and neighbour counts (built on first use for the mesh of meshGen topoGen)
+    DevBuf dDivGrad;
+    unsigned long topoGen = 0;
+    std::vector<unsigned char> isBnd;
+    std::vector<int> nbrCount;
     std::vector<long long> tripOff;  // per mesh triangle
     std::vector<unsigned char> vFree;
     long long nTripElem = 0;
@@ -584,6 +591,56 @@ int system_energy(oc_ctx* c, double* E, double* Emesh, double* Eair)
     return OC_OK;
 }
 
+// Split-query keys at the current UV into c->dDivGrad (mesh vertices): the
+// element pass's weighted gradient rows (w = triArea / surfaceArea, as
+// SymDirichletEnergy::computeLocalGradient), then their per-vertex spread.
+int div_grad_keys(oc_ctx* c)
+{
+    if (ensure_scratch(c, c->nF)) return OC_ERR_HIP;
+    OC_HIP(c, c->dDivGrad.ensure(sizeof(double) * (size_t)c->nV));
+    oc::launch_element(c->stream, mesh_set(c, 0), c->dUV.as<double>(), nullptr, c->dElemG.as<double>(), nullptr, 0,
+                       nullptr, nullptr);
+    oc::launch_div_grad(c->stream, c->nV, c->dMvcPtr.as<int>(), c->dMvcSrc.as<int>(), c->dElemG.as<double>(),
+                        c->dDivGrad.as<double>());
+    OC_HIP(c, hipGetLastError());
+    return OC_OK;
+}
+
+// Boundary flags and neighbour counts of the mesh (TriMesh::computeFeatures
+// builds vNeighbor and the directed-edge map edge2Tri, TriMesh.cpp:437-446;
+// isBoundaryVert, TriMesh.cpp:2069-2083, asks for a neighbour w with (v, w) or
+// (w, v) missing from edge2Tri). Around vertex v the directed edges leaving it
+// end at the corners after v and those entering it start at the corners
+// before v, so v is on the boundary when those two sets differ.
+void mesh_topology(oc_ctx* c)
+{
+    if (c->topoGen == c->meshGen && (int)c->isBnd.size() == c->nV) return;
+    const int nV = c->nV;
+    c->isBnd.assign(nV, 0);
+    c->nbrCount.assign(nV, 0);
+    oc::parallel_row_chunks(nV, oc::setup_threads(3L * c->nF), [&](int, int r0, int r1) {
+        std::vector<int> nx, pv, un;
+        for (int v = r0; v < r1; ++v) {
+            nx.clear();
+            pv.clear();
+            for (int q = c->mvcPtr[v]; q < c->mvcPtr[v + 1]; ++q) {
+                const int t = c->mvcSrc[q] / 3, k = c->mvcSrc[q] % 3;
+                nx.push_back(c->Fh[3 * (size_t)t + (k + 1) % 3]);
+                pv.push_back(c->Fh[3 * (size_t)t + (k + 2) % 3]);
+            }
+            std::sort(nx.begin(), nx.end());
+            nx.erase(std::unique(nx.begin(), nx.end()), nx.end());
+            std::sort(pv.begin(), pv.end());
+            pv.erase(std::unique(pv.begin(), pv.end()), pv.end());
+            un.clear();
+            std::set_union(nx.begin(), nx.end(), pv.begin(), pv.end(), std::back_inserter(un));
+            c->nbrCount[v] = (int)un.size();
+            c->isBnd[v] = nx != pv;
+        }
+    });
+    c->topoGen = c->meshGen;
+}
+
 }  // namespace
 
 // =====================================================================================
@@ -876,6 +933,72 @@ int oc_check_inversion(oc_ctx* c, int* ok)
     unsigned int n = 0;
     if (int st = count_inverted(c, mesh_set(c, 0), &n)) return st;
     *ok = (n == 0) ? 1 : 0;
+    return OC_OK;
+}
+
+int oc_div_grad_per_vert(oc_ctx* c, double* out)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && out, "oc_div_grad_per_vert: no mesh");
+    (void)hipSetDevice(c->device);
+    if (int st = div_grad_keys(c)) return st;
+    OC_HIP(c, hipMemcpyAsync(out, c->dDivGrad.p, sizeof(double) * (size_t)c->nV, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int oc_split_candidates(oc_ctx* c, int splitInterior, double filterExpIn, const double* vertWeight, int capacity,
+                        int* cand, int* nCand, int* nSorted)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && nCand && nSorted && capacity >= 0 && (capacity == 0 || cand),
+           "oc_split_candidates: bad arguments");
+    (void)hipSetDevice(c->device);
+    if (int st = div_grad_keys(c)) return st;
+    std::vector<double> dg(c->nV);
+    OC_HIP(c, hipMemcpyAsync(dg.data(), c->dDivGrad.p, sizeof(double) * (size_t)c->nV, hipMemcpyDeviceToHost,
+                             c->stream));
+    mesh_topology(c);
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    // std::map<double, int> semantics: keys ascending, a later vertex with an
+    // equal key replaces the earlier one
+    std::vector<std::pair<double, int>> keys;
+    for (int v = 0; v < c->nV; ++v) {
+        if (c->nbrCount[v] <= 2) continue;
+        if (splitInterior) {
+            if (c->isBnd[v]) continue;
+            bool toBound = false;
+            for (int q = c->mvcPtr[v]; q < c->mvcPtr[v + 1] && !toBound; ++q) {
+                const int t = c->mvcSrc[q] / 3;
+                for (int k = 0; k < 3; ++k) toBound = toBound || c->isBnd[c->Fh[3 * (size_t)t + k]];
+            }
+            if (toBound) continue;
+        } else if (!c->isBnd[v]) {
+            continue;
+        }
+        const double key = -dg[v] / (vertWeight ? vertWeight[v] : 1.0);
+        if (std::isnan(key)) {
+            c->err = "oc_split_candidates: a split key is NaN (degenerate or inverted triangles)";
+            return OC_ERR_DEGENERATE;
+        }
+        keys.push_back({key, v});
+    }
+    std::sort(keys.begin(), keys.end());
+    size_t m = 0;
+    for (size_t i = 0; i < keys.size(); ++i) {
+        if (i + 1 < keys.size() && keys[i + 1].first == keys[i].first) continue;
+        keys[m++] = keys[i];
+    }
+    keys.resize(m);
+    *nSorted = (int)m;
+    *nCand = 0;
+    if (m == 0) return OC_OK;
+    int amt = splitInterior ? static_cast<int>(std::pow((double)m, filterExpIn))
+                            : static_cast<int>(std::pow((double)m, 0.8) * 1.0);
+    if (amt < 2) amt = 2;
+    const int n = std::min<int>(amt, (int)m);
+    *nCand = n;
+    for (int i = 0; i < n && i < capacity; ++i) cand[i] = keys[i].second;
     return OC_OK;
 }
 

@@ -317,6 +317,43 @@ void launch_gather_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSr
 }
 
 // ----------------------------------------------------------------------------
+// Candidate key of the split query: per vertex, the sample standard deviation
+// of its triangles' gradient rows (SymDirichletEnergy::computeDivGradPerVert,
+// SymDirichletEnergy.cpp:108-154): the mean accumulated in triangle order and
+// divided by the count, then sqrt(sum |row - mean|^2 / (count - 1)); 0 for a
+// vertex in one triangle. Operations in the reference's order, uncontracted.
+__global__ void __launch_bounds__(kBlock) k_div_grad(int nV, const int* __restrict__ vcPtr,
+                                                     const int* __restrict__ vcSrc,
+                                                     const double2* __restrict__ elemG, double* out)
+{
+#pragma clang fp contract(off)
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    const int k0 = vcPtr[v], k1 = vcPtr[v + 1], n = k1 - k0;
+    double mx = 0.0, my = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const double2 e = elemG[vcSrc[k]];
+        mx += e.x;
+        my += e.y;
+    }
+    mx /= n;
+    my /= n;
+    double sd = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const double2 e = elemG[vcSrc[k]];
+        const double dx = e.x - mx, dy = e.y - my;
+        sd += dx * dx + dy * dy;
+    }
+    out[v] = n == 1 ? 0.0 : sqrt(sd / (n - 1.0));
+}
+
+void launch_div_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSrc, const double* elemG, double* out)
+{
+    hipLaunchKernelGGL(k_div_grad, dim3(grid_for(nV)), dim3(kBlock), 0, s, nV, vcPtr, vcSrc,
+                       reinterpret_cast<const double2*>(elemG), out);
+}
+
+// ----------------------------------------------------------------------------
 template <typename OV>
 __device__ __forceinline__ OV cvt4(double x, double y, double z, double w)
 {

@@ -53,6 +53,10 @@ void launch_triplets(hipStream_t s, const TriSet& T, const double* UV, const lon
 void launch_gather_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSrc, const unsigned char* fixedMask,
                         const double* elemG, double* g, double* partials, double* gsq);
 
+// out[v] = sample standard deviation of the contributions of vertex v's
+// triangles (0 for a vertex in one triangle): the split-query key.
+void launch_div_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSrc, const double* elemG, double* out);
+
 // Upper-block assembly into the full 2x2-block CSR; diagonal blocks also
 // produce the Jacobi preconditioner (blockJacobi: inverse 2x2 block,
 // else inverse diagonal entries). Sums are FP64; vals is float when vals32.

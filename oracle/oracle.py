@@ -47,6 +47,13 @@ def lib():
         L.orc_gradient.restype = None
         L.orc_gradient.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int,
                                    C.c_int, _ip, _dp]
+        L.orc_local_gradients.restype = None
+        L.orc_local_gradients.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp]
+        L.orc_div_grad_per_vert.restype = None
+        L.orc_div_grad_per_vert.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp]
+        L.orc_split_candidates.restype = C.c_int
+        L.orc_split_candidates.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, C.c_int, C.c_double, _ip,
+                                           C.POINTER(C.c_int)]
         L.orc_make_pd.restype = None
         L.orc_make_pd.argtypes = [C.c_int, _dp]
         L.orc_eigh.restype = None
@@ -167,6 +174,37 @@ def gradient(F, UV, feat, fixed=(0,), uniform=False):
     lib().orc_gradient(nV, F.shape[0], F, UV, feat.triArea, feat.triAreaSq, feat.e0SqLen, feat.e1SqLen,
                        feat.e0dote1, feat.surfaceArea, int(uniform), len(fx), fx, g)
     return g
+
+
+def local_gradients(F, UV, feat):
+    """SymDirichletEnergy::computeLocalGradient (SymDirichletEnergy.cpp:215-256): (nF, 3, 2)."""
+    F = _i32(F)
+    out = np.zeros((F.shape[0], 3, 2))
+    lib().orc_local_gradients(F.shape[0], F, _f64(UV), feat.triArea, feat.triAreaSq, feat.e0SqLen, feat.e1SqLen,
+                              feat.e0dote1, feat.surfaceArea, out)
+    return out
+
+
+def div_grad_per_vert(F, UV, feat):
+    """SymDirichletEnergy::computeDivGradPerVert (SymDirichletEnergy.cpp:108-154): (nV,)."""
+    F = _i32(F)
+    UV = _f64(UV)
+    out = np.zeros(UV.shape[0])
+    lib().orc_div_grad_per_vert(UV.shape[0], F.shape[0], F, UV, feat.triArea, feat.triAreaSq, feat.e0SqLen,
+                                feat.e1SqLen, feat.e0dote1, feat.surfaceArea, out)
+    return out
+
+
+def split_candidates(F, nV, div_grad, vert_weight=None, split_interior=False, filter_exp_in=0.6):
+    """The candidate filter of TriMesh::querySplit (TriMesh.cpp:556-633, propagate = false):
+    (candidates in key order, size of the sorted candidate map)."""
+    F = _i32(F)
+    w = np.ones(nV) if vert_weight is None else _f64(vert_weight)
+    cand = np.zeros(max(nV, 2), np.int32)
+    ns = C.c_int(0)
+    n = lib().orc_split_candidates(nV, F.shape[0], F, _f64(div_grad), w, int(split_interior), float(filter_exp_in),
+                                   cand, C.byref(ns))
+    return cand[:n].copy(), ns.value
 
 
 def element_hessians(F, UV, feat, uniform=False, project=True):

@@ -121,6 +121,57 @@ double orc_energy(int nF, const int* F, const double* UV, const double* triArea,
 }
 
 // ---------------------------------------------------------------------------
+// Per-triangle gradient rows w (dLeft_k rightTerm + dRight_k leftTerm) of
+// SymDirichletEnergy::computeGradient (SymDirichletEnergy.cpp:258-304), the
+// same rows computeLocalGradient stores (:215-256). r: 3 x 2.
+static void local_gradient(int t, const int* F, const double* UV, const double* triArea, const double* triAreaSq,
+                           const double* e0SqLen, const double* e1SqLen, const double* e0dote1, double surfaceArea,
+                           int uniform, double r[6])
+{
+    const int i1 = F[3 * t + 0], i2 = F[3 * t + 1], i3 = F[3 * t + 2];
+    const double* U1 = UV + 2 * i1;
+    const double* U2 = UV + 2 * i2;
+    const double* U3 = UV + 2 * i3;
+    const double U2m1[2] = {U2[0] - U1[0], U2[1] - U1[1]};
+    const double U3m1[2] = {U3[0] - U1[0], U3[1] - U1[1]};
+    const double area_U = 0.5 * (U2m1[0] * U3m1[1] - U2m1[1] * U3m1[0]);
+    const double leftTerm = 1.0 + triAreaSq[t] / area_U / area_U;
+    const double sq3 = U3m1[0] * U3m1[0] + U3m1[1] * U3m1[1];
+    const double sq2 = U2m1[0] * U2m1[0] + U2m1[1] * U2m1[1];
+    const double dt = U3m1[0] * U2m1[0] + U3m1[1] * U2m1[1];
+    const double rightTerm = (sq3 * e0SqLen[t] + sq2 * e1SqLen[t]) / 4 / triAreaSq[t] -
+                             dt * e0dote1[t] / 2 / triAreaSq[t];
+    const double areaRatio = triAreaSq[t] / area_U / area_U / area_U;
+    const double w = uniform ? 1.0 : (triArea[t] / surfaceArea);
+    const double e01 = e0dote1[t], e0 = e0SqLen[t], e1 = e1SqLen[t], A2 = triAreaSq[t];
+
+    // vertex 1: edge opposite is U3-U2
+    {
+        const double eo[2] = {U3[0] - U2[0], U3[1] - U2[1]};
+        const double dL[2] = {areaRatio * eo[1], areaRatio * -eo[0]};
+        const double dR[2] = {((e01 - e0) * U3m1[0] + (e01 - e1) * U2m1[0]) / 2.0 / A2,
+                              ((e01 - e0) * U3m1[1] + (e01 - e1) * U2m1[1]) / 2.0 / A2};
+        r[0] = w * (dL[0] * rightTerm + dR[0] * leftTerm);
+        r[1] = w * (dL[1] * rightTerm + dR[1] * leftTerm);
+    }
+    // vertex 2: edge opposite is U1-U3
+    {
+        const double eo[2] = {U1[0] - U3[0], U1[1] - U3[1]};
+        const double dL[2] = {areaRatio * eo[1], areaRatio * -eo[0]};
+        const double dR[2] = {(e1 * U2m1[0] - e01 * U3m1[0]) / 2.0 / A2, (e1 * U2m1[1] - e01 * U3m1[1]) / 2.0 / A2};
+        r[2] = w * (dL[0] * rightTerm + dR[0] * leftTerm);
+        r[3] = w * (dL[1] * rightTerm + dR[1] * leftTerm);
+    }
+    // vertex 3: edge opposite is U2-U1
+    {
+        const double eo[2] = {U2[0] - U1[0], U2[1] - U1[1]};
+        const double dL[2] = {areaRatio * eo[1], areaRatio * -eo[0]};
+        const double dR[2] = {(e0 * U3m1[0] - e01 * U2m1[0]) / 2.0 / A2, (e0 * U3m1[1] - e01 * U2m1[1]) / 2.0 / A2};
+        r[4] = w * (dL[0] * rightTerm + dR[0] * leftTerm);
+        r[5] = w * (dL[1] * rightTerm + dR[1] * leftTerm);
+    }
+}
+
 // a3: analytic gradient, scatter-add in triangle order, fixed rows zeroed.
 // SymDirichletEnergy::computeGradient, SymDirichletEnergy.cpp:258-304.
 void orc_gradient(int nV, int nF, const int* F, const double* UV, const double* triArea,
@@ -130,55 +181,117 @@ void orc_gradient(int nV, int nF, const int* F, const double* UV, const double* 
 {
     std::memset(g, 0, sizeof(double) * 2 * (size_t)nV);
     for (int t = 0; t < nF; ++t) {
-        const int i1 = F[3 * t + 0], i2 = F[3 * t + 1], i3 = F[3 * t + 2];
-        const double* U1 = UV + 2 * i1;
-        const double* U2 = UV + 2 * i2;
-        const double* U3 = UV + 2 * i3;
-        const double U2m1[2] = {U2[0] - U1[0], U2[1] - U1[1]};
-        const double U3m1[2] = {U3[0] - U1[0], U3[1] - U1[1]};
-        const double area_U = 0.5 * (U2m1[0] * U3m1[1] - U2m1[1] * U3m1[0]);
-        const double leftTerm = 1.0 + triAreaSq[t] / area_U / area_U;
-        const double sq3 = U3m1[0] * U3m1[0] + U3m1[1] * U3m1[1];
-        const double sq2 = U2m1[0] * U2m1[0] + U2m1[1] * U2m1[1];
-        const double dt = U3m1[0] * U2m1[0] + U3m1[1] * U2m1[1];
-        const double rightTerm = (sq3 * e0SqLen[t] + sq2 * e1SqLen[t]) / 4 / triAreaSq[t] -
-                                 dt * e0dote1[t] / 2 / triAreaSq[t];
-        const double areaRatio = triAreaSq[t] / area_U / area_U / area_U;
-        const double w = uniform ? 1.0 : (triArea[t] / surfaceArea);
-        const double e01 = e0dote1[t], e0 = e0SqLen[t], e1 = e1SqLen[t], A2 = triAreaSq[t];
-
-        // vertex 1: edge opposite is U3-U2
-        {
-            const double eo[2] = {U3[0] - U2[0], U3[1] - U2[1]};
-            const double dL[2] = {areaRatio * eo[1], areaRatio * -eo[0]};
-            const double dR[2] = {((e01 - e0) * U3m1[0] + (e01 - e1) * U2m1[0]) / 2.0 / A2,
-                                  ((e01 - e0) * U3m1[1] + (e01 - e1) * U2m1[1]) / 2.0 / A2};
-            g[2 * i1 + 0] += w * (dL[0] * rightTerm + dR[0] * leftTerm);
-            g[2 * i1 + 1] += w * (dL[1] * rightTerm + dR[1] * leftTerm);
-        }
-        // vertex 2: edge opposite is U1-U3
-        {
-            const double eo[2] = {U1[0] - U3[0], U1[1] - U3[1]};
-            const double dL[2] = {areaRatio * eo[1], areaRatio * -eo[0]};
-            const double dR[2] = {(e1 * U2m1[0] - e01 * U3m1[0]) / 2.0 / A2,
-                                  (e1 * U2m1[1] - e01 * U3m1[1]) / 2.0 / A2};
-            g[2 * i2 + 0] += w * (dL[0] * rightTerm + dR[0] * leftTerm);
-            g[2 * i2 + 1] += w * (dL[1] * rightTerm + dR[1] * leftTerm);
-        }
-        // vertex 3: edge opposite is U2-U1
-        {
-            const double eo[2] = {U2[0] - U1[0], U2[1] - U1[1]};
-            const double dL[2] = {areaRatio * eo[1], areaRatio * -eo[0]};
-            const double dR[2] = {(e0 * U3m1[0] - e01 * U2m1[0]) / 2.0 / A2,
-                                  (e0 * U3m1[1] - e01 * U2m1[1]) / 2.0 / A2};
-            g[2 * i3 + 0] += w * (dL[0] * rightTerm + dR[0] * leftTerm);
-            g[2 * i3 + 1] += w * (dL[1] * rightTerm + dR[1] * leftTerm);
+        double r[6];
+        local_gradient(t, F, UV, triArea, triAreaSq, e0SqLen, e1SqLen, e0dote1, surfaceArea, uniform, r);
+        for (int k = 0; k < 3; ++k) {
+            g[2 * F[3 * t + k] + 0] += r[2 * k];
+            g[2 * F[3 * t + k] + 1] += r[2 * k + 1];
         }
     }
     for (int k = 0; k < nFixed; ++k) {
         g[2 * fixedV[k] + 0] = 0.0;
         g[2 * fixedV[k] + 1] = 0.0;
     }
+}
+
+// SymDirichletEnergy::computeLocalGradient, SymDirichletEnergy.cpp:215-256
+// (weights triArea / surfaceArea): out nF x 3 x 2.
+void orc_local_gradients(int nF, const int* F, const double* UV, const double* triArea, const double* triAreaSq,
+                         const double* e0SqLen, const double* e1SqLen, const double* e0dote1, double surfaceArea,
+                         double* out)
+{
+    for (int t = 0; t < nF; ++t)
+        local_gradient(t, F, UV, triArea, triAreaSq, e0SqLen, e1SqLen, e0dote1, surfaceArea, 0, out + 6 * (size_t)t);
+}
+
+// SymDirichletEnergy::computeDivGradPerVert, SymDirichletEnergy.cpp:108-154
+// (the STANDARD_DEVIATION_FILTERING branch it compiles): per vertex the mean of
+// its triangles' local gradient rows (accumulated in triangle order), then
+// sqrt(sum |row - mean|^2 / (count - 1)); 0 for a vertex in one triangle.
+void orc_div_grad_per_vert(int nV, int nF, const int* F, const double* UV, const double* triArea,
+                           const double* triAreaSq, const double* e0SqLen, const double* e1SqLen,
+                           const double* e0dote1, double surfaceArea, double* out)
+{
+    std::vector<double> lg(6 * (size_t)nF), mean(2 * (size_t)nV, 0.0), sd(nV, 0.0);
+    std::vector<int> cnt(nV, 0);
+    orc_local_gradients(nF, F, UV, triArea, triAreaSq, e0SqLen, e1SqLen, e0dote1, surfaceArea, lg.data());
+    for (int t = 0; t < nF; ++t)
+        for (int i = 0; i < 3; ++i) {
+            const int v = F[3 * t + i];
+            mean[2 * v] += lg[6 * t + 2 * i];
+            mean[2 * v + 1] += lg[6 * t + 2 * i + 1];
+            cnt[v]++;
+        }
+    for (int v = 0; v < nV; ++v) {
+        mean[2 * v] /= cnt[v];
+        mean[2 * v + 1] /= cnt[v];
+    }
+    for (int t = 0; t < nF; ++t)
+        for (int i = 0; i < 3; ++i) {
+            const int v = F[3 * t + i];
+            const double dx = lg[6 * t + 2 * i] - mean[2 * v], dy = lg[6 * t + 2 * i + 1] - mean[2 * v + 1];
+            sd[v] += dx * dx + dy * dy;
+        }
+    for (int v = 0; v < nV; ++v) out[v] = cnt[v] == 1 ? 0.0 : std::sqrt(sd[v] / (cnt[v] - 1.0));
+}
+
+// The candidate filter of TriMesh::querySplit (TriMesh.cpp:556-633) for
+// propagate = false. vNeighbor and edge2Tri as TriMesh::computeFeatures builds
+// them (TriMesh.cpp:437-446); isBoundaryVert as TriMesh.cpp:2069-2083.
+// Boundary splits: boundary vertices with more than two neighbours; interior
+// splits: non-boundary vertices with more than two neighbours and no boundary
+// neighbour. Each goes into a std::map keyed by -divGrad / vertWeight (a later
+// vertex with an equal key replaces the earlier), and the first
+// max(2, (int)(size^e)) in key order are the candidates (e = 0.8 for boundary
+// splits, filterExpIn for interior ones). cand: the candidates in that order;
+// returns their number; *nSorted = map size.
+int orc_split_candidates(int nV, int nF, const int* F, const double* divGrad, const double* vertWeight,
+                         int splitInterior, double filterExpIn, int* cand, int* nSorted)
+{
+    std::vector<std::set<int>> nbr(nV);
+    std::set<std::pair<int, int>> edge2Tri;
+    for (int t = 0; t < nF; ++t)
+        for (int i = 0; i < 3; ++i) {
+            const int a = F[3 * t + i], b = F[3 * t + (i + 1) % 3];
+            edge2Tri.insert({a, b});
+            nbr[a].insert(b);
+            nbr[b].insert(a);
+        }
+    std::vector<char> bnd(nV, 0);
+    for (int v = 0; v < nV; ++v)
+        for (int w : nbr[v])
+            if (!edge2Tri.count({v, w}) || !edge2Tri.count({w, v})) {
+                bnd[v] = 1;
+                break;
+            }
+    std::map<double, int> sorted;
+    for (int v = 0; v < nV; ++v) {
+        if (nbr[v].size() <= 2) continue;
+        if (splitInterior) {
+            if (bnd[v]) continue;
+            bool toBound = false;
+            for (int w : nbr[v])
+                if (bnd[w]) {
+                    toBound = true;
+                    break;
+                }
+            if (toBound) continue;
+        } else if (!bnd[v]) {
+            continue;
+        }
+        sorted[-divGrad[v] / vertWeight[v]] = v;
+    }
+    *nSorted = (int)sorted.size();
+    if (sorted.empty()) return 0;
+    int amt = splitInterior ? static_cast<int>(std::pow(sorted.size(), filterExpIn))
+                            : static_cast<int>(std::pow(sorted.size(), 0.8) * 1.0);
+    if (amt < 2) amt = 2;
+    int n = 0;
+    for (const auto& kv : sorted) {
+        cand[n++] = kv.second;
+        if (n >= amt) break;
+    }
+    return n;
 }
 
 // ---------------------------------------------------------------------------

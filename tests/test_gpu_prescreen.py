@@ -1,0 +1,91 @@
+"""GPU split-query pre-screen (TriMesh::querySplit's candidate filter and
+SymDirichletEnergy::computeDivGradPerVert) against the oracle.
+
+Tolerances: keys 1e-12 relative (the element pass contracts multiply-adds, the
+reference rounds each operation); candidate lists equal (the distorted maps
+used here have no keys that close together)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from optcuts_amd import capi
+from optcuts_amd.meshgen import height_field_disc
+
+from _helpers import distort
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("m,seed", [(51, 0), (31, 3)])
+def test_div_grad_matches_oracle(ctx, m, seed):
+    V, F, UV = height_field_disc(m, seed=seed)
+    UV = distort(UV, seed, amp=0.05)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    got = ctx.div_grad_per_vert()
+    want = O.div_grad_per_vert(F, UV, O.RestFeatures(V, F))
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    np.testing.assert_array_equal(got == 0, want == 0)
+
+
+@pytest.mark.parametrize("interior", [False, True])
+def test_split_candidates_match_oracle(ctx, interior):
+    m = 51
+    V, F, UV = height_field_disc(m, seed=1)
+    UV = distort(UV, 1, amp=0.05)
+    nV = V.shape[0]
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    feat = O.RestFeatures(V, F)
+    dg = O.div_grad_per_vert(F, UV, feat)
+    w = np.random.default_rng(5).uniform(0.5, 2.0, nV)
+    for weight in (None, w):
+        got, ns = ctx.split_candidates(interior, 0.6, weight)
+        want, wns = O.split_candidates(F, nV, dg, weight, interior, 0.6)
+        print(interior, ns, len(got), flush=True)
+        assert ns == wns and len(got) == len(want) >= 2
+        np.testing.assert_array_equal(got, want)
+
+
+def test_split_candidates_after_newton_steps(ctx):
+    """Keys at the UV the device-resident Newton driver reached (the state the topology step queries)."""
+    V, F, UV = height_field_disc(41, seed=2)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                    pcg_rel_tol=1e-8, pcg_max_iter=1000)
+    for _ in range(4):
+        ctx.newton_step()
+    UVn = ctx.get_uv()
+    got = ctx.div_grad_per_vert()
+    want = O.div_grad_per_vert(F, UVn, O.RestFeatures(V, F))
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    c_got, ns = ctx.split_candidates(False)
+    c_want, wns = O.split_candidates(F, V.shape[0], want)
+    assert ns == wns and list(c_got) == list(c_want)
+
+
+def test_split_candidates_200k_faces(ctx):
+    """Full size: 1,264 boundary vertices; those with more than two neighbours are the map, its first
+    int(size^0.8) the candidates; interior: the (317-4)^2 vertices away from the boundary, int(size^0.6)."""
+    m = 317
+    V, F, UV = height_field_disc(m, seed=0)
+    UV = distort(UV, 9, amp=0.03)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    deg = np.bincount(F.ravel(), minlength=V.shape[0])
+    corners_single = int((deg[[0, m - 1, m * (m - 1), m * m - 1]] == 1).sum())
+    cand, ns = ctx.split_candidates(False)
+    assert ns == 4 * (m - 1) - corners_single
+    assert len(cand) == int(ns ** 0.8) and len(set(cand)) == len(cand)
+    dg = ctx.div_grad_per_vert()
+    assert np.all(np.diff(-dg[cand]) >= 0)  # key order
+    cand_in, ns_in = ctx.split_candidates(True, 0.6)
+    assert ns_in == (m - 4) ** 2 and len(cand_in) == int(ns_in ** 0.6)

@@ -506,3 +506,109 @@ def test_tutte_loops_and_components():
     assert [int(np.searchsorted(vs, l[0], side="right") - 1) for l in l1] == [0, 1, 2]
     assert [int(np.searchsorted(vs, l[0], side="right") - 1) for l in l0] == first
     assert [len(l) for l in l1] == [24, 32, 16]
+
+
+# --------------------------------------------------------------------------- split-query pre-screen
+def test_local_gradients_central_differences_and_scatter():
+    """computeLocalGradient rows: the derivative of each triangle's own weighted energy with respect to its
+    corners (central differences of energy_per_elem), and their scatter-sum is computeGradient's vector."""
+    V, F, UV = height_field_disc(6, seed=7)
+    UV = _distort(UV, 7, 0.1)
+    feat = O.RestFeatures(V, F)
+    lg = O.local_gradients(F, UV, feat)
+    G = np.zeros_like(UV)
+    np.add.at(G, F.ravel(), lg.reshape(-1, 2))
+    np.testing.assert_array_equal(G.ravel(), O.gradient(F, UV, feat, fixed=()))
+    h = 1e-6
+    for t in (0, 5, F.shape[0] - 1):
+        for k in range(3):
+            for d in range(2):
+                Up, Um = UV.copy(), UV.copy()
+                Up[F[t, k], d] += h
+                Um[F[t, k], d] -= h
+                fd = (O.energy_per_elem(F, Up, feat)[t] - O.energy_per_elem(F, Um, feat)[t]) / (2 * h)
+                assert lg[t, k, d] == pytest.approx(fd, rel=1e-6, abs=1e-9)
+
+
+def test_div_grad_is_the_sample_spread_of_local_gradients():
+    """computeDivGradPerVert: sqrt(var_x + var_y) of a vertex's local gradient rows with the n - 1 divisor
+    (numpy's var, ddof = 1), 0 for vertices in one triangle."""
+    V, F, UV = height_field_disc(9, seed=2)
+    UV = _distort(UV, 2, 0.08)
+    feat = O.RestFeatures(V, F)
+    lg = O.local_gradients(F, UV, feat)
+    dg = O.div_grad_per_vert(F, UV, feat)
+    for v in range(V.shape[0]):
+        rows = lg[np.nonzero(F == v)]
+        want = 0.0 if rows.shape[0] == 1 else np.sqrt(np.var(rows[:, 0], ddof=1) + np.var(rows[:, 1], ddof=1))
+        assert dg[v] == pytest.approx(want, rel=1e-12, abs=1e-300)
+    assert (dg == 0).sum() == (np.bincount(F.ravel()) == 1).sum()
+
+
+def test_div_grad_vanishes_at_an_isometry():
+    """At a rigid map every triangle sits at its energy minimum, so every local gradient and every spread is 0
+    (to rounding)."""
+    V, F, _ = height_field_disc(9, seed=2)
+    V = V.copy()
+    V[:, 2] = 0.0
+    c, s = np.cos(0.7), np.sin(0.7)
+    UV = V[:, :2] @ np.array([[c, s], [-s, c]]) + np.array([0.3, -1.2])
+    feat = O.RestFeatures(V, F)
+    assert np.abs(O.div_grad_per_vert(F, UV, feat)).max() < 1e-12
+
+
+def _brute_candidates(F, nV, keys_div, weight, interior, e_in):
+    """The querySplit filter restated with Python sets and a dict (edge counts for the boundary)."""
+    edges = {}
+    nbr = [set() for _ in range(nV)]
+    for a, b, c in F:
+        for u, w in ((a, b), (b, c), (c, a)):
+            e = (min(u, w), max(u, w))
+            edges[e] = edges.get(e, 0) + 1
+            nbr[u].add(w)
+            nbr[w].add(u)
+    bnd = np.zeros(nV, bool)
+    for (u, w), n in edges.items():
+        if n == 1:
+            bnd[u] = bnd[w] = True
+    table = {}
+    for v in range(nV):
+        if len(nbr[v]) <= 2:
+            continue
+        if interior and (bnd[v] or any(bnd[w] for w in nbr[v])):
+            continue
+        if not interior and not bnd[v]:
+            continue
+        table[-keys_div[v] / weight[v]] = v
+    amt = max(2, int(len(table) ** (e_in if interior else 0.8)))
+    return [table[k] for k in sorted(table)][:amt], len(table)
+
+
+@pytest.mark.parametrize("interior", [False, True])
+def test_split_candidates_brute_force(interior):
+    V, F, UV = height_field_disc(21, seed=4)
+    UV = _distort(UV, 4, 0.06)
+    feat = O.RestFeatures(V, F)
+    dg = O.div_grad_per_vert(F, UV, feat)
+    w = np.random.default_rng(1).uniform(0.5, 2.0, V.shape[0])
+    for weight in (np.ones(V.shape[0]), w):
+        cand, ns = O.split_candidates(F, V.shape[0], dg, weight, interior, 0.6)
+        bc, bns = _brute_candidates(F, V.shape[0], dg, weight, interior, 0.6)
+        assert ns == bns and list(cand) == bc
+    # 21 x 21 grid: 80 boundary vertices (one corner in a single triangle has two neighbours), 17 x 17 interior
+    # vertices away from the boundary
+    assert ns == (17 * 17 if interior else 80 - (np.bincount(F.ravel())[[0, 20, 420, 440]] == 1).sum())
+
+
+def test_split_candidates_equal_keys_collapse():
+    """Equal keys share one map entry, held by the last vertex inserted (the largest id)."""
+    V, F, _ = height_field_disc(11, seed=0)
+    nV = V.shape[0]
+    keys = np.zeros(nV)
+    cand, ns = O.split_candidates(F, nV, keys)
+    bc, bns = _brute_candidates(F, nV, keys, np.ones(nV), False, 0.6)
+    assert ns == 1 == bns and list(cand) == bc and cand[0] >= nV - 2
+    keys = np.arange(nV, dtype=float) % 3  # three distinct keys: 2, 1, 0 in key order (-2 < -1 < 0)
+    cand, ns = O.split_candidates(F, nV, keys)
+    bc, bns = _brute_candidates(F, nV, keys, np.ones(nV), False, 0.6)
+    assert ns == 3 == bns and list(cand) == bc and len(cand) == 2
