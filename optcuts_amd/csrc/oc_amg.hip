@@ -1369,11 +1369,14 @@ template <typename T>
 int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
                   const std::vector<unsigned char>& fixedMask, std::string& err)
 {
-    // The previous hierarchy's finest rows are reused where they come out
-    // the same (an air-mesh change leaves most of the mesh rows as they were).
-    std::vector<AmgHostLevel> H;
-    const AmgPrevious prev{L.size() > 1 ? L[0] : nullptr, L.size() > 1 ? L[1] : nullptr};
+    // The previous hierarchy's rows are reused where they come out the same
+    // (an air-mesh change leaves most of the mesh rows as they were).
+    std::vector<AmgHostLevel> H = std::move(spare_);
+    spare_.clear();
+    AmgPrevious prev;
+    for (Level* l : L) prev.levels.push_back(l);
     const int st = amg_build_structure(nV, rowPtr, colIdx, fixedMask, kCoarseMaxDofs, kMaxLevels, H, err, &prev);
+    for (Level* l : L) spare_.push_back(std::move(static_cast<AmgHostLevel&>(*l)));
     release();
     if (st != 0) return -1;
     for (auto& h : H) {

@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "oc_amg_structure.hpp"
 #include "oc_precond.hpp"
 
 #include <string>
@@ -106,6 +107,9 @@ private:
         void trim();   // free the buffers nobody took back
         void clear();  // free everything
     } pool_;
+    // Host storage of the hierarchy before the previous one: the next
+    // structure build writes into it instead of fresh allocations.
+    std::vector<AmgHostLevel> spare_;
     bool ready_ = false;
     void release();
     void vcycle(hipStream_t s, int l, const int* stop);

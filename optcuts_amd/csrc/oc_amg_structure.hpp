@@ -338,13 +338,12 @@ inline int aggregate_nodes(int n, const std::vector<int>& rowPtr, const std::vec
 }
 
 
-// Levels 0 and 1 of an earlier structure build: rows of the new finest level
-// that come out the same (same matrix row, same aggregates around it) are
-// copied from it instead of regenerated; the result is identical to a build
-// without it.
+// The levels of an earlier structure build: rows of a new level that come out
+// the same as that build's level of the same depth (same matrix row, same
+// aggregates around it) are copied from it instead of regenerated; the result
+// is identical to a build without it.
 struct AmgPrevious {
-    const AmgHostLevel* fine = nullptr;
-    const AmgHostLevel* coarse = nullptr;
+    std::vector<const AmgHostLevel*> levels;
 };
 
 // Builds every level down to at most coarseMaxDofs unknowns (or maxLevels
@@ -353,8 +352,17 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
                                const std::vector<unsigned char>& fixedMask, long coarseMaxDofs, int maxLevels,
                                std::vector<AmgHostLevel>& L, std::string& err, const AmgPrevious* prev = nullptr)
 {
+    // Levels reuse the storage of L's elements on entry (a hierarchy two
+    // set-ups old); every array is rewritten, and nc marks the levels that
+    // get a transfer.
+    std::vector<AmgHostLevel> store = std::move(L);
     L.clear();
-    L.emplace_back();
+    const auto take = [&](size_t l) {
+        AmgHostLevel x = l < store.size() ? std::move(store[l]) : AmgHostLevel();
+        x.nc = 0;
+        return x;
+    };
+    L.push_back(take(0));
     L[0].n = nV;
     L[0].bs = 2;
     L[0].rowPtr = rowPtr;
@@ -362,7 +370,6 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
     L[0].excluded = fixedMask;
     for (;;) {
         AmgHostLevel& A = L.back();
-        const bool fine = L.size() == 1;
         A.diagPos.assign(A.n, -1);
         for (int u = 0; u < A.n; ++u)
             for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k)
@@ -389,12 +396,15 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
         const int nc = aggregate_nodes(A.n, A.rowPtr, A.colIdx, A.excluded, agg);
         if (nc == 0 || (long)(nc - nIso) * 10 > (long)(nIn - nIso) * 9) break;  // coupled part stopped coarsening
         A.nc = nc;
-        // Rows of an earlier finest level that come out the same: matrix row
+        // Rows of the earlier build's level that come out the same: matrix row
         // (cleanA), prolongator row (cleanP: same aggregates in its closed
         // neighbourhood), A P row (cleanT: cleanA and every neighbour cleanP);
         // dA/dP/dT: how far each such row moved in its CSR arrays.
-        const AmgHostLevel* O =
-            (fine && prev && prev->fine && prev->fine->nc > 0 && prev->coarse) ? prev->fine : nullptr;
+        const size_t lv = L.size() - 1;
+        const AmgHostLevel* O = (prev && lv + 1 < prev->levels.size() && prev->levels[lv]->nc > 0)
+                                    ? prev->levels[lv]
+                                    : nullptr;
+        const AmgHostLevel* Onext = O ? prev->levels[lv + 1] : nullptr;
         std::vector<unsigned char> cleanA, cleanP, cleanT;
         std::vector<int> dA, dP, dT;
         const int nt0 = setup_threads((long)A.colIdx.size());
@@ -513,7 +523,7 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
         // node: over the P blocks (u, c1) in increasing block order (the
         // restriction lists), the T entries of row u. A coarse row is reused
         // when its restriction list has the same rows, each cleanP and cleanT.
-        AmgHostLevel nxl;
+        AmgHostLevel nxl = take(L.size());
         AmgHostLevel* nx = &nxl;
         nx->n = nc;
         nx->bs = kAmgNullDim;
@@ -547,7 +557,7 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
             const auto mapB = [&](int, int j, int t) { return t + dT[O->pRow[j]]; };
             if (O)
                 product_rows(nc, nc, nt, gen, [&](int c1) { return cleanC[c1] != 0; },
-                             ProductRef{&prev->coarse->rowPtr, &prev->coarse->colIdx, &O->acPtr, &O->acP, &O->acT},
+                             ProductRef{&Onext->rowPtr, &Onext->colIdx, &O->acPtr, &O->acP, &O->acT},
                              mapA, mapB, nx->rowPtr, nx->colIdx, A.acPtr, A.acP, &A.acT);
             else
                 product_rows(nc, nc, nt, gen, none, ProductRef{}, mapA, mapB, nx->rowPtr, nx->colIdx, A.acPtr,
@@ -555,6 +565,12 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
         }
         L.push_back(std::move(nxl));
     }
+    // the coarsest level has no transfer: drop what its storage held
+    AmgHostLevel& C = L.back();
+    for (auto* v : {&C.agg, &C.aggPtr, &C.aggMem, &C.pPtr, &C.pCol, &C.pRow, &C.psPtr, &C.psA, &C.rPtr, &C.rP,
+                    &C.tPtr, &C.tCol, &C.tcPtr, &C.tcA, &C.tcP, &C.acPtr, &C.acP, &C.acT})
+        v->clear();
+    C.pIsTent.clear();
     return 0;
 }
 

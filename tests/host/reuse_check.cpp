@@ -21,8 +21,10 @@ struct Build {
 
 // Combined system of nV mesh vertices with triangles F (nF) and air triangles
 // Fa (nFa, combined ids) over nVt vertices; fixed mesh vertices fixedV.
+// storage (optional): levels of another build whose arrays the new levels
+// take over (rewritten), as the library reuses the hierarchy two set-ups old.
 void build(int nV, int nF, const int* F, int nVt, int nFa, const int* Fa, int nFixed, const int* fixedV,
-           const Build* prev, Build& B, std::string& err)
+           const Build* prev, Build& B, std::string& err, std::vector<oc::AmgHostLevel>* storage = nullptr)
 {
     std::vector<int> Ft(F, F + 3 * (size_t)nF);
     Ft.insert(Ft.end(), Fa, Fa + 3 * (size_t)nFa);
@@ -40,10 +42,9 @@ void build(int nV, int nF, const int* F, int nVt, int nFa, const int* Fa, int nF
     oc::build_system_maps(nVt, Ft, fixed.data(), B.vcPtr, B.vcSrc, B.P, B.M, prev ? &prev->P : nullptr,
                           prev ? &prev->M : nullptr, prev ? keep.data() : nullptr, (int)keep.size());
     oc::AmgPrevious ap;
-    if (prev && prev->L.size() > 1) {
-        ap.fine = &prev->L[0];
-        ap.coarse = &prev->L[1];
-    }
+    if (prev)
+        for (const auto& l : prev->L) ap.levels.push_back(&l);
+    if (storage) B.L = std::move(*storage);
     if (oc::amg_build_structure(nVt, B.P.rowPtr, B.P.colIdx, fixed, 64, 10, B.L, err, prev ? &ap : nullptr) != 0)
         B.L.clear();
 }
@@ -99,10 +100,11 @@ int reuse_check(int nV, int nF, const int* F, int nFixed, const int* fixedV, int
                 int nVt2, int nFa2, const int* Fa2, int* stats, char* what, int whatLen)
 {
     std::string err;
-    Build b1, full, inc;
+    Build b1, full, inc, donor;
     build(nV, nF, F, nVt1, nFa1, Fa1, nFixed, fixedV, nullptr, b1, err);
     build(nV, nF, F, nVt2, nFa2, Fa2, nFixed, fixedV, nullptr, full, err);
-    build(nV, nF, F, nVt2, nFa2, Fa2, nFixed, fixedV, &b1, inc, err);
+    build(nV, nF, F, nVt1, nFa1, Fa1, nFixed, fixedV, nullptr, donor, err);
+    build(nV, nF, F, nVt2, nFa2, Fa2, nFixed, fixedV, &b1, inc, err, &donor.L);
     int kept = 0;
     for (int v = 0; v < nV; ++v) kept += !b1.touched[v] && !full.touched[v];
     stats[0] = (int)full.L.size();
