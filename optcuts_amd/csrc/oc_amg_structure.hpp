@@ -371,9 +371,11 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
     for (;;) {
         AmgHostLevel& A = L.back();
         A.diagPos.assign(A.n, -1);
-        for (int u = 0; u < A.n; ++u)
-            for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k)
-                if (A.colIdx[k] == u) A.diagPos[u] = k;
+        parallel_row_chunks(A.n, setup_threads((long)A.colIdx.size()), [&](int, int r0, int r1) {
+            for (int u = r0; u < r1; ++u)
+                for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k)
+                    if (A.colIdx[k] == u) A.diagPos[u] = k;
+        });
         for (int u = 0; u < A.n; ++u)
             if (A.diagPos[u] < 0) {
                 err = "AMG: a row has no diagonal block";
@@ -477,16 +479,17 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
                              nullptr);
         }
         const int nP = (int)A.pCol.size();
-        A.pRow.assign(nP, 0);
-        A.pIsTent.assign(nP, 0);
-        for (int u = 0; u < A.n; ++u)
-            for (int j = A.pPtr[u]; j < A.pPtr[u + 1]; ++j) {
-                A.pRow[j] = u;
-                A.pIsTent[j] = A.pCol[j] == agg[u] ? 1 : 0;
+        A.pRow.resize(nP);
+        A.pIsTent.resize(nP);
+        parallel_row_chunks(A.n, nt0, [&](int, int r0, int r1) {
+            for (int u = r0; u < r1; ++u) {
+                for (int j = A.pPtr[u]; j < A.pPtr[u + 1]; ++j) {
+                    A.pRow[j] = u;
+                    A.pIsTent[j] = A.pCol[j] == agg[u] ? 1 : 0;
+                }
+                if (O && cleanP[u]) dP[u] = A.pPtr[u] - O->pPtr[u];
             }
-        if (O)
-            for (int u = 0; u < A.n; ++u)
-                if (cleanP[u]) dP[u] = A.pPtr[u] - O->pPtr[u];
+        });
         // restriction: P blocks by column
         A.rPtr.assign(nc + 1, 0);
         for (int j = 0; j < nP; ++j) A.rPtr[A.pCol[j] + 1]++;
