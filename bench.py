@@ -272,7 +272,31 @@ def main():
                          "the initial map is the time-to-target goal of the other forms")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only time plain device reads (sum) and copies at several sizes: what the GPU streams at "
+                         "the row kernels' size, for comparison with their roofline fraction")
     args = ap.parse_args()
+    if args.stream_probe:
+        import torch
+        out = {}
+        for mb in (16, 36, 72, 144, 512):
+            x = torch.ones(mb * 1000000 // 8, dtype=torch.float64, device="cuda")
+            y = torch.empty_like(x)
+            r = torch.empty((), dtype=torch.float64, device="cuda")
+            for name, fn, nbytes in (("read_sum", lambda: torch.sum(x, dim=0, out=r), x.numel() * 8),
+                                     ("copy", lambda: y.copy_(x), 2 * x.numel() * 8)):
+                for _ in range(5):
+                    fn()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(50):
+                    fn()
+                e1.record()
+                torch.cuda.synchronize()
+                us = e0.elapsed_time(e1) * 1e3 / 50
+                out["%s_%dMB" % (name, mb)] = {"us": us, "GB_per_s": nbytes / (us * 1e-6) / 1e9}
+        print(json.dumps({"stream_probe": out}), flush=True)
+        return
     if args.copy_probe:
         import torch
         a = torch.ones(args.copy_probe * (1 << 20) // 8, dtype=torch.float64, device="cuda")
