@@ -88,12 +88,14 @@ def time_to_target(ctx, V, F, UV, pcg_tol, precond, precision, e_star, max_iter)
     t0 = time.perf_counter()
     ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16 * nV / (nV - 1), allow_edec_rel_tol=False, precond=precond,
                     pcg_rel_tol=pcg_tol, pcg_max_iter=max_iter_for(precond), precision=precision)
-    fb = {"retry_safe_amg": 0, "jacobi": 0, "truncated": 0}
+    fb = {"retry_fp64_elements": 0, "retry_safe_amg": 0, "jacobi": 0, "truncated": 0}
     t = None
     fixed = e_star is None  # no target: run max_iter iterations, the reached E_SD becomes the target
     for k in range(1, max_iter + 1):
         s, done = ctx.newton_step()
-        if s["pcg_fallback"] == 1:
+        if s["pcg_fallback"] == 4:
+            fb["retry_fp64_elements"] += 1
+        elif s["pcg_fallback"] == 1:
             fb["retry_safe_amg"] += 1
         elif s["pcg_fallback"] == 2:
             fb["jacobi"] += 1

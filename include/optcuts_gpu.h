@@ -168,11 +168,13 @@ typedef struct {
     int pcg_iters;    /* PCG iterations of this Newton step */
     double pcg_rel_res;
     int pcg_fallback; /* 0: the selected preconditioner solved the step. Otherwise its first attempt broke down
-                         (r.z or p.Ap not positive) or hit pcgMaxIter, and 1: multigrid rebuilt with its
-                         guaranteed smoother interval solved it; 2: Jacobi-PCG solved it; 3: Jacobi-PCG met
-                         negative curvature (p.Ap <= 0, possible only when FP32 element Hessians leave the
-                         assembled matrix slightly indefinite) and the step uses its last iterate, or -g when
-                         it had none (truncated Newton) */
+                         (r.z or p.Ap not positive) or hit pcgMaxIter, and 4 (OC_PRECOND_LEARNED with
+                         OC_PRECISION_MIXED): the matrix rebuilt from FP64 element terms with a newly predicted
+                         factor solved it; 1: multigrid
+                         rebuilt with its guaranteed smoother interval solved it; 2: Jacobi-PCG solved it; 3:
+                         Jacobi-PCG met negative curvature (p.Ap <= 0, possible only when FP32 element Hessians
+                         leave the assembled matrix slightly indefinite) and the step uses its last iterate, or
+                         -g when it had none (truncated Newton) */
     int pcg_iters_first; /* iterations of the first attempt when pcg_fallback != 0 (pcg_iters counts all) */
     int precond_setups;  /* multigrid hierarchies built on this step (1, or 2 after a safe-interval retry) */
 } oc_iter_stats;

@@ -1786,6 +1786,8 @@ void system_element_pass(oc_ctx* c, const TriSet& T)
                        nullptr, f32);
 }
 
+void system_assemble(oc_ctx* c);
+
 // Newton direction H d = b (b in c->dB, d in c->dX) with the selected
 // preconditioner and its fallbacks; fills the PCG fields of S.
 int solve_direction(oc_ctx* c, oc_iter_stats* S)
@@ -1805,12 +1807,31 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S)
     int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext, c->pcgHint);
     c->pcgHint = st == OC_OK ? it : 0;
     if (c->ext && st != OC_OK) {
-        // The preconditioned solve broke down or hit its cap. Multigrid first
-        // retries with its guaranteed (Gershgorin) smoother interval; what
-        // still fails falls back to Jacobi-PCG at the same tolerance.
+        // The preconditioned solve broke down or hit its cap. With the
+        // learned factor in the mixed form the likely cause is the matrix
+        // itself: FP32 element Hessians, each PSD to FP32 rounding, can sum to
+        // a matrix slightly indefinite along its near-null directions, which
+        // the long Krylov runs of that preconditioner find. The matrix is then
+        // rebuilt from FP64 element terms at the same UV (the right-hand side
+        // stays), the factor predicted again, and the solve retried. (With
+        // multigrid the FP32-stored hierarchy is what loses definiteness, and
+        // the FP64 rebuild was measured not to help.) Multigrid retries with
+        // its guaranteed (Gershgorin) smoother interval; what still fails falls
+        // back to Jacobi-PCG at the same tolerance.
         S->pcg_iters_first = it;
         S->pcg_fallback = 2;
-        if (c->amgValid) {
+        if (c->sysPrecision == OC_PRECISION_MIXED && c->nPrecond == OC_PRECOND_LEARNED) {
+            c->sysPrecision = OC_PRECISION_FP64;
+            system_element_pass(c, system_set(c));
+            system_assemble(c);
+            c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
+            c->sysPrecision = OC_PRECISION_MIXED;
+            int it2 = 0;
+            st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, c->ext);
+            it += it2;
+            if (st == OC_OK) S->pcg_fallback = 4;
+        }
+        if (st != OC_OK && c->amgValid) {
             amg_numeric(c, 1);
             ++S->precond_setups;
             int it2 = 0;
