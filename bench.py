@@ -267,6 +267,7 @@ def main():
     ap.add_argument("--no-tte", action="store_true", help="skip the time-to-E_SD run")
     ap.add_argument("--no-scaffold", action="store_true", help="skip the scaffolded loop")
     ap.add_argument("--no-prescreen", action="store_true", help="skip the split-query pre-screen timing")
+    ap.add_argument("--no-graphs", action="store_true", help="launch PCG iterations kernel by kernel")
     ap.add_argument("--scaffold-iters", type=int, default=10)
     ap.add_argument("--tte-max-iter", type=int, default=60)
     ap.add_argument("--target-iters", type=int, default=0, metavar="K",
@@ -335,6 +336,8 @@ def main():
     else:
         run = [args.form]
     ctx = capi.Context(0)
+    if args.no_graphs:
+        ctx.pcg_graphs(False)
     net = None
     if any(f.startswith("learned") for f in run):
         net = capi.Gnn(0)

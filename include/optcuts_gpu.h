@@ -185,6 +185,11 @@ typedef struct {
  * pcgRelTol/pcgMaxIter/precond select the Newton-system solve. */
 int oc_newton_init(oc_ctx* ctx, double a0, double relGL2Tol, int allowEDecRelTol, int precond, double pcgRelTol,
                    int pcgMaxIter, double* E0);
+/* PCG iterations of the Newton solve (and of the other solves through this context) as replays of captured HIP
+ * graphs (enable != 0, the default: an even iteration, an odd one and an even+odd pair are captured once per
+ * set of buffers) or launched kernel by kernel (0). The results are identical either way. No reference
+ * counterpart. */
+int oc_pcg_graphs(oc_ctx* ctx, int enable);
 /* Selects OC_PRECISION_FP64 (default) or OC_PRECISION_MIXED for the next oc_newton_init. No reference
  * counterpart (the reference is FP64 throughout). */
 int oc_newton_set_precision(oc_ctx* ctx, int precision);

@@ -69,6 +69,7 @@ _SIGS = {
     "oc_init_step_size": (C.c_int, [_ctx, _dp, C.POINTER(C.c_double)]),
     "oc_check_inversion": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
     "oc_div_grad_per_vert": (C.c_int, [_ctx, _dp]),
+    "oc_pcg_graphs": (C.c_int, [_ctx, C.c_int]),
     "oc_split_candidates": (C.c_int, [_ctx, C.c_int, C.c_double, _vp, C.c_int, _ip, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "oc_set_air_mesh": (C.c_int, [_ctx, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_double]),
@@ -275,6 +276,10 @@ class Context:
         ok = C.c_int()
         self._chk(self.L.oc_check_inversion(self.h, C.byref(ok)))
         return bool(ok.value)
+
+    def pcg_graphs(self, enable):
+        """PCG iterations from captured HIP graphs (True, the default) or launched one by one."""
+        self._chk(self.L.oc_pcg_graphs(self.h, int(bool(enable))))
 
     def div_grad_per_vert(self):
         """SymDirichletEnergy::computeDivGradPerVert at the current UV: (nV,)."""

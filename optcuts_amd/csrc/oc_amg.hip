@@ -1369,6 +1369,7 @@ template <typename T>
 int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
                   const std::vector<unsigned char>& fixedMask, std::string& err)
 {
+    ++generation_;
     // The previous hierarchy's rows are reused where they come out the same
     // (an air-mesh change leaves most of the mesh rows as they were).
     std::vector<AmgHostLevel> H = std::move(spare_);

@@ -143,6 +143,21 @@ struct oc_ctx {
 
     // ---- PCG workspace ---------------------------------------------------------
     DevBuf dX, dR, dZ, dP0, dP1, dQ, dB, dCtrl, dPcgPart;
+    // PCG iterations replayed from captured HIP graphs (an even iteration,
+    // an odd one, an even+odd pair, a run of pairs), valid for the buffers
+    // and sizes recorded in pcgSig
+    bool pcgGraphs = true;
+    std::vector<uintptr_t> pcgSig;
+    hipGraphExec_t pcgG[4] = {nullptr, nullptr, nullptr, nullptr};
+    void clear_pcg_graphs()
+    {
+        for (auto& g : pcgG)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        pcgSig.clear();
+    }
     int lastIters = 0;
     double lastRelRes = 0.0;
 
@@ -342,6 +357,81 @@ int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxI
                         ctrl);
     if (amg) amg->apply(c->stream, c->dR.as<double>(), c->dZ.as<double>(), partRZ, &ctrl->done);
     double* pbuf[2] = {c->dP0.as<double>(), c->dP1.as<double>()};
+    // One iteration's launches. The kernels take the iteration number from
+    // the control record, so the launches depend only on its parity (which
+    // buffers alternate) and every iteration of one parity is the same graph.
+    const auto enqueue = [&](int par) {
+        double* pOld = pbuf[par];
+        double* pNew = pbuf[par ^ 1];
+        const double* pu = partU[par];
+        if (amg)
+            oc::launch_pcg_spmv(c->stream, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.p, P.vals32,
+                                c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), partRZ, amg->rz_records(), 1,
+                                pu + 1, GU, 2, partS, ctrl);
+        else
+            oc::launch_pcg_spmv(c->stream, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.p, P.vals32,
+                                c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), pu, GU, 2, pu + 1, GU, 2, partS,
+                                ctrl);
+        oc::launch_pcg_update(c->stream, nV, pNew, c->dQ.as<double>(), Minv, c->dX.as<double>(), c->dR.as<double>(),
+                              c->dZ.as<double>(), partS, partU[par ^ 1], ctrl);
+        if (amg) amg->apply(c->stream, c->dR.as<double>(), c->dZ.as<double>(), partRZ, &ctrl->done);
+    };
+    // Graphs: 0 an even iteration, 1 an odd one, 2 an even+odd pair, 3 a run
+    // of kChain pairs. They name these buffers and sizes; anything else
+    // recaptures them.
+    constexpr int kChain = 8;
+    if (c->pcgGraphs) {
+        const uintptr_t sig[] = {(uintptr_t)nV,           (uintptr_t)P.dRowPtr.p, (uintptr_t)P.dColIdx.p,
+                                 (uintptr_t)P.dVals.p,    (uintptr_t)P.vals32,    (uintptr_t)Minv,
+                                 (uintptr_t)c->dX.p,      (uintptr_t)c->dR.p,     (uintptr_t)c->dZ.p,
+                                 (uintptr_t)c->dP0.p,     (uintptr_t)c->dP1.p,    (uintptr_t)c->dQ.p,
+                                 (uintptr_t)c->dPcgPart.p, (uintptr_t)ctrl,       (uintptr_t)amg,
+                                 amg ? (uintptr_t)amg->generation() : 0, amg ? (uintptr_t)amg->rz_records() : 0};
+        const size_t ns = sizeof(sig) / sizeof(sig[0]);
+        if (c->pcgSig.size() != ns || !std::equal(sig, sig + ns, c->pcgSig.begin())) {
+            c->clear_pcg_graphs();
+            c->pcgSig.assign(sig, sig + ns);
+        }
+    }
+    const auto graph = [&](int kind) -> hipGraphExec_t {
+        if (c->pcgG[kind]) return c->pcgG[kind];
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+        if (kind == 0 || kind == 2) enqueue(0);
+        if (kind == 1 || kind == 2) enqueue(1);
+        if (kind == 3)
+            for (int q = 0; q < 2 * kChain; ++q) enqueue(q & 1);
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(c->stream, &g) != hipSuccess || !g) return nullptr;
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return nullptr;
+        c->pcgG[kind] = ex;
+        return ex;
+    };
+    // Queues iterations [it, it + n): replayed graphs, or launches one by one
+    // (graphs off, or a capture that failed, which turns them off).
+    const auto queue = [&](int it, int n) -> hipError_t {
+        for (int k = 0; k < n;) {
+            const int par = (it + k) & 1;
+            const int kind = par ? 1 : (n - k >= 2 * kChain ? 3 : (n - k >= 2 ? 2 : 0));
+            const int span = kind == 3 ? 2 * kChain : (kind == 2 ? 2 : 1);
+            hipGraphExec_t ex = c->pcgGraphs ? graph(kind) : nullptr;
+            if (c->pcgGraphs && !ex) {
+                c->pcgGraphs = false;
+                c->clear_pcg_graphs();
+                (void)hipGetLastError();
+            }
+            if (ex) {
+                const hipError_t e = hipGraphLaunch(ex, c->stream);
+                if (e != hipSuccess) return e;
+            } else {
+                for (int q = 0; q < span; ++q) enqueue((par + q) & 1);
+            }
+            k += span;
+        }
+        return hipGetLastError();
+    };
     int it = 0;
     // Iterations are queued in batches and the control record read back after
     // each. Queued iterations past convergence still launch (and return at
@@ -351,23 +441,8 @@ int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxI
     int next = amg ? 2 : 32;
     const int maxBatch = amg ? 32 : 512;
     for (;;) {
-        for (int k = 0; k < batch; ++k, ++it) {
-            double* pOld = pbuf[it & 1];
-            double* pNew = pbuf[(it + 1) & 1];
-            const double* pu = partU[it & 1];
-            if (amg)
-                oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.p, P.vals32,
-                                    c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), partRZ, amg->rz_records(), 1,
-                                    pu + 1, GU, 2, partS, ctrl);
-            else
-                oc::launch_pcg_spmv(c->stream, it, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.p, P.vals32,
-                                    c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), pu, GU, 2, pu + 1, GU, 2,
-                                    partS, ctrl);
-            oc::launch_pcg_update(c->stream, it, nV, pNew, c->dQ.as<double>(), Minv, c->dX.as<double>(),
-                                  c->dR.as<double>(), c->dZ.as<double>(), partS, partU[(it + 1) & 1], ctrl);
-            if (amg) amg->apply(c->stream, c->dR.as<double>(), c->dZ.as<double>(), partRZ, &ctrl->done);
-        }
-        OC_HIP(c, hipGetLastError());
+        OC_HIP(c, queue(it, batch));
+        it += batch;
         OC_HIP(c, hipMemcpyAsync(&h, ctrl, sizeof(PcgCtrl), hipMemcpyDeviceToHost, c->stream));
         OC_HIP(c, hipStreamSynchronize(c->stream));
         if (h.done) break;
@@ -673,6 +748,7 @@ int oc_ctx_destroy(oc_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
+        c->clear_pcg_graphs();
         (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -933,6 +1009,16 @@ int oc_check_inversion(oc_ctx* c, int* ok)
     unsigned int n = 0;
     if (int st = count_inverted(c, mesh_set(c, 0), &n)) return st;
     *ok = (n == 0) ? 1 : 0;
+    return OC_OK;
+}
+
+int oc_pcg_graphs(oc_ctx* c, int enable)
+{
+    if (!c) return OC_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->clear_pcg_graphs();
+    c->pcgGraphs = enable != 0;
     return OC_OK;
 }
 
@@ -2188,12 +2274,12 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
     double* tUo = tS + GS;                   // 2*GU written by update
     double* tSo = tUo + 2 * (size_t)GU;      // GS   written by spmv
     auto spmv = [&]() {
-        oc::launch_pcg_spmv(s, 1, c->nVt, c->sys.dRowPtr.as<int>(), c->sys.dColIdx.as<int>(), c->sys.dVals.p,
+        oc::launch_pcg_spmv(s, c->nVt, c->sys.dRowPtr.as<int>(), c->sys.dColIdx.as<int>(), c->sys.dVals.p,
                             c->sys.vals32, c->dZ.as<double>(), c->dP0.as<double>(), c->dP1.as<double>(),
                             c->dQ.as<double>(), tU, GU, 2, tU + 1, GU, 2, tSo, ctrl);
     };
     auto update = [&]() {
-        oc::launch_pcg_update(s, 1, c->nVt, c->dP1.as<double>(), c->dQ.as<double>(), c->sys.dMinv.as<double>(),
+        oc::launch_pcg_update(s, c->nVt, c->dP1.as<double>(), c->dQ.as<double>(), c->sys.dMinv.as<double>(),
                               c->dTmpVec.as<double>(), c->dR.as<double>(), c->dZ.as<double>(), tS, tUo, ctrl);
     };
     auto launch = [&]() {
@@ -2249,6 +2335,7 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
         OC_HIP(c, hipStreamSynchronize(s));
         h.done = 0;
         h.iter = 1;
+        h.itNext = h.itCur = 1;
         h.maxit = 1 << 30;
         h.tol2 = 0.0;
         h.bb = 1.0;

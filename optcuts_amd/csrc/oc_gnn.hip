@@ -911,6 +911,7 @@ void Gnn::release_pattern()
 int Gnn::set_pattern(hipStream_t s, int nVin, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
                      const std::vector<unsigned char>& fixedMask, std::string& err)
 {
+    ++generation_;
     release_pattern();
     const int n = nVin;
     if (n <= 0 || (int)rowPtr.size() != n + 1 || (int)fixedMask.size() != n) {

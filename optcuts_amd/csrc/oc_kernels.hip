@@ -671,6 +671,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init(int nV, const double2* __re
         if (blockIdx.x == 0) {
             ctrl->done = 0;
             ctrl->iter = 0;
+            ctrl->itNext = 0;
+            ctrl->itCur = 0;
         }
     }
 }
@@ -742,7 +744,7 @@ __device__ __forceinline__ double4 to_double4(const float4& v)
 // row chunk (b % 8) * (G / 8) + b / 8 so each XCD owns a contiguous eighth of
 // the rows (G = pcg_grid_spmv, a multiple of 8).
 template <typename V4>
-__global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* __restrict__ rowPtr,
+__global__ void __launch_bounds__(kBlock) k_pcg_spmv(int nV, const int* __restrict__ rowPtr,
                                                      const int* __restrict__ colIdx,
                                                      const V4* __restrict__ vals,
                                                      const double2* __restrict__ z,
@@ -752,6 +754,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* 
                                                      double* partS, PcgCtrl* ctrl)
 {
     if (ctrl->done) return;
+    const int it = ctrl->itNext;
     constexpr int LANES = kPcgLanes, UNROLL = kPcgRows, RPP = kBlock / LANES;
     const int G = gridDim.x;
     const int c = (nV + G - 1) / G;
@@ -800,6 +803,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* 
                 ctrl->rzSlot[it & 1] = rz;
                 ctrl->rr = rr;
                 ctrl->iter = it;
+                ctrl->itCur = it;
                 if (conv) ctrl->done = 1;
                 else if (indef) ctrl->done = 3;
                 else if (capped) ctrl->done = 2;
@@ -843,24 +847,24 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int it, int nV, const int* 
     if (threadIdx.x == 0) partS[blockIdx.x] = acc[0];
 }
 
-void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const void* vals,
+void launch_pcg_spmv(hipStream_t s, int nV, const int* rowPtr, const int* colIdx, const void* vals,
                      int vals32, const double* z, const double* pOld, double* pNew, double* q, const double* partRz,
                      int nRz, int sRz, const double* partRr, int nRr, int sRr, double* partS, PcgCtrl* ctrl)
 {
     if (vals32)
-        hipLaunchKernelGGL(k_pcg_spmv<float4>, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, it, nV, rowPtr, colIdx,
+        hipLaunchKernelGGL(k_pcg_spmv<float4>, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, nV, rowPtr, colIdx,
                            reinterpret_cast<const float4*>(vals), reinterpret_cast<const double2*>(z),
                            reinterpret_cast<const double2*>(pOld), reinterpret_cast<double2*>(pNew),
                            reinterpret_cast<double2*>(q), partRz, nRz, sRz, partRr, nRr, sRr, partS, ctrl);
     else
-        hipLaunchKernelGGL(k_pcg_spmv<double4>, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, it, nV, rowPtr, colIdx,
+        hipLaunchKernelGGL(k_pcg_spmv<double4>, dim3(pcg_grid_spmv(nV)), dim3(kBlock), 0, s, nV, rowPtr, colIdx,
                            reinterpret_cast<const double4*>(vals), reinterpret_cast<const double2*>(z),
                            reinterpret_cast<const double2*>(pOld), reinterpret_cast<double2*>(pNew),
                            reinterpret_cast<double2*>(q), partRz, nRz, sRz, partRr, nRr, sRr, partS, ctrl);
 }
 
 template <bool APPLY_M>
-__global__ void __launch_bounds__(kBlock) k_pcg_update(int it, int nV, const double2* __restrict__ p,
+__global__ void __launch_bounds__(kBlock) k_pcg_update(int nV, const double2* __restrict__ p,
                                                        const double2* __restrict__ q,
                                                        const double* __restrict__ Minv, double2* x, double2* r,
                                                        double2* z, const double* __restrict__ partS, int nPartS,
@@ -873,7 +877,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int it, int nV, const dou
         if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->done = 3;
         return;
     }
+    const int it = ctrl->itCur;
     const double alpha = ctrl->rzSlot[it & 1] / pq[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->itNext = it + 1;
     int r0, r1;
     row_chunk(nV, r0, r1);
     double acc[2] = {0.0, 0.0};
@@ -900,16 +906,16 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int it, int nV, const dou
     }
 }
 
-void launch_pcg_update(hipStream_t s, int it, int nV, const double* pNew, const double* q, const double* Minv,
+void launch_pcg_update(hipStream_t s, int nV, const double* pNew, const double* q, const double* Minv,
                        double* x, double* r, double* z, const double* partS, double* partUout, PcgCtrl* ctrl)
 {
     if (Minv)
-        hipLaunchKernelGGL(k_pcg_update<true>, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, it, nV,
+        hipLaunchKernelGGL(k_pcg_update<true>, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, nV,
                            reinterpret_cast<const double2*>(pNew), reinterpret_cast<const double2*>(q), Minv,
                            reinterpret_cast<double2*>(x), reinterpret_cast<double2*>(r),
                            reinterpret_cast<double2*>(z), partS, pcg_grid_spmv(nV), partUout, ctrl);
     else
-        hipLaunchKernelGGL(k_pcg_update<false>, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, it, nV,
+        hipLaunchKernelGGL(k_pcg_update<false>, dim3(pcg_grid_update(nV)), dim3(kBlock), 0, s, nV,
                            reinterpret_cast<const double2*>(pNew), reinterpret_cast<const double2*>(q), Minv,
                            reinterpret_cast<double2*>(x), reinterpret_cast<double2*>(r),
                            reinterpret_cast<double2*>(z), partS, pcg_grid_spmv(nV), partUout, ctrl);

@@ -89,10 +89,16 @@ constexpr int kPcgLanes = 4;      // lanes per block-row in the PCG product
 constexpr int kPcgRows = 2;       // block rows per lane group and pass in the PCG product
 constexpr int kPcgMaxGrid = 1024;  // workgroups of the PCG kernels (bounds the partial records)
 
+// The iteration number lives here, not in launch arguments, so that every
+// iteration of one parity launches identical arguments (graph replay):
+// k_pcg_init sets itNext = 0; k_pcg_spmv reads itNext and records it in itCur
+// (and iter); k_pcg_update reads itCur and sets itNext = itCur + 1. Each field
+// is written by one kernel and read only by the next.
 struct PcgCtrl {
     double bb, tol2, rr, pad0;
     double rzSlot[2];
-    int iter, maxit, done, pad1;
+    int iter, maxit, done, itNext;
+    int itCur, pad1, pad2, pad3;
 };
 
 // Workgroups of the update/init kernels and of the SpMV kernel for nV rows.
@@ -106,10 +112,10 @@ int pcg_grid_spmv(int nV);
 void launch_pcg_init(hipStream_t s, int nV, const double* b, const double* Minv, double* x, double* r, double* z,
                      double* partU, PcgCtrl* ctrl);
 // vals: double4 blocks, or float4 blocks when vals32 (FP64 arithmetic either way).
-void launch_pcg_spmv(hipStream_t s, int it, int nV, const int* rowPtr, const int* colIdx, const void* vals,
+void launch_pcg_spmv(hipStream_t s, int nV, const int* rowPtr, const int* colIdx, const void* vals,
                      int vals32, const double* z, const double* pOld, double* pNew, double* q, const double* partRz,
                      int nRz, int sRz, const double* partRr, int nRr, int sRr, double* partS, PcgCtrl* ctrl);
-void launch_pcg_update(hipStream_t s, int it, int nV, const double* pNew, const double* q, const double* Minv,
+void launch_pcg_update(hipStream_t s, int nV, const double* pNew, const double* q, const double* Minv,
                        double* x, double* r, double* z, const double* partS, double* partUout, PcgCtrl* ctrl);
 
 // y = A x (2x2-block CSR; float blocks when vals32).

@@ -12,6 +12,13 @@ struct Preconditioner {
     // once while *stop is nonzero (a converged PCG); stop may be null.
     virtual void apply(hipStream_t s, const double* r, double* z, double* rzPart, const int* stop) = 0;
     virtual int rz_records() const = 0;
+    // Changes whenever the device buffers apply() launches on may have
+    // changed (a new set-up); launches captured into a graph stay valid while
+    // it holds.
+    unsigned long generation() const { return generation_; }
+
+protected:
+    unsigned long generation_ = 0;
 };
 
 }  // namespace oc

@@ -296,6 +296,40 @@ def test_air_mesh_change_reuses_rows_bit_exactly(ctx, precond):
     assert np.array_equal(uva, uvb)
 
 
+@pytest.mark.parametrize("precond", [capi.PRECOND_AMG, capi.PRECOND_JACOBI])
+def test_pcg_graph_replay_matches_eager_launches(ctx, precond):
+    """PCG iterations replayed from captured graphs (three graphs, replayed many times per solve, with the
+    Newton step's eager kernels, read-backs and synchronisations between solves) give bit-identical Newton
+    steps to the same iterations launched kernel by kernel, for several steps and after a re-initialisation."""
+    V, F, UV = height_field_disc(101, seed=3)
+    UV = distort(UV, 3, amp=0.03)
+    kw = dict(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=precond,
+              pcg_rel_tol=1e-6 if precond == capi.PRECOND_AMG else 1e-8, pcg_max_iter=50000)
+    runs = []
+    for graphs in (True, False):
+        c = capi.Context(0)
+        try:
+            c.pcg_graphs(graphs)
+            c.set_mesh(V, F, fixed=(0,))
+            c.set_uv(UV)
+            trace = [c.newton_init(**kw)]
+            for k in range(6):
+                s, _ = c.newton_step()
+                trace.append((s["E_total"], s["step"], s["pcg_iters"], s["pcg_rel_res"], s["pcg_fallback"]))
+                if k == 2:  # a fresh system on the same context: new buffers, graphs recaptured
+                    UVk = c.get_uv()
+                    c.set_uv(UVk)
+                    trace.append(c.newton_init(**kw))
+            runs.append((trace, c.get_uv()))
+        finally:
+            c.close()
+    (ta, ua), (tb, ub) = runs
+    for x, y in zip(ta, tb):
+        print(x, flush=True)
+        assert x == y
+    assert np.array_equal(ua, ub)
+
+
 def test_newton_200k_faces_properties(ctx):
     """Full-size workload (317x317 grid, 199,712 faces): size-independent properties."""
     V, F, UV = height_field_disc(317, seed=0)
