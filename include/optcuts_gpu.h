@@ -101,6 +101,14 @@ int oc_init_step_size(oc_ctx* ctx, const double* dir, double* stepSize);
 /* TriMesh::checkInversion (TriMesh.cpp:1736-1756): *ok = 1 if no triangle has negative signed UV area. */
 int oc_check_inversion(oc_ctx* ctx, int* ok);
 
+/* ---- seam length ----------------------------------------------------------- */
+/* TriMesh::computeSeamSparsity (TriMesh.cpp:1542-1558) at the context's UV, the E_se column of the energy trace
+ * (Optimizer.cpp:706-728): the sum of the rest lengths (edgeLen, TriMesh.cpp:340) of the cohesive edges cohE
+ * (nCoh x 4 vertex ids; a row with a negative id is a boundary edge and is skipped, TriMesh.cpp:334-339), in
+ * tri-soup mode (triSoup != 0) only of those with a vertex pair (cohE[0], cohE[2]) or (cohE[1], cohE[3]) more
+ * than 1e-2 avgEdgeLen apart in UV, plus initSeamLen. */
+int oc_seam_sparsity(oc_ctx* ctx, int nCoh, const int* cohE, double initSeamLen, int triSoup, double* sparsity);
+
 /* ---- split-query pre-screen (topology step) ------------------------------ */
 /* SymDirichletEnergy::computeDivGradPerVert (SymDirichletEnergy.cpp:108-154, the STANDARD_DEVIATION_FILTERING
  * branch it compiles) at the context's UV: per mesh vertex the sample standard deviation of its triangles'

@@ -70,6 +70,7 @@ _SIGS = {
     "oc_check_inversion": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
     "oc_div_grad_per_vert": (C.c_int, [_ctx, _dp]),
     "oc_pcg_graphs": (C.c_int, [_ctx, C.c_int]),
+    "oc_seam_sparsity": (C.c_int, [_ctx, C.c_int, _ip, C.c_double, C.c_int, C.POINTER(C.c_double)]),
     "oc_split_candidates": (C.c_int, [_ctx, C.c_int, C.c_double, _vp, C.c_int, _ip, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "oc_set_air_mesh": (C.c_int, [_ctx, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_double]),
@@ -276,6 +277,14 @@ class Context:
         ok = C.c_int()
         self._chk(self.L.oc_check_inversion(self.h, C.byref(ok)))
         return bool(ok.value)
+
+    def seam_sparsity(self, cohE, init_seam_len=0.0, tri_soup=False):
+        """TriMesh::computeSeamSparsity at the current UV (cohE: (nCoh, 4) vertex ids, -1 on boundary edges)."""
+        cohE = _i32(np.asarray(cohE).reshape(-1, 4))
+        out = C.c_double()
+        self._chk(self.L.oc_seam_sparsity(self.h, cohE.shape[0], cohE, float(init_seam_len), int(bool(tri_soup)),
+                                          C.byref(out)))
+        return out.value
 
     def pcg_graphs(self, enable):
         """PCG iterations from captured HIP graphs (True, the default) or launched one by one."""

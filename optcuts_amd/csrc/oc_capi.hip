@@ -83,7 +83,7 @@ struct oc_ctx {
     DevBuf dMvcPtr, dMvcSrc;
     // split-query pre-screen: per-vertex keys, and the mesh's boundary flags
     // and neighbour counts (built on first use for the mesh of meshGen topoGen)
-    DevBuf dDivGrad;
+    DevBuf dDivGrad, dCohE, dSeam;
     unsigned long topoGen = 0;
     std::vector<unsigned char> isBnd;
     std::vector<int> nbrCount;
@@ -1019,6 +1019,28 @@ int oc_pcg_graphs(oc_ctx* c, int enable)
     OC_HIP(c, hipStreamSynchronize(c->stream));
     c->clear_pcg_graphs();
     c->pcgGraphs = enable != 0;
+    return OC_OK;
+}
+
+int oc_seam_sparsity(oc_ctx* c, int nCoh, const int* cohE, double initSeamLen, int triSoup, double* sparsity)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && nCoh >= 0 && (nCoh == 0 || cohE) && sparsity, "oc_seam_sparsity: bad arguments");
+    for (long k = 0; k < 4L * nCoh; ++k)
+        OC_ARG(c, cohE[k] < c->nV, "oc_seam_sparsity: cohesive vertex id out of range");
+    (void)hipSetDevice(c->device);
+    if (ensure_scratch(c, std::max(1, nCoh))) return OC_ERR_HIP;
+    OC_HIP(c, c->dCohE.ensure(sizeof(int) * 4 * (size_t)std::max(1, nCoh)));
+    OC_HIP(c, c->dSeam.ensure(sizeof(double)));
+    if (nCoh)
+        OC_HIP(c, hipMemcpyAsync(c->dCohE.p, cohE, sizeof(int) * 4 * (size_t)nCoh, hipMemcpyHostToDevice, c->stream));
+    oc::launch_seam_len(c->stream, nCoh, c->dCohE.as<int>(), c->dVrest.as<double>(), c->dUV.as<double>(),
+                        c->avgEdgeLen, triSoup, c->dPartials.as<double>(), c->dSeam.as<double>());
+    OC_HIP(c, hipGetLastError());
+    double s = 0.0;
+    OC_HIP(c, hipMemcpyAsync(&s, c->dSeam.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    *sparsity = s + initSeamLen;
     return OC_OK;
 }
 

@@ -317,6 +317,48 @@ void launch_gather_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSr
 }
 
 // ----------------------------------------------------------------------------
+// Seam length (TriMesh::computeSeamSparsity, TriMesh.cpp:1542-1558): each
+// interior cohesive edge (no negative id) adds its rest length, in tri-soup
+// mode only when one of its vertex pairs (0, 2), (1, 3) lies more than
+// 1e-2 avgEdgeLen apart in UV. Per-block compensated sums, then one finisher.
+__global__ void __launch_bounds__(kBlock) k_seam_len(int nC, const int4* __restrict__ cohE,
+                                                     const double* __restrict__ Vrest, const double2* __restrict__ UV,
+                                                     double avgEdgeLen, int triSoup, double* partials)
+{
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    CSum acc[1] = {{0.0, 0.0}};
+    if (c < nC) {
+        const int4 e = cohE[c];
+        if (min(min(e.x, e.y), min(e.z, e.w)) >= 0) {
+            bool take = !triSoup;
+            if (!take) {
+                const double2 a = UV[e.x], b = UV[e.z], p = UV[e.y], q = UV[e.w];
+                const double d0 = sqrt((a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y));
+                const double d1 = sqrt((p.x - q.x) * (p.x - q.x) + (p.y - q.y) * (p.y - q.y));
+                take = d0 / avgEdgeLen > 1.0e-2 || d1 / avgEdgeLen > 1.0e-2;
+            }
+            if (take) {
+                const double* u = Vrest + 3 * (size_t)e.x;
+                const double* w = Vrest + 3 * (size_t)e.y;
+                const double dx = u[0] - w[0], dy = u[1] - w[1], dz = u[2] - w[2];
+                acc[0].s = sqrt(dx * dx + dy * dy + dz * dz);
+            }
+        }
+    }
+    publish_csum<1>(acc, partials);
+}
+
+void launch_seam_len(hipStream_t s, int nC, const int* cohE, const double* Vrest, const double* UV, double avgEdgeLen,
+                     int triSoup, double* partials, double* out)
+{
+    const int grid = grid_for(nC);
+    hipLaunchKernelGGL(k_seam_len, dim3(grid), dim3(kBlock), 0, s, nC, reinterpret_cast<const int4*>(cohE), Vrest,
+                       reinterpret_cast<const double2*>(UV), avgEdgeLen, triSoup, partials);
+    hipLaunchKernelGGL(k_finish_csum<1>, dim3(1), dim3(kBlock), 0, s, partials, grid, out);
+}
+
+// ----------------------------------------------------------------------------
 // Candidate key of the split query: per vertex, the sample standard deviation
 // of its triangles' gradient rows (SymDirichletEnergy::computeDivGradPerVert,
 // SymDirichletEnergy.cpp:108-154): the mean accumulated in triangle order and

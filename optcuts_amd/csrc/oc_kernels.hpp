@@ -53,6 +53,12 @@ void launch_triplets(hipStream_t s, const TriSet& T, const double* UV, const lon
 void launch_gather_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSrc, const unsigned char* fixedMask,
                         const double* elemG, double* g, double* partials, double* gsq);
 
+// out = total rest length of the cohesive edges that count as seam (cohE:
+// nC x 4 ids, rows with a negative id skipped; tri-soup mode: only edges
+// whose sides lie apart in UV), via per-block partials.
+void launch_seam_len(hipStream_t s, int nC, const int* cohE, const double* Vrest, const double* UV, double avgEdgeLen,
+                     int triSoup, double* partials, double* out);
+
 // out[v] = sample standard deviation of the contributions of vertex v's
 // triangles (0 for a vertex in one triangle): the split-query key.
 void launch_div_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSrc, const double* elemG, double* out);

@@ -89,3 +89,28 @@ def test_split_candidates_200k_faces(ctx):
     assert np.all(np.diff(-dg[cand]) >= 0)  # key order
     cand_in, ns_in = ctx.split_candidates(True, 0.6)
     assert ns_in == (m - 4) ** 2 and len(cand_in) == int(ns_in ** 0.6)
+
+
+def test_seam_sparsity_matches_oracle(ctx):
+    """TriMesh::computeSeamSparsity on the GPU: random cohesive edges over a 5k-face disc (some boundary rows
+    with -1, some whose sides coincide in UV), both modes, against the oracle (1e-12; the GPU sums in a
+    different order)."""
+    V, F, UV = height_field_disc(51, seed=2)
+    UV = distort(UV, 2, amp=0.04)
+    nV = V.shape[0]
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    feat = O.RestFeatures(V, F)
+    rng = np.random.default_rng(3)
+    coh = rng.integers(0, nV, size=(3000, 4)).astype(np.int32)
+    coh[::7, 2:] = -1                    # boundary cohesive edges
+    coh[1::5, 2] = coh[1::5, 0]          # sides that coincide in UV: not seam in tri-soup mode
+    coh[1::5, 3] = coh[1::5, 1]
+    coh[2::11, 2] = coh[2::11, 0]        # one side apart, the other together: still seam
+    for soup in (False, True):
+        for init in (0.0, 0.75):
+            got = ctx.seam_sparsity(coh, init, soup)
+            want = O.seam_sparsity(coh, V, UV, feat.avgEdgeLen, initSeamLen=init, triSoup=soup)
+            print(soup, init, got, want, flush=True)
+            assert got == pytest.approx(want, rel=1e-12)
+    assert ctx.seam_sparsity(np.zeros((0, 4), np.int32), 0.5) == 0.5
