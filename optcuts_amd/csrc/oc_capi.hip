@@ -1069,29 +1069,50 @@ int oc_split_candidates(oc_ctx* c, int splitInterior, double filterExpIn, const 
     mesh_topology(c);
     OC_HIP(c, hipStreamSynchronize(c->stream));
     // std::map<double, int> semantics: keys ascending, a later vertex with an
-    // equal key replaces the earlier one
-    std::vector<std::pair<double, int>> keys;
-    for (int v = 0; v < c->nV; ++v) {
-        if (c->nbrCount[v] <= 2) continue;
-        if (splitInterior) {
-            if (c->isBnd[v]) continue;
-            bool toBound = false;
-            for (int q = c->mvcPtr[v]; q < c->mvcPtr[v + 1] && !toBound; ++q) {
-                const int t = c->mvcSrc[q] / 3;
-                for (int k = 0; k < 3; ++k) toBound = toBound || c->isBnd[c->Fh[3 * (size_t)t + k]];
+    // equal key replaces the earlier one. The (key, vertex) pairs are
+    // distinct, so their sorted order is unique; it is built as sorted row
+    // chunks merged pairwise, on the host workers.
+    const int nth = oc::setup_threads(4L * c->nV);
+    std::vector<std::vector<std::pair<double, int>>> part(nth);
+    std::vector<char> nanKey(nth, 0);
+    oc::parallel_row_chunks(c->nV, nth, [&](int ch, int r0, int r1) {
+        auto& out = part[ch];
+        for (int v = r0; v < r1; ++v) {
+            if (c->nbrCount[v] <= 2) continue;
+            if (splitInterior) {
+                if (c->isBnd[v]) continue;
+                bool toBound = false;
+                for (int q = c->mvcPtr[v]; q < c->mvcPtr[v + 1] && !toBound; ++q) {
+                    const int t = c->mvcSrc[q] / 3;
+                    for (int k = 0; k < 3; ++k) toBound = toBound || c->isBnd[c->Fh[3 * (size_t)t + k]];
+                }
+                if (toBound) continue;
+            } else if (!c->isBnd[v]) {
+                continue;
             }
-            if (toBound) continue;
-        } else if (!c->isBnd[v]) {
-            continue;
+            const double key = -dg[v] / (vertWeight ? vertWeight[v] : 1.0);
+            if (std::isnan(key)) nanKey[ch] = 1;
+            out.push_back({key, v});
         }
-        const double key = -dg[v] / (vertWeight ? vertWeight[v] : 1.0);
-        if (std::isnan(key)) {
+        std::sort(out.begin(), out.end());
+    });
+    for (char b : nanKey)
+        if (b) {
             c->err = "oc_split_candidates: a split key is NaN (degenerate or inverted triangles)";
             return OC_ERR_DEGENERATE;
         }
-        keys.push_back({key, v});
+    std::vector<size_t> bound(1, 0);
+    for (auto& p : part) bound.push_back(bound.back() + p.size());
+    std::vector<std::pair<double, int>> keys(bound.back());
+    for (int ch = 0; ch < nth; ++ch) std::copy(part[ch].begin(), part[ch].end(), keys.begin() + bound[ch]);
+    for (size_t w = 1; w < (size_t)nth; w *= 2) {  // merge runs [i, i+w) and [i+w, i+2w) of chunks
+        const int nm = (int)((nth + 2 * w - 1) / (2 * w));
+        oc::parallel_row_chunks(nm, nm, [&](int j, int, int) {
+            const size_t a = 2 * w * j, b = std::min(a + w, (size_t)nth), e = std::min(a + 2 * w, (size_t)nth);
+            if (b < e)
+                std::inplace_merge(keys.begin() + bound[a], keys.begin() + bound[b], keys.begin() + bound[e]);
+        });
     }
-    std::sort(keys.begin(), keys.end());
     size_t m = 0;
     for (size_t i = 0; i < keys.size(); ++i) {
         if (i + 1 < keys.size() && keys[i + 1].first == keys[i].first) continue;

@@ -89,6 +89,11 @@ def test_split_candidates_200k_faces(ctx):
     assert np.all(np.diff(-dg[cand]) >= 0)  # key order
     cand_in, ns_in = ctx.split_candidates(True, 0.6)
     assert ns_in == (m - 4) ** 2 and len(cand_in) == int(ns_in ** 0.6)
+    # the host selection (filters, map order, cut; chunk-sorted and merged at this size) against the oracle's
+    # filter applied to the same GPU keys
+    for interior, got in ((False, cand), (True, cand_in)):
+        want, wns = O.split_candidates(F, V.shape[0], dg, None, interior, 0.6)
+        assert wns == (ns_in if interior else ns) and list(got) == list(want)
 
 
 def test_seam_sparsity_matches_oracle(ctx):
