@@ -119,3 +119,21 @@ def test_seam_sparsity_matches_oracle(ctx):
             print(soup, init, got, want, flush=True)
             assert got == pytest.approx(want, rel=1e-12)
     assert ctx.seam_sparsity(np.zeros((0, 4), np.int32), 0.5) == 0.5
+
+
+def test_prescreen_and_seam_argument_errors(ctx):
+    """Bad ids are rejected before any launch; a short candidate buffer receives the first entries and the
+    full count."""
+    V, F, UV = height_field_disc(21, seed=0)
+    nV = V.shape[0]
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(distort(UV, 0, amp=0.03))
+    with pytest.raises(capi.OCError):
+        ctx.seam_sparsity(np.array([[0, 1, 2, nV]], np.int32))
+    assert ctx.seam_sparsity(np.array([[0, 1, -1, -1]], np.int32), 0.25) == 0.25
+    full, ns = ctx.split_candidates(False)
+    cand = np.full(3, -7, np.int32)
+    n, m = capi.C.c_int(), capi.C.c_int()
+    st = ctx.L.oc_split_candidates(ctx.h, 0, 0.6, None, 3, cand, capi.C.byref(n), capi.C.byref(m))
+    assert st == capi.OC_OK and n.value == len(full) > 3 and m.value == ns
+    np.testing.assert_array_equal(cand, full[:3])
