@@ -178,7 +178,8 @@ typedef struct {
     int pcg_fallback; /* 0: the selected preconditioner solved the step. Otherwise its first attempt broke down
                          (r.z or p.Ap not positive) or hit pcgMaxIter, and 4 (OC_PRECOND_LEARNED with
                          OC_PRECISION_MIXED): the matrix rebuilt from FP64 element terms with a newly predicted
-                         factor solved it; 1: multigrid
+                         factor solved it (the Newton system's element terms then stay FP64 until the next
+                         oc_newton_init; line-search energies stay FP32); 1: multigrid
                          rebuilt with its guaranteed smoother interval solved it; 2: Jacobi-PCG solved it; 3:
                          Jacobi-PCG met negative curvature (p.Ap <= 0, possible only when FP32 element Hessians
                          leave the assembled matrix slightly indefinite) and the step uses its last iterate, or

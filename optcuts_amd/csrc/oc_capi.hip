@@ -131,6 +131,10 @@ struct oc_ctx {
     double pcgTol = 1e-10;
     int pcgMaxIter = 10000;
     int pcgHint = 0;  // iterations of the previous Newton step's solve (first-batch size)
+    // Mixed form with the learned factor: element terms of the Newton system
+    // (gradient, Hessian) in FP64 from the step whose FP32-assembled system
+    // broke the solve until the next oc_newton_init
+    bool elemFp64 = false;
 
     int precision = OC_PRECISION_FP64;  // requested for the next oc_newton_init
     int sysPrecision = OC_PRECISION_FP64;
@@ -1842,6 +1846,7 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     c->pcgTol = pcgRelTol;
     c->pcgMaxIter = pcgMaxIter;
     c->pcgHint = 0;
+    c->elemFp64 = false;
     c->sysPrecision = c->precision;
     // Optimizer constructor: exit(-1) on an inverted initial map (Optimizer.cpp:65-67)
     if (ensure_scratch(c, c->nF + c->nFair)) return OC_ERR_HIP;
@@ -1908,9 +1913,12 @@ int oc_newton_set_precision(oc_ctx* c, int precision)
 
 namespace {
 // Element pass + Hessian assembly of the Newton system in the driver's precision.
+// FP32 element terms in the Newton system (mixed form, not escalated)
+bool system_f32(const oc_ctx* c) { return c->sysPrecision == OC_PRECISION_MIXED && !c->elemFp64; }
+
 void system_element_pass(oc_ctx* c, const TriSet& T)
 {
-    const int f32 = c->sysPrecision == OC_PRECISION_MIXED;
+    const int f32 = system_f32(c);
     oc::launch_element(c->stream, T, c->dUV.as<double>(), nullptr, c->dElemG.as<double>(), c->dElemH.p, 1, nullptr,
                        nullptr, f32);
 }
@@ -1942,19 +1950,20 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S)
         // a matrix slightly indefinite along its near-null directions, which
         // the long Krylov runs of that preconditioner find. The matrix is then
         // rebuilt from FP64 element terms at the same UV (the right-hand side
-        // stays), the factor predicted again, and the solve retried. (With
+        // stays), the factor predicted again, and the solve retried; later
+        // steps of this solve build their Newton system from FP64 element
+        // terms directly (the line search keeps FP32 energies). (With
         // multigrid the FP32-stored hierarchy is what loses definiteness, and
         // the FP64 rebuild was measured not to help.) Multigrid retries with
         // its guaranteed (Gershgorin) smoother interval; what still fails falls
         // back to Jacobi-PCG at the same tolerance.
         S->pcg_iters_first = it;
         S->pcg_fallback = 2;
-        if (c->sysPrecision == OC_PRECISION_MIXED && c->nPrecond == OC_PRECOND_LEARNED) {
-            c->sysPrecision = OC_PRECISION_FP64;
+        if (system_f32(c) && c->nPrecond == OC_PRECOND_LEARNED) {
+            c->elemFp64 = true;
             system_element_pass(c, system_set(c));
             system_assemble(c);
             c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
-            c->sysPrecision = OC_PRECISION_MIXED;
             int it2 = 0;
             st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, c->ext);
             it += it2;
@@ -2131,7 +2140,7 @@ void system_assemble(oc_ctx* c)
 {
     const int nUB = (int)c->sm.ubRow.size();
     const int bj = c->nPrecond == OC_PRECOND_BLOCK_JACOBI;
-    if (c->sysPrecision == OC_PRECISION_MIXED)
+    if (system_f32(c))
         oc::launch_assemble<float>(c->stream, nUB, c->dUbRow.as<int>(), c->dUbCol.as<int>(), c->dUbPosU.as<int>(),
                                    c->dUbPosL.as<int>(), c->dUbCPtr.as<int>(), c->dUbCSrc.as<int>(),
                                    c->dFixedMaskSys.as<unsigned char>(), c->dFixedDiag.as<double>(),

@@ -193,7 +193,8 @@ def test_newton_learned_matches_reference_trace():
 def test_mixed_form_200k_reaches_target_without_jacobi(precond):
     """Learned factor in the mixed form at 200k faces, run to the workload's E_SD target (1e-6): a solve that
     breaks down on the FP32-assembled matrix is retried on the matrix rebuilt from FP64 element terms with a
-    newly predicted factor (pcg_fallback 4), so no step falls back to Jacobi-PCG or truncates."""
+    newly predicted factor (pcg_fallback 4, after which the solve's Newton systems stay FP64), so no step falls
+    back to Jacobi-PCG or truncates."""
     import json
     import os
     e_star = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "workload_targets.json")))["E_SD_star"]
