@@ -48,8 +48,8 @@ def lanczos0_bytes(nnzb, nV):
 # make the same counters read the known size of a plain 512 MiB device copy.
 # Summaries with commands: profiles/pmc_traffic_200k.md.
 TRAFFIC_200K = {
-    "k_lanczos_fine": (2.0 * 17083.3 + 3188.7) * 1024,
-    "k_pcg_spmv": (2.0 * 14334.1 + 3207.3) * 1024,
+    "k_lanczos_fine": (2.0 * 17072.8 + 3188.8) * 1024,
+    "k_pcg_spmv": (2.0 * 14334.2 + 3209.3) * 1024,
 }
 
 
