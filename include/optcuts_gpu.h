@@ -133,7 +133,9 @@ int oc_split_candidates(oc_ctx* ctx, int splitInterior, double filterExpIn, cons
  * appended after the mesh vertices (Scaffold.cpp:179-184). areaThres is airMesh.areaThres_AM. Air vertices that
  * are fixed mesh vertices are fixed (Scaffold.cpp:186-192). wScaf is the scaffold weight w_scaf
  * (Optimizer.cpp:87); each air energy is scaled by wScaf / nFair (Optimizer.cpp:776,795,839). Pass nFair = 0 to
- * drop the scaffold. */
+ * drop the scaffold. The next oc_newton_init builds the new pattern; on the same mesh it copies the pattern,
+ * slot-map and multigrid rows of mesh vertices outside the old and new air triangles from the previous build
+ * (identical result, a fraction of the time). */
 int oc_set_air_mesh(oc_ctx* ctx, int nVair, const double* airUV, int nFair, const int* Fair, int nBnd,
                     const int* bnd, double areaThres, double wScaf);
 
