@@ -161,13 +161,14 @@ def charts_bench(ctx, max_iter=60):
             "max_rel_diff_final_E_SD": float(abs(e_b - e_s).max() / e_s.min())}
 
 
-def scaffold_bench(ctx, V, F, UV, forms, iters=10, w_scaf=0.01):
+def scaffold_bench(ctx, V, F, UV, forms, iters=10, w_scaf=0.01, e_star=None, max_iter=40):
     """The geometry loop with the bijectivity scaffold, as the reference runs it by default: before every
     Newton step a new air mesh is made at the current UV (Optimizer.cpp:251-257; the reference re-meshes the
     air region with Triangle, here a two-row band outside the boundary with fresh random diagonals stands in
     for it), set (oc_set_air_mesh), and the system re-initialised (new pattern, slot maps and multigrid
     structure: oc_newton_init), then one step. Per-iteration wall times of the library calls; the air-mesh
-    generation is timed apart and left out of iters_per_s."""
+    generation is timed apart and left out of iters_per_s and of the time to target. With e_star, each form
+    runs until its mesh E_SD is within 1e-6 of it (at most max_iter iterations), else iters iterations."""
     import numpy as np
     from optcuts_amd.meshgen import band_air_mesh, boundary_loop_ccw
     loop = boundary_loop_ccw(F)
@@ -178,7 +179,8 @@ def scaffold_bench(ctx, V, F, UV, forms, iters=10, w_scaf=0.01):
         ctx.set_mesh(V, F, fixed=(0,))
         ctx.set_uv(UV)
         rows = []
-        for k in range(iters):
+        hit = None
+        for k in range(max_iter if e_star is not None else iters):
             t0 = time.perf_counter()
             airUV, Fair, bnd = band_air_mesh(ctx.get_uv(), loop, rings=2, seed=k)
             t1 = time.perf_counter()
@@ -189,7 +191,9 @@ def scaffold_bench(ctx, V, F, UV, forms, iters=10, w_scaf=0.01):
             s, done = ctx.newton_step()
             t3 = time.perf_counter()
             rows.append((t1 - t0, t2 - t1, t3 - t2, s["pcg_iters"], s["pcg_fallback"], s["E_SD"]))
-            if done:
+            if e_star is not None and hit is None and abs(s["E_SD"] - e_star) <= 1e-6 * e_star:
+                hit = len(rows)
+            if done or (hit is not None and len(rows) >= iters):
                 break
         a = np.array([r[:3] for r in rows]) * 1e3
         out[name] = {"newton_iterations": len(rows), "pcg_rel_tol": tol,
@@ -201,8 +205,15 @@ def scaffold_bench(ctx, V, F, UV, forms, iters=10, w_scaf=0.01):
                      "pcg_iters": [int(r[3]) for r in rows], "pcg_fallback_codes": [int(r[4]) for r in rows],
                      "E_SD_last": rows[-1][5],
                      "air_triangles": int(Fair.shape[0]), "air_vertices": int(airUV.shape[0])}
+        if e_star is not None:
+            out[name]["time_to_E_SD_1e-6"] = (
+                {"seconds": float((a[:hit, 1].sum() + a[:hit, 2].sum()) * 1e-3), "newton_iterations": hit}
+                if hit else None)
     if "chosen" in out and "plain" in out:
         out["speedup_iters_per_s_chosen_vs_plain"] = out["chosen"]["iters_per_s"] / out["plain"]["iters_per_s"]
+        tc, tp = out["chosen"].get("time_to_E_SD_1e-6"), out["plain"].get("time_to_E_SD_1e-6")
+        if tc and tp:
+            out["time_to_E_SD_speedup_chosen_vs_plain"] = tp["seconds"] / tc["seconds"]
     return out
 
 
@@ -390,7 +401,7 @@ def main():
             sf["chosen"] = (capi.PRECOND_AMG, args.chosen_tol)
         if "plain" in res:
             sf["plain"] = (capi.PRECOND_JACOBI, args.pcg_tol)
-        scaf = scaffold_bench(ctx, V, F, UV, sf, iters=args.scaffold_iters) if sf else None
+        scaf = scaffold_bench(ctx, V, F, UV, sf, iters=args.scaffold_iters, e_star=e_star) if sf else None
     pre = None if args.no_prescreen else prescreen_bench(ctx, V, F, UV)
     ctx.close()
     if net is not None:
