@@ -118,7 +118,7 @@ def time_to_target(ctx, V, F, UV, pcg_tol, precond, precision, e_star, max_iter)
 def charts_bench(ctx, max_iter=60):
     """The 64-chart workload (height-field discs of 500-20000 faces, log-uniform
     sizes, seeds 1000+i): all charts batched in one context versus the same
-    charts solved one after another, chosen solve form, each chart run until
+    charts solved one after another, multigrid form at PCG tolerance 1e-4, each chart run until
     its own stop (gradient tolerance or line-search stop) or max_iter."""
     from optcuts_amd import capi
     from optcuts_amd.meshgen import batched_disc_charts, union_of_charts
@@ -269,7 +269,14 @@ def main():
     ap.add_argument("--grid", type=int, default=317, help="vertex grid side (317 -> 199,712 faces)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--pcg-tol", type=float, default=1e-10, help="plain form PCG relative tolerance")
-    ap.add_argument("--chosen-tol", type=float, default=1e-4, help="chosen form PCG relative tolerance")
+    ap.add_argument("--chosen-tol", type=float, default=1e-3,
+                    help="chosen form (FP64 multigrid) PCG relative tolerance")
+    ap.add_argument("--mixed-tol", type=float, default=1e-4,
+                    help="mixed form PCG relative tolerance (at 1e-3 its FP32-stored hierarchy falls back to "
+                         "Jacobi on this workload)")
+    ap.add_argument("--learned-tol", type=float, default=5e-3,
+                    help="PCG relative tolerance of both learned forms (a factor that needs hundreds of PCG "
+                         "iterations for 1e-4 reaches the target sooner with looser inner solves)")
     ap.add_argument("--form", choices=["all", "both", "plain", "chosen", "mixed", "learned", "learned_mixed"],
                     default="all", help="all = every form (learned ones when trained weights exist); "
                                         "both = plain and chosen")
@@ -335,9 +342,9 @@ def main():
     V, F, UV = height_field_disc(args.grid, seed=args.seed)
     forms = {"plain": (capi.PRECOND_JACOBI, args.pcg_tol, capi.PRECISION_FP64),
              "chosen": (capi.PRECOND_AMG, args.chosen_tol, capi.PRECISION_FP64),
-             "mixed": (capi.PRECOND_AMG, args.chosen_tol, capi.PRECISION_MIXED),
-             "learned": (capi.PRECOND_LEARNED, args.chosen_tol, capi.PRECISION_FP64),
-             "learned_mixed": (capi.PRECOND_LEARNED, args.chosen_tol, capi.PRECISION_MIXED)}
+             "mixed": (capi.PRECOND_AMG, args.mixed_tol, capi.PRECISION_MIXED),
+             "learned": (capi.PRECOND_LEARNED, args.learned_tol, capi.PRECISION_FP64),
+             "learned_mixed": (capi.PRECOND_LEARNED, args.learned_tol, capi.PRECISION_MIXED)}
     from optcuts_amd import learned
     have_weights = os.path.exists(learned.DEFAULT_WEIGHTS)
     if args.form == "all":
