@@ -189,12 +189,13 @@ def test_newton_learned_matches_reference_trace():
         net.close()
 
 
-@pytest.mark.parametrize("precond", ["learned"])
-def test_mixed_form_200k_reaches_target_without_jacobi(precond):
+@pytest.mark.parametrize("precond,tol", [("learned", 1e-4), ("learned", 5e-3)])
+def test_mixed_form_200k_reaches_target_without_jacobi(precond, tol):
     """Learned factor in the mixed form at 200k faces, run to the workload's E_SD target (1e-6): a solve that
     breaks down on the FP32-assembled matrix is retried on the matrix rebuilt from FP64 element terms with a
     newly predicted factor (pcg_fallback 4, after which the solve's Newton systems stay FP64), so no step falls
-    back to Jacobi-PCG or truncates."""
+    back to Jacobi-PCG or truncates. 5e-3 is the benchmark's PCG tolerance for this form (inexact Newton: about
+    twice the Newton iterations of 1e-4, each much cheaper)."""
     import json
     import os
     e_star = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "workload_targets.json")))["E_SD_star"]
@@ -211,10 +212,10 @@ def test_mixed_form_200k_reaches_target_without_jacobi(precond):
         ctx.set_uv(UV)
         ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16 * nV / (nV - 1), allow_edec_rel_tol=False,
                         precond=capi.PRECOND_LEARNED if precond == "learned" else capi.PRECOND_AMG,
-                        pcg_rel_tol=1e-4, pcg_max_iter=200000 if precond == "learned" else 1000,
+                        pcg_rel_tol=tol, pcg_max_iter=200000 if precond == "learned" else 1000,
                         precision=capi.PRECISION_MIXED)
         codes = []
-        for k in range(1, 31):
+        for k in range(1, 61):
             s, done = ctx.newton_step()
             codes.append(s["pcg_fallback"])
             print(precond, "mixed iter", k, s["E_SD"], "pcg", s["pcg_iters"], "fallback", s["pcg_fallback"],

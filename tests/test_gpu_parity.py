@@ -350,6 +350,35 @@ def test_newton_200k_faces_properties(ctx):
     assert O.energy(F, UVg, feat) == pytest.approx(E_prev, rel=1e-12)
 
 
+def test_chosen_form_200k_faces_reaches_oracle_optimum(ctx):
+    """The benchmark's chosen form (FP64 multigrid PCG stopped at 1e-3 relative residual, an inexact Newton
+    step) at 199,712 faces reaches the oracle's converged E_SD (tests/golden/workload_targets.json) to 1e-6
+    without a PCG fallback; the final map is inversion-free and its energy under the oracle is the one
+    reported."""
+    import json
+    import os
+    e_star = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "workload_targets.json")))["E_SD_star"]
+    V, F, UV = height_field_disc(317, seed=0)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                    pcg_rel_tol=1e-3, pcg_max_iter=1000)
+    hit = None
+    for k in range(1, 31):
+        s, done = ctx.newton_step()
+        print("chosen 1e-3 iter", k, s["E_SD"], "pcg", s["pcg_iters"], "fallback", s["pcg_fallback"], flush=True)
+        assert s["pcg_fallback"] == 0, s
+        if abs(s["E_SD"] - e_star) <= 1e-6 * e_star:
+            hit = k
+            break
+        if done:
+            break
+    assert hit is not None and hit <= 20, hit
+    UVg = ctx.get_uv()
+    assert O.check_inversion(F, UVg)
+    assert O.energy(F, UVg, O.RestFeatures(V, F)) == pytest.approx(ctx.energy(), rel=1e-12)
+
+
 def test_newton_trace_amg_matches_oracle_5k_faces(ctx):
     """Chosen-form solve (FP32 smoothed-aggregation AMG under FP64 PCG) at a tight tolerance: the Newton
     trajectory stays on the oracle's (LDL^T) trajectory to 1e-9 per iteration."""
