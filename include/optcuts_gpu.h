@@ -95,6 +95,13 @@ int oc_gradient(oc_ctx* ctx, int uniformWeight, double* g);
  * length of I/J/V. */
 int oc_hessian_triplets(oc_ctx* ctx, int uniformWeight, long long capacity, int* I, int* J, double* V,
                         long long* count);
+/* The per-triangle terms of SymDirichletEnergy (SymDirichletEnergy.cpp:24-46, 258-304, 438-523 with
+ * IglUtils::makePD) as the device Newton driver's element pass computes them in the given precision
+ * (OC_PRECISION_FP64, or OC_PRECISION_MIXED: FP32 element arithmetic on positions relative to each triangle's first
+ * vertex), weights triArea / surfaceArea: E nF energies, G nF x 6 gradient rows [u0 v0 u1 v1 u2 v2], H nF x 36
+ * projected Hessians (row-major 6x6). Any pointer may be null. No reference counterpart: it exposes the element
+ * arithmetic of both precisions for error measurement against an FP64 evaluation. */
+int oc_element_terms(oc_ctx* ctx, int precision, double* E, double* G, double* H);
 /* SymDirichletEnergy::initStepSize (SymDirichletEnergy.cpp:551-610): *stepSize = min(*stepSize, first
  * positive root of det(U + a dir) over the triangles); dir has 2*nV values. */
 int oc_init_step_size(oc_ctx* ctx, const double* dir, double* stepSize);
@@ -138,6 +145,22 @@ int oc_split_candidates(oc_ctx* ctx, int splitInterior, double filterExpIn, cons
  * (identical result, a fraction of the time). */
 int oc_set_air_mesh(oc_ctx* ctx, int nVair, const double* airUV, int nFair, const int* Fair, int nBnd,
                     const int* bnd, double areaThres, double wScaf);
+/* The air mesh between Newton steps, in place of the reference's new Triangle air mesh after every step
+ * (Optimizer.cpp:251-257, Scaffold.cpp:27-176). The air vertices have moved with the steps (the step bound keeps
+ * the air triangles positively oriented). Every mode resets the air mesh's rest shape to its current UV (z = 0,
+ * areaThres_AM clamp as set), as a freshly built air mesh has, and evaluates *E = a0 E_SD + scaffold term.
+ *   OC_SCAFFOLD_REFRESH: the triangulation is kept; the Newton system's pattern is unchanged.
+ *   OC_SCAFFOLD_REMESH_WHEN_NEEDED: when the smallest angle of the air triangles is below minAngleDeg, the air mesh
+ *     is first re-meshed by Lawson edge flips (the Delaunay triangulation of the current air vertices constrained
+ *     by the map's boundary and the outer box) and the system re-patterned (only air rows change).
+ *   OC_SCAFFOLD_REMESH: flips at every call.
+ * *flips (may be null) = edges flipped (0: pattern unchanged). Needs oc_newton_init with an air mesh. Writes no
+ * trace line and keeps the state oc_newton_rebuild keeps. */
+enum { OC_SCAFFOLD_REFRESH = 0, OC_SCAFFOLD_REMESH_WHEN_NEEDED = 1, OC_SCAFFOLD_REMESH = 2 };
+int oc_scaffold_update(oc_ctx* ctx, int mode, double minAngleDeg, double* E, int* flips);
+/* The current air mesh: airUV nVair x 2 (air-mesh numbering: the boundary vertices first), Fair nFair x 3 (air
+ * ids, after any flips). Either pointer may be null. */
+int oc_air_mesh_get(oc_ctx* ctx, double* airUV, int* Fair);
 
 /* ---- LinSysSolver interface (LinSysSolver.hpp:22-256) -------------------- */
 /* LinSysSolver::set_pattern(vNeighbor, fixedVert) (LinSysSolver.hpp:37-135) + EigenLibSolver::set_pattern
@@ -196,6 +219,12 @@ typedef struct {
  * pcgRelTol/pcgMaxIter/precond select the Newton-system solve. */
 int oc_newton_init(oc_ctx* ctx, double a0, double relGL2Tol, int allowEDecRelTol, int precond, double pcgRelTol,
                    int pcgMaxIter, double* E0);
+/* The scaffold rebuild inside Optimizer::solve (Optimizer.cpp:251-257, re-patterned at :522-530): after
+ * oc_set_air_mesh replaced the air mesh of an initialised problem, rebuilds the Newton system with the parameters
+ * of the last oc_newton_init and evaluates *E = a0 E_SD + scaffold term. Unlike oc_newton_init it writes no trace
+ * line and keeps the solver state that carries across iterations (the previous solve's iteration count, the
+ * mixed learned form's switch to FP64 element terms). */
+int oc_newton_rebuild(oc_ctx* ctx, double* E);
 /* PCG iterations of the Newton solve (and of the other solves through this context) as replays of captured HIP
  * graphs (enable != 0, the default: an even iteration, an odd one and an even+odd pair are captured once per
  * set of buffers) or launched kernel by kernel (0). The results are identical either way. No reference
@@ -258,7 +287,21 @@ int oc_obj_write_mesh(const char* path, int nV, const double* Vrest, const doubl
  * files. */
 int oc_trace_open(oc_ctx* ctx, const char* folder);
 int oc_trace_set_seam(oc_ctx* ctx, double seamTerm);
+/* The seam column as Optimizer::writeEnergyValToFile computes it (Optimizer.cpp:706-728): every energy line
+ * recomputes E_se = TriMesh::computeSeamSparsity(E_se, !fractureMode) / virtualRadius at the current UV over the
+ * cohesive edges cohE (nCoh x 4, as oc_seam_sparsity) plus initSeamLen, virtualRadius = sqrt(surfaceArea / pi)
+ * (TriMesh.cpp:404); in fracture mode the first column is lastEnergyVal + (1 - a0) E_se. Replaces the constant of
+ * oc_trace_set_seam (and vice versa). */
+int oc_trace_set_seams(oc_ctx* ctx, int nCoh, const int* cohE, double initSeamLen, int fractureMode);
 int oc_trace_close(oc_ctx* ctx);
+/* info.txt as saveInfoForPresent writes it (main.cpp:346-387) at the context's UV: the caller's counts (input
+ * vertices and faces, iterations, topology iterations), lambda_init and timings[15] = {timing_total, secPast, topo,
+ * desc, scaf, enUp, mtrComp, mtrAssem, symFac, numFac, backSolve, lineSearch, bSplit, iSplit, cMerge}; the
+ * library's distortion (E_SD), seam length / virtualRadius (half the boundary length when a0 = 1, else the seams and
+ * mode of oc_trace_set_seams), TriMesh::outputStandardStretch (TriMesh.cpp:1585-1658) and the initial seams
+ * (nInitSeams x 4, printed as Eigen prints a matrix). 1 - energyParams[0] is 1 - a0 of the last oc_newton_init. */
+int oc_info_write(oc_ctx* ctx, const char* path, int vertAmtInput, int nFInitial, int iterNum, int topoIter,
+                  double lambdaInit, const double* timings, int nInitSeams, const int* initSeams);
 
 /* ---- batched charts ----------------------------------------------------------------------------------
  * Many independent charts in one context: after oc_set_mesh with the union of the charts (chart k owns the

@@ -28,6 +28,9 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 TUTTE_COMPONENTS = 0
 TUTTE_ALL_LOOPS = 1
+SCAFFOLD_REFRESH = 0
+SCAFFOLD_REMESH_WHEN_NEEDED = 1
+SCAFFOLD_REMESH = 2
 
 
 class OCError(RuntimeError):
@@ -66,6 +69,7 @@ _SIGS = {
     "oc_energy_per_elem": (C.c_int, [_ctx, C.c_int, _dp]),
     "oc_gradient": (C.c_int, [_ctx, C.c_int, _dp]),
     "oc_hessian_triplets": (C.c_int, [_ctx, C.c_int, C.c_longlong, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
+    "oc_element_terms": (C.c_int, [_ctx, C.c_int, _vp, _vp, _vp]),
     "oc_init_step_size": (C.c_int, [_ctx, _dp, C.POINTER(C.c_double)]),
     "oc_check_inversion": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
     "oc_div_grad_per_vert": (C.c_int, [_ctx, _dp]),
@@ -74,6 +78,8 @@ _SIGS = {
     "oc_split_candidates": (C.c_int, [_ctx, C.c_int, C.c_double, _vp, C.c_int, _ip, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "oc_set_air_mesh": (C.c_int, [_ctx, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_double]),
+    "oc_scaffold_update": (C.c_int, [_ctx, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "oc_air_mesh_get": (C.c_int, [_ctx, _vp, _vp]),
     "oc_solver_set_pattern": (C.c_int, [_ctx, C.c_int, _ip, _ip, C.c_int, _ip, C.POINTER(C.c_int)]),
     "oc_solver_get_pattern": (C.c_int, [_ctx, _ip, _ip]),
     "oc_solver_update_a": (C.c_int, [_ctx, C.c_longlong, _ip, _ip, _dp]),
@@ -108,6 +114,7 @@ _SIGS = {
     "oc_newton_init": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.POINTER(C.c_double)]),
     "oc_newton_step": (C.c_int, [_ctx, C.POINTER(IterStats), C.POINTER(C.c_int)]),
+    "oc_newton_rebuild": (C.c_int, [_ctx, C.POINTER(C.c_double)]),
     "oc_time_kernel": (C.c_int, [_ctx, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "oc_system_sizes": (C.c_int, [_ctx] + [C.POINTER(C.c_longlong)] * 4),
     "oc_io_last_error": (C.c_char_p, []),
@@ -117,6 +124,9 @@ _SIGS = {
     "oc_obj_write_mesh": (C.c_int, [C.c_char_p, C.c_int, _dp, _dp, C.c_int, _ip, _ip, C.c_int]),
     "oc_trace_open": (C.c_int, [_ctx, C.c_char_p]),
     "oc_trace_set_seam": (C.c_int, [_ctx, C.c_double]),
+    "oc_trace_set_seams": (C.c_int, [_ctx, C.c_int, _ip, C.c_double, C.c_int]),
+    "oc_info_write": (C.c_int, [_ctx, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int,
+                                _ip]),
     "oc_trace_close": (C.c_int, [_ctx]),
 }
 
@@ -268,6 +278,14 @@ class Context:
                                              V.ctypes.data, C.byref(n)))
         return I, J, V
 
+    def element_terms(self, precision=PRECISION_FP64):
+        """Per-triangle (E (nF,), G (nF, 6), H (nF, 6, 6)) of the element pass in the given precision."""
+        E = np.zeros(self.nF)
+        G = np.zeros((self.nF, 6))
+        H = np.zeros((self.nF, 6, 6))
+        self._chk(self.L.oc_element_terms(self.h, int(precision), E.ctypes.data, G.ctypes.data, H.ctypes.data))
+        return E, G, H
+
     def init_step_size(self, d, step=1.0):
         s = C.c_double(step)
         self._chk(self.L.oc_init_step_size(self.h, _f64(d), C.byref(s)))
@@ -310,6 +328,21 @@ class Context:
         airUV, Fair, bnd = _f64(airUV), _i32(Fair), _i32(bnd)
         self._chk(self.L.oc_set_air_mesh(self.h, airUV.shape[0], airUV, Fair.shape[0], Fair, len(bnd), bnd,
                                          float(area_thres), float(w_scaf)))
+        self.nVair, self.nFair = airUV.shape[0], Fair.shape[0]
+
+    def scaffold_update(self, mode=SCAFFOLD_REMESH_WHEN_NEEDED, min_angle_deg=10.0):
+        """The air mesh between steps (rest shape reset to the current UV; Lawson re-mesh per mode). Returns
+        (energy, edges flipped)."""
+        e, n = C.c_double(), C.c_int()
+        self._chk(self.L.oc_scaffold_update(self.h, int(mode), float(min_angle_deg), C.byref(e), C.byref(n)))
+        return e.value, n.value
+
+    def air_mesh_get(self):
+        """(airUV (nVair, 2), Fair (nFair, 3)) of the current air mesh."""
+        uv = np.zeros((self.nVair, 2))
+        F = np.zeros((self.nFair, 3), np.int32)
+        self._chk(self.L.oc_air_mesh_get(self.h, uv.ctypes.data, F.ctypes.data))
+        return uv, F
 
     # ---- LinSysSolver --------------------------------------------------------
     def solver_set_pattern(self, nV, nbr_ptr, nbr, fixed=(0,)):
@@ -380,6 +413,22 @@ class Context:
     def trace_close(self):
         self._chk(self.L.oc_trace_close(self.h))
 
+    def trace_set_seams(self, cohE, init_seam_len=0.0, fracture_mode=False):
+        """Seam column recomputed per energy line from the cohesive edges (Optimizer.cpp:706-728)."""
+        cohE = _i32(np.asarray(cohE).reshape(-1, 4))
+        self._chk(self.L.oc_trace_set_seams(self.h, cohE.shape[0], cohE, float(init_seam_len),
+                                            int(bool(fracture_mode))))
+
+    def info_write(self, path, vert_amt_input, nF_initial, iter_num, topo_iter, lambda_init, timings,
+                   init_seams=None):
+        """info.txt as saveInfoForPresent writes it (main.cpp:346-387); timings: 15 values."""
+        t = _f64(np.asarray(timings, dtype=np.float64).reshape(-1))
+        assert t.size == 15
+        s = _i32(np.zeros((0, 4)) if init_seams is None else np.asarray(init_seams).reshape(-1, 4))
+        self._chk(self.L.oc_info_write(self.h, os.fsencode(path), int(vert_amt_input), int(nF_initial),
+                                       int(iter_num), int(topo_iter), float(lambda_init), t, s.shape[0],
+                                       s if s.size else np.zeros(4, np.int32)))
+
     def set_charts(self, vertex_start, face_start):
         """Batched charts: contiguous vertex / triangle ranges of the mesh set by set_mesh."""
         vs, fs = _i32(vertex_start), _i32(face_start)
@@ -404,6 +453,13 @@ class Context:
         e = C.c_double()
         self._chk(self.L.oc_newton_init(self.h, float(a0), float(rel_gl2_tol), int(allow_edec_rel_tol),
                                         int(precond), float(pcg_rel_tol), int(pcg_max_iter), C.byref(e)))
+        return e.value
+
+    def newton_rebuild(self):
+        """After set_air_mesh: the scaffold rebuild of Optimizer::solve (new system, same parameters, no trace
+        line). Returns the energy at the current UV."""
+        e = C.c_double()
+        self._chk(self.L.oc_newton_rebuild(self.h, C.byref(e)))
         return e.value
 
     def newton_step(self):

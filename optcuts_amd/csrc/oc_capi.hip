@@ -2,6 +2,7 @@
 // layout, sparsity-pattern and slot-map construction, and the
 // device-resident Newton step. Kernels live in oc_kernels.hip.
 #include "../../include/optcuts_gpu.h"
+#include "oc_air_mesh.hpp"
 #include "oc_amg.hpp"
 #include "oc_amg_structure.hpp"
 #include "oc_common.hpp"
@@ -11,6 +12,7 @@
 #include "oc_system_pattern.hpp"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -70,6 +72,12 @@ struct oc_ctx {
     FILE* trE = nullptr;
     FILE* trG = nullptr;
     double trSeam = 0.0;
+    // the seam column recomputed per line from the cohesive edges (oc_trace_set_seams)
+    bool trSeams = false;
+    int trFracture = 0, trNCoh = 0;
+    double trInitSeamLen = 0.0;
+    std::vector<int> trCohE;
+    DevBuf dTrCohE;
 
     // ---- mesh block (block 0) -----------------------------------------------
     int nV = 0, nF = 0;
@@ -94,7 +102,10 @@ struct oc_ctx {
 
     // ---- air block (block 1) -------------------------------------------------
     int nVair = 0, nFair = 0, nBnd = 0;
-    double wScaf = 0.0;
+    double wScaf = 0.0, airAreaThres = 0.0;
+    std::vector<int> FairLocal;         // nFair x 3 in air-mesh numbering
+    std::vector<int> airL2G;            // air vertex -> combined vertex (localVI2Global)
+    DevBuf dAirL2G, dAirUV2;            // the same on the device; air-vertex UV copy-out
     std::vector<int> FairGlobal;        // nFair x 3 in combined numbering
     std::vector<unsigned char> airFixedGlobal;  // per combined vertex: fixed in the air mesh
 
@@ -103,10 +114,15 @@ struct oc_ctx {
     DevBuf dVrest, dTmpVrest, dTmpF;
     DevBuf dMeshSums, dAirSums;  // 3 doubles each
     DevBuf dUV;                  // combined vertices x 2
+    DevBuf dUVspare;             // the other UV buffer of an air-mesh change
     DevBuf dFixedMaskMesh;
 
     // scratch
-    DevBuf dPartials, dCounters, dScalars, dOrdMin, dLsOut, dEelem, dElemG, dElemH, dG, dTmpVec, dDir;
+    DevBuf dPartials, dCounters, dScalars, dOrdMin, dLsOut, dEelem, dElemG, dElemH, dG, dTmpVec, dDir, dLsDec;
+    // pinned host words the Newton step reads back asynchronously (the gradient norm, the line-search outcome),
+    // and the event that marks the gradient norm's arrival
+    double* mail = nullptr;
+    hipEvent_t evGsq = nullptr;
     DevBuf dTrI, dTrJ, dTrV;
 
     // ---- Newton system (combined) ---------------------------------------------
@@ -217,6 +233,7 @@ namespace {
 
 constexpr int kMaxPartialBlocks = 1 << 16;
 constexpr int kRzRecords = 2048;  // r.z partial records of an external preconditioner
+constexpr size_t kMailBytes = 4096;  // pinned read-back words of a context
 
 int ensure_scratch(oc_ctx* c, long nItems)
 {
@@ -233,6 +250,7 @@ int ensure_scratch(oc_ctx* c, long nItems)
     }
     OC_HIP(c, c->dOrdMin.ensure(sizeof(unsigned long long) * 8));
     OC_HIP(c, c->dLsOut.ensure(sizeof(double) * 4 * oc::kLSCand));
+    OC_HIP(c, c->dLsDec.ensure(sizeof(oc::LsDecision)));
     (void)kMaxPartialBlocks;
     return OC_OK;
 }
@@ -617,8 +635,9 @@ int build_system(oc_ctx* c)
     OC_HIP(c, c->dElemG.ensure(sizeof(double) * 6 * (size_t)std::max(1, nFt)));
     OC_HIP(c, c->dElemH.ensure(sizeof(double) * 24 * (size_t)std::max(1, nFt)));
     OC_HIP(c, c->dG.ensure(sizeof(double) * 2 * (size_t)std::max(1, nVt)));
+    OC_HIP(c, c->dTmpVec.ensure(sizeof(double) * 2 * (size_t)std::max(1, nVt)));
     if (ensure_pcg(c, nVt)) return OC_ERR_HIP;
-    if (ensure_scratch(c, std::max(nFt, (int)c->sm.ubRow.size()))) return OC_ERR_HIP;
+    if (ensure_scratch(c, std::max(std::max(nFt, nVt), (int)c->sm.ubRow.size()))) return OC_ERR_HIP;
     c->amgValid = false;
     c->amg = nullptr;
     c->ext = nullptr;
@@ -685,6 +704,38 @@ int div_grad_keys(oc_ctx* c)
     return OC_OK;
 }
 
+// E_se / virtualRadius at the current UV: TriMesh::computeSeamSparsity(E_se, !fractureMode) over the cohesive
+// edges of oc_trace_set_seams plus initSeamLen, divided by sqrt(surfaceArea / pi) (TriMesh.cpp:404,1542-1558;
+// Optimizer.cpp:708-710).
+int seam_term(oc_ctx* c, double* out)
+{
+    double s = 0.0;
+    if (c->trNCoh > 0) {
+        if (ensure_scratch(c, c->trNCoh)) return OC_ERR_HIP;
+        oc::launch_seam_len(c->stream, c->trNCoh, c->dTrCohE.as<int>(), c->dVrest.as<double>(), c->dUV.as<double>(),
+                            c->avgEdgeLen, !c->trFracture, c->dPartials.as<double>(), c->dSeam.as<double>());
+        OC_HIP(c, hipGetLastError());
+        OC_HIP(c, hipMemcpyAsync(&s, c->dSeam.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        OC_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    *out = (s + c->trInitSeamLen) / std::sqrt(c->surfaceArea / M_PI);
+    return OC_OK;
+}
+
+// One energyValPerIter.txt line (Optimizer::writeEnergyValToFile, Optimizer.cpp:706-728): in fracture mode the
+// first column is lastEnergyVal + (1 - energyParams[0]) E_se, else lastEnergyVal; then E_SD, E_se, a0. Without
+// oc_trace_set_seams the seam column is the caller's constant (oc_trace_set_seam).
+int write_energy_trace(oc_ctx* c, double E, double Esd)
+{
+    double seam = c->trSeam, col0 = E;
+    if (c->trSeams) {
+        if (int st = seam_term(c, &seam)) return st;
+        if (c->trFracture) col0 = E + (1.0 - c->a0) * seam;
+    }
+    oc::io::trace_energy_line(c->trE, col0, Esd, seam, c->a0);
+    return OC_OK;
+}
+
 // Boundary flags and neighbour counts of the mesh (TriMesh::computeFeatures
 // builds vNeighbor and the directed-edge map edge2Tri, TriMesh.cpp:437-446;
 // isBoundaryVert, TriMesh.cpp:2069-2083, asks for a neighbour w with (v, w) or
@@ -741,6 +792,11 @@ int oc_ctx_create(int device, oc_ctx** out)
         delete c;
         return OC_ERR_HIP;
     }
+    if (hipHostMalloc((void**)&c->mail, kMailBytes, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evGsq, hipEventDisableTiming) != hipSuccess) {
+        (void)oc_ctx_destroy(c);
+        return OC_ERR_HIP;
+    }
     *out = c;
     return OC_OK;
 }
@@ -755,6 +811,8 @@ int oc_ctx_destroy(oc_ctx* c)
         c->clear_pcg_graphs();
         (void)hipStreamDestroy(c->stream);
     }
+    if (c->evGsq) (void)hipEventDestroy(c->evGsq);
+    if (c->mail) (void)hipHostFree(c->mail);
     delete c;
     return OC_OK;
 }
@@ -798,6 +856,24 @@ int oc_trace_set_seam(oc_ctx* c, double seamTerm)
 {
     if (!c) return OC_ERR_ARG;
     c->trSeam = seamTerm;
+    c->trSeams = false;
+    return OC_OK;
+}
+
+int oc_trace_set_seams(oc_ctx* c, int nCoh, const int* cohE, double initSeamLen, int fractureMode)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && nCoh >= 0 && (nCoh == 0 || cohE), "oc_trace_set_seams: bad arguments");
+    for (long k = 0; k < 4L * nCoh; ++k) OC_ARG(c, cohE[k] < c->nV, "oc_trace_set_seams: vertex id out of range");
+    (void)hipSetDevice(c->device);
+    c->trCohE.assign(cohE, cohE + 4 * (size_t)nCoh);
+    if (upload(c, c->dTrCohE, c->trCohE)) return OC_ERR_HIP;
+    OC_HIP(c, c->dSeam.ensure(sizeof(double)));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->trNCoh = nCoh;
+    c->trInitSeamLen = initSeamLen;
+    c->trFracture = fractureMode ? 1 : 0;
+    c->trSeams = true;
     return OC_OK;
 }
 
@@ -991,6 +1067,52 @@ int oc_hessian_triplets(oc_ctx* c, int uniformWeight, long long capacity, int* I
     return OC_OK;
 }
 
+int oc_element_terms(oc_ctx* c, int precision, double* E, double* G, double* H)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0, "oc_element_terms: no mesh");
+    OC_ARG(c, precision == OC_PRECISION_FP64 || precision == OC_PRECISION_MIXED, "oc_element_terms: bad precision");
+    (void)hipSetDevice(c->device);
+    const size_t nF = (size_t)c->nF;
+    const int f32 = precision == OC_PRECISION_MIXED;
+    OC_HIP(c, c->dElemH.ensure(sizeof(double) * 24 * nF));
+    oc::launch_element(c->stream, mesh_set(c, 0), c->dUV.as<double>(), c->dEelem.as<double>(), c->dElemG.as<double>(),
+                       H ? c->dElemH.p : nullptr, 1, nullptr, nullptr, f32);
+    OC_HIP(c, hipGetLastError());
+    if (E) OC_HIP(c, hipMemcpyAsync(E, c->dEelem.p, sizeof(double) * nF, hipMemcpyDeviceToHost, c->stream));
+    if (G) OC_HIP(c, hipMemcpyAsync(G, c->dElemG.p, sizeof(double) * 6 * nF, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> blk;
+    if (H) {
+        blk.resize(24 * nF);
+        if (f32) {
+            std::vector<float> b32(24 * nF);
+            OC_HIP(c, hipMemcpyAsync(b32.data(), c->dElemH.p, sizeof(float) * 24 * nF, hipMemcpyDeviceToHost,
+                                     c->stream));
+            OC_HIP(c, hipStreamSynchronize(c->stream));
+            std::copy(b32.begin(), b32.end(), blk.begin());
+        } else {
+            OC_HIP(c, hipMemcpyAsync(blk.data(), c->dElemH.p, sizeof(double) * 24 * nF, hipMemcpyDeviceToHost,
+                                     c->stream));
+        }
+    }
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    if (H) {
+        // upper 2x2 blocks (vertex pairs (0,0) (0,1) (0,2) (1,1) (1,2) (2,2), each [xx, xy, yx, yy]) -> full 6x6
+        static const int bi[6] = {0, 0, 0, 1, 1, 2}, bj[6] = {0, 1, 2, 1, 2, 2};
+        for (size_t t = 0; t < nF; ++t) {
+            double* M = H + 36 * t;
+            for (int b = 0; b < 6; ++b)
+                for (int r = 0; r < 2; ++r)
+                    for (int q = 0; q < 2; ++q) {
+                        const double v = blk[24 * t + 4 * b + 2 * r + q];
+                        M[(2 * bi[b] + r) * 6 + 2 * bj[b] + q] = v;
+                        M[(2 * bj[b] + q) * 6 + 2 * bi[b] + r] = v;
+                    }
+        }
+    }
+    return OC_OK;
+}
+
 int oc_init_step_size(oc_ctx* c, const double* dir, double* stepSize)
 {
     if (!c) return OC_ERR_ARG;
@@ -1045,6 +1167,121 @@ int oc_seam_sparsity(oc_ctx* c, int nCoh, const int* cohE, double initSeamLen, i
     OC_HIP(c, hipMemcpyAsync(&s, c->dSeam.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     OC_HIP(c, hipStreamSynchronize(c->stream));
     *sparsity = s + initSeamLen;
+    return OC_OK;
+}
+
+int oc_info_write(oc_ctx* c, const char* path, int vertAmtInput, int nFInitial, int iterNum, int topoIter,
+                  double lambdaInit, const double* timings, int nInitSeams, const int* initSeams)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && path && timings && nInitSeams >= 0 && (nInitSeams == 0 || initSeams),
+           "oc_info_write: bad arguments");
+    (void)hipSetDevice(c->device);
+    const int nV = c->nV, nF = c->nF;
+    std::vector<double> UV(2 * (size_t)nV);
+    OC_HIP(c, hipMemcpyAsync(UV.data(), c->dUV.p, sizeof(double) * UV.size(), hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    // distortion: energyTerms[0]->computeEnergyVal at the result (main.cpp:372-373)
+    double distortion, unused;
+    if (int st = energy_sums(c, mesh_set(c, 0), &distortion, &unused)) return st;
+    // seam length (main.cpp:360-370): half the boundary length in pure distortion mode (energyParams[0] == 1),
+    // else computeSeamSparsity(!fractureMode) over the cohesive edges of oc_trace_set_seams
+    const double virtualRadius = std::sqrt(c->surfaceArea / M_PI);
+    double seamTerm = 0.0;  // seamLen / virtualRadius
+    const double* Vr = c->Vh.data();
+    auto restLen = [&](int a, int b) {
+        const double dx = Vr[3 * (size_t)a] - Vr[3 * (size_t)b], dy = Vr[3 * (size_t)a + 1] - Vr[3 * (size_t)b + 1],
+                     dz = Vr[3 * (size_t)a + 2] - Vr[3 * (size_t)b + 2];
+        return std::sqrt(dx * dx + dy * dy + dz * dz);
+    };
+    if (c->a0 == 1.0) {
+        // TriMesh::computeBoundaryLen (TriMesh.cpp:1533-1541): directed edges whose reverse is no edge
+        std::vector<std::pair<int, int>> he;
+        he.reserve(3 * (size_t)nF);
+        for (int t = 0; t < nF; ++t)
+            for (int k = 0; k < 3; ++k) he.emplace_back(c->Fh[3 * (size_t)t + k], c->Fh[3 * (size_t)t + (k + 1) % 3]);
+        std::vector<std::pair<int, int>> sorted(he);
+        std::sort(sorted.begin(), sorted.end());
+        sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
+        double seamLen = 0.0;
+        for (const auto& e : sorted)
+            if (!std::binary_search(sorted.begin(), sorted.end(), std::make_pair(e.second, e.first)))
+                seamLen += restLen(e.second, e.first);
+        seamLen /= 2.0;
+        seamTerm = seamLen / virtualRadius;
+    } else if (c->trSeams) {
+        if (int st = seam_term(c, &seamTerm)) return st;
+    }
+    // TriMesh::computeStandardStretch (TriMesh.cpp:1585-1652) with IglUtils::computeDeformationGradient and
+    // mapTriangleTo2D (IglUtils.cpp:172-194)
+    double l2 = 0.0, sinf = -DBL_MAX, shear = 0.0, cinf = DBL_MAX, areaUV = 0.0, area = 0.0;
+    for (int t = 0; t < nF; ++t) {
+        const int* f = &c->Fh[3 * (size_t)t];
+        double e0[3], e1[3];
+        for (int d = 0; d < 3; ++d) {
+            e0[d] = Vr[3 * (size_t)f[1] + d] - Vr[3 * (size_t)f[0] + d];
+            e1[d] = Vr[3 * (size_t)f[2] + d] - Vr[3 * (size_t)f[0] + d];
+        }
+        const double e0n = std::sqrt(e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2]);
+        const double cx = e0[1] * e1[2] - e0[2] * e1[1], cy = e0[2] * e1[0] - e0[0] * e1[2],
+                     cz = e0[0] * e1[1] - e0[1] * e1[0];
+        const double x1 = e0n, x2x = (e0[0] * e1[0] + e0[1] * e1[1] + e0[2] * e1[2]) / e0n,
+                     x2y = std::sqrt(cx * cx + cy * cy + cz * cz) / e0n;
+        const double u01[2] = {UV[2 * (size_t)f[1]] - UV[2 * (size_t)f[0]], UV[2 * (size_t)f[1] + 1] - UV[2 * (size_t)f[0] + 1]};
+        const double u02[2] = {UV[2 * (size_t)f[2]] - UV[2 * (size_t)f[0]], UV[2 * (size_t)f[2] + 1] - UV[2 * (size_t)f[0] + 1]};
+        const double u01Len = std::sqrt(u01[0] * u01[0] + u01[1] * u01[1]);
+        // U = [[u01Len, u01.u02 / u01Len], [0, (u01 x u02) / u01Len]], Vm = [[x1, x2x], [0, x2y]], F = Vm U^-1
+        const double U00 = u01Len, U01 = (u01[0] * u02[0] + u01[1] * u02[1]) / u01Len,
+                     U11 = (u01[0] * u02[1] - u01[1] * u02[0]) / u01Len;
+        const double invdet = 1.0 / (U00 * U11);  // Eigen's 2x2 inverse: cofactors times 1/det
+        const double I00 = U11 * invdet, I01 = -U01 * invdet, I11 = U00 * invdet;
+        const double F00 = x1 * I00, F01 = x1 * I01 + x2x * I11, F10 = 0.0, F11 = x2y * I11;
+        const double a = F00 * F00 + F10 * F10, b = F00 * F01 + F10 * F11, cc = F01 * F01 + F11 * F11;
+        const double t0 = a + cc, t1 = std::sqrt((a - cc) * (a - cc) + 4. * b * b);
+        const double tau = std::sqrt((t0 + t1) / 2.), gamma = std::sqrt((t0 - t1) / 2.);
+        const double triArea = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);  // TriMesh.cpp:364-367
+        l2 += t0 / 2.0 * triArea;
+        if (tau > sinf) sinf = tau;
+        shear += b * b / a / cc * triArea;
+        if (gamma < cinf) cinf = gamma;
+        areaUV += 0.5 * (u01[0] * u02[1] - u01[1] * u02[0]);
+        area += triArea;
+    }
+    l2 = std::sqrt(l2 / c->surfaceArea);
+    shear = std::sqrt(shear / c->surfaceArea);
+    const double scale = std::sqrt(areaUV / c->surfaceArea);
+    l2 *= scale;
+    sinf *= scale;
+    cinf *= scale;
+    (void)area;
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) {
+        c->err = std::string("oc_info_write: cannot create ") + path;
+        return OC_ERR_ARG;
+    }
+    static const char* lbl[13] = {"topo", "desc", "scaf", "enUp", "mtrComp", "mtrAssem", "symFac", "numFac",
+                                  "backSolve", "lineSearch", "bSplit", "iSplit", "cMerge"};
+    std::fprintf(fp, "%d %d\n", vertAmtInput, nFInitial);
+    std::fprintf(fp, "%d %d 0 0 %g %g\n", iterNum, topoIter, lambdaInit, 1.0 - c->a0);
+    std::fprintf(fp, "0.0 0.0 %g %g", timings[0], timings[1]);
+    for (int k = 0; k < 13; ++k) std::fprintf(fp, " %s%g", lbl[k], timings[2 + k]);
+    std::fprintf(fp, "\n%g %g\n", distortion, seamTerm);
+    std::fprintf(fp, "%g %g %g %g\n", l2, sinf, shear, cinf);
+    std::fprintf(fp, "initialSeams %d\n", nInitSeams);
+    // Eigen's default matrix output: every entry right-aligned to the widest entry, one space apart
+    int width = 0;
+    for (long k = 0; k < 4L * nInitSeams; ++k) width = std::max(width, std::snprintf(nullptr, 0, "%d", initSeams[k]));
+    for (int r = 0; r < nInitSeams; ++r) {
+        for (int q = 0; q < 4; ++q) std::fprintf(fp, q ? " %*d" : "%*d", width, initSeams[4 * (size_t)r + q]);
+        if (r + 1 < nInitSeams) std::fputc('\n', fp);
+    }
+    std::fputc('\n', fp);
+    const bool ok = std::ferror(fp) == 0;
+    std::fclose(fp);
+    if (!ok) {
+        c->err = std::string("oc_info_write: write error on ") + path;
+        return OC_ERR_ARG;
+    }
     return OC_OK;
 }
 
@@ -1162,6 +1399,11 @@ int oc_set_air_mesh(oc_ctx* c, int nVair, const double* airUV, int nFair, const 
     for (int l = 0; l < nVair; ++l) l2g[l] = l < nBnd ? bnd[l] : c->nV + l - nBnd;
     c->FairGlobal.resize(3 * (size_t)nFair);
     for (long k = 0; k < 3L * nFair; ++k) c->FairGlobal[k] = l2g[Fair[k]];
+    c->FairLocal.assign(Fair, Fair + 3 * (size_t)nFair);
+    c->airAreaThres = areaThres;
+    c->airL2G = l2g;
+    if (upload(c, c->dAirL2G, c->airL2G)) return OC_ERR_HIP;
+    OC_HIP(c, c->dAirUV2.ensure(sizeof(double) * 2 * (size_t)nVair));
     c->airFixedGlobal.assign(nVt, 0);
     for (int l = 0; l < nBnd; ++l)
         if (c->fixedMaskMesh[bnd[l]]) c->airFixedGlobal[bnd[l]] = 1;
@@ -1192,8 +1434,9 @@ int oc_set_air_mesh(oc_ctx* c, int nVair, const double* airUV, int nFair, const 
         return OC_ERR_DEGENERATE;
     }
     if (int st = upload_tri_ids(c, c->nF, nFair, c->FairGlobal.data())) return st;
-    // combined UV: keep the mesh part, append the interior air vertices
-    DevBuf nu;
+    // combined UV: keep the mesh part, append the interior air vertices (two buffers alternate, so an air-mesh
+    // change allocates only when the combined size outgrows them)
+    DevBuf& nu = c->dUVspare;
     OC_HIP(c, nu.ensure(sizeof(double) * 2 * (size_t)nVt));
     OC_HIP(c, hipMemcpyAsync(nu.p, c->dUV.p, sizeof(double) * 2 * (size_t)c->nV, hipMemcpyDeviceToDevice, c->stream));
     if (nVair > nBnd)
@@ -1889,7 +2132,100 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
         c->lastE = e;
     }
     if (E0) *E0 = c->lastE;
-    if (c->trE && c->nCharts == 0) oc::io::trace_energy_line(c->trE, c->lastE, c->lastESD, c->trSeam, c->a0);
+    if (c->trE && c->nCharts == 0)
+        if (int st = write_energy_trace(c, c->lastE, c->lastESD)) return st;
+    return OC_OK;
+}
+
+int oc_newton_rebuild(oc_ctx* c, double* E)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0, "oc_newton_rebuild: no mesh");
+    if (!c->sysBuilt || c->nCharts > 0) {
+        c->err = "oc_newton_rebuild: needs an oc_newton_init of this mesh (single problem)";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    // Optimizer::solve rebuilds the scaffold after a step (Optimizer.cpp:251-257); the next iteration only
+    // re-patterns the system (:522-530). Parameters, trace files, the previous solve's iteration count and the
+    // precision escalation of the mixed learned form carry over.
+    if (ensure_scratch(c, c->nF + c->nFair)) return OC_ERR_HIP;
+    if (int st = build_system(c)) return st;
+    if (int st = system_energy(c, &c->lastE, &c->lastESD, &c->lastEScaf)) return st;
+    if (E) *E = c->lastE;
+    return OC_OK;
+}
+
+int oc_air_mesh_get(oc_ctx* c, double* airUV, int* Fair)
+{
+    if (!c) return OC_ERR_ARG;
+    if (c->nFair == 0) {
+        c->err = "oc_air_mesh_get: no air mesh";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    if (airUV) {
+        oc::launch_uv_gather(c->stream, c->nVair, c->dAirL2G.as<int>(), c->dUV.as<double>(), nullptr,
+                             c->dAirUV2.as<double>());
+        OC_HIP(c, hipGetLastError());
+        OC_HIP(c, hipMemcpyAsync(airUV, c->dAirUV2.p, sizeof(double) * 2 * (size_t)c->nVair, hipMemcpyDeviceToHost,
+                                 c->stream));
+        OC_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    if (Fair) std::copy(c->FairLocal.begin(), c->FairLocal.end(), Fair);
+    return OC_OK;
+}
+
+int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, double* E, int* flips)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, mode == OC_SCAFFOLD_REFRESH || mode == OC_SCAFFOLD_REMESH_WHEN_NEEDED || mode == OC_SCAFFOLD_REMESH,
+           "oc_scaffold_update: unknown mode");
+    if (c->nFair == 0 || !c->sysBuilt || c->nCharts > 0) {
+        c->err = "oc_scaffold_update: needs an air mesh and an oc_newton_init";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    hipStream_t s = c->stream;
+    long nFlips = 0;
+    if (mode != OC_SCAFFOLD_REFRESH) {
+        std::vector<double> P(2 * (size_t)c->nVair);
+        oc::launch_uv_gather(s, c->nVair, c->dAirL2G.as<int>(), c->dUV.as<double>(), nullptr,
+                             c->dAirUV2.as<double>());
+        OC_HIP(c, hipGetLastError());
+        OC_HIP(c, hipMemcpyAsync(P.data(), c->dAirUV2.p, sizeof(double) * P.size(), hipMemcpyDeviceToHost, s));
+        OC_HIP(c, hipStreamSynchronize(s));
+        if (mode == OC_SCAFFOLD_REMESH || oc::air_min_angle(P, c->FairLocal) < minAngleDeg * M_PI / 180.0) {
+            nFlips = oc::lawson_flips(P, c->FairLocal, 64L * c->nFair + 1024);
+            if (nFlips > 0) {
+                for (size_t k = 0; k < c->FairLocal.size(); ++k) c->FairGlobal[k] = c->airL2G[c->FairLocal[k]];
+                if (int st = upload_tri_ids(c, c->nF, c->nFair, c->FairGlobal.data())) return st;
+                c->sysReady = false;
+                if (int st = build_system(c)) return st;
+            }
+        }
+    }
+    // the air mesh's rest shape becomes its current UV (z = 0), as a freshly built air mesh has
+    // (Scaffold.cpp:174-177); triangles below areaThres_AM take the equilateral clamp
+    const int nVt = c->nVt;
+    OC_HIP(c, c->dTmpVrest.ensure(sizeof(double) * 3 * (size_t)nVt));
+    oc::launch_uv_gather(s, nVt, nullptr, c->dUV.as<double>(), c->dTmpVrest.as<double>(), nullptr);
+    const int o = c->nF;
+    oc::launch_rest_features(s, c->nFair, c->dF0.as<int>() + o, c->dF1.as<int>() + o, c->dF2.as<int>() + o,
+                             c->dTmpVrest.as<double>(), c->airAreaThres, c->dTriArea.as<double>() + o,
+                             c->dA2.as<double>() + o, c->dE0.as<double>() + o, c->dE1.as<double>() + o,
+                             c->dE01.as<double>() + o, c->dPartials.as<double>(), c->dAirSums.as<double>());
+    OC_HIP(c, hipGetLastError());
+    double sums[3];
+    OC_HIP(c, hipMemcpyAsync(sums, c->dAirSums.p, sizeof(sums), hipMemcpyDeviceToHost, s));
+    OC_HIP(c, hipStreamSynchronize(s));
+    if (sums[2] > 0.0) {
+        c->err = "oc_scaffold_update: an air-mesh triangle has zero area";
+        return OC_ERR_DEGENERATE;
+    }
+    if (int st = system_energy(c, &c->lastE, &c->lastESD, &c->lastEScaf)) return st;
+    if (E) *E = c->lastE;
+    if (flips) *flips = (int)nFlips;
     return OC_OK;
 }
 
@@ -1925,11 +2261,9 @@ void system_element_pass(oc_ctx* c, const TriSet& T)
 
 void system_assemble(oc_ctx* c);
 
-// Newton direction H d = b (b in c->dB, d in c->dX) with the selected
-// preconditioner and its fallbacks; fills the PCG fields of S.
-int solve_direction(oc_ctx* c, oc_iter_stats* S)
+// The preconditioner's numeric set-up for the assembled Newton matrix (no host synchronisation).
+void launch_precond_setup(oc_ctx* c, oc_iter_stats* S)
 {
-    hipStream_t s = c->stream;
     // The multigrid hierarchy is rebuilt for every Newton matrix: reusing the
     // previous one stays a valid (SPD) preconditioner but, measured at 200k
     // faces, early Newton steps change the matrix enough to multiply the PCG
@@ -1938,7 +2272,16 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S)
         amg_numeric(c, 0);
         ++S->precond_setups;
     }
-    if (c->nPrecond == OC_PRECOND_LEARNED) c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
+    if (c->nPrecond == OC_PRECOND_LEARNED) c->learned->net.forward(c->stream, c->sys.dVals.p, c->sys.vals32);
+}
+
+// Newton direction H d = b (b in c->dB, d in c->dX) with the selected
+// preconditioner and its fallbacks; fills the PCG fields of S. setupLaunched:
+// launch_precond_setup was already queued for this matrix.
+int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
+{
+    hipStream_t s = c->stream;
+    if (!setupLaunched) launch_precond_setup(c, S);
     int it = 0;
     double rr = 0.0;
     int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext, c->pcgHint);
@@ -2175,9 +2518,16 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     oc::launch_gather_grad(s, nVt, c->dVcPtr.as<int>(), c->dVcSrc.as<int>(), c->dFixedMaskSys.as<unsigned char>(),
                            c->dElemG.as<double>(), c->dG.as<double>(), c->dPartials.as<double>(), scalar(c, 6));
     OC_HIP(c, hipGetLastError());
-    double gsq;
-    OC_HIP(c, hipMemcpyAsync(&gsq, scalar(c, 6), sizeof(double), hipMemcpyDeviceToHost, s));
-    OC_HIP(c, hipStreamSynchronize(s));
+    // ||g||^2 travels to the host while the Newton system is assembled and the preconditioner set up (work
+    // that is simply dropped on the one step whose gradient test succeeds)
+    OC_HIP(c, hipMemcpyAsync(c->mail, scalar(c, 6), sizeof(double), hipMemcpyDeviceToHost, s));
+    OC_HIP(c, hipEventRecord(c->evGsq, s));
+    system_assemble(c);
+    oc::launch_scale_copy(s, 2L * nVt, -1.0, c->dG.as<double>(), c->dB.as<double>());
+    launch_precond_setup(c, S);
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipEventSynchronize(c->evGsq));
+    const double gsq = c->mail[0];
     S->grad_sq = gsq;
     S->E_total = c->lastE;
     S->E_SD = c->lastESD;
@@ -2185,8 +2535,7 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     if (c->trG) {  // Optimizer.cpp:211-214: |g|^2 and the SD term's own |g_SD|^2
         double gsd = gsq / (c->a0 * c->a0);
         if (c->nFair) {
-            if (ensure_scratch(c, c->nV)) return OC_ERR_HIP;
-            OC_HIP(c, c->dTmpVec.ensure(sizeof(double) * 2 * (size_t)nVt));
+            // dTmpVec (2 nVt) and the partial records (nVt rows) were sized by the system build
             oc::launch_element(s, mesh_set(c, 0), c->dUV.as<double>(), nullptr, c->dElemG.as<double>(), nullptr, 0,
                                nullptr, nullptr);
             oc::launch_gather_grad(s, c->nV, c->dMvcPtr.as<int>(), c->dMvcSrc.as<int>(),
@@ -2203,29 +2552,46 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
         *done = 1;
         return OC_OK;
     }
-    // Hessian assembly + direction: H d = -g
-    system_assemble(c);
-    oc::launch_scale_copy(s, 2L * nVt, -1.0, c->dG.as<double>(), c->dB.as<double>());
-    if (int st = solve_direction(c, S)) return st;
+    // direction: H d = -g (system assembled and preconditioner set up above)
+    if (int st = solve_direction(c, S, true)) return st;
     const double* dir = c->dX.as<double>();
-    // line search (Optimizer.cpp:575-652)
-    double bound;
-    if (int e = step_bound(c, T, dir, 1.0, &bound)) return e;
-    double stepSize = bound * 0.99;
-    double lastE = c->lastE, lastEScafStart = c->lastEScaf;
-    if (c->nFair) {  // the scaffold changed: recompute the energy (Optimizer.cpp:588-592)
-        if (int e = system_energy(c, &lastE, &c->lastESD, &lastEScafStart)) return e;
-        c->lastE = lastE;
-        c->lastEScaf = lastEScafStart;
-    }
+    // line search (Optimizer.cpp:575-652) on the device: the injectivity bound, the energies at the first
+    // kLSCand halvings from 0.99 x the bound, the reference's rule applied to them and the step committed, then
+    // one read-back; more halvings than that continue on the host below
     constexpr int K = oc::kLSCand;
-    double cand[4 * K];
-    int k0 = -1;  // first index of the candidates held in cand
+    unsigned long long* om = c->dOrdMin.as<unsigned long long>();
+    oc::launch_set_u64(s, om, oc::dbl_to_ordered_host(1.0));
+    oc::launch_step_bound(s, T, c->dUV.as<double>(), dir, om, 1.0);
+    const double* esum = nullptr;
+    if (c->nFair) {  // the scaffold changed: the energy is recomputed (Optimizer.cpp:588-592)
+        oc::launch_element(s, T, c->dUV.as<double>(), nullptr, nullptr, nullptr, 0, c->dPartials.as<double>(),
+                           scalar(c, 4), f32);
+        esum = scalar(c, 4);
+    }
+    oc::launch_linesearch(s, T, c->dUV.as<double>(), dir, 0.0, 0, c->dPartials.as<double>(), c->dLsOut.as<double>(),
+                          f32, om);
+    oc::LsDecision* dDec = c->dLsDec.as<oc::LsDecision>();
+    oc::launch_ls_decide_commit(s, c->dLsOut.as<double>(), om, esum, c->lastE, c->a0, T.sAir, dDec, 2L * nVt, dir,
+                                c->dUV.as<double>());
+    OC_HIP(c, hipGetLastError());
+    double* cand = c->mail;
+    oc::LsDecision* D = reinterpret_cast<oc::LsDecision*>(c->mail + 4 * K);
+    OC_HIP(c, hipMemcpyAsync(cand, c->dLsOut.p, sizeof(double) * 4 * K, hipMemcpyDeviceToHost, s));
+    OC_HIP(c, hipMemcpyAsync(D, dDec, sizeof(oc::LsDecision), hipMemcpyDeviceToHost, s));
+    OC_HIP(c, hipStreamSynchronize(s));
+    double lastE = c->lastE, lastEScafStart = c->lastEScaf;
+    if (c->nFair) {
+        lastE = c->lastE = D->lastE;
+        c->lastESD = D->lastESD;
+        lastEScafStart = c->lastEScaf = D->lastEScaf;
+    }
+    const double stepSize = D->bound * 0.99;
+    int k0 = 0;  // first index of the candidates held in cand
     auto fetch = [&](int kstart) -> int {
         oc::launch_linesearch(s, T, c->dUV.as<double>(), dir, stepSize, kstart, c->dPartials.as<double>(),
                               c->dLsOut.as<double>(), f32);
         OC_HIP(c, hipGetLastError());
-        OC_HIP(c, hipMemcpyAsync(cand, c->dLsOut.p, sizeof(cand), hipMemcpyDeviceToHost, s));
+        OC_HIP(c, hipMemcpyAsync(cand, c->dLsOut.p, sizeof(double) * 4 * K, hipMemcpyDeviceToHost, s));
         OC_HIP(c, hipStreamSynchronize(s));
         k0 = kstart;
         return OC_OK;
@@ -2246,37 +2612,49 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     double alphaCommit = alpha;  // step of the UV left in place
     double testE, testESD, testES;
     bool inv;
-    if (int e = get(0, &testE, &testESD, &testES, &inv)) return e;
     int halvings = 0;
-    while (testE > lastE) {
-        const double na = alpha / 2.0;
-        ++halvings;
-        if (na == 0.0) {
-            stopped = true;
-            alpha = 0.0;
-            break;
+    if (D->status != 1) {
+        // decided and committed on the device
+        stopped = D->status == 2;
+        alpha = D->alpha;
+        alphaCommit = D->alphaCommit;
+        testE = D->testE;
+        testESD = D->testESD;
+        testES = D->testES;
+        halvings = D->halvings;
+    } else {
+        // more than kLSCand halvings: the rule continues on the host from the first candidate
+        if (int e = get(0, &testE, &testESD, &testES, &inv)) return e;
+        while (testE > lastE) {
+            const double na = alpha / 2.0;
+            ++halvings;
+            if (na == 0.0) {
+                stopped = true;
+                alpha = 0.0;
+                break;
+            }
+            alpha = na;
+            alphaCommit = alpha;
+            ++k;
+            if (int e = get(k, &testE, &testESD, &testES, &inv)) return e;
         }
-        alpha = na;
-        alphaCommit = alpha;
-        ++k;
-        if (int e = get(k, &testE, &testESD, &testES, &inv)) return e;
-    }
-    // post-check of inversions (Optimizer.cpp:615-629)
-    while (!stopped && inv) {
-        const double na = alpha / 2.0;
-        ++halvings;
-        if (na == 0.0) {
-            stopped = true;
-            alpha = 0.0;
-            break;
+        // post-check of inversions (Optimizer.cpp:615-629)
+        while (!stopped && inv) {
+            const double na = alpha / 2.0;
+            ++halvings;
+            if (na == 0.0) {
+                stopped = true;
+                alpha = 0.0;
+                break;
+            }
+            alpha = na;
+            alphaCommit = alpha;
+            ++k;
+            if (int e = get(k, &testE, &testESD, &testES, &inv)) return e;
         }
-        alpha = na;
-        alphaCommit = alpha;
-        ++k;
-        if (int e = get(k, &testE, &testESD, &testES, &inv)) return e;
+        oc::launch_axpy_commit(s, 2L * nVt, alphaCommit, dir, c->dUV.as<double>());
+        OC_HIP(c, hipGetLastError());
     }
-    oc::launch_axpy_commit(s, 2L * nVt, alphaCommit, dir, c->dUV.as<double>());
-    OC_HIP(c, hipGetLastError());
     double lastEDec = lastE - testE;
     if (c->nFair) lastEDec += (-lastEScafStart + testES);
     if (c->allowEDecRelTol && (lastEDec / lastE < 1.0e-6 * alpha) && (alpha > 1.0e-3)) stopped = true;
@@ -2292,7 +2670,7 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     S->stopped = stopped ? 1 : 0;
     *done = stopped ? 1 : 0;
     if (c->trE && !stopped)  // Optimizer.cpp:645-648
-        oc::io::trace_energy_line(c->trE, testE, testESD, c->trSeam, c->a0);
+        if (int st = write_energy_trace(c, testE, testESD)) return st;
     return OC_OK;
 }
 
@@ -2320,6 +2698,8 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
     // and the SpMV a live (r.z, r.r) record; the kernels write into scratch
     // records so the values they read stay fixed across repetitions.
     const int GU = oc::pcg_grid_update(c->nVt), GS = oc::pcg_grid_spmv(c->nVt);
+    // size the partial records before any pointer into them is taken
+    if (ensure_scratch(c, std::max<long>(4L * GU + 2L * GS, 2L * c->nVt))) return OC_ERR_HIP;
     double* tp = c->dPartials.as<double>();
     double* tU = tp;                         // 2*GU read by spmv
     double* tS = tp + 2 * (size_t)GU;        // GS   read by update
@@ -2398,8 +2778,7 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
         rec[0] = 1.0;
         rec[1] = 1.0;
         rec[2 * (size_t)GU] = 1.0;
-        if (ensure_scratch(c, std::max<long>(4L * GU + 2L * GS, 2L * c->nVt))) return OC_ERR_HIP;
-        OC_HIP(c, hipMemcpyAsync(tU, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice, s));
+        OC_HIP(c, hipMemcpyAsync(tU,rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice, s));
         OC_HIP(c, hipStreamSynchronize(s));
     }
     hipEvent_t e0, e1;

@@ -124,6 +124,12 @@ __device__ __forceinline__ unsigned long long dbl_to_ordered(double x)
     return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
 }
 
+__device__ __forceinline__ double ordered_to_dbl(unsigned long long u)
+{
+    u = (u & 0x8000000000000000ull) ? (u & 0x7fffffffffffffffull) : ~u;
+    return __longlong_as_double((long long)u);
+}
+
 inline double ordered_to_dbl_host(unsigned long long u)
 {
     u = (u & 0x8000000000000000ull) ? (u & 0x7fffffffffffffffull) : ~u;

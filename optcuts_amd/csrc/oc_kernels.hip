@@ -530,9 +530,11 @@ void launch_step_bound(hipStream_t s, const TriSet& T, const double* UV, const d
 template <typename EP>
 __global__ void __launch_bounds__(kBlock) k_linesearch(TriSet T, const double2* __restrict__ UV,
                                                        const double2* __restrict__ dir, double alphaK0,
+                                                       const unsigned long long* __restrict__ boundOrd,
                                                        double* partials)
 {
     constexpr int K = kLSCand;
+    if (boundOrd) alphaK0 = ordered_to_dbl(*boundOrd) * 0.99;  // Optimizer.cpp:578: 0.99 x injectivity bound
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     CSum acc[3 * K];
 #pragma unroll
@@ -595,17 +597,19 @@ __global__ void __launch_bounds__(kBlock) k_finish_linesearch(const double* __re
 }
 
 void launch_linesearch(hipStream_t s, const TriSet& T, const double* UV, const double* dir, double alpha0, int k0,
-                       double* partials, double* out, int f32)
+                       double* partials, double* out, int f32, const unsigned long long* boundOrd)
 {
     double a = alpha0;
     for (int k = 0; k < k0; ++k) a = a / 2.0;
     const int grid = grid_for(T.nF);
     if (f32)
         hipLaunchKernelGGL(k_linesearch<float>, dim3(grid), dim3(kBlock), 0, s, T,
-                           reinterpret_cast<const double2*>(UV), reinterpret_cast<const double2*>(dir), a, partials);
+                           reinterpret_cast<const double2*>(UV), reinterpret_cast<const double2*>(dir), a, boundOrd,
+                           partials);
     else
         hipLaunchKernelGGL(k_linesearch<double>, dim3(grid), dim3(kBlock), 0, s, T,
-                           reinterpret_cast<const double2*>(UV), reinterpret_cast<const double2*>(dir), a, partials);
+                           reinterpret_cast<const double2*>(UV), reinterpret_cast<const double2*>(dir), a, boundOrd,
+                           partials);
     hipLaunchKernelGGL(k_finish_linesearch, dim3(1), dim3(kBlock), 0, s, partials, grid, T.sMesh, T.sAir, out);
 }
 
@@ -620,6 +624,73 @@ __global__ void __launch_bounds__(kBlock) k_axpy_commit(long n, double alpha, co
 void launch_axpy_commit(hipStream_t s, long n, double alpha, const double* d, double* UV)
 {
     hipLaunchKernelGGL(k_axpy_commit, dim3(grid_for(n)), dim3(kBlock), 0, s, n, alpha, d, UV);
+}
+
+// The reference's line-search rule (Optimizer.cpp:575-629) applied to the K candidates of one k_linesearch
+// launch from the bound: halve while E(alpha) > lastE, then while a triangle is inverted; alpha reaching 0 stops.
+// lastE is the host's value, or (esum non-null: the scaffold changed since it was computed, :588-592)
+// sMesh esum[0] + sAir esum[1] of the energy pass at the current UV. One thread.
+__global__ void k_ls_decide(const double* __restrict__ cand, const unsigned long long* __restrict__ boundOrd,
+                            const double* __restrict__ esum, double lastEHost, double sMesh, double sAir,
+                            LsDecision* out)
+{
+    constexpr int K = kLSCand;
+    LsDecision r{};
+    if (esum) {
+        r.lastESD = esum[0];
+        r.lastEScaf = esum[1] * sAir;
+        r.lastE = sMesh * esum[0] + r.lastEScaf;
+    } else {
+        r.lastE = lastEHost;
+    }
+    r.bound = ordered_to_dbl(*boundOrd);
+    double alpha = r.bound * 0.99, alphaCommit = alpha;
+    int k = 0, halv = 0, status = 0;
+    double testE = cand[0];
+    bool inv = cand[3 * K] > 0.0;
+    for (int phase = 0; phase < 2 && status == 0; ++phase) {
+        while (status == 0 && (phase == 0 ? (testE > r.lastE) : inv)) {
+            const double na = alpha / 2.0;
+            ++halv;
+            if (na == 0.0) {
+                status = 2;
+                alpha = 0.0;
+                break;
+            }
+            alpha = na;
+            alphaCommit = alpha;
+            if (++k >= K) {
+                status = 1;
+                break;
+            }
+            testE = cand[k];
+            inv = cand[3 * K + k] > 0.0;
+        }
+    }
+    r.alpha = alpha;
+    r.alphaCommit = alphaCommit;
+    const int kk = k < K ? k : K - 1;
+    r.testE = testE;
+    r.testESD = cand[K + kk];
+    r.testES = cand[2 * K + kk] * sAir;
+    r.halvings = halv;
+    r.status = status;
+    *out = r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_axpy_commit_dev(long n, const LsDecision* __restrict__ dec,
+                                                            const double* __restrict__ d, double* UV)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && dec->status != 1) UV[i] = add_rn(UV[i], mul_rn(dec->alphaCommit, d[i]));
+}
+
+void launch_ls_decide_commit(hipStream_t s, const double* cand, const unsigned long long* boundOrd,
+                             const double* esum, double lastEHost, double sMesh, double sAir, LsDecision* dec, long n,
+                             const double* d, double* UV)
+{
+    hipLaunchKernelGGL(k_ls_decide, dim3(1), dim3(1), 0, s, cand, boundOrd, esum, lastEHost, sMesh, sAir, dec);
+    hipLaunchKernelGGL(k_axpy_commit_dev, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dec, d, UV);
 }
 
 // ----------------------------------------------------------------------------
@@ -1065,6 +1136,26 @@ __global__ void __launch_bounds__(kBlock) k_scale_copy(long n, double sign, cons
 void launch_scale_copy(hipStream_t s, long n, double sign, const double* g, double* b)
 {
     hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(n)), dim3(kBlock), 0, s, n, sign, g, b);
+}
+
+__global__ void __launch_bounds__(kBlock) k_uv_gather(int n, const int* __restrict__ ids,
+                                                      const double2* __restrict__ UV, double* out3, double2* out2)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double2 p = UV[ids ? ids[k] : k];
+    if (out3) {
+        out3[3 * (size_t)k] = p.x;
+        out3[3 * (size_t)k + 1] = p.y;
+        out3[3 * (size_t)k + 2] = 0.0;
+    }
+    if (out2) out2[k] = p;
+}
+
+void launch_uv_gather(hipStream_t s, int n, const int* ids, const double* UV, double* out3, double* out2)
+{
+    hipLaunchKernelGGL(k_uv_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, n, ids,
+                       reinterpret_cast<const double2*>(UV), out3, reinterpret_cast<double2*>(out2));
 }
 
 }  // namespace oc

@@ -81,9 +81,29 @@ void launch_step_bound(hipStream_t s, const TriSet& T, const double* UV, const d
 // k = 0..K-1: out[k] = sMesh*E_mesh + sAir*E_air, out[K+k] = E_mesh,
 // out[2K+k] = E_air, out[3K+k] = number of triangles with negative
 // doubled area (always FP64). f32: FP32 element energies, FP64 sums.
+// boundOrd non-null: alpha0 = 0.99 x the step bound it holds (order-preserving uint64, as launch_step_bound
+// leaves it), read on the device; k0 must then be 0.
 constexpr int kLSCand = 8;
 void launch_linesearch(hipStream_t s, const TriSet& T, const double* UV, const double* dir, double alpha0, int k0,
-                       double* partials, double* out, int f32 = 0);
+                       double* partials, double* out, int f32 = 0, const unsigned long long* boundOrd = nullptr);
+
+// Outcome of the line-search rule applied on the device to one batch of candidates (k_ls_decide).
+struct LsDecision {
+    double alphaCommit;  // step the UV moved by (status 0 and 2)
+    double alpha;        // accepted step (0 when stopped)
+    double testE, testESD, testES;  // energies at the accepted candidate
+    double lastE, lastESD, lastEScaf;  // the energy the candidates were compared with
+    double bound;        // the injectivity bound (before the 0.99 factor)
+    int halvings;
+    int status;  // 0 accepted, 1 more than kLSCand halvings needed (nothing committed), 2 alpha underflowed to 0
+    int pad[2];
+};
+// Applies Optimizer::lineSearch's rule (:597-629) to the candidates `cand` of a launch_linesearch from boundOrd
+// and, unless more candidates are needed, commits UV += alphaCommit d (n doubles). lastE: lastEHost, or with
+// esum non-null sMesh esum[0] + sAir esum[1].
+void launch_ls_decide_commit(hipStream_t s, const double* cand, const unsigned long long* boundOrd,
+                             const double* esum, double lastEHost, double sMesh, double sAir, LsDecision* dec, long n,
+                             const double* d, double* UV);
 
 // UV[i] = UV[i] + alpha * d[i] for i < n (two roundings, as the reference's
 // stepForward, Optimizer.cpp:666-669).
@@ -164,4 +184,7 @@ void launch_count_inverted(hipStream_t s, const TriSet& T, const double* UV, uns
 void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v);
 // b = sign * g for n doubles.
 void launch_scale_copy(hipStream_t s, long n, double sign, const double* g, double* b);
+// For k < n, p = UV row ids[k] (row k when ids is null): out3[k] = (p.x, p.y, 0) and out2[k] = p (either may be
+// null). Turns UV rows into the 3-vector rest positions of a planar mesh, or copies out a subset of rows.
+void launch_uv_gather(hipStream_t s, int n, const int* ids, const double* UV, double* out3, double* out2);
 }  // namespace oc

@@ -315,6 +315,42 @@ def seam_sparsity(cohE, Vrest, UV, avgEdgeLen, initSeamLen=0.0, triSoup=False):
                                    float(initSeamLen), int(triSoup))
 
 
+def standard_stretch(Vrest, F, UV):
+    """TriMesh::computeStandardStretch (TriMesh.cpp:1585-1652): (stretch_l2, stretch_inf, stretch_shear,
+    compress_inf) of the map, each triangle's deformation gradient from IglUtils::computeDeformationGradient
+    (IglUtils.cpp:172-194: the rest triangle laid flat with its first edge on the x axis, the UV triangle likewise,
+    F = X U^-1), L2 and shear averaged by rest area and square-rooted, stretches scaled by sqrt(UV area / rest area)."""
+    P = np.asarray(Vrest, np.float64)[F]
+    Q = np.asarray(UV, np.float64)[F]
+    e0, e1 = P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]
+    cr = np.cross(e0, e1)
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    x1 = np.linalg.norm(e0, axis=1)
+    x2 = np.stack([(e0 * e1).sum(1) / x1, np.linalg.norm(cr, axis=1) / x1], 1)
+    u01, u02 = Q[:, 1] - Q[:, 0], Q[:, 2] - Q[:, 0]
+    l01 = np.linalg.norm(u01, axis=1)
+    U = np.zeros((len(F), 2, 2))
+    U[:, 0, 0] = l01
+    U[:, 0, 1] = (u01 * u02).sum(1) / l01
+    U[:, 1, 1] = (u01[:, 0] * u02[:, 1] - u01[:, 1] * u02[:, 0]) / l01
+    X = np.zeros((len(F), 2, 2))
+    X[:, 0, 0] = x1
+    X[:, :, 1] = x2
+    G = X @ np.linalg.inv(U)
+    a = (G[:, :, 0] ** 2).sum(1)
+    b = (G[:, :, 0] * G[:, :, 1]).sum(1)
+    c = (G[:, :, 1] ** 2).sum(1)
+    t0 = a + c
+    t1 = np.sqrt((a - c) ** 2 + 4.0 * b * b)
+    tau = np.sqrt((t0 + t1) / 2.0)
+    with np.errstate(invalid="ignore"):
+        gamma = np.sqrt((t0 - t1) / 2.0)  # NaN when rounding makes t0 < t1: the reference's '<' skips it
+    sa = area.sum()
+    scale = np.sqrt(0.5 * (u01[:, 0] * u02[:, 1] - u01[:, 1] * u02[:, 0]).sum() / sa)
+    return (np.sqrt((t0 / 2.0 * area).sum() / sa) * scale, tau.max() * scale,
+            np.sqrt((b * b / a / c * area).sum() / sa), np.nanmin(gamma) * scale)
+
+
 def newton_run(Vrest, F, UV, a0=1.0, max_iter=100, rel_gl2_tol=1e-12, allow_edec_rel_tol=True):
     """Plain Newton + own LDLT, no scaffold. Returns (trace rows, final UV, exit reason)."""
     UV = _f64(UV).copy()

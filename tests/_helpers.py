@@ -105,12 +105,13 @@ class ScaffoldOracle:
             g[2 * G: 2 * G + 2] += self.sAir * ga[2 * l: 2 * l + 2]
         return g
 
-    def hessian_upper(self, X):
-        """Combined upper CSR (ia, ja, a) as Optimizer::computeHessian + update_a build it."""
+    def hessian_upper(self, X, project=True):
+        """Combined upper CSR (ia, ja, a) as Optimizer::computeHessian + update_a build it (project=False keeps
+        the element Hessians unprojected, for derivative checks)."""
         U, A = self.split(X)
-        I1, J1, V1 = O.hessian_triplets(self.F, O.element_hessians(self.F, U, self.feat), fixed=(0,),
-                                        scale=self.a0)
-        Ha = O.element_hessians(self.Fair, A, self.afeat, uniform=True)
+        I1, J1, V1 = O.hessian_triplets(self.F, O.element_hessians(self.F, U, self.feat, project=project),
+                                        fixed=(0,), scale=self.a0)
+        Ha = O.element_hessians(self.Fair, A, self.afeat, uniform=True, project=project)
         I2, J2, V2 = O.hessian_triplets(self.Fair, Ha, fixed=tuple(self.air_fixed), scale=self.sAir)
         I2 = (self.l2g[I2 // 2] * 2 + I2 % 2).astype(np.int32)
         J2 = (self.l2g[J2 // 2] * 2 + J2 % 2).astype(np.int32)

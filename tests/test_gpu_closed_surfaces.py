@@ -1,0 +1,153 @@
+"""Closed-surface workloads on the GPU: perturbed icosphere (genus 0) and fused tori (genus 3), cut to a disc,
+Tutte map, then the joint objective's geometry step (a0 = 1 - 0.999, w_scaf = 0.01 a0) with the bijectivity
+scaffold kept between steps by oc_scaffold_update (rest shape reset to the current UV, Lawson re-mesh when the
+smallest air angle drops below 20 degrees).
+
+Parity: on small instances every scaffolded Newton step matches the oracle's step (tests/_helpers.py
+ScaffoldOracle with its own LDL^T) taken from the same state and the same air mesh, to 1e-9 relative in energy;
+the Tutte map matches the oracle's to 1e-10. Full size (100k and 200k faces): properties (energy decreases, no
+inverted mesh or air triangle, no PCG fallback).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from optcuts_amd import capi
+from optcuts_amd import scaffold as SC
+from optcuts_amd import surfaces as S
+from optcuts_amd import workloads as W
+
+from _helpers import ScaffoldOracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def signed_areas(P, F):
+    a, b, c = P[F[:, 0]], P[F[:, 1]], P[F[:, 2]]
+    return 0.5 * ((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))
+
+
+@pytest.mark.parametrize("name", ["icosphere_nu10", "tori_h0.12"])
+def test_closed_surface_scaffolded_steps_match_oracle(ctx, name):
+    Vc, Fc, cohE, meta = S.workload(name)
+    print(name, meta, flush=True)
+    assert meta["genus"] == (0 if name.startswith("ico") else 3)
+    assert S.euler_characteristic(Fc) == 1 and len(S.boundary_loops(Fc)) == 1
+    ctx.set_mesh(Vc, Fc, fixed=(0,))
+    UV, nl, nb, it = ctx.tutte(capi.TUTTE_COMPONENTS, precond=capi.PRECOND_JACOBI, rel_tol=1e-13)
+    UVo = O.tutte_uv(Vc, Fc)[0]
+    assert np.abs(UV - UVo).max() <= 1e-10
+    assert O.check_inversion(Fc, UV)
+    a0 = 1.0 - 0.999
+    w = 0.01 * a0
+    ael = O.RestFeatures(Vc, Fc).avgEdgeLen
+    thr = SC.area_threshold(ael)
+    air = SC.initial_air_mesh(UV, Fc, ael)
+    ctx.set_air_mesh(*air, thr, w)
+    E0 = ctx.newton_init(a0=a0, rel_gl2_tol=1e-30, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                         pcg_rel_tol=1e-13, pcg_max_iter=5000)
+    airUV, Fair, bnd = air
+    so = ScaffoldOracle(Vc, Fc, airUV, Fair, bnd, thr, a0, w)
+    X = np.concatenate([UV, airUV[len(bnd):]])
+    assert E0 == pytest.approx(so.energy(X)[0], rel=1e-12)
+    total_flips = 0
+    for k in range(6):
+        Xo, Eo, step, halv, gsq = so.newton_step(X, None)
+        s, done = ctx.newton_step()
+        print(name, k, s["E_total"], Eo, s["step"], step, s["pcg_iters"], flush=True)
+        assert s["pcg_fallback"] == 0
+        assert s["E_total"] == pytest.approx(Eo, rel=1e-9), k
+        assert s["step"] == pytest.approx(step, rel=1e-7), k
+        assert s["halvings"] == halv
+        # the state after the step, then the air-mesh update
+        uv = ctx.get_uv()
+        auv, _ = ctx.air_mesh_get()
+        assert np.abs(uv - Xo[: Vc.shape[0]]).max() <= 1e-7
+        E_upd, flips = ctx.scaffold_update(capi.SCAFFOLD_REMESH_WHEN_NEEDED, 20.0)
+        total_flips += flips
+        auv2, Fair2 = ctx.air_mesh_get()
+        np.testing.assert_array_equal(auv2, auv)
+        assert (signed_areas(auv2, Fair2) > 0).all()
+        # the oracle composition with the updated air mesh: rest shape = the current air UV
+        so = ScaffoldOracle(Vc, Fc, auv2, Fair2, bnd, thr, a0, w)
+        X = np.concatenate([uv, auv2[len(bnd):]])
+        assert E_upd == pytest.approx(so.energy(X)[0], rel=1e-12)
+        # a refreshed air mesh is at rest: its energy is 4 per triangle unless clamped by areaThres
+        Ea = so.energy(X)[2]
+        assert Ea <= 4.0 * w * (1 + 1e-9) or (signed_areas(auv2, Fair2) < thr).any()
+    print(name, "air flips", total_flips, flush=True)
+
+
+def test_scaffold_remesh_restores_quality(ctx):
+    """Moving the interior air vertices into a poor triangulation and asking for a re-mesh: the flips raise the
+    smallest air angle, leave every air triangle positively oriented and keep the energy finite; a refresh-only
+    update changes no triangle."""
+    Vc, Fc, cohE, meta = S.workload("icosphere_nu10")
+    ctx.set_mesh(Vc, Fc, fixed=(0,))
+    UV = ctx.tutte(capi.TUTTE_COMPONENTS, precond=capi.PRECOND_JACOBI, rel_tol=1e-13)[0]
+    ael = O.RestFeatures(Vc, Fc).avgEdgeLen
+    airUV, Fair, bnd = SC.initial_air_mesh(UV, Fc, ael)
+    # shear the interior air vertices (orientation kept)
+    nb = len(bnd)
+    rng = np.random.default_rng(1)
+    mv = rng.normal(size=(airUV.shape[0] - nb, 2)) * ael
+    for f in (1.0, 0.5, 0.25, 0.125, 0.0625):
+        A2 = airUV.copy()
+        A2[nb:] += f * mv
+        if (signed_areas(A2, Fair) > 0).all():
+            break
+    assert (signed_areas(A2, Fair) > 0).all()
+    ctx.set_air_mesh(A2, Fair, bnd, SC.area_threshold(ael), 1e-5)
+    ctx.newton_init(a0=1e-3, rel_gl2_tol=1e-12, precond=capi.PRECOND_AMG, pcg_rel_tol=1e-6, pcg_max_iter=2000)
+    E, flips = ctx.scaffold_update(capi.SCAFFOLD_REFRESH)
+    assert flips == 0
+    np.testing.assert_array_equal(ctx.air_mesh_get()[1], Fair)
+
+    def min_angle(P, F):
+        out = np.pi
+        for k in range(3):
+            a = P[F[:, (k + 1) % 3]] - P[F[:, k]]
+            b = P[F[:, (k + 2) % 3]] - P[F[:, k]]
+            out = min(out, np.arctan2(np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]), (a * b).sum(1)).min())
+        return np.degrees(out)
+    before = min_angle(A2, Fair)
+    E2, flips = ctx.scaffold_update(capi.SCAFFOLD_REMESH)
+    auv, F2 = ctx.air_mesh_get()
+    after = min_angle(auv, F2)
+    print("air min angle", before, "->", after, "flips", flips, flush=True)
+    assert flips > 0 and after > before
+    assert (signed_areas(auv, F2) > 0).all()
+    assert np.isfinite(E2)
+    s, _ = ctx.newton_step()
+    assert s["pcg_fallback"] == 0 and s["E_total"] < E2
+
+
+@pytest.mark.parametrize("name", ["icosphere_100k", "tori_200k"])
+def test_closed_surfaces_full_size_properties(ctx, name):
+    w = W.prepare_closed(ctx, name)
+    print(name, w["meta"], "tutte pcg", w["tutte_pcg_iters"], flush=True)
+    assert O.check_inversion(w["F"], w["UV0"])
+    E_prev = ctx.newton_init(a0=w["a0"], rel_gl2_tol=1e-12, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                             pcg_rel_tol=1e-3, pcg_max_iter=1000)
+    for k in range(6):
+        s, done = ctx.newton_step()
+        E_upd, flips = ctx.scaffold_update(capi.SCAFFOLD_REMESH_WHEN_NEEDED, 20.0)
+        print(name, k, "E", s["E_total"], "E_SD", s["E_SD"], "pcg", s["pcg_iters"], "fallback", s["pcg_fallback"],
+              "flips", flips, flush=True)
+        assert s["pcg_fallback"] == 0
+        assert s["E_total"] < E_prev
+        E_prev = E_upd
+        if done:
+            break
+    UVg = ctx.get_uv()
+    assert O.check_inversion(w["F"], UVg)
+    auv, Fa = ctx.air_mesh_get()
+    assert (signed_areas(auv, Fa) > 0).all()
+    assert O.energy(w["F"], UVg, O.RestFeatures(w["V"], w["F"])) == pytest.approx(ctx.energy(), rel=1e-12)

@@ -116,3 +116,78 @@ def test_obj_to_tutte_to_newton_to_obj(ctx, tmp_path):
     span = (UVf.max(0) - UVf.min(0)).max()
     np.testing.assert_allclose(UV2, (UVf - UVf.min(0)) / span, rtol=0, atol=1e-6)
     assert len(_read(tmp_path / "gradientPerIter.txt")) >= 2
+
+
+def _fields(line):
+    return [float(x) for x in line.split()]
+
+
+@pytest.mark.parametrize("fracture", [False, True])
+def test_energy_trace_seam_column_matches_oracle(ctx, tmp_path, fracture):
+    """A cut closed surface (icosphere, genus 0) with its cohesive edges: every energy line recomputes
+    E_se = computeSeamSparsity(!fractureMode) / virtualRadius at the current UV, and in fracture mode the first
+    column is lastEnergyVal + (1 - a0) E_se (Optimizer.cpp:706-728); checked against the oracle's seam length at the
+    UV of each line, both modes."""
+    from optcuts_amd import surfaces as S
+    Vc, Fc, cohE, meta = S.workload("icosphere_nu10")
+    ctx.set_mesh(Vc, Fc, fixed=(0,))
+    ctx.tutte(capi.TUTTE_COMPONENTS, precond=capi.PRECOND_JACOBI, rel_tol=1e-13)
+    feat = O.RestFeatures(Vc, Fc)
+    vr = np.sqrt(feat.surfaceArea / np.pi)
+    a0, init = 0.25, 0.125
+    ctx.trace_open(str(tmp_path))
+    ctx.trace_set_seams(cohE, init_seam_len=init, fracture_mode=fracture)
+    E0 = ctx.newton_init(a0=a0, rel_gl2_tol=1e-12, allow_edec_rel_tol=False, pcg_rel_tol=1e-12,
+                         pcg_max_iter=20000)
+    states = [(E0, ctx.energy(), ctx.get_uv())]
+    for k in range(4):
+        s, done = ctx.newton_step()
+        assert not s["stopped"]
+        states.append((s["E_total"], s["E_SD"], ctx.get_uv()))
+    ctx.trace_close()
+    lines = _read(str(tmp_path / "energyValPerIter.txt"))
+    assert len(lines) == len(states)
+    for line, (E, esd, uv) in zip(lines, states):
+        se = O.seam_sparsity(cohE, Vc, uv, feat.avgEdgeLen, init, triSoup=not fracture) / vr
+        col0 = E + (1.0 - a0) * se if fracture else E
+        got = _fields(line)
+        assert len(got) == 4 and line == " ".join("%g" % x for x in got)
+        np.testing.assert_allclose(got, [col0, esd, se, a0], rtol=2e-5)
+    # the seams open up as the map relaxes, so the tri-soup (non-fracture) count is not constant
+    if not fracture:
+        assert len({ln.split()[2] for ln in lines}) >= 1
+
+
+def test_info_file_matches_oracle(ctx, tmp_path):
+    """info.txt (main.cpp:346-387) after a few steps: the caller's counters and timings, E_SD, seam length over
+    virtualRadius, the standard stretch metrics and the initial seams printed as Eigen prints an integer matrix."""
+    from optcuts_amd import surfaces as S
+    Vc, Fc, cohE, meta = S.workload("icosphere_nu10")
+    ctx.set_mesh(Vc, Fc, fixed=(0,))
+    ctx.tutte(capi.TUTTE_COMPONENTS, precond=capi.PRECOND_JACOBI, rel_tol=1e-13)
+    ctx.trace_set_seams(cohE, fracture_mode=True)
+    ctx.newton_init(a0=0.001, rel_gl2_tol=1e-12, allow_edec_rel_tol=False, pcg_rel_tol=1e-10, pcg_max_iter=20000)
+    for _ in range(3):
+        ctx.newton_step()
+    timings = np.arange(15) * 0.5 + 0.25
+    p = str(tmp_path / "info.txt")
+    ctx.info_write(p, meta["vertices"], meta["faces"], 3, 0, 0.999, timings, cohE)
+    lines = open(p).read().split("\n")
+    feat = O.RestFeatures(Vc, Fc)
+    uv = ctx.get_uv()
+    assert lines[0] == "%d %d" % (meta["vertices"], meta["faces"])
+    assert lines[1] == "3 0 0 0 %g %g" % (0.999, 1.0 - 0.001)
+    lbl = ["topo", "desc", "scaf", "enUp", "mtrComp", "mtrAssem", "symFac", "numFac", "backSolve", "lineSearch",
+           "bSplit", "iSplit", "cMerge"]
+    assert lines[2] == "0.0 0.0 %g %g " % (timings[0], timings[1]) + " ".join(
+        "%s%g" % (l, t) for l, t in zip(lbl, timings[2:]))
+    d, sl = _fields(lines[3])
+    assert d == pytest.approx(O.energy(Fc, uv, feat), rel=1e-5)
+    assert sl == pytest.approx(O.seam_sparsity(cohE, Vc, uv, feat.avgEdgeLen, 0.0, triSoup=False) /
+                               np.sqrt(feat.surfaceArea / np.pi), rel=1e-5)
+    np.testing.assert_allclose(_fields(lines[4]), O.standard_stretch(Vc, Fc, uv), rtol=1e-5)
+    assert lines[5] == "initialSeams %d" % cohE.shape[0]
+    w = max(len(str(int(x))) for x in cohE.ravel())
+    for r in range(cohE.shape[0]):
+        assert lines[6 + r] == " ".join(str(int(x)).rjust(w) for x in cohE[r])
+    assert lines[6 + cohE.shape[0]] == "" and len(lines) == 7 + cohE.shape[0]
