@@ -85,6 +85,13 @@ int oc_mesh_stats(oc_ctx* ctx, double* surfaceArea, double* avgEdgeLen);
 /* ---- Energy interface (Energy.hpp:16-46, SymDirichletEnergy.hpp:15-42) ---- */
 /* Energy::computeEnergyVal (Energy.cpp:35-40): sum of per-element SD. */
 int oc_energy(oc_ctx* ctx, int uniformWeight, double* E);
+/* oc_energy without any device allocation, for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). No reference counterpart. The same kernels and result as
+ * oc_energy, with the reduction's workgroup records in a fixed device array of the library (2 MiB per device, held
+ * until the process ends) instead of the context's scratch: meshes of at most 16,777,216 triangles (OC_ERR_ARG
+ * beyond). Every call on a device shares that array, so calls that overlap on one device (contexts on other
+ * streams or threads) race. */
+int oc_energy_noalloc(oc_ctx* ctx, int uniformWeight, double* E);
 /* SymDirichletEnergy::getEnergyValPerElem (SymDirichletEnergy.cpp:24-46), nF values. */
 int oc_energy_per_elem(oc_ctx* ctx, int uniformWeight, double* Eelem);
 /* SymDirichletEnergy::computeGradient (SymDirichletEnergy.cpp:258-304), 2*nV values, fixed rows 0. */

@@ -67,6 +67,7 @@ _SIGS = {
     "oc_mesh_stats": (C.c_int, [_ctx, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "oc_energy": (C.c_int, [_ctx, C.c_int, C.POINTER(C.c_double)]),
     "oc_energy_per_elem": (C.c_int, [_ctx, C.c_int, _dp]),
+    "oc_energy_noalloc": (C.c_int, [_ctx, C.c_int, C.POINTER(C.c_double)]),
     "oc_gradient": (C.c_int, [_ctx, C.c_int, _dp]),
     "oc_hessian_triplets": (C.c_int, [_ctx, C.c_int, C.c_longlong, _vp, _vp, _vp, C.POINTER(C.c_longlong)]),
     "oc_element_terms": (C.c_int, [_ctx, C.c_int, _vp, _vp, _vp]),
@@ -256,6 +257,12 @@ class Context:
     def energy(self, uniform=False):
         e = C.c_double()
         self._chk(self.L.oc_energy(self.h, int(uniform), C.byref(e)))
+        return e.value
+
+    def energy_noalloc(self, uniform=False):
+        """oc_energy without device allocation (the library's fixed scratch array)."""
+        e = C.c_double()
+        self._chk(self.L.oc_energy_noalloc(self.h, int(uniform), C.byref(e)))
         return e.value
 
     def energy_per_elem(self, uniform=False):

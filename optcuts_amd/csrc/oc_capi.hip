@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iterator>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -773,6 +774,41 @@ void mesh_topology(oc_ctx* c)
 
 }  // namespace
 
+// ---- scratch of the no-allocation entry point (oc_energy_noalloc) --------------------------------------------
+// The energy reduction's workgroup records (2 compensated sums = 4 doubles each, up to kStaticBlocks workgroups of
+// kBlock triangles) and its two finished sums, in a file-scope device array: 2 MiB + 64 B held on every device the
+// library runs on. Host code reaches each device's copy only through the address hipGetSymbolAddress returns for
+// it (looked up once per device, at the first context on that device).
+namespace {
+constexpr int kStaticBlocks = 1 << 16;
+constexpr long kStaticMaxTri = (long)kStaticBlocks * oc::kBlock;  // 16,777,216 triangles
+constexpr size_t kStaticRecordDoubles = 4 * (size_t)kStaticBlocks;
+}  // namespace
+static __device__ __align__(256) double scratch[kStaticRecordDoubles + 8];
+
+namespace {
+std::mutex g_scratchMu;
+std::vector<double*> g_scratchPtr;  // per device
+
+// With `device` current: its copy of `scratch`, looked up on the first call for the device.
+int static_scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> g(g_scratchMu);
+    if ((int)g_scratchPtr.size() <= device) g_scratchPtr.resize(device + 1, nullptr);
+    if (g_scratchPtr[device]) return OC_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return OC_ERR_HIP;
+    g_scratchPtr[device] = static_cast<double*>(p);
+    return OC_OK;
+}
+
+double* static_scratch(int device)
+{
+    std::lock_guard<std::mutex> g(g_scratchMu);
+    return device < (int)g_scratchPtr.size() ? g_scratchPtr[device] : nullptr;
+}
+}  // namespace
+
 // =====================================================================================
 extern "C" {
 
@@ -793,7 +829,7 @@ int oc_ctx_create(int device, oc_ctx** out)
         return OC_ERR_HIP;
     }
     if (hipHostMalloc((void**)&c->mail, kMailBytes, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evGsq, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->evGsq, hipEventDisableTiming) != hipSuccess || static_scratch_setup(device)) {
         (void)oc_ctx_destroy(c);
         return OC_ERR_HIP;
     }
@@ -1001,6 +1037,30 @@ int oc_energy(oc_ctx* c, int uniformWeight, double* E)
     double em, ea;
     if (int st = energy_sums(c, mesh_set(c, uniformWeight), &em, &ea)) return st;
     *E = em;
+    return OC_OK;
+}
+
+int oc_energy_noalloc(oc_ctx* c, int uniformWeight, double* E)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && E, "oc_energy_noalloc: no mesh");
+    OC_ARG(c, (long)c->nF <= kStaticMaxTri,
+           "oc_energy_noalloc: more than 16,777,216 triangles (the static scratch holds 65,536 workgroup records)");
+    double* p = static_scratch(c->device);
+    if (!p) {
+        c->err = "oc_energy_noalloc: the static scratch of this device was not found";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    // the same kernels as oc_energy: per-workgroup records, then the one-workgroup finisher; every record is
+    // written before it is read, so nothing needs clearing between calls
+    double* sums = p + kStaticRecordDoubles;
+    oc::launch_element(c->stream, mesh_set(c, uniformWeight), c->dUV.as<double>(), nullptr, nullptr, nullptr, 0, p,
+                       sums);
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipMemcpyAsync(c->mail, sums, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    *E = c->mail[0];
     return OC_OK;
 }
 
