@@ -1,16 +1,18 @@
-"""Geometry-step benchmark: Newton iterations/s and time-to-E_SD(1e-6 rel) on
-the 200k-face synthetic workload (317x317 height-field disc, seed 0,
-199,712 faces), one MI355X.
+"""Geometry-step benchmark on one MI355X: Newton iterations/s and time to the reference's final E_SD (1e-6 rel)
+at 200k faces, chosen versus plain GPU form.
 
-A step is one Newton iteration of the symmetric Dirichlet geometry step
-(Optimizer::solve(1)): element pass (energy, gradient, PSD-projected
-Hessian), block-CSR assembly, PCG direction, injectivity step bound and line
-search, all device-resident behind the C ABI.
+Headline workload (config C2): three fused tori (genus 3), 200,216 faces, cut to a disc, Tutte map, OptCuts joint
+mode's geometry step (energyParams[0] = 1 - 0.999, main.cpp:1569-1570) with the bijectivity scaffold on from the
+start (w_scaf = 0.01 energyParams[0], Optimizer.cpp:87). A step is one Newton iteration (Optimizer::solve(1)):
+element pass (energy, gradient, PSD-projected Hessian), block-CSR assembly, PCG direction, injectivity bound and
+line search, device-resident behind the C ABI, followed by the air-mesh update that the reference does with a new
+Triangle mesh (here oc_scaffold_update: rest shape reset, Lawson re-mesh when an air triangle set the step bound
+or every 2nd step). The timed region starts with the mesh, its Tutte map and the first air mesh in HBM.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 1]
 
-Prints one JSON line. The timed region starts with the mesh, UV and rest
-features already in HBM.
+Prints one JSON line. Secondary sections: C1 (perturbed icosphere, 100k faces), C4 (icosphere, 1M faces), the
+200k height-field disc with all five solver forms, C3 (64 batched charts), the split-query pre-screen.
 """
 import argparse
 import json
@@ -23,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Newton iters/sec + time-to-E_SD(1e-6 rel) at 200k faces; chosen vs plain GPU"
 HBM_PEAK_GBS = 8000.0
+TARGETS = os.path.join(ROOT, "tests", "golden", "workload_targets.json")
 
 
 def spmv_bytes(nnzb, nV):
@@ -41,17 +44,153 @@ def lanczos0_bytes(nnzb, nV):
     return 36 * nnzb + 4 * (nV + 1) + 100 * nV
 
 
-# HBM traffic per launch of the roofline kernels at the 200k-face workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch, mean over
-# full dispatches: those queued past PCG convergence return at once and are
-# left out), FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0: the factors that
-# make the same counters read the known size of a plain 512 MiB device copy.
-# Summaries with commands: profiles/pmc_traffic_200k.md.
-TRAFFIC_200K = {
-    "k_lanczos_fine": (2.0 * 17072.8 + 3188.8) * 1024,
-    "k_pcg_spmv": (2.0 * 14334.2 + 3209.3) * 1024,
+# HBM traffic per launch of the roofline kernels, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per
+# dispatch, mean over full dispatches), FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0: the factors that make the
+# same counters read the known size of a plain 512 MiB device copy. Summaries with commands: profiles/.
+TRAFFIC = {
+    ("disc_200k", "k_lanczos_fine"): (2.0 * 17072.8 + 3188.8) * 1024,
+    ("disc_200k", "k_pcg_spmv"): (2.0 * 14334.2 + 3209.3) * 1024,
 }
 
+FALLBACK_NAMES = {1: "retry_safe_amg", 2: "jacobi", 3: "truncated", 4: "retry_fp64_elements", 5: "jacobi_direct"}
+
+
+def count_fallbacks(stats):
+    out = {v: 0 for v in FALLBACK_NAMES.values()}
+    for s in stats:
+        if s["pcg_fallback"]:
+            out[FALLBACK_NAMES[s["pcg_fallback"]]] += 1
+    return out
+
+
+def load_targets():
+    return json.load(open(TARGETS)) if os.path.exists(TARGETS) else {}
+
+
+# ------------------------------------------------------------------------------------------------------------
+# closed surfaces with the scaffold (C1, C2, C4)
+
+def closed_forms(args):
+    """name -> (preconditioner, PCG relative tolerance, PCG iteration cap, element precision) on the closed
+    surfaces. The fast forms cap PCG per step (inexact Newton; on the Tutte start of a genus-3 map no iterative
+    solve captures the exact step and long solves only cost time); the plain form runs Jacobi-PCG to 1e-10."""
+    from optcuts_amd import capi
+    fp64, mixed = capi.PRECISION_FP64, capi.PRECISION_MIXED
+    return {"chosen": (capi.PRECOND_AMG, args.chosen_tol, 100, fp64),
+            "mixed": (capi.PRECOND_AMG, args.chosen_tol, 100, mixed),
+            "learned": (capi.PRECOND_LEARNED, args.learned_tol, 2000, fp64),
+            "learned_mixed": (capi.PRECOND_LEARNED, args.learned_tol, 2000, mixed),
+            "block_jacobi_at_learned_tol": (capi.PRECOND_BLOCK_JACOBI, args.learned_tol, 2000, fp64),
+            "plain": (capi.PRECOND_JACOBI, args.pcg_tol, 200000, fp64)}
+
+
+def closed_step(ctx, args):
+    """One Newton step then the air-mesh update; returns (stats, done, step s, update s)."""
+    from optcuts_amd import capi
+    t0 = time.perf_counter()
+    s, done = ctx.newton_step()
+    t1 = time.perf_counter()
+    ctx.scaffold_update(capi.SCAFFOLD_REMESH_WHEN_NEEDED, 0.0, args.air_interval)
+    return s, done, t1 - t0, time.perf_counter() - t1
+
+
+def closed_run(ctx, w, form, args, steps, warmup, e_star=None, budget=None):
+    """From the Tutte map: warmup untimed steps, then `steps` timed ones (or, with e_star, until E_SD <= (1+1e-6)
+    E* or the budget in seconds). Returns a dict of timings and per-step records."""
+    from optcuts_amd import capi
+    from optcuts_amd import workloads as W
+    pc, tol, mx, prec = closed_forms(args)[form]
+    W.restart(ctx, w)
+    t0 = time.perf_counter()
+    ctx.newton_init(a0=w["a0"], rel_gl2_tol=1e-12, allow_edec_rel_tol=False, precond=pc, pcg_rel_tol=tol,
+                    pcg_max_iter=mx, precision=prec)
+    t_init = time.perf_counter() - t0
+    for _ in range(warmup):
+        closed_step(ctx, args)
+    rows, stats = [], []
+    t0 = time.perf_counter()
+    hit = None
+    k = 0
+    while True:
+        s, done, ts, tu = closed_step(ctx, args)
+        k += 1
+        stats.append(s)
+        rows.append((ts, tu, s["E_SD"], s["pcg_iters"]))
+        el = time.perf_counter() - t0
+        if e_star is not None and hit is None and s["E_SD"] <= (1.0 + 1e-6) * e_star:
+            hit = (k, el + t_init)
+        if e_star is None and k >= steps:
+            break
+        if e_star is not None and (hit is not None or k >= steps or (budget and el > budget)):
+            break
+        if done:
+            break
+    el = time.perf_counter() - t0
+    import numpy as np
+    a = np.array(rows)
+    out = {"newton_iterations": k, "seconds": el, "newton_iters_per_s": k / el, "ms_per_step": 1e3 * el / k,
+           "step_ms_median": float(np.median(a[:, 0]) * 1e3), "air_update_ms_median": float(np.median(a[:, 1]) * 1e3),
+           "air_update_share": float(a[:, 1].sum() / el), "init_ms": t_init * 1e3,
+           "pcg_iters_median": float(np.median(a[:, 3])), "pcg_fallbacks": count_fallbacks(stats),
+           "E_SD_first": float(a[0, 2]), "E_SD_last": float(a[-1, 2])}
+    if e_star is not None:
+        out["reached"] = hit is not None
+        if hit is not None:
+            out["time_to_target_s"] = hit[1]
+            out["iterations_to_target"] = hit[0]
+        else:
+            out["time_to_target_s"] = None
+            out["stopped_after_s"] = el + t_init
+        e64 = ctx.energy()
+        out["E_SD_reached_fp64"] = e64
+        out["rel_err_vs_E_star"] = (e64 - e_star) / e_star
+    return out
+
+
+def closed_bench(ctx, name, args, targets, forms=("chosen", "plain"), timed_steps=None, plain_steps=3,
+                 plain_budget=30.0, tte_max=400, form_budget=30.0):
+    """The named closed workload: every form's timed steps from the Tutte map, then (with a target) its time to
+    E_SD <= (1+1e-6) E*. The plain form runs plain_steps timed steps and its time-to-target stops after
+    plain_budget seconds; the other non-chosen forms stop after form_budget seconds."""
+    from optcuts_amd import workloads as W
+    w = W.prepare_closed(ctx, name)
+    e_star = targets.get("closed", {}).get(name, {}).get("E_SD_star")
+    out = {"workload": name, "faces": w["meta"]["faces"], "genus": w["meta"]["genus"],
+           "cut_vertices": w["meta"]["cut_vertices"], "seam_edges": w["meta"]["seam_edges"],
+           "air_triangles": int(w["air"][1].shape[0]), "a0": w["a0"], "w_scaf": w["w_scaf"],
+           "generate_and_cut_s": w["generate_s"], "upload_and_tutte_s": w["upload_tutte_s"],
+           "tutte_pcg_iters": w["tutte_pcg_iters"], "first_air_mesh_s": w["air_s"], "E_SD_star": e_star,
+           "E_SD_star_source": "smallest E_SD of the oracle's exact-Newton run (own LDL^T) from the same Tutte map "
+                               "and first air mesh, air rest shape reset after every step, no re-mesh; "
+                               "tests/golden/workload_targets.json" if e_star else None, "forms": {}}
+    for form in forms:
+        print("bench: %s form %s" % (name, form), file=sys.stderr, flush=True)
+        pc, tol, mx, prec = closed_forms(args)[form]
+        n = min(plain_steps, timed_steps) if form == "plain" else timed_steps
+        r = closed_run(ctx, w, form, args, n, args.warmup)
+        r.update({"pcg_rel_tol": tol, "pcg_max_iter": mx, "precision": {0: "fp64", 1: "mixed"}[prec],
+                  "precond": {0: "jacobi", 1: "block-jacobi", 2: "sa-amg", 3: "learned-factor"}[pc]})
+        if e_star is not None and not args.no_tte:
+            budget = None if form == "chosen" else (plain_budget if form == "plain" else form_budget)
+            r["time_to_E_SD_1e-6"] = closed_run(ctx, w, form, args, tte_max, 0, e_star=e_star, budget=budget)
+        out["forms"][form] = r
+        out["system_sizes"] = ctx.system_sizes()
+    tp = out["forms"].get("plain", {}).get("time_to_E_SD_1e-6")
+    speed = {}
+    for form, r in out["forms"].items():
+        tf = r.get("time_to_E_SD_1e-6")
+        if not tp or not tf or not tf["reached"]:
+            speed[form] = None
+        elif tp["reached"]:
+            speed[form] = {"speedup": tp["time_to_target_s"] / tf["time_to_target_s"]}
+        else:
+            speed[form] = {"speedup_at_least": tp["stopped_after_s"] / tf["time_to_target_s"]}
+    out["time_to_E_SD_vs_plain"] = speed
+    return out, w
+
+
+# ------------------------------------------------------------------------------------------------------------
+# the 200k-face height-field disc (DistMin, no scaffold): all five solver forms
 
 def max_iter_for(precond):
     from optcuts_amd import capi
@@ -64,62 +203,100 @@ def run_form(ctx, V, F, UV, pcg_tol, steps, warmup, precond, precision):
     nV = V.shape[0]
     ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16 * nV / (nV - 1), allow_edec_rel_tol=False, precond=precond,
                     pcg_rel_tol=pcg_tol, pcg_max_iter=max_iter_for(precond), precision=precision)
-    stats = []
     for _ in range(warmup):
-        s, done = ctx.newton_step()
-        stats.append(s)
+        ctx.newton_step()
     t0 = time.perf_counter()
-    timed = []
-    for _ in range(steps):
-        s, done = ctx.newton_step()
-        timed.append(s)
-    t1 = time.perf_counter()
-    return t1 - t0, stats, timed
+    timed = [ctx.newton_step()[0] for _ in range(steps)]
+    return time.perf_counter() - t0, timed
 
 
 def time_to_target(ctx, V, F, UV, pcg_tol, precond, precision, e_star, max_iter):
-    """Wall time of Newton iterations until |E_SD - E*| <= 1e-6 E* (the driver's
-    own energy: FP64 or, in the mixed form, FP32 elements with compensated
-    FP64 sums). The reached map is then re-evaluated in FP64 and that value
-    is checked too, so the mixed form's energy error is stated per result."""
+    """Wall time of Newton iterations until E_SD <= (1+1e-6) E* (the driver's own energy: FP64 or, in the mixed
+    form, FP32 elements with compensated FP64 sums); the reached map is then re-evaluated in FP64."""
     nV = V.shape[0]
     ctx.set_mesh(V, F, fixed=(0,))
     ctx.set_uv(UV)
     t0 = time.perf_counter()
     ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16 * nV / (nV - 1), allow_edec_rel_tol=False, precond=precond,
                     pcg_rel_tol=pcg_tol, pcg_max_iter=max_iter_for(precond), precision=precision)
-    fb = {"retry_fp64_elements": 0, "retry_safe_amg": 0, "jacobi": 0, "truncated": 0}
-    t = None
-    fixed = e_star is None  # no target: run max_iter iterations, the reached E_SD becomes the target
+    stats, t = [], None
     for k in range(1, max_iter + 1):
         s, done = ctx.newton_step()
-        if s["pcg_fallback"] == 4:
-            fb["retry_fp64_elements"] += 1
-        elif s["pcg_fallback"] == 1:
-            fb["retry_safe_amg"] += 1
-        elif s["pcg_fallback"] == 2:
-            fb["jacobi"] += 1
-        elif s["pcg_fallback"] == 3:
-            fb["truncated"] += 1
-        if not fixed and abs(s["E_SD"] - e_star) <= 1e-6 * e_star:
+        stats.append(s)
+        if s["E_SD"] <= (1.0 + 1e-6) * e_star:
             t = time.perf_counter() - t0
             break
         if done:
             break
-    if fixed:
-        t = time.perf_counter() - t0
-        e_star = s["E_SD"]
     e64 = ctx.energy()
     return {"seconds": t, "newton_iterations": k, "E_SD_reached": s["E_SD"], "E_SD_reached_fp64": e64,
-            "rel_err_vs_E_star_fp64": abs(e64 - e_star) / e_star,
-            "driver_energy_minus_fp64": s["E_SD"] - e64, "pcg_fallbacks": fb}
+            "rel_err_vs_E_star_fp64": abs(e64 - e_star) / e_star, "driver_energy_minus_fp64": s["E_SD"] - e64,
+            "pcg_fallbacks": count_fallbacks(stats)}
+
+
+def disc_bench(ctx, args, targets):
+    import numpy as np
+    from optcuts_amd import capi, learned
+    from optcuts_amd.meshgen import height_field_disc
+    V, F, UV = height_field_disc(317, seed=0)
+    e_star = targets.get("E_SD_star")
+    have_weights = os.path.exists(learned.DEFAULT_WEIGHTS)
+    forms = {"plain": (capi.PRECOND_JACOBI, args.pcg_tol, capi.PRECISION_FP64),
+             "chosen": (capi.PRECOND_AMG, args.chosen_tol, capi.PRECISION_FP64),
+             "mixed": (capi.PRECOND_AMG, args.mixed_tol, capi.PRECISION_MIXED),
+             # controls at the loose tolerances of the faster forms: what inexact Newton alone gives
+             "plain_at_chosen_tol": (capi.PRECOND_JACOBI, args.chosen_tol, capi.PRECISION_FP64),
+             "block_jacobi_at_learned_tol": (capi.PRECOND_BLOCK_JACOBI, args.learned_tol, capi.PRECISION_FP64)}
+    if have_weights:
+        forms["learned"] = (capi.PRECOND_LEARNED, args.learned_tol, capi.PRECISION_FP64)
+        forms["learned_mixed"] = (capi.PRECOND_LEARNED, args.learned_tol, capi.PRECISION_MIXED)
+    net = None
+    if have_weights:
+        net = capi.Gnn(0)
+        net.set_params(learned.load_weights())
+        ctx.set_learned(net)
+    res = {}
+    for name, (precond, tol, precision) in forms.items():
+        print("bench: disc_200k form %s" % name, file=sys.stderr, flush=True)
+        steps = args.disc_steps if not name.startswith("plain") else min(args.disc_steps, 5)
+        el, timed = run_form(ctx, V, F, UV, tol, steps, 1, precond, precision)
+        r = {"newton_iters_per_s": steps / el, "ms_per_step": 1e3 * el / steps,
+             "pcg_iters_per_newton": float(np.mean([s["pcg_iters"] for s in timed])),
+             "pcg_fallbacks": count_fallbacks(timed), "precond": {0: "jacobi", 1: "block-jacobi", 2: "sa-amg",
+                                                                   3: "learned-factor"}[precond],
+             "precision": {0: "fp64", 1: "mixed"}[precision], "pcg_rel_tol": tol}
+        r["kernel_ms"] = {"element_pass": ctx.time_kernel(0, reps=10), "assembly": ctx.time_kernel(1, reps=10),
+                          "pcg_spmv": ctx.time_kernel(5, reps=50), "pcg_update": ctx.time_kernel(6, reps=50),
+                          "linesearch_candidates": ctx.time_kernel(3, reps=10),
+                          "step_bound": ctx.time_kernel(4, reps=10)}
+        if precond in (capi.PRECOND_AMG, capi.PRECOND_LEARNED):
+            r["kernel_ms"]["precond_apply"] = ctx.time_kernel(7, reps=10)
+            r["kernel_ms"]["precond_setup"] = ctx.time_kernel(8, reps=5)
+        if precond == capi.PRECOND_AMG:
+            r["kernel_ms"]["lanczos_fine_step"] = ctx.time_kernel(9, reps=50)
+        if e_star is not None and not args.no_tte:
+            r["time_to_E_SD_1e-6"] = time_to_target(ctx, V, F, UV, tol, precond, precision, e_star, args.tte_max_iter)
+        res[name] = r
+    if net is not None:
+        ctx.set_learned(None)
+        net.close()
+    nF, nVt, nnzb, nub = ctx.system_sizes()
+    speeds = {}
+    if "plain" in res and e_star is not None and not args.no_tte:
+        tp = res["plain"]["time_to_E_SD_1e-6"]["seconds"]
+        for name, r in res.items():
+            tf = r["time_to_E_SD_1e-6"]["seconds"]
+            speeds[name] = (tp / tf) if (tp and tf) else None
+    return {"workload": "height-field disc 317x317, seed 0, 199,712 faces; DistMin (lambda = 0), no scaffold",
+            "E_SD_star": e_star, "forms": res, "time_to_E_SD_speedup_vs_plain": speeds,
+            "system_sizes": [nF, nVt, nnzb, nub]}, (V, F, UV)
 
 
 def charts_bench(ctx, max_iter=60):
-    """The 64-chart workload (height-field discs of 500-20000 faces, log-uniform
-    sizes, seeds 1000+i): all charts batched in one context versus the same
-    charts solved one after another, multigrid form at PCG tolerance 1e-4, each chart run until
-    its own stop (gradient tolerance or line-search stop) or max_iter."""
+    """The 64-chart workload (height-field discs of 500-20000 faces, log-uniform sizes, seeds 1000+i): all charts
+    batched in one context versus the same charts solved one after another, multigrid form at PCG tolerance 1e-4,
+    each chart run until its own stop (gradient tolerance or line-search stop) or max_iter."""
+    import numpy as np
     from optcuts_amd import capi
     from optcuts_amd.meshgen import batched_disc_charts, union_of_charts
     charts = batched_disc_charts(64, 500, 20000, base_seed=1000)
@@ -139,8 +316,7 @@ def charts_bench(ctx, max_iter=60):
     st = ctx.charts_state()
     e_b = st["E_SD"].copy()
     t0 = time.perf_counter()
-    e_s = []
-    its_s = 0
+    e_s, its_s = [], 0
     for Vk, Fk, UVk in charts:
         ctx.set_mesh(Vk, Fk, fixed=(0,))
         ctx.set_uv(UVk)
@@ -152,74 +328,17 @@ def charts_bench(ctx, max_iter=60):
         its_s += k
         e_s.append(s["E_SD"])
     ts = time.perf_counter() - t0
-    e_s = __import__("numpy").array(e_s)
+    e_s = np.array(e_s)
     return {"charts": len(charts), "faces": int(F.shape[0]), "vertices": int(V.shape[0]),
             "batched_seconds": tb, "batched_newton_iterations": it,
             "sequential_seconds": ts, "sequential_newton_iterations_total": its_s,
-            "speedup_batched_vs_sequential": ts / tb,
-            "charts_converged": int((st["state"] != 0).sum()),
+            "speedup_batched_vs_sequential": ts / tb, "charts_converged": int((st["state"] != 0).sum()),
             "max_rel_diff_final_E_SD": float(abs(e_b - e_s).max() / e_s.min())}
 
 
-def scaffold_bench(ctx, V, F, UV, forms, iters=10, w_scaf=0.01, e_star=None, max_iter=40):
-    """The geometry loop with the bijectivity scaffold, as the reference runs it by default: before every
-    Newton step a new air mesh is made at the current UV (Optimizer.cpp:251-257; the reference re-meshes the
-    air region with Triangle, here a two-row band outside the boundary with fresh random diagonals stands in
-    for it), set (oc_set_air_mesh), and the system re-initialised (new pattern, slot maps and multigrid
-    structure: oc_newton_init), then one step. Per-iteration wall times of the library calls; the air-mesh
-    generation is timed apart and left out of iters_per_s and of the time to target. With e_star, each form
-    runs until its mesh E_SD is within 1e-6 of it (at most max_iter iterations), else iters iterations."""
-    import numpy as np
-    from optcuts_amd.meshgen import band_air_mesh, boundary_loop_ccw
-    loop = boundary_loop_ccw(F)
-    e = np.linalg.norm(UV[F[:, [1, 2, 0]]] - UV[F], axis=2).mean()
-    area_thres = np.sqrt(3.0) / 4.0 * (0.5 * e) ** 2
-    out = {}
-    for name, (precond, tol) in forms.items():
-        ctx.set_mesh(V, F, fixed=(0,))
-        ctx.set_uv(UV)
-        rows = []
-        hit = None
-        for k in range(max_iter if e_star is not None else iters):
-            t0 = time.perf_counter()
-            airUV, Fair, bnd = band_air_mesh(ctx.get_uv(), loop, rings=2, seed=k)
-            t1 = time.perf_counter()
-            ctx.set_air_mesh(airUV, Fair, bnd, area_thres, w_scaf)
-            ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=precond, pcg_rel_tol=tol,
-                            pcg_max_iter=max_iter_for(precond))
-            t2 = time.perf_counter()
-            s, done = ctx.newton_step()
-            t3 = time.perf_counter()
-            rows.append((t1 - t0, t2 - t1, t3 - t2, s["pcg_iters"], s["pcg_fallback"], s["E_SD"]))
-            if e_star is not None and hit is None and abs(s["E_SD"] - e_star) <= 1e-6 * e_star:
-                hit = len(rows)
-            if done or (hit is not None and len(rows) >= iters):
-                break
-        a = np.array([r[:3] for r in rows]) * 1e3
-        out[name] = {"newton_iterations": len(rows), "pcg_rel_tol": tol,
-                     "iters_per_s": len(rows) / (a[:, 1].sum() + a[:, 2].sum()) * 1e3,
-                     "reinit_ms_median": float(np.median(a[1:, 1])) if len(rows) > 1 else float(a[0, 1]),
-                     "step_ms_median": float(np.median(a[1:, 2])) if len(rows) > 1 else float(a[0, 2]),
-                     "first_iteration_ms": float(a[0, 1] + a[0, 2]),
-                     "air_mesh_generation_ms_median": float(np.median(a[:, 0])),
-                     "pcg_iters": [int(r[3]) for r in rows], "pcg_fallback_codes": [int(r[4]) for r in rows],
-                     "E_SD_last": rows[-1][5],
-                     "air_triangles": int(Fair.shape[0]), "air_vertices": int(airUV.shape[0])}
-        if e_star is not None:
-            out[name]["time_to_E_SD_1e-6"] = (
-                {"seconds": float((a[:hit, 1].sum() + a[:hit, 2].sum()) * 1e-3), "newton_iterations": hit}
-                if hit else None)
-    if "chosen" in out and "plain" in out:
-        out["speedup_iters_per_s_chosen_vs_plain"] = out["chosen"]["iters_per_s"] / out["plain"]["iters_per_s"]
-        tc, tp = out["chosen"].get("time_to_E_SD_1e-6"), out["plain"].get("time_to_E_SD_1e-6")
-        if tc and tp:
-            out["time_to_E_SD_speedup_chosen_vs_plain"] = tp["seconds"] / tc["seconds"]
-    return out
-
-
 def prescreen_bench(ctx, V, F, UV, reps=5):
-    """Topology-step pre-screen at the workload's size: oc_split_candidates (GPU keys, host selection) for
-    boundary and interior splits, median wall time of reps calls."""
+    """Topology-step pre-screen: oc_split_candidates (GPU keys, host selection) for boundary and interior splits,
+    median wall time of reps calls."""
     import numpy as np
     ctx.set_mesh(V, F, fixed=(0,))
     ctx.set_uv(UV)
@@ -236,8 +355,7 @@ def prescreen_bench(ctx, V, F, UV, reps=5):
 
 
 def prescreen_cpu_baseline(V, F, UV):
-    """The same pre-screen by the oracle's restatement of computeDivGradPerVert + querySplit's filter
-    (1 thread)."""
+    """The same pre-screen by the oracle's restatement of computeDivGradPerVert + querySplit's filter (1 thread)."""
     from oracle import oracle as O
     feat = O.RestFeatures(V, F)
     out = {}
@@ -250,52 +368,54 @@ def prescreen_cpu_baseline(V, F, UV):
     return out
 
 
-def cpu_baseline(V, F, UV, iters=3):
-    from oracle import oracle as O
+def cpu_baseline_closed(w):
+    """One scaffolded Newton iteration of the headline workload from its Tutte map by the oracle (the project's FP64
+    CPU restatement: element terms, assembly, its own sparse LDL^T, line search), 1 thread."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _helpers import ScaffoldOracle
+    airUV, Fair, bnd = w["air"]
     t = time.perf_counter()
-    tr, _, _ = O.newton_run(V, F, UV, a0=1.0, max_iter=iters, rel_gl2_tol=1e-30, allow_edec_rel_tol=False)
+    so = ScaffoldOracle(w["V"], w["F"], airUV, Fair, bnd, w["area_thres"], w["a0"], w["w_scaf"])
+    X = np.concatenate([w["UV0"], airUV[len(bnd):]])
+    so.newton_step(X, None)
     el = time.perf_counter() - t
-    return {"value": (len(tr) - 1) / el, "unit": "Newton iters/s", "cores": 1, "kind": "port",
-            "sample": "%d Newton iterations of the same 200k-face workload from the same initial UV: the "
-                      "project's FP64 CPU restatement of the reference (own sparse LDL^T, nested-dissection "
-                      "order), 1 thread, %.1f s" % (len(tr) - 1, el)}
+    return {"value": 1.0 / el, "unit": "Newton iters/s", "cores": 1, "kind": "port",
+            "sample": "1 scaffolded Newton iteration of the same workload (three fused tori, 200,216 faces, from its "
+                      "Tutte map with the same first air mesh): the project's FP64 CPU restatement of the reference "
+                      "(own sparse LDL^T, nested-dissection order), 1 thread, %.1f s" % el}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--grid", type=int, default=317, help="vertex grid side (317 -> 199,712 faces)")
-    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=100, help="timed Newton steps of the headline chosen form")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="tori_200k", choices=["tori_200k", "icosphere_100k", "icosphere_1m"])
     ap.add_argument("--pcg-tol", type=float, default=1e-10, help="plain form PCG relative tolerance")
-    ap.add_argument("--chosen-tol", type=float, default=1e-3,
-                    help="chosen form (FP64 multigrid) PCG relative tolerance")
-    ap.add_argument("--mixed-tol", type=float, default=1e-4,
-                    help="mixed form PCG relative tolerance (at 1e-3 its FP32-stored hierarchy falls back to "
-                         "Jacobi on this workload)")
-    ap.add_argument("--learned-tol", type=float, default=5e-3,
-                    help="PCG relative tolerance of both learned forms (a factor that needs hundreds of PCG "
-                         "iterations for 1e-4 reaches the target sooner with looser inner solves)")
-    ap.add_argument("--form", choices=["all", "both", "plain", "chosen", "mixed", "learned", "learned_mixed"],
-                    default="all", help="all = every form (learned ones when trained weights exist); "
-                                        "both = plain and chosen")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-charts", action="store_true", help="skip the 64-chart batched run")
-    ap.add_argument("--no-tte", action="store_true", help="skip the time-to-E_SD run")
-    ap.add_argument("--no-scaffold", action="store_true", help="skip the scaffolded loop")
-    ap.add_argument("--no-prescreen", action="store_true", help="skip the split-query pre-screen timing")
-    ap.add_argument("--no-graphs", action="store_true", help="launch PCG iterations kernel by kernel")
-    ap.add_argument("--scaffold-iters", type=int, default=10)
+    ap.add_argument("--chosen-tol", type=float, default=1e-3, help="chosen form (FP64 multigrid) PCG tolerance")
+    ap.add_argument("--mixed-tol", type=float, default=1e-4, help="mixed form PCG tolerance (disc section)")
+    ap.add_argument("--learned-tol", type=float, default=5e-3, help="learned forms' PCG tolerance (disc section)")
+    ap.add_argument("--air-interval", type=int, default=2,
+                    help="re-mesh the air mesh at least every N steps (besides whenever an air triangle set the "
+                         "step bound)")
+    ap.add_argument("--plain-budget", type=float, default=30.0,
+                    help="wall-clock budget (s) of the plain form's time-to-target run on the closed surfaces")
+    ap.add_argument("--disc-steps", type=int, default=10)
     ap.add_argument("--tte-max-iter", type=int, default=60)
-    ap.add_argument("--target-iters", type=int, default=0, metavar="K",
-                    help="workloads without a stored E*: the plain form's E_SD after K Newton iterations from "
-                         "the initial map is the time-to-target goal of the other forms")
+    ap.add_argument("--no-tte", action="store_true", help="skip the time-to-E_SD runs")
+    ap.add_argument("--no-secondary", action="store_true", help="headline workload only")
+    ap.add_argument("--no-disc", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--no-charts", action="store_true")
+    ap.add_argument("--no-prescreen", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch PCG iterations kernel by kernel")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--stream-probe", action="store_true",
-                    help="only time plain device reads (sum) and copies at several sizes: what the GPU streams at "
-                         "the row kernels' size, for comparison with their roofline fraction")
+                    help="only time plain device reads (sum) and copies at several sizes")
     args = ap.parse_args()
     if args.stream_probe:
         import torch
@@ -335,142 +455,93 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     from optcuts_amd import capi
-    from optcuts_amd.meshgen import height_field_disc
-
-    V, F, UV = height_field_disc(args.grid, seed=args.seed)
-    forms = {"plain": (capi.PRECOND_JACOBI, args.pcg_tol, capi.PRECISION_FP64),
-             "chosen": (capi.PRECOND_AMG, args.chosen_tol, capi.PRECISION_FP64),
-             "mixed": (capi.PRECOND_AMG, args.mixed_tol, capi.PRECISION_MIXED),
-             "learned": (capi.PRECOND_LEARNED, args.learned_tol, capi.PRECISION_FP64),
-             "learned_mixed": (capi.PRECOND_LEARNED, args.learned_tol, capi.PRECISION_MIXED)}
-    from optcuts_amd import learned
-    have_weights = os.path.exists(learned.DEFAULT_WEIGHTS)
-    if args.form == "all":
-        run = ["plain", "chosen", "mixed"] + (["learned", "learned_mixed"] if have_weights else [])
-    elif args.form == "both":
-        run = ["plain", "chosen"]
-    else:
-        run = [args.form]
+    targets = load_targets()
     ctx = capi.Context(0)
     if args.no_graphs:
         ctx.pcg_graphs(False)
+    arch = ctx.arch()
+    from optcuts_amd import learned
     net = None
-    if any(f.startswith("learned") for f in run):
+    fast = ("chosen", "mixed")
+    if os.path.exists(learned.DEFAULT_WEIGHTS):
         net = capi.Gnn(0)
         net.set_params(learned.load_weights())
         ctx.set_learned(net)
-    arch = ctx.arch()
-    targets_path = os.path.join(ROOT, "tests", "golden", "workload_targets.json")
-    e_star = None
-    if os.path.exists(targets_path) and args.grid == 317 and args.seed == 0:
-        e_star = json.load(open(targets_path))["E_SD_star"]
-    e_star_source = "oracle run to convergence (tests/golden/workload_targets.json)" if e_star is not None else None
-    if e_star is None and args.target_iters > 0 and "plain" in run:
-        run = ["plain"] + [f for f in run if f != "plain"]
-    res = {}
-    for name in run:
-        precond, tol, precision = forms[name]
-        el, warm, timed = run_form(ctx, V, F, UV, tol, args.steps, args.warmup, precond, precision)
-        r = {"newton_iters_per_s": args.steps / el, "ms_per_step": 1e3 * el / args.steps,
-             "pcg_iters_per_newton": float(np.mean([s["pcg_iters"] for s in timed])),
-             "pcg_fallback_codes": [s["pcg_fallback"] for s in timed],
-             "precond_setups": [s["precond_setups"] for s in timed],
-             "precond": {capi.PRECOND_JACOBI: "jacobi", capi.PRECOND_AMG: "sa-amg",
-                         capi.PRECOND_LEARNED: "learned-factor"}[precond],
-             "precision": {capi.PRECISION_FP64: "fp64", capi.PRECISION_MIXED: "mixed"}[precision],
-             "pcg_rel_tol": tol}
-        nF, nVt, nnzb, nub = ctx.system_sizes()
-        r["kernel_ms"] = {"element_pass": ctx.time_kernel(0, reps=10), "assembly": ctx.time_kernel(1, reps=10),
-                          "pcg_spmv": ctx.time_kernel(5, reps=50), "pcg_update": ctx.time_kernel(6, reps=50),
-                          "linesearch_candidates": ctx.time_kernel(3, reps=10),
-                          "step_bound": ctx.time_kernel(4, reps=10)}
-        if precond in (capi.PRECOND_AMG, capi.PRECOND_LEARNED):
-            r["kernel_ms"]["precond_apply"] = ctx.time_kernel(7, reps=10)
-            r["kernel_ms"]["precond_setup"] = ctx.time_kernel(8, reps=5)
-        if precond == capi.PRECOND_AMG:
-            r["kernel_ms"]["lanczos_fine_step"] = ctx.time_kernel(9, reps=50)
-        if e_star is None and name == "plain" and args.target_iters > 0 and not args.no_tte:
-            r["time_to_E_SD_1e-6"] = time_to_target(ctx, V, F, UV, tol, precond, precision, None,
-                                                    args.target_iters)
-            e_star = r["time_to_E_SD_1e-6"]["E_SD_reached"]
-            e_star_source = "plain form after %d Newton iterations" % args.target_iters
-        elif e_star is not None and not args.no_tte:
-            r["time_to_E_SD_1e-6"] = time_to_target(ctx, V, F, UV, tol, precond, precision, e_star,
-                                                    args.tte_max_iter)
-        res[name] = r
-    charts = None if args.no_charts else charts_bench(ctx)
-    scaf = None
-    if not args.no_scaffold:
-        sf = {}
-        if "chosen" in res:
-            sf["chosen"] = (capi.PRECOND_AMG, args.chosen_tol)
-        if "plain" in res:
-            sf["plain"] = (capi.PRECOND_JACOBI, args.pcg_tol)
-        scaf = scaffold_bench(ctx, V, F, UV, sf, iters=args.scaffold_iters, e_star=e_star) if sf else None
-    pre = None if args.no_prescreen else prescreen_bench(ctx, V, F, UV)
-    ctx.close()
+        fast = fast + ("learned", "learned_mixed", "block_jacobi_at_learned_tol")
+    print("bench: headline %s" % args.workload, file=sys.stderr, flush=True)
+    head, w = closed_bench(ctx, args.workload, args, targets, forms=fast + ("plain",), timed_steps=args.steps,
+                           plain_budget=args.plain_budget)
+    # the chosen form's dominant kernel on the headline workload is the PCG product (the Jacobi-PCG iterations of
+    # the steps multigrid cannot precondition and every PCG iteration after them): timed live on the last system
+    nF, nVt, nnzb, nub = head["system_sizes"]
+    from optcuts_amd import workloads as W
+    W.restart(ctx, w)
+    pc, tol, mx, prec = closed_forms(args)["chosen"]
+    ctx.newton_init(a0=w["a0"], rel_gl2_tol=1e-12, allow_edec_rel_tol=False, precond=pc, pcg_rel_tol=tol,
+                    pcg_max_iter=mx, precision=prec)
+    closed_step(ctx, args)
+    nF, nVt, nnzb, nub = ctx.system_sizes()
+    spmv_ms = ctx.time_kernel(5, reps=50)
+    nbytes = spmv_bytes(nnzb, nVt)
+    achieved = nbytes / (spmv_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_pcg_spmv (PCG product on the 2x2-block Newton matrix, mesh + air)",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": TRAFFIC.get((args.workload, "k_pcg_spmv")), "algorithmic_bytes_per_launch": nbytes,
+                "avg_launch_ms": spmv_ms, "system": {"triangles": nF, "vertices": nVt, "blocks": nnzb}}
+    secondary = {}
+    if not args.no_secondary:
+        if not args.no_c1 and args.workload != "icosphere_100k":
+            print("bench: icosphere_100k", file=sys.stderr, flush=True)
+            secondary["icosphere_100k"] = closed_bench(ctx, "icosphere_100k", args, targets,
+                                                       forms=("chosen", "mixed", "plain"), timed_steps=12,
+                                                       plain_steps=5, plain_budget=30.0)[0]
+        if not args.no_c4 and args.workload != "icosphere_1m":
+            print("bench: icosphere_1m", file=sys.stderr, flush=True)
+            secondary["icosphere_1m"] = closed_bench(ctx, "icosphere_1m", args, targets, forms=fast + ("plain",),
+                                                     timed_steps=12, plain_steps=2, plain_budget=20.0)[0]
+        if not args.no_disc:
+            print("bench: disc_200k", file=sys.stderr, flush=True)
+            secondary["disc_200k"], (V, F, UV) = disc_bench(ctx, args, targets)
+            if not args.no_prescreen:
+                secondary["split_prescreen_disc_200k"] = prescreen_bench(ctx, V, F, UV)
+                if not args.no_cpu_baseline:
+                    secondary["split_prescreen_disc_200k"]["cpu_oracle"] = prescreen_cpu_baseline(V, F, UV)
+        if not args.no_charts:
+            print("bench: charts", file=sys.stderr, flush=True)
+            secondary["batched_charts"] = charts_bench(ctx)
     if net is not None:
+        ctx.set_learned(None)
         net.close()
-
-    head = res["chosen"] if "chosen" in res else (res["plain"] if "plain" in res else next(iter(res.values())))
-    plain = res.get("plain")
-    is_200k = args.grid == 317 and args.seed == 0
-
-    def roofline(kernel, nbytes, ms, traffic_key):
-        achieved = nbytes / (ms * 1e-3) / 1e9
-        return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_200K[traffic_key] if is_200k else None,
-                "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": ms}
-
-    if "chosen" in res:
-        # the chosen form's dominant kernel (kernel trace: k_lanczos, fine level)
-        rl = roofline("k_lanczos<double,2> (fine-level Lanczos step of the multigrid set-up)",
-                      lanczos0_bytes(nnzb, nVt), res["chosen"]["kernel_ms"]["lanczos_fine_step"], "k_lanczos_fine")
-    else:
-        rl = roofline("k_pcg_spmv", spmv_bytes(nnzb, nVt), head["kernel_ms"]["pcg_spmv"], "k_pcg_spmv")
-    rl_plain = roofline("k_pcg_spmv", spmv_bytes(nnzb, nVt), plain["kernel_ms"]["pcg_spmv"],
-                        "k_pcg_spmv") if plain else None
-    speeds = {}
-    if "plain" in res and e_star is not None and not args.no_tte:
-        tp = res["plain"]["time_to_E_SD_1e-6"]["seconds"]
-        for name, r in res.items():
-            tf = r["time_to_E_SD_1e-6"]["seconds"]
-            speeds[name] = (tp / tf) if (tp and tf) else None
-    speed = speeds.get("chosen")
+    ctx.close()
+    chosen = head["forms"]["chosen"]
     out = {
         "metric": METRIC,
-        "value": head["newton_iters_per_s"],
+        "value": chosen["newton_iters_per_s"],
         "unit": "Newton iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
+        "ms_per_step": chosen["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "height-field disc %dx%d grid, seed %d, %d faces, %d vertices; DistMin (lambda=0), "
-                               "no scaffold; value is the %s form" % (args.grid, args.grid, args.seed, F.shape[0],
-                                                                      V.shape[0], "chosen" if "chosen" in res
-                                                                      else "plain"),
-                   "device": arch, "E_SD_star": e_star, "E_SD_star_source": e_star_source},
-        "forms": res,
-        "time_to_E_SD_speedup_chosen_vs_plain": speed,
-        "time_to_E_SD_speedup_vs_plain": speeds,
-        "batched_charts": charts,
-        "scaffold": scaf,
-        "split_prescreen": pre,
-        "roofline": rl,
-        "roofline_plain_form": rl_plain,
+        "config": {"workload": "%s: %s; cut to a disc, Tutte map, OptCuts joint mode geometry step (lambda_init "
+                               "0.999), bijectivity scaffold on; value is the chosen form (FP64, multigrid PCG to "
+                               "1e-3) over %d Newton steps from the Tutte map, air-mesh updates included"
+                               % (args.workload, W.CLOSED[args.workload], args.steps),
+                   "device": arch, "E_SD_star": head["E_SD_star"], "E_SD_star_source": head["E_SD_star_source"]},
+        "headline": head,
+        "time_to_E_SD_vs_plain": head.get("time_to_E_SD_vs_plain"),
+        "secondary": secondary,
+        "roofline": roofline,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(V, F, UV)
-        if pre is not None:
-            pre["cpu_oracle"] = prescreen_cpu_baseline(V, F, UV)
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_closed(w)
     print(json.dumps(out), flush=True)
 
 

@@ -157,14 +157,16 @@ int oc_set_air_mesh(oc_ctx* ctx, int nVair, const double* airUV, int nFair, cons
  * the air triangles positively oriented). Every mode resets the air mesh's rest shape to its current UV (z = 0,
  * areaThres_AM clamp as set), as a freshly built air mesh has, and evaluates *E = a0 E_SD + scaffold term.
  *   OC_SCAFFOLD_REFRESH: the triangulation is kept; the Newton system's pattern is unchanged.
- *   OC_SCAFFOLD_REMESH_WHEN_NEEDED: when the smallest angle of the air triangles is below minAngleDeg, the air mesh
- *     is first re-meshed by Lawson edge flips (the Delaunay triangulation of the current air vertices constrained
- *     by the map's boundary and the outer box) and the system re-patterned (only air rows change).
+ *   OC_SCAFFOLD_REMESH_WHEN_NEEDED: when the last step's injectivity bound was set by an air triangle (the air mesh
+ *     held the step back), when maxInterval > 0 calls have passed since the last re-mesh, or when minAngleDeg > 0
+ *     and the smallest angle of the air triangles is below it, the air mesh is first re-meshed by Lawson edge flips
+ *     (the Delaunay triangulation of the current air vertices constrained by the map's boundary and the outer
+ *     box) and the system re-patterned (only air rows change).
  *   OC_SCAFFOLD_REMESH: flips at every call.
  * *flips (may be null) = edges flipped (0: pattern unchanged). Needs oc_newton_init with an air mesh. Writes no
  * trace line and keeps the state oc_newton_rebuild keeps. */
 enum { OC_SCAFFOLD_REFRESH = 0, OC_SCAFFOLD_REMESH_WHEN_NEEDED = 1, OC_SCAFFOLD_REMESH = 2 };
-int oc_scaffold_update(oc_ctx* ctx, int mode, double minAngleDeg, double* E, int* flips);
+int oc_scaffold_update(oc_ctx* ctx, int mode, double minAngleDeg, int maxInterval, double* E, int* flips);
 /* The current air mesh: airUV nVair x 2 (air-mesh numbering: the boundary vertices first), Fair nFair x 3 (air
  * ids, after any flips). Either pointer may be null. */
 int oc_air_mesh_get(oc_ctx* ctx, double* airUV, int* Fair);
@@ -213,9 +215,11 @@ typedef struct {
                          factor solved it (the Newton system's element terms then stay FP64 until the next
                          oc_newton_init; line-search energies stay FP32); 1: multigrid
                          rebuilt with its guaranteed smoother interval solved it; 2: Jacobi-PCG solved it; 3:
-                         Jacobi-PCG met negative curvature (p.Ap <= 0, possible only when FP32 element Hessians
-                         leave the assembled matrix slightly indefinite) and the step uses its last iterate, or
-                         -g when it had none (truncated Newton) */
+                         Jacobi-PCG (the plain form's own solve, or the fallback) met p.Ap <= 0 (a singular Newton
+                         matrix, as for one triangle free to rotate about its fixed vertex, or FP32 element
+                         Hessians that leave it slightly indefinite) and the step uses its last iterate, or -g
+                         when it had none (truncated Newton); 5: Jacobi-PCG was used directly because
+                         multigrid failed (code 2) within the last 3 steps (it is tried again after them) */
     int pcg_iters_first; /* iterations of the first attempt when pcg_fallback != 0 (pcg_iters counts all) */
     int precond_setups;  /* multigrid hierarchies built on this step (1, or 2 after a safe-interval retry) */
 } oc_iter_stats;

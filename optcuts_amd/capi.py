@@ -79,7 +79,8 @@ _SIGS = {
     "oc_split_candidates": (C.c_int, [_ctx, C.c_int, C.c_double, _vp, C.c_int, _ip, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "oc_set_air_mesh": (C.c_int, [_ctx, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_double]),
-    "oc_scaffold_update": (C.c_int, [_ctx, C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "oc_scaffold_update": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int)]),
     "oc_air_mesh_get": (C.c_int, [_ctx, _vp, _vp]),
     "oc_solver_set_pattern": (C.c_int, [_ctx, C.c_int, _ip, _ip, C.c_int, _ip, C.POINTER(C.c_int)]),
     "oc_solver_get_pattern": (C.c_int, [_ctx, _ip, _ip]),
@@ -337,11 +338,14 @@ class Context:
                                          float(area_thres), float(w_scaf)))
         self.nVair, self.nFair = airUV.shape[0], Fair.shape[0]
 
-    def scaffold_update(self, mode=SCAFFOLD_REMESH_WHEN_NEEDED, min_angle_deg=10.0):
-        """The air mesh between steps (rest shape reset to the current UV; Lawson re-mesh per mode). Returns
-        (energy, edges flipped)."""
+    def scaffold_update(self, mode=SCAFFOLD_REMESH_WHEN_NEEDED, min_angle_deg=0.0, max_interval=2):
+        """The air mesh between steps (rest shape reset to the current UV; Lawson re-mesh per mode: when needed =
+        the last step's bound came from an air triangle, max_interval calls have passed since the last re-mesh
+        (0: no count), or an air angle is below min_angle_deg (0: no angle test)). Returns (energy, edges
+        flipped)."""
         e, n = C.c_double(), C.c_int()
-        self._chk(self.L.oc_scaffold_update(self.h, int(mode), float(min_angle_deg), C.byref(e), C.byref(n)))
+        self._chk(self.L.oc_scaffold_update(self.h, int(mode), float(min_angle_deg), int(max_interval), C.byref(e),
+                                            C.byref(n)))
         return e.value, n.value
 
     def air_mesh_get(self):

@@ -148,6 +148,10 @@ struct oc_ctx {
     double pcgTol = 1e-10;
     int pcgMaxIter = 10000;
     int pcgHint = 0;  // iterations of the previous Newton step's solve (first-batch size)
+    int amgSkip = 0;  // Newton steps left that go to Jacobi-PCG directly after multigrid failed
+    bool airLimited = false;  // the last step's injectivity bound came from an air triangle
+    bool amgPending = false;  // multigrid selected, its structure not yet built for the current pattern
+    int airSinceRemesh = 0;   // oc_scaffold_update calls since the air mesh was last re-meshed
     // Mixed form with the learned factor: element terms of the Newton system
     // (gradient, Hessian) in FP64 from the step whose FP32-assembled system
     // broke the solve until the next oc_newton_init
@@ -235,6 +239,7 @@ namespace {
 constexpr int kMaxPartialBlocks = 1 << 16;
 constexpr int kRzRecords = 2048;  // r.z partial records of an external preconditioner
 constexpr size_t kMailBytes = 4096;  // pinned read-back words of a context
+constexpr int kAmgSkip = 3;          // steps without multigrid after it failed on a step
 
 int ensure_scratch(oc_ctx* c, long nItems)
 {
@@ -576,6 +581,8 @@ void amg_numeric(oc_ctx* c, int safe)
     c->amg->numeric(c->stream, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>(), safe);
 }
 
+int amg_structure(oc_ctx* c);
+
 int build_system(oc_ctx* c)
 {
     const int nVt = c->nV + (c->nFair ? c->nVair - c->nBnd : 0);
@@ -659,23 +666,34 @@ int build_system(oc_ctx* c)
         }
         c->ext = &c->learned->net;
     }
+    c->amgPending = false;
     if (c->nPrecond == OC_PRECOND_AMG) {
-        std::string e;
         c->amg = c->sysPrecision == OC_PRECISION_MIXED ? static_cast<oc::AmgBase*>(&c->amg32)
                                                       : static_cast<oc::AmgBase*>(&c->amg64);
-        if (c->amg->setup(c->stream, nVt, c->sys.rowPtr, c->sys.colIdx, c->fixedMaskSys, e) != 0) {
-            c->err = e;
-            return OC_ERR_HIP;
-        }
-        c->ext = c->amg;
-        if (c->ext->rz_records() > kRzRecords) {
-            c->err = "AMG: too many partial records";
-            return OC_ERR_STATE;
-        }
-        c->amgValid = true;
+        // while multigrid is being skipped after a failure, its structure is built when it is next used
+        if (c->amgSkip > 0) c->amgPending = true;
+        else if (int st = amg_structure(c)) return st;
     }
     OC_HIP(c, hipStreamSynchronize(c->stream));
     c->sysReady = true;
+    return OC_OK;
+}
+
+// The multigrid structure (aggregates, prolongator pattern, product maps) of the current Newton pattern.
+int amg_structure(oc_ctx* c)
+{
+    std::string e;
+    if (c->amg->setup(c->stream, c->nVt, c->sys.rowPtr, c->sys.colIdx, c->fixedMaskSys, e) != 0) {
+        c->err = e;
+        return OC_ERR_HIP;
+    }
+    c->ext = c->amg;
+    if (c->ext->rz_records() > kRzRecords) {
+        c->err = "AMG: too many partial records";
+        return OC_ERR_STATE;
+    }
+    c->amgValid = true;
+    c->amgPending = false;
     return OC_OK;
 }
 
@@ -1461,6 +1479,8 @@ int oc_set_air_mesh(oc_ctx* c, int nVair, const double* airUV, int nFair, const 
     for (long k = 0; k < 3L * nFair; ++k) c->FairGlobal[k] = l2g[Fair[k]];
     c->FairLocal.assign(Fair, Fair + 3 * (size_t)nFair);
     c->airAreaThres = areaThres;
+    c->airSinceRemesh = 0;
+    c->airLimited = false;
     c->airL2G = l2g;
     if (upload(c, c->dAirL2G, c->airL2G)) return OC_ERR_HIP;
     OC_HIP(c, c->dAirUV2.ensure(sizeof(double) * 2 * (size_t)nVair));
@@ -2149,6 +2169,7 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     c->pcgTol = pcgRelTol;
     c->pcgMaxIter = pcgMaxIter;
     c->pcgHint = 0;
+    c->amgSkip = 0;
     c->elemFp64 = false;
     c->sysPrecision = c->precision;
     // Optimizer constructor: exit(-1) on an inverted initial map (Optimizer.cpp:65-67)
@@ -2236,7 +2257,7 @@ int oc_air_mesh_get(oc_ctx* c, double* airUV, int* Fair)
     return OC_OK;
 }
 
-int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, double* E, int* flips)
+int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, int maxInterval, double* E, int* flips)
 {
     if (!c) return OC_ERR_ARG;
     OC_ARG(c, mode == OC_SCAFFOLD_REFRESH || mode == OC_SCAFFOLD_REMESH_WHEN_NEEDED || mode == OC_SCAFFOLD_REMESH,
@@ -2248,6 +2269,7 @@ int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, double* E, int* 
     (void)hipSetDevice(c->device);
     hipStream_t s = c->stream;
     long nFlips = 0;
+    ++c->airSinceRemesh;
     if (mode != OC_SCAFFOLD_REFRESH) {
         std::vector<double> P(2 * (size_t)c->nVair);
         oc::launch_uv_gather(s, c->nVair, c->dAirL2G.as<int>(), c->dUV.as<double>(), nullptr,
@@ -2255,7 +2277,10 @@ int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, double* E, int* 
         OC_HIP(c, hipGetLastError());
         OC_HIP(c, hipMemcpyAsync(P.data(), c->dAirUV2.p, sizeof(double) * P.size(), hipMemcpyDeviceToHost, s));
         OC_HIP(c, hipStreamSynchronize(s));
-        if (mode == OC_SCAFFOLD_REMESH || oc::air_min_angle(P, c->FairLocal) < minAngleDeg * M_PI / 180.0) {
+        if (mode == OC_SCAFFOLD_REMESH || c->airLimited || (maxInterval > 0 && c->airSinceRemesh >= maxInterval) ||
+            (minAngleDeg > 0.0 && oc::air_min_angle(P, c->FairLocal) < minAngleDeg * M_PI / 180.0)) {
+            c->airLimited = false;
+            c->airSinceRemesh = 0;
             nFlips = oc::lawson_flips(P, c->FairLocal, 64L * c->nFair + 1024);
             if (nFlips > 0) {
                 for (size_t k = 0; k < c->FairLocal.size(); ++k) c->FairGlobal[k] = c->airL2G[c->FairLocal[k]];
@@ -2322,17 +2347,41 @@ void system_element_pass(oc_ctx* c, const TriSet& T)
 void system_assemble(oc_ctx* c);
 
 // The preconditioner's numeric set-up for the assembled Newton matrix (no host synchronisation).
-void launch_precond_setup(oc_ctx* c, oc_iter_stats* S)
+int launch_precond_setup(oc_ctx* c, oc_iter_stats* S)
 {
+    if (c->amgPending && c->amgSkip == 0)
+        if (int st = amg_structure(c)) return st;
     // The multigrid hierarchy is rebuilt for every Newton matrix: reusing the
     // previous one stays a valid (SPD) preconditioner but, measured at 200k
     // faces, early Newton steps change the matrix enough to multiply the PCG
     // iterations (12 -> 733), which costs more than the set-up saved.
-    if (c->amgValid) {
+    if (c->amgValid && c->amgSkip == 0) {
         amg_numeric(c, 0);
         ++S->precond_setups;
     }
     if (c->nPrecond == OC_PRECOND_LEARNED) c->learned->net.forward(c->stream, c->sys.dVals.p, c->sys.vals32);
+    return OC_OK;
+}
+
+// Jacobi-PCG for the direction (the fallback of every preconditioner), with the truncated-Newton rule when it
+// meets negative curvature; iterations are added to it.
+int jacobi_direction(oc_ctx* c, oc_iter_stats* S, int& it, double& rr)
+{
+    int it2 = 0;
+    int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, std::max(c->pcgMaxIter, 200000), it2, rr, nullptr);
+    it += it2;
+    if (st == OC_ERR_BREAKDOWN) {
+        // Negative curvature along p (the assembled matrix is not positive
+        // definite): truncated Newton. The iterate before the breakdown
+        // minimises the quadratic model over the Krylov space so far and is
+        // a descent direction; with no iterate, steepest descent.
+        if (it2 == 0)
+            OC_HIP(c, hipMemcpyAsync(c->dX.p, c->dB.p, sizeof(double) * 2 * (size_t)c->sys.nV,
+                                     hipMemcpyDeviceToDevice, c->stream));
+        S->pcg_fallback = 3;
+        st = OC_OK;
+    }
+    return st;
 }
 
 // Newton direction H d = b (b in c->dB, d in c->dX) with the selected
@@ -2341,11 +2390,31 @@ void launch_precond_setup(oc_ctx* c, oc_iter_stats* S)
 int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
 {
     hipStream_t s = c->stream;
-    if (!setupLaunched) launch_precond_setup(c, S);
+    if (!setupLaunched)
+        if (int st = launch_precond_setup(c, S)) return st;
     int it = 0;
     double rr = 0.0;
+    if ((c->amgValid || c->amgPending) && c->amgSkip > 0) {
+        // multigrid failed on a recent step (and no set-up was launched): Jacobi-PCG directly
+        --c->amgSkip;
+        S->pcg_fallback = 5;
+        const int st = jacobi_direction(c, S, it, rr);
+        S->pcg_iters = it;
+        S->pcg_rel_res = rr;
+        return st != OC_OK && st != OC_ERR_NOT_CONVERGED ? st : OC_OK;
+    }
     int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext, c->pcgHint);
     c->pcgHint = st == OC_OK ? it : 0;
+    if (!c->ext && st == OC_ERR_BREAKDOWN) {
+        // Jacobi-PCG met p.Ap <= 0: the Newton matrix is singular along p (one triangle whose only fixed vertex
+        // leaves it free to rotate, or FP32 element Hessians that left it slightly indefinite). Truncated Newton
+        // with the iterate so far (steepest descent when there is none), as the fallback chain below.
+        if (it == 0)
+            OC_HIP(c, hipMemcpyAsync(c->dX.p, c->dB.p, sizeof(double) * 2 * (size_t)c->sys.nV,
+                                     hipMemcpyDeviceToDevice, s));
+        S->pcg_fallback = 3;
+        st = OC_OK;
+    }
     if (c->ext && st != OC_OK) {
         // The preconditioned solve broke down or hit its cap. With the
         // learned factor in the mixed form the likely cause is the matrix
@@ -2381,21 +2450,11 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
             if (st == OC_OK) S->pcg_fallback = 1;
         }
         if (st != OC_OK) {
-            int it2 = 0;
-            st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, std::max(c->pcgMaxIter, 200000), it2, rr,
-                         nullptr);
-            it += it2;
-            if (st == OC_ERR_BREAKDOWN) {
-                // Negative curvature along p (the assembled matrix is not positive
-                // definite): truncated Newton. The iterate before the breakdown
-                // minimises the quadratic model over the Krylov space so far and is
-                // a descent direction; with no iterate, steepest descent.
-                if (it2 == 0)
-                    OC_HIP(c, hipMemcpyAsync(c->dX.p, c->dB.p, sizeof(double) * 2 * (size_t)c->sys.nV,
-                                             hipMemcpyDeviceToDevice, s));
-                S->pcg_fallback = 3;
-                st = OC_OK;
-            }
+            st = jacobi_direction(c, S, it, rr);
+            // multigrid that failed twice on this matrix is skipped for the next kAmgSkip steps (retried
+            // afterwards): on maps whose Newton matrices it cannot precondition (the Tutte map of a cut genus-3
+            // surface squeezes triangles to 1e-4 of the mean area) its two attempts would be paid on every step
+            if (c->amgValid) c->amgSkip = kAmgSkip;
         }
     }
     S->pcg_iters = it;
@@ -2584,7 +2643,7 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     OC_HIP(c, hipEventRecord(c->evGsq, s));
     system_assemble(c);
     oc::launch_scale_copy(s, 2L * nVt, -1.0, c->dG.as<double>(), c->dB.as<double>());
-    launch_precond_setup(c, S);
+    if (int st = launch_precond_setup(c, S)) return st;
     OC_HIP(c, hipGetLastError());
     OC_HIP(c, hipEventSynchronize(c->evGsq));
     const double gsq = c->mail[0];
@@ -2620,8 +2679,10 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     // one read-back; more halvings than that continue on the host below
     constexpr int K = oc::kLSCand;
     unsigned long long* om = c->dOrdMin.as<unsigned long long>();
+    unsigned long long* omAir = c->nFair ? om + 1 : nullptr;  // the air triangles' own bound
     oc::launch_set_u64(s, om, oc::dbl_to_ordered_host(1.0));
-    oc::launch_step_bound(s, T, c->dUV.as<double>(), dir, om, 1.0);
+    if (omAir) oc::launch_set_u64(s, omAir, oc::dbl_to_ordered_host(1.0));
+    oc::launch_step_bound(s, T, c->dUV.as<double>(), dir, om, 1.0, omAir);
     const double* esum = nullptr;
     if (c->nFair) {  // the scaffold changed: the energy is recomputed (Optimizer.cpp:588-592)
         oc::launch_element(s, T, c->dUV.as<double>(), nullptr, nullptr, nullptr, 0, c->dPartials.as<double>(),
@@ -2631,8 +2692,8 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     oc::launch_linesearch(s, T, c->dUV.as<double>(), dir, 0.0, 0, c->dPartials.as<double>(), c->dLsOut.as<double>(),
                           f32, om);
     oc::LsDecision* dDec = c->dLsDec.as<oc::LsDecision>();
-    oc::launch_ls_decide_commit(s, c->dLsOut.as<double>(), om, esum, c->lastE, c->a0, T.sAir, dDec, 2L * nVt, dir,
-                                c->dUV.as<double>());
+    oc::launch_ls_decide_commit(s, c->dLsOut.as<double>(), om, omAir, esum, c->lastE, c->a0, T.sAir, dDec, 2L * nVt,
+                                dir, c->dUV.as<double>());
     OC_HIP(c, hipGetLastError());
     double* cand = c->mail;
     oc::LsDecision* D = reinterpret_cast<oc::LsDecision*>(c->mail + 4 * K);
@@ -2646,6 +2707,8 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
         lastEScafStart = c->lastEScaf = D->lastEScaf;
     }
     const double stepSize = D->bound * 0.99;
+    // an air triangle set the injectivity bound (below 1): the air mesh held the step back
+    c->airLimited = c->nFair > 0 && D->bound < 1.0 && D->airBound <= D->bound;
     int k0 = 0;  // first index of the candidates held in cand
     auto fetch = [&](int kstart) -> int {
         oc::launch_linesearch(s, T, c->dUV.as<double>(), dir, stepSize, kstart, c->dPartials.as<double>(),

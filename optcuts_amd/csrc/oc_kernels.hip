@@ -497,7 +497,7 @@ template void launch_assemble<float>(hipStream_t, int, const int*, const int*, c
 // ----------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_step_bound(TriSet T, const double2* __restrict__ UV,
                                                        const double2* __restrict__ dir, unsigned long long* ordMin,
-                                                       double cap)
+                                                       unsigned long long* ordMinAir, double cap)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double bound = cap;
@@ -515,15 +515,22 @@ __global__ void __launch_bounds__(kBlock) k_step_bound(TriSet T, const double2* 
         bound = sd_step_bound(U, Vd, cap);
     }
     // NaN bounds never lower the step (the reference's 'bound < stepSize' is false)
-    bound = wave_min(bound != bound ? cap : bound);
+    bound = bound != bound ? cap : bound;
+    if (ordMinAir) {
+        // the air triangles' own minimum as well: whether the air mesh set the step
+        const double b = wave_min((t < T.nF && t >= T.nFmesh) ? bound : cap);
+        if ((threadIdx.x % kWave) == 0 && b < cap) atomicMin(ordMinAir, dbl_to_ordered(b));
+    }
+    bound = wave_min(bound);
     if ((threadIdx.x % kWave) == 0 && bound < cap) atomicMin(ordMin, dbl_to_ordered(bound));
 }
 
 void launch_step_bound(hipStream_t s, const TriSet& T, const double* UV, const double* dir,
-                       unsigned long long* ordMin, double cap)
+                       unsigned long long* ordMin, double cap, unsigned long long* ordMinAir)
 {
     hipLaunchKernelGGL(k_step_bound, dim3(grid_for(T.nF)), dim3(kBlock), 0, s, T,
-                       reinterpret_cast<const double2*>(UV), reinterpret_cast<const double2*>(dir), ordMin, cap);
+                       reinterpret_cast<const double2*>(UV), reinterpret_cast<const double2*>(dir), ordMin,
+                       ordMinAir, cap);
 }
 
 // ----------------------------------------------------------------------------
@@ -631,8 +638,8 @@ void launch_axpy_commit(hipStream_t s, long n, double alpha, const double* d, do
 // lastE is the host's value, or (esum non-null: the scaffold changed since it was computed, :588-592)
 // sMesh esum[0] + sAir esum[1] of the energy pass at the current UV. One thread.
 __global__ void k_ls_decide(const double* __restrict__ cand, const unsigned long long* __restrict__ boundOrd,
-                            const double* __restrict__ esum, double lastEHost, double sMesh, double sAir,
-                            LsDecision* out)
+                            const unsigned long long* __restrict__ airOrd, const double* __restrict__ esum,
+                            double lastEHost, double sMesh, double sAir, LsDecision* out)
 {
     constexpr int K = kLSCand;
     LsDecision r{};
@@ -644,6 +651,7 @@ __global__ void k_ls_decide(const double* __restrict__ cand, const unsigned long
         r.lastE = lastEHost;
     }
     r.bound = ordered_to_dbl(*boundOrd);
+    r.airBound = airOrd ? ordered_to_dbl(*airOrd) : r.bound;
     double alpha = r.bound * 0.99, alphaCommit = alpha;
     int k = 0, halv = 0, status = 0;
     double testE = cand[0];
@@ -686,10 +694,11 @@ __global__ void __launch_bounds__(kBlock) k_axpy_commit_dev(long n, const LsDeci
 }
 
 void launch_ls_decide_commit(hipStream_t s, const double* cand, const unsigned long long* boundOrd,
-                             const double* esum, double lastEHost, double sMesh, double sAir, LsDecision* dec, long n,
-                             const double* d, double* UV)
+                             const unsigned long long* airOrd, const double* esum, double lastEHost, double sMesh,
+                             double sAir, LsDecision* dec, long n, const double* d, double* UV)
 {
-    hipLaunchKernelGGL(k_ls_decide, dim3(1), dim3(1), 0, s, cand, boundOrd, esum, lastEHost, sMesh, sAir, dec);
+    hipLaunchKernelGGL(k_ls_decide, dim3(1), dim3(1), 0, s, cand, boundOrd, airOrd, esum, lastEHost, sMesh, sAir,
+                       dec);
     hipLaunchKernelGGL(k_axpy_commit_dev, dim3(grid_for(n)), dim3(kBlock), 0, s, n, dec, d, UV);
 }
 

@@ -73,9 +73,10 @@ void launch_assemble(hipStream_t s, int nUB, const int* ubRow, const int* ubCol,
                      int blockJacobi);
 
 // Minimum over triangles of the injectivity step bound, folded into
-// *ordMin (order-preserving uint64 of a double; min is order-independent).
+// *ordMin (order-preserving uint64 of a double; min is order-independent);
+// with ordMinAir, the minimum over the air triangles alone as well.
 void launch_step_bound(hipStream_t s, const TriSet& T, const double* UV, const double* dir,
-                       unsigned long long* ordMin, double cap);
+                       unsigned long long* ordMin, double cap, unsigned long long* ordMinAir = nullptr);
 
 // Line-search energies at K candidate steps alpha_k = alpha0 / 2^(k0+k),
 // k = 0..K-1: out[k] = sMesh*E_mesh + sAir*E_air, out[K+k] = E_mesh,
@@ -94,16 +95,17 @@ struct LsDecision {
     double testE, testESD, testES;  // energies at the accepted candidate
     double lastE, lastESD, lastEScaf;  // the energy the candidates were compared with
     double bound;        // the injectivity bound (before the 0.99 factor)
+    double airBound;     // its minimum over the air triangles alone (cap when none)
     int halvings;
     int status;  // 0 accepted, 1 more than kLSCand halvings needed (nothing committed), 2 alpha underflowed to 0
     int pad[2];
 };
 // Applies Optimizer::lineSearch's rule (:597-629) to the candidates `cand` of a launch_linesearch from boundOrd
 // and, unless more candidates are needed, commits UV += alphaCommit d (n doubles). lastE: lastEHost, or with
-// esum non-null sMesh esum[0] + sAir esum[1].
+// esum non-null sMesh esum[0] + sAir esum[1]. airOrd (may be null): the air triangles' bound, reported.
 void launch_ls_decide_commit(hipStream_t s, const double* cand, const unsigned long long* boundOrd,
-                             const double* esum, double lastEHost, double sMesh, double sAir, LsDecision* dec, long n,
-                             const double* d, double* UV);
+                             const unsigned long long* airOrd, const double* esum, double lastEHost, double sMesh,
+                             double sAir, LsDecision* dec, long n, const double* d, double* UV);
 
 // UV[i] = UV[i] + alpha * d[i] for i < n (two roundings, as the reference's
 // stepForward, Optimizer.cpp:666-669).
