@@ -139,6 +139,14 @@ int oc_div_grad_per_vert(oc_ctx* ctx, double* out);
  * splits). The keys come from the GPU; the selection runs on the host. */
 int oc_split_candidates(oc_ctx* ctx, int splitInterior, double filterExpIn, const double* vertWeight, int capacity,
                         int* cand, int* nCand, int* nSorted);
+/* The per-pair screen of TriMesh::queryMerge (TriMesh.cpp:801-908) for propagate = false, before each pair's
+ * computeLocalLDec, at the context's UV over the cohesive edges cohE (nCoh x 4, [a' b' a'' b''], mesh ids):
+ * a row is a zipper bottom when a' == a'' or b' == b''; its two other vertices move to their midpoint, which is
+ * projected by the reference's relaxation method (70 sweeps, eps = 1e-6 avgEdgeLen) when it inverts one of their
+ * triangles. status[r]: 0 candidate (mergedPos[2r..2r+1] = the position computeLocalLDec starts from), 1 not a
+ * zipper bottom, 2 no feasible position (skipped by the reference). *nCand = rows with status 0. Positions and
+ * decisions are bit-identical to the reference's arithmetic. */
+int oc_merge_candidates(oc_ctx* ctx, int nCoh, const int* cohE, int* status, double* mergedPos, int* nCand);
 
 /* ---- bijectivity scaffold (Scaffold.cpp:27-313) ------------------------- */
 /* The air mesh built by the Scaffold constructor: nVair vertices with UV airUV (nVair x 2; its rest shape is
@@ -194,8 +202,20 @@ int oc_solver_factorize(oc_ctx* ctx);
 int oc_solver_solve(oc_ctx* ctx, const double* rhs, double* x);
 /* LinSysSolver::multiply (LinSysSolver.hpp:172-187): Ax = A x with the full symmetric matrix. */
 int oc_solver_multiply(oc_ctx* ctx, const double* x, double* Ax);
+/* The solve's parameters (EigenLibSolver has none: its factorization is exact): OC_PRECOND_JACOBI,
+ * OC_PRECOND_BLOCK_JACOBI or OC_PRECOND_AMG (smoothed-aggregation multigrid built by factorize; a breakdown
+ * rebuilds it with the Gershgorin interval, then falls back to Jacobi-PCG), relative residual relTol, at most
+ * maxIter PCG iterations. */
 int oc_solver_set_params(oc_ctx* ctx, int precond, double relTol, int maxIter);
+/* Coordinates (2*nV, [u0 v0 u1 v1 ...], e.g. the current UV of TriMesh::V) for OC_PRECOND_AMG's near-null
+ * space (translations and the rotation of the plane per aggregate); without them the multigrid uses
+ * translations only. Takes effect at the next factorize. No reference counterpart: LinSysSolver passes no
+ * geometry. */
+int oc_solver_set_coords(oc_ctx* ctx, const double* UV);
 int oc_solver_last_stats(oc_ctx* ctx, int* iters, double* relResidual);
+/* What the last oc_solver_solve fell back to: 0 none, 1 multigrid rebuilt with the Gershgorin interval,
+ * 2 Jacobi-PCG. No reference counterpart. */
+int oc_solver_last_fallback(oc_ctx* ctx, int* fallback);
 
 /* ---- device-resident Newton driver (Optimizer.cpp:154-261,505-704) ------- */
 typedef struct {

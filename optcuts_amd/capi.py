@@ -78,6 +78,7 @@ _SIGS = {
     "oc_seam_sparsity": (C.c_int, [_ctx, C.c_int, _ip, C.c_double, C.c_int, C.POINTER(C.c_double)]),
     "oc_split_candidates": (C.c_int, [_ctx, C.c_int, C.c_double, _vp, C.c_int, _ip, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
+    "oc_merge_candidates": (C.c_int, [_ctx, C.c_int, _ip, _ip, _dp, C.POINTER(C.c_int)]),
     "oc_set_air_mesh": (C.c_int, [_ctx, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, C.c_double, C.c_double]),
     "oc_scaffold_update": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double),
                                      C.POINTER(C.c_int)]),
@@ -93,6 +94,8 @@ _SIGS = {
     "oc_solver_multiply": (C.c_int, [_ctx, _dp, _dp]),
     "oc_solver_set_params": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int]),
     "oc_solver_last_stats": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "oc_solver_set_coords": (C.c_int, [_ctx, _dp]),
+    "oc_solver_last_fallback": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
     "oc_newton_get_system": (C.c_int, [_ctx, _vp, _vp, _vp, _vp, _vp]),
     "oc_tutte": (C.c_int, [_ctx, C.c_int, C.c_int, C.c_double, C.c_int, _vp, C.POINTER(C.c_int),
                            C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -332,6 +335,16 @@ class Context:
                                              C.byref(ns)))
         return cand[:n.value].copy(), ns.value
 
+    def merge_candidates(self, cohE):
+        """TriMesh::queryMerge's per-pair screen: (status per row: 0 candidate, 1 not a zipper bottom,
+        2 infeasible; merged positions (nCoh, 2); number of candidates)."""
+        cohE = _i32(cohE).reshape(-1, 4)
+        st = np.zeros(cohE.shape[0], np.int32)
+        mp = np.zeros((cohE.shape[0], 2))
+        n = C.c_int()
+        self._chk(self.L.oc_merge_candidates(self.h, cohE.shape[0], cohE, st, mp.reshape(-1), C.byref(n)))
+        return st, mp, n.value
+
     def set_air_mesh(self, airUV, Fair, bnd, area_thres, w_scaf):
         airUV, Fair, bnd = _f64(airUV), _i32(Fair), _i32(bnd)
         self._chk(self.L.oc_set_air_mesh(self.h, airUV.shape[0], airUV, Fair.shape[0], Fair, len(bnd), bnd,
@@ -403,6 +416,18 @@ class Context:
         it, rr = C.c_int(), C.c_double()
         self._chk(self.L.oc_solver_last_stats(self.h, C.byref(it), C.byref(rr)))
         return it.value, rr.value
+
+    def solver_set_coords(self, UV):
+        """Coordinates (nV, 2) for the multigrid's rigid near-null space (PRECOND_AMG)."""
+        UV = _f64(UV)
+        assert UV.size == self._sol_n
+        self._chk(self.L.oc_solver_set_coords(self.h, UV))
+
+    def solver_last_fallback(self):
+        """0 none, 1 multigrid rebuilt with the Gershgorin interval, 2 Jacobi-PCG."""
+        f = C.c_int()
+        self._chk(self.L.oc_solver_last_fallback(self.h, C.byref(f)))
+        return f.value
 
     # ---- Newton driver -------------------------------------------------------
     def tutte(self, layout=TUTTE_COMPONENTS, precond=PRECOND_JACOBI, rel_tol=1e-12, max_iter=200000):

@@ -12,6 +12,7 @@
 #include "oc_system_pattern.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -93,6 +94,7 @@ struct oc_ctx {
     // split-query pre-screen: per-vertex keys, and the mesh's boundary flags
     // and neighbour counts (built on first use for the mesh of meshGen topoGen)
     DevBuf dDivGrad, dCohE, dSeam;
+    DevBuf dMrgA, dMrgB, dMrgOff, dMrgTri, dMrgCorner, dMrgStatus, dMrgPos;  // merge-query screen
     unsigned long topoGen = 0;
     std::vector<unsigned char> isBnd;
     std::vector<int> nbrCount;
@@ -209,6 +211,14 @@ struct oc_ctx {
     double solTol = 1e-10;
     int solMaxIter = 100000;
     bool solFactorized = false;
+    // multigrid for the drop-in: structure per pattern, hierarchy per factorize; near-null space from the
+    // coordinates of oc_solver_set_coords (rigid motions), translations only until they are set
+    oc::Amg<double> amgSol;
+    std::vector<unsigned char> solFixed;
+    bool solAmgStruct = false;
+    DevBuf dSolUV;
+    bool solHaveUV = false;
+    int solFallback = 0;  // what the last solve fell back to (0 none, 1 safe-interval multigrid, 2 Jacobi)
 
     const char* fail(const std::string& m)
     {
@@ -1450,6 +1460,76 @@ int oc_split_candidates(oc_ctx* c, int splitInterior, double filterExpIn, const 
     return OC_OK;
 }
 
+int oc_merge_candidates(oc_ctx* c, int nCoh, const int* cohE, int* status, double* mergedPos, int* nCand)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->nV > 0 && nCoh >= 0 && (nCoh == 0 || (cohE && status && mergedPos)) && nCand,
+           "oc_merge_candidates: bad arguments");
+    for (long k = 0; k < 4L * nCoh; ++k)
+        OC_ARG(c, cohE[k] >= 0 && cohE[k] < c->nV, "oc_merge_candidates: cohesive vertex out of range");
+    (void)hipSetDevice(c->device);
+    // zipper-bottom rows and their triangle lists: for each merge vertex v, the triangle holding the directed
+    // edge (v, w) for every neighbour w in ascending order (vNeighbor, edge2Tri; the last triangle when two hold
+    // the same directed edge, as the map assignment leaves it)
+    std::vector<int> rowOf, rA, rB, off(1, 0), tl, cl;
+    std::vector<std::array<int, 3>> inc;
+    for (int r = 0; r < nCoh; ++r) {
+        const int* row = cohE + 4 * (size_t)r;
+        int fork;
+        if (row[0] == row[2]) fork = 0;
+        else if (row[1] == row[3]) fork = 1;
+        else {
+            status[r] = 1;
+            mergedPos[2 * (size_t)r] = mergedPos[2 * (size_t)r + 1] = 0.0;
+            continue;
+        }
+        const int va = row[1 - fork], vb = row[3 - fork];
+        for (int v : {va, vb}) {
+            inc.clear();
+            for (int q = c->mvcPtr[v]; q < c->mvcPtr[v + 1]; ++q) {
+                const int t = c->mvcSrc[q] / 3, k = c->mvcSrc[q] % 3;
+                inc.push_back({c->Fh[3 * (size_t)t + (k + 1) % 3], t, k});
+            }
+            std::sort(inc.begin(), inc.end());
+            for (size_t i = 0; i < inc.size(); ++i) {
+                if (i + 1 < inc.size() && inc[i + 1][0] == inc[i][0]) continue;
+                tl.push_back(inc[i][1]);
+                cl.push_back(inc[i][2]);
+            }
+        }
+        rowOf.push_back(r);
+        rA.push_back(va);
+        rB.push_back(vb);
+        off.push_back((int)tl.size());
+    }
+    const int n = (int)rowOf.size();
+    *nCand = 0;
+    if (n == 0) return OC_OK;
+    if (upload(c, c->dMrgA, rA) || upload(c, c->dMrgB, rB) || upload(c, c->dMrgOff, off) ||
+        upload(c, c->dMrgTri, tl) || upload(c, c->dMrgCorner, cl))
+        return OC_ERR_HIP;
+    OC_HIP(c, c->dMrgStatus.ensure(sizeof(int) * (size_t)n));
+    OC_HIP(c, c->dMrgPos.ensure(sizeof(double) * 2 * (size_t)n));
+    oc::launch_merge_screen(c->stream, n, c->dMrgA.as<int>(), c->dMrgB.as<int>(), c->dMrgOff.as<int>(),
+                            c->dMrgTri.as<int>(), c->dMrgCorner.as<int>(), c->dF0.as<int>(), c->dF1.as<int>(),
+                            c->dF2.as<int>(), c->dUV.as<double>(), 1.0e-6 * c->avgEdgeLen, c->dMrgStatus.as<int>(),
+                            c->dMrgPos.as<double>());
+    OC_HIP(c, hipGetLastError());
+    std::vector<int> st(n);
+    std::vector<double> mp(2 * (size_t)n);
+    OC_HIP(c, hipMemcpyAsync(st.data(), c->dMrgStatus.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipMemcpyAsync(mp.data(), c->dMrgPos.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+        const int r = rowOf[i];
+        status[r] = st[i];
+        mergedPos[2 * (size_t)r] = mp[2 * (size_t)i];
+        mergedPos[2 * (size_t)r + 1] = mp[2 * (size_t)i + 1];
+        *nCand += st[i] == 0;
+    }
+    return OC_OK;
+}
+
 int oc_set_air_mesh(oc_ctx* c, int nVair, const double* airUV, int nFair, const int* Fair, int nBnd, const int* bnd,
                     double areaThres, double wScaf)
 {
@@ -1598,11 +1678,16 @@ int oc_solver_set_pattern(oc_ctx* c, int nV, const int* nbrPtr, const int* nbr, 
         }
     c->solN = 2 * nV;
     c->solA.assign(nnzU, 0.0);
+    c->solFixed = isFixed;
+    c->solAmgStruct = false;
+    c->solHaveUV = false;
     if (upload_pattern(c, c->sol)) return OC_ERR_HIP;
     if (upload(c, c->dSolDst, c->solDst) || upload(c, c->dSolMir, c->solMir)) return OC_ERR_HIP;
     OC_HIP(c, c->dSolA.ensure(sizeof(double) * (size_t)std::max(1, nnzU)));
     OC_HIP(c, c->dSolB.ensure(sizeof(double) * 2 * (size_t)nV));
     OC_HIP(c, c->dSolX.ensure(sizeof(double) * 2 * (size_t)nV));
+    OC_HIP(c, c->dSolUV.ensure(sizeof(double) * 2 * (size_t)nV));
+    OC_HIP(c, hipMemsetAsync(c->dSolUV.p, 0, sizeof(double) * 2 * (size_t)nV, c->stream));
     if (ensure_pcg(c, nV)) return OC_ERR_HIP;
     OC_HIP(c, hipStreamSynchronize(c->stream));
     c->solFactorized = false;
@@ -1695,8 +1780,32 @@ int oc_solver_factorize(oc_ctx* c)
     oc::launch_jacobi_from_bsr(c->stream, c->sol.nV, c->sol.dDiagPos.as<int>(), c->sol.dVals.as<double>(),
                                c->sol.dMinv.as<double>(), c->solPrecond == OC_PRECOND_BLOCK_JACOBI);
     OC_HIP(c, hipGetLastError());
+    if (c->solPrecond == OC_PRECOND_AMG) {
+        if (!c->solAmgStruct) {
+            std::string e;
+            if (c->amgSol.setup(c->stream, c->sol.nV, c->sol.rowPtr, c->sol.colIdx, c->solFixed, e) != 0) {
+                c->err = "oc_solver_factorize: " + e;
+                return OC_ERR_HIP;
+            }
+            c->solAmgStruct = true;
+        }
+        c->amgSol.numeric(c->stream, c->sol.dVals.p, 0, c->dSolUV.as<double>());
+        OC_HIP(c, hipGetLastError());
+    }
     OC_HIP(c, hipStreamSynchronize(c->stream));
     c->solFactorized = true;
+    return OC_OK;
+}
+
+int oc_solver_set_coords(oc_ctx* c, const double* UV)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0 && UV, "oc_solver_set_coords: no pattern");
+    (void)hipSetDevice(c->device);
+    OC_HIP(c, hipMemcpyAsync(c->dSolUV.p, UV, sizeof(double) * c->solN, hipMemcpyHostToDevice, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->solHaveUV = true;
+    c->solFactorized = false;
     return OC_OK;
 }
 
@@ -1712,7 +1821,29 @@ int oc_solver_solve(oc_ctx* c, const double* rhs, double* x)
     OC_HIP(c, hipMemcpyAsync(c->dSolB.p, rhs, sizeof(double) * c->solN, hipMemcpyHostToDevice, c->stream));
     int it;
     double rr;
-    const int st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it, rr);
+    c->solFallback = 0;
+    int st;
+    if (c->solPrecond == OC_PRECOND_AMG) {
+        // multigrid; on a breakdown (an indefinite V-cycle from a low eigenvalue estimate) the hierarchy is
+        // rebuilt with the guaranteed Gershgorin interval, then Jacobi-PCG takes over
+        st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it, rr, &c->amgSol);
+        if (st == OC_ERR_BREAKDOWN) {
+            int it2;
+            c->amgSol.numeric(c->stream, c->sol.dVals.p, 0, c->dSolUV.as<double>(), 1);
+            OC_HIP(c, hipGetLastError());
+            c->solFallback = 1;
+            st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it2, rr, &c->amgSol);
+            it += it2;
+        }
+        if (st == OC_ERR_BREAKDOWN) {
+            int it2;
+            c->solFallback = 2;
+            st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it2, rr);
+            it += it2;
+        }
+    } else {
+        st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it, rr);
+    }
     c->lastIters = it;
     c->lastRelRes = rr;
     if (st != OC_OK && st != OC_ERR_NOT_CONVERGED) return st;
@@ -1738,7 +1869,8 @@ int oc_solver_multiply(oc_ctx* c, const double* x, double* Ax)
 int oc_solver_set_params(oc_ctx* c, int precond, double relTol, int maxIter)
 {
     if (!c) return OC_ERR_ARG;
-    OC_ARG(c, (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI) && relTol > 0.0 && maxIter > 0,
+    OC_ARG(c, (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI || precond == OC_PRECOND_AMG) &&
+                  relTol > 0.0 && maxIter > 0,
            "oc_solver_set_params: bad parameters");
     c->solPrecond = precond;
     c->solTol = relTol;
@@ -1752,6 +1884,13 @@ int oc_solver_last_stats(oc_ctx* c, int* iters, double* relResidual)
     if (!c) return OC_ERR_ARG;
     if (iters) *iters = c->lastIters;
     if (relResidual) *relResidual = c->lastRelRes;
+    return OC_OK;
+}
+
+int oc_solver_last_fallback(oc_ctx* c, int* fallback)
+{
+    if (!c || !fallback) return OC_ERR_ARG;
+    *fallback = c->solFallback;
     return OC_OK;
 }
 

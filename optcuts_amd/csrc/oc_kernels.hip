@@ -14,6 +14,7 @@
 #include "oc_kernels.hpp"
 #include "oc_sd_element.hpp"
 
+#include <cfloat>
 #include <type_traits>
 
 namespace oc {
@@ -356,6 +357,79 @@ void launch_seam_len(hipStream_t s, int nC, const int* cohE, const double* Vrest
     hipLaunchKernelGGL(k_seam_len, dim3(grid), dim3(kBlock), 0, s, nC, reinterpret_cast<const int4*>(cohE), Vrest,
                        reinterpret_cast<const double2*>(UV), avgEdgeLen, triSoup, partials);
     hipLaunchKernelGGL(k_finish_csum<1>, dim3(1), dim3(kBlock), 0, s, partials, grid, out);
+}
+
+// ----------------------------------------------------------------------------
+// Per-pair screen of the merge query (TriMesh::queryMerge, TriMesh.cpp:836-908), one lane per zipper-bottom
+// cohesive pair: the two vertices to merge (va, vb) move to their midpoint; when that inverts one of their
+// incident triangles (doubled signed area < 0, TriMesh.cpp:1710-1756), the midpoint is projected onto the
+// half-planes of the edges opposite the merge vertex (relaxation method: at most 70 sweeps in list order, a
+// violated or nearly active row moves the point to eps inside; half-planes built with both merge vertices at
+// the midpoint) and the pair is infeasible when the result still inverts a triangle. Triangle lists
+// tl[off[r]..off[r+1]) hold va's triangles, then vb's, each with the corner of its own merge vertex in cl.
+// Operations in the reference's order, uncontracted, so positions and decisions are bit-identical.
+// status: 0 candidate, 2 infeasible; merged: the position.
+__global__ void __launch_bounds__(kBlock) k_merge_screen(int nRows, const int* __restrict__ rA,
+                                                         const int* __restrict__ rB, const int* __restrict__ off,
+                                                         const int* __restrict__ tl,
+                                                         const int* __restrict__ cl, const int* __restrict__ F0,
+                                                         const int* __restrict__ F1, const int* __restrict__ F2,
+                                                         const double2* __restrict__ UV, double eps, int* status,
+                                                         double2* merged)
+{
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nRows) return;
+    const int va = rA[r], vb = rB[r], k0 = off[r], k1 = off[r + 1];
+    const double2 pa = UV[va], pb = UV[vb];
+    double2 m = make_double2((pa.x + pb.x) / 2.0, (pa.y + pb.y) / 2.0);
+    const double2 mid = m;
+    const auto pos = [&](int v, double2 at) { return (v == va || v == vb) ? at : UV[v]; };
+    const auto corner = [&](int t, int k) { return k == 0 ? F0[t] : (k == 1 ? F1[t] : F2[t]); };
+    const auto inverts = [&](double2 at) {
+        for (int k = k0; k < k1; ++k) {
+            const int t = tl[k];
+            const double2 p0 = pos(F0[t], at), p1 = pos(F1[t], at), p2 = pos(F2[t], at);
+            const double e0x = p1.x - p0.x, e0y = p1.y - p0.y, e1x = p2.x - p0.x, e1y = p2.y - p0.y;
+            const double dbArea = e0x * e1y - e0y * e1x;
+            if (dbArea < 0.0) return true;
+        }
+        return false;
+    };
+    int st = 0;
+    if (inverts(m)) {
+        for (int it = 0; it < 70; ++it) {
+            double maxRes = -DBL_MAX;
+            for (int k = k0; k < k1; ++k) {
+                const int t = tl[k], i = cl[k];
+                const double2 v1 = pos(corner(t, (i + 1) % 3), mid), v2 = pos(corner(t, (i + 2) % 3), mid);
+                const double cx = v2.y - v1.y, cy = v1.x - v2.x;
+                const double len = sqrt(cx * cx + cy * cy);
+                const double nx = cx / len, ny = cy / len, c = (v1.x * v2.y - v1.y * v2.x) / len;
+                const double res = (nx * m.x + ny * m.y) - c;
+                if (res > -eps) {
+                    const double s = res + eps;
+                    m.x -= s * nx;
+                    m.y -= s * ny;
+                }
+                if (res > maxRes) maxRes = res;
+            }
+            if (maxRes < 0.0) break;
+        }
+        st = inverts(m) ? 2 : 0;
+    }
+    status[r] = st;
+    merged[r] = m;
+}
+
+void launch_merge_screen(hipStream_t s, int nRows, const int* rA, const int* rB, const int* off, const int* tl,
+                         const int* cl, const int* F0, const int* F1, const int* F2, const double* UV, double eps,
+                         int* status, double* merged)
+{
+    if (nRows <= 0) return;
+    hipLaunchKernelGGL(k_merge_screen, dim3(grid_for(nRows)), dim3(kBlock), 0, s, nRows, rA, rB, off, tl, cl, F0,
+                       F1, F2, reinterpret_cast<const double2*>(UV), eps, status,
+                       reinterpret_cast<double2*>(merged));
 }
 
 // ----------------------------------------------------------------------------

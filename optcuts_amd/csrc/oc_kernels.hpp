@@ -59,6 +59,13 @@ void launch_gather_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSr
 void launch_seam_len(hipStream_t s, int nC, const int* cohE, const double* Vrest, const double* UV, double avgEdgeLen,
                      int triSoup, double* partials, double* out);
 
+// The merge query's per-pair screen (TriMesh::queryMerge, TriMesh.cpp:836-908) over nRows zipper-bottom pairs
+// (merge vertices rA/rB, incident triangles tl[off[r]..off[r+1]) with the merge vertex's corner in cl): status 0
+// candidate / 2 infeasible, merged position (2 doubles per pair).
+void launch_merge_screen(hipStream_t s, int nRows, const int* rA, const int* rB, const int* off, const int* tl,
+                         const int* cl, const int* F0, const int* F1, const int* F2, const double* UV, double eps,
+                         int* status, double* merged);
+
 // out[v] = sample standard deviation of the contributions of vertex v's
 // triangles (0 for a vertex in one triangle): the split-query key.
 void launch_div_grad(hipStream_t s, int nV, const int* vcPtr, const int* vcSrc, const double* elemG, double* out);

@@ -54,6 +54,8 @@ def lib():
         L.orc_split_candidates.restype = C.c_int
         L.orc_split_candidates.argtypes = [C.c_int, C.c_int, _ip, _dp, _dp, C.c_int, C.c_double, _ip,
                                            C.POINTER(C.c_int)]
+        L.orc_merge_screen.restype = C.c_int
+        L.orc_merge_screen.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_double, C.c_int, _ip, _ip, _dp]
         L.orc_make_pd.restype = None
         L.orc_make_pd.argtypes = [C.c_int, _dp]
         L.orc_eigh.restype = None
@@ -205,6 +207,20 @@ def split_candidates(F, nV, div_grad, vert_weight=None, split_interior=False, fi
     n = lib().orc_split_candidates(nV, F.shape[0], F, _f64(div_grad), w, int(split_interior), float(filter_exp_in),
                                    cand, C.byref(ns))
     return cand[:n].copy(), ns.value
+
+
+def merge_screen(F, UV, cohE, avg_edge_len):
+    """The per-pair screen of TriMesh::queryMerge (TriMesh.cpp:801-905, propagate = false) before its local
+    evaluation: (status per cohesive row: 0 candidate, 1 not a zipper bottom, 2 infeasible; merged positions
+    (nCoh, 2); number of candidates)."""
+    F = _i32(F)
+    UV = _f64(UV)
+    cohE = _i32(cohE).reshape(-1, 4)
+    st = np.zeros(cohE.shape[0], np.int32)
+    mp = np.zeros((cohE.shape[0], 2))
+    n = lib().orc_merge_screen(UV.shape[0], F.shape[0], F, UV, float(avg_edge_len), cohE.shape[0], cohE, st,
+                               mp.reshape(-1))
+    return st, mp, n
 
 
 def element_hessians(F, UV, feat, uniform=False, project=True):

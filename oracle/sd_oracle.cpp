@@ -294,6 +294,111 @@ int orc_split_candidates(int nV, int nF, const int* F, const double* divGrad, co
     return n;
 }
 
+// The per-pair screen of TriMesh::queryMerge (TriMesh.cpp:801-905) for propagate = false, before its
+// computeLocalLDec: a cohesive row [a', b', a'', b''] is a "zipper bottom" when a' == a'' (fork 0) or
+// b' == b'' (fork 1); the two vertices to merge are row[1 - fork] and row[3 - fork]. Their incident triangles
+// come from vNeighbor (ascending) and edge2Tri (TriMesh.cpp:836-848); the merged position starts at their
+// midpoint; if that inverts an incident triangle (checkInversion, TriMesh.cpp:1710-1756: doubled area < 0), it is
+// projected by the relaxation method for the half-planes of the opposite edges (at most 70 sweeps,
+// eps = 1e-6 avgEdgeLen, built with both merge vertices at the midpoint, TriMesh.cpp:861-899) and the pair is
+// dropped if the projected position still inverts one (:901-908). status: 0 candidate (merged = its position),
+// 1 not a zipper bottom, 2 no feasible position. Returns the number of candidates.
+int orc_merge_screen(int nV, int nF, const int* F, const double* UV, double avgEdgeLen, int nCoh, const int* cohE,
+                     int* status, double* merged)
+{
+    std::vector<std::set<int>> nbr(nV);
+    std::map<std::pair<int, int>, int> edge2Tri;
+    for (int t = 0; t < nF; ++t)
+        for (int i = 0; i < 3; ++i) {
+            const int a = F[3 * t + i], b = F[3 * t + (i + 1) % 3];
+            edge2Tri[{a, b}] = t;
+            nbr[a].insert(b);
+            nbr[b].insert(a);
+        }
+    std::vector<double> V(UV, UV + 2 * (size_t)nV);
+    const auto inverted = [&](const std::vector<int>& tris) {
+        for (int t : tris) {
+            const int* f = F + 3 * (size_t)t;
+            const double e0x = V[2 * f[1]] - V[2 * f[0]], e0y = V[2 * f[1] + 1] - V[2 * f[0] + 1];
+            const double e1x = V[2 * f[2]] - V[2 * f[0]], e1y = V[2 * f[2] + 1] - V[2 * f[0] + 1];
+            const double dbArea = e0x * e1y - e0y * e1x;
+            if (dbArea < 0.0) return true;
+        }
+        return false;
+    };
+    int nCand = 0;
+    for (int r = 0; r < nCoh; ++r) {
+        const int* row = cohE + 4 * (size_t)r;
+        int fork;
+        if (row[0] == row[2]) fork = 0;
+        else if (row[1] == row[3]) fork = 1;
+        else {
+            status[r] = 1;
+            continue;
+        }
+        const int va = row[1 - fork], vb = row[3 - fork];
+        std::vector<int> tris;
+        int nFirst = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            const int v = pass ? vb : va;
+            for (int w : nbr[v]) {
+                auto it = edge2Tri.find({v, w});
+                if (it != edge2Tri.end()) tris.push_back(it->second);
+            }
+            if (!pass) nFirst = (int)tris.size();
+        }
+        double mx = (V[2 * va] + V[2 * vb]) / 2.0, my = (V[2 * va + 1] + V[2 * vb + 1]) / 2.0;
+        const double b0x = V[2 * va], b0y = V[2 * va + 1], b1x = V[2 * vb], b1y = V[2 * vb + 1];
+        V[2 * va] = V[2 * vb] = mx;
+        V[2 * va + 1] = V[2 * vb + 1] = my;
+        bool ok = true;
+        if (inverted(tris)) {
+            // half-plane rows (unit normal, offset) of the edge opposite the merge vertex, one per triangle
+            std::vector<double> nx(tris.size()), ny(tris.size()), c(tris.size());
+            for (size_t k = 0; k < tris.size(); ++k) {
+                const int t = tris[k], vm = (int)k < nFirst ? va : vb;
+                for (int i = 0; i < 3; ++i)
+                    if (F[3 * t + i] == vm) {
+                        const int p1 = F[3 * t + (i + 1) % 3], p2 = F[3 * t + (i + 2) % 3];
+                        const double x1 = V[2 * p1], y1 = V[2 * p1 + 1], x2 = V[2 * p2], y2 = V[2 * p2 + 1];
+                        const double cx = y2 - y1, cy = x1 - x2;
+                        const double len = std::sqrt(cx * cx + cy * cy);
+                        nx[k] = cx / len;
+                        ny[k] = cy / len;
+                        c[k] = (x1 * y2 - y1 * x2) / len;
+                        break;
+                    }
+            }
+            const double eps = 1.0e-6 * avgEdgeLen;
+            for (int iter = 0; iter < 70; ++iter) {
+                double maxRes = -__DBL_MAX__;
+                for (size_t k = 0; k < tris.size(); ++k) {
+                    const double res = (nx[k] * mx + ny[k] * my) - c[k];
+                    if (res > -eps) {
+                        const double s = res + eps;
+                        mx -= s * nx[k];
+                        my -= s * ny[k];
+                    }
+                    if (res > maxRes) maxRes = res;
+                }
+                if (maxRes < 0.0) break;
+            }
+            V[2 * va] = V[2 * vb] = mx;
+            V[2 * va + 1] = V[2 * vb + 1] = my;
+            ok = !inverted(tris);
+        }
+        V[2 * va] = b0x;
+        V[2 * va + 1] = b0y;
+        V[2 * vb] = b1x;
+        V[2 * vb + 1] = b1y;
+        status[r] = ok ? 0 : 2;
+        merged[2 * r] = mx;
+        merged[2 * r + 1] = my;
+        nCand += ok;
+    }
+    return nCand;
+}
+
 // ---------------------------------------------------------------------------
 // a5: symmetric eigen-decomposition of a small dense matrix by the cyclic
 // Jacobi method (Golub & Van Loan, Matrix Computations 4th ed., Alg. 8.5.3),

@@ -111,15 +111,53 @@ def test_solver_matches_reference_golden(ctx, golden, m):
     np.testing.assert_array_equal(ctx.solver_get_a(), O.update_a(ia, ja, I, J, Vt))
     np.testing.assert_allclose(ctx.solver_get_a(), golden[p + "a"], rtol=0,
                                atol=1e-13 * np.abs(golden[p + "a"]).max())
-    for prec in (capi.PRECOND_JACOBI, capi.PRECOND_BLOCK_JACOBI):
+    for prec, coords in ((capi.PRECOND_JACOBI, False), (capi.PRECOND_BLOCK_JACOBI, False),
+                         (capi.PRECOND_AMG, False), (capi.PRECOND_AMG, True)):
         ctx.solver_set_params(prec, rel_tol=1e-13, max_iter=200000)
+        if coords:
+            ctx.solver_set_coords(UV)
         ctx.solver_analyze_pattern()
         ctx.solver_factorize()
         for rhs, sol in ((-golden[p + "g"], golden[p + "d"]), (golden[p + "b2"], golden[p + "d2"])):
             x = ctx.solver_solve(rhs)
             it, rr = ctx.solver_last_stats()
             assert rr <= 1e-13
-            assert np.linalg.norm(x - sol) <= 1e-8 * np.linalg.norm(sol), (m, prec, it)
+            assert np.linalg.norm(x - sol) <= 1e-8 * np.linalg.norm(sol), (m, prec, coords, it)
+            if prec == capi.PRECOND_AMG:
+                assert ctx.solver_last_fallback() == 0, (m, coords)
+
+
+def test_solver_amg_drop_in_200k_faces(ctx):
+    """The drop-in path with multigrid at 200k faces (the reference's Optimizer hands the assembled Newton
+    matrix to LinSysSolver): the direction equals Jacobi-PCG's, in a mesh-size independent number of
+    iterations, with and without coordinates for the rotation mode."""
+    V, F, UV = height_field_disc(317, seed=0)
+    nV = V.shape[0]
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    I, J, Vt = ctx.hessian_triplets()
+    g = ctx.gradient()
+    ptr, nbr = O.vertex_neighbors(nV, F)
+    ctx.solver_set_pattern(nV, ptr, nbr, fixed=(0,))
+    ctx.solver_update_a(I, J, Vt)
+    out = {}
+    for key, prec, coords in (("jacobi", capi.PRECOND_JACOBI, False), ("amg", capi.PRECOND_AMG, False),
+                              ("amg_uv", capi.PRECOND_AMG, True)):
+        ctx.solver_set_params(prec, rel_tol=1e-10, max_iter=200000)
+        if coords:
+            ctx.solver_set_coords(UV)
+        ctx.solver_factorize()
+        x = ctx.solver_solve(-g)
+        it, rr = ctx.solver_last_stats()
+        print(key, "pcg", it, "rel res %.3g" % rr, flush=True)
+        out[key] = (x, it)
+        if prec == capi.PRECOND_AMG:
+            assert ctx.solver_last_fallback() == 0
+    xj = out["jacobi"][0]
+    for key in ("amg", "amg_uv"):
+        assert np.linalg.norm(out[key][0] - xj) <= 1e-7 * np.linalg.norm(xj), key
+    assert out["amg_uv"][1] < 100
+    assert out["amg_uv"][1] * 20 < out["jacobi"][1] and out["amg"][1] * 5 < out["jacobi"][1]
 
 
 def test_solver_multiply_and_bad_triplet(ctx):

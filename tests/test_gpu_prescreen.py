@@ -137,3 +137,44 @@ def test_prescreen_and_seam_argument_errors(ctx):
     st = ctx.L.oc_split_candidates(ctx.h, 0, 0.6, None, 3, cand, capi.C.byref(n), capi.C.byref(m))
     assert st == capi.OC_OK and n.value == len(full) > 3 and m.value == ns
     np.testing.assert_array_equal(cand, full[:3])
+
+
+@pytest.mark.parametrize("seed,amp", [(0, 0.0), (1, 0.1), (2, 0.15)])
+def test_merge_candidates_match_oracle_bit_exactly(ctx, seed, amp):
+    """TriMesh::queryMerge's per-pair screen on random zipper-bottom pairs (a vertex and two of its neighbours,
+    both fork columns, plus rows that are not zipper bottoms): status and merged positions bit-identical to the
+    oracle, including the pairs projected by the relaxation method and the infeasible ones."""
+    from test_oracle_merge import random_rows
+    V, F, UV = height_field_disc(41, seed=seed)
+    if amp:
+        UV = distort(UV, seed, amp=amp)
+    rows = random_rows(F, V.shape[0], 3000, seed)
+    ctx.set_mesh(V, F, fixed=(0,))
+    ctx.set_uv(UV)
+    st, mp, n = ctx.merge_candidates(rows)
+    ost, omp, on = O.merge_screen(F, UV, rows, O.RestFeatures(V, F).avgEdgeLen)
+    print(seed, amp, "candidates", n, "infeasible", int((st == 2).sum()), flush=True)
+    np.testing.assert_array_equal(st, ost)
+    assert n == on
+    zipper = st != 1
+    np.testing.assert_array_equal(mp[zipper], omp[zipper])
+    assert (st == 2).any() and (st == 0).any()
+
+
+def test_merge_candidates_on_a_cut_surface(ctx):
+    """The cut icosphere's own cohesive edges at its Tutte map and after Newton steps: the two rows at the ends of
+    the cut path are zipper bottoms; the screen equals the oracle's."""
+    from optcuts_amd import surfaces as S
+    Vc, Fc, cohE, meta = S.workload("icosphere_nu10")
+    ctx.set_mesh(Vc, Fc, fixed=(0,))
+    UV = ctx.tutte(capi.TUTTE_COMPONENTS, precond=capi.PRECOND_JACOBI, rel_tol=1e-13)[0]
+    ael = O.RestFeatures(Vc, Fc).avgEdgeLen
+    ctx.newton_init(a0=1.0, rel_gl2_tol=1e-12, precond=capi.PRECOND_AMG, pcg_rel_tol=1e-6, pcg_max_iter=2000)
+    for k in range(4):
+        UVk = ctx.get_uv()
+        st, mp, n = ctx.merge_candidates(cohE)
+        ost, omp, on = O.merge_screen(Fc, UVk, cohE, ael)
+        assert (st != 1).sum() == 2
+        np.testing.assert_array_equal(st, ost)
+        np.testing.assert_array_equal(mp[st != 1], omp[st != 1])
+        ctx.newton_step()
