@@ -7,7 +7,7 @@ start (w_scaf = 0.01 energyParams[0], Optimizer.cpp:87). A step is one Newton it
 element pass (energy, gradient, PSD-projected Hessian), block-CSR assembly, PCG direction, injectivity bound and
 line search, device-resident behind the C ABI, followed by the air-mesh update that the reference does with a new
 Triangle mesh (here oc_scaffold_update: rest shape reset, Lawson re-mesh when an air triangle set the step bound
-or every 2nd step). The timed region starts with the mesh, its Tutte map and the first air mesh in HBM.
+or every 8th step). The timed region starts with the mesh, its Tutte map and the first air mesh in HBM.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 1]
 
@@ -52,7 +52,8 @@ TRAFFIC = {
     ("disc_200k", "k_pcg_spmv"): (2.0 * 14334.2 + 3209.3) * 1024,
 }
 
-FALLBACK_NAMES = {1: "retry_safe_amg", 2: "jacobi", 3: "truncated", 4: "retry_fp64_elements", 5: "jacobi_direct"}
+FALLBACK_NAMES = {1: "retry_safe_amg", 2: "jacobi", 3: "truncated", 4: "retry_fp64_elements", 5: "jacobi_direct",
+                  6: "truncated_at_cap"}
 
 
 def count_fallbacks(stats):
@@ -85,13 +86,20 @@ def closed_forms(args):
 
 
 def closed_step(ctx, args):
-    """One Newton step then the air-mesh update; returns (stats, done, step s, update s)."""
+    """One Newton step then the air-mesh update; returns (stats, done, step s, update s, edges flipped). The
+    library's set-up phase times of each re-meshing update go to closed_step.remesh_times."""
     from optcuts_amd import capi
     t0 = time.perf_counter()
     s, done = ctx.newton_step()
     t1 = time.perf_counter()
-    ctx.scaffold_update(capi.SCAFFOLD_REMESH_WHEN_NEEDED, 0.0, args.air_interval)
-    return s, done, t1 - t0, time.perf_counter() - t1
+    _, flips = ctx.scaffold_update(capi.SCAFFOLD_REMESH_WHEN_NEEDED, 0.0, args.air_interval)
+    t2 = time.perf_counter()
+    if flips:
+        closed_step.remesh_times.append(ctx.setup_times())
+    return s, done, t1 - t0, t2 - t1, flips
+
+
+closed_step.remesh_times = []
 
 
 def closed_run(ctx, w, form, args, steps, warmup, e_star=None, budget=None):
@@ -108,14 +116,15 @@ def closed_run(ctx, w, form, args, steps, warmup, e_star=None, budget=None):
     for _ in range(warmup):
         closed_step(ctx, args)
     rows, stats = [], []
+    closed_step.remesh_times = []
     t0 = time.perf_counter()
     hit = None
     k = 0
     while True:
-        s, done, ts, tu = closed_step(ctx, args)
+        s, done, ts, tu, nflip = closed_step(ctx, args)
         k += 1
         stats.append(s)
-        rows.append((ts, tu, s["E_SD"], s["pcg_iters"]))
+        rows.append((ts, tu, s["E_SD"], s["pcg_iters"], nflip))
         el = time.perf_counter() - t0
         if e_star is not None and hit is None and s["E_SD"] <= (1.0 + 1e-6) * e_star:
             hit = (k, el + t_init)
@@ -131,6 +140,13 @@ def closed_run(ctx, w, form, args, steps, warmup, e_star=None, budget=None):
     out = {"newton_iterations": k, "seconds": el, "newton_iters_per_s": k / el, "ms_per_step": 1e3 * el / k,
            "step_ms_median": float(np.median(a[:, 0]) * 1e3), "air_update_ms_median": float(np.median(a[:, 1]) * 1e3),
            "air_update_share": float(a[:, 1].sum() / el), "init_ms": t_init * 1e3,
+           "air_remeshes": int((a[:, 4] > 0).sum()),
+           "air_update_ms_median_remesh": float(np.median(a[a[:, 4] > 0, 1]) * 1e3) if (a[:, 4] > 0).any() else None,
+           "air_update_ms_median_refresh": float(np.median(a[a[:, 4] == 0, 1]) * 1e3) if (a[:, 4] == 0).any()
+           else None,
+           "step_ms_max": float(a[:, 0].max() * 1e3), "pcg_iters_max": int(a[:, 3].max()),
+           "remesh_phase_ms_median": ({k: float(np.median([t[k] for t in closed_step.remesh_times]))
+                                       for k in closed_step.remesh_times[0]} if closed_step.remesh_times else None),
            "pcg_iters_median": float(np.median(a[:, 3])), "pcg_fallbacks": count_fallbacks(stats),
            "E_SD_first": float(a[0, 2]), "E_SD_last": float(a[-1, 2])}
     if e_star is not None:
@@ -255,6 +271,9 @@ def disc_bench(ctx, args, targets):
         net = capi.Gnn(0)
         net.set_params(learned.load_weights())
         ctx.set_learned(net)
+    if args.disc_forms:
+        keep = args.disc_forms.split(",")
+        forms = {k: v for k, v in forms.items() if k in keep}
     res = {}
     for name, (precond, tol, precision) in forms.items():
         print("bench: disc_200k form %s" % name, file=sys.stderr, flush=True)
@@ -386,6 +405,33 @@ def cpu_baseline_closed(w):
                       "(own sparse LDL^T, nested-dissection order), 1 thread, %.1f s" % el}
 
 
+def headline(ctx, args, targets, fast):
+    """The headline workload's forms, then the roofline of its dominant kernel: the PCG product (the Jacobi-PCG
+    iterations of the steps multigrid cannot precondition and every PCG iteration after them), timed live with
+    HIP events on the system after one chosen-form step from the Tutte map."""
+    from optcuts_amd import workloads as W
+    forms = fast + ("plain",)
+    if args.headline_forms:
+        forms = tuple(f for f in forms if f in args.headline_forms.split(","))
+    print("bench: headline %s" % args.workload, file=sys.stderr, flush=True)
+    head, w = closed_bench(ctx, args.workload, args, targets, forms=forms, timed_steps=args.steps,
+                           plain_budget=args.plain_budget)
+    W.restart(ctx, w)
+    pc, tol, mx, prec = closed_forms(args)["chosen"]
+    ctx.newton_init(a0=w["a0"], rel_gl2_tol=1e-12, allow_edec_rel_tol=False, precond=pc, pcg_rel_tol=tol,
+                    pcg_max_iter=mx, precision=prec)
+    closed_step(ctx, args)
+    nF, nVt, nnzb, nub = ctx.system_sizes()
+    spmv_ms = ctx.time_kernel(5, reps=50)
+    nbytes = spmv_bytes(nnzb, nVt)
+    achieved = nbytes / (spmv_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_pcg_spmv (PCG product on the 2x2-block Newton matrix, mesh + air)",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": TRAFFIC.get((args.workload, "k_pcg_spmv")), "algorithmic_bytes_per_launch": nbytes,
+                "avg_launch_ms": spmv_ms, "system": {"triangles": nF, "vertices": nVt, "blocks": nnzb}}
+    return head, w, roofline
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,7 +442,7 @@ def main():
     ap.add_argument("--chosen-tol", type=float, default=1e-3, help="chosen form (FP64 multigrid) PCG tolerance")
     ap.add_argument("--mixed-tol", type=float, default=1e-4, help="mixed form PCG tolerance (disc section)")
     ap.add_argument("--learned-tol", type=float, default=5e-3, help="learned forms' PCG tolerance (disc section)")
-    ap.add_argument("--air-interval", type=int, default=2,
+    ap.add_argument("--air-interval", type=int, default=8,
                     help="re-mesh the air mesh at least every N steps (besides whenever an air triangle set the "
                          "step bound)")
     ap.add_argument("--plain-budget", type=float, default=30.0,
@@ -405,6 +451,10 @@ def main():
     ap.add_argument("--tte-max-iter", type=int, default=60)
     ap.add_argument("--no-tte", action="store_true", help="skip the time-to-E_SD runs")
     ap.add_argument("--no-secondary", action="store_true", help="headline workload only")
+    ap.add_argument("--no-headline", action="store_true",
+                    help="secondary sections only (profiling runs; the JSON line then has no value)")
+    ap.add_argument("--headline-forms", default="", help="comma list of the headline's forms (default: all)")
+    ap.add_argument("--disc-forms", default="", help="comma list of the disc section's forms (default: all)")
     ap.add_argument("--no-disc", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
@@ -469,26 +519,10 @@ def main():
         net.set_params(learned.load_weights())
         ctx.set_learned(net)
         fast = fast + ("learned", "learned_mixed", "block_jacobi_at_learned_tol")
-    print("bench: headline %s" % args.workload, file=sys.stderr, flush=True)
-    head, w = closed_bench(ctx, args.workload, args, targets, forms=fast + ("plain",), timed_steps=args.steps,
-                           plain_budget=args.plain_budget)
-    # the chosen form's dominant kernel on the headline workload is the PCG product (the Jacobi-PCG iterations of
-    # the steps multigrid cannot precondition and every PCG iteration after them): timed live on the last system
-    nF, nVt, nnzb, nub = head["system_sizes"]
     from optcuts_amd import workloads as W
-    W.restart(ctx, w)
-    pc, tol, mx, prec = closed_forms(args)["chosen"]
-    ctx.newton_init(a0=w["a0"], rel_gl2_tol=1e-12, allow_edec_rel_tol=False, precond=pc, pcg_rel_tol=tol,
-                    pcg_max_iter=mx, precision=prec)
-    closed_step(ctx, args)
-    nF, nVt, nnzb, nub = ctx.system_sizes()
-    spmv_ms = ctx.time_kernel(5, reps=50)
-    nbytes = spmv_bytes(nnzb, nVt)
-    achieved = nbytes / (spmv_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_pcg_spmv (PCG product on the 2x2-block Newton matrix, mesh + air)",
-                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": TRAFFIC.get((args.workload, "k_pcg_spmv")), "algorithmic_bytes_per_launch": nbytes,
-                "avg_launch_ms": spmv_ms, "system": {"triangles": nF, "vertices": nVt, "blocks": nnzb}}
+    head, w, roofline = None, None, None
+    if not args.no_headline:
+        head, w, roofline = headline(ctx, args, targets, fast)
     secondary = {}
     if not args.no_secondary:
         if not args.no_c1 and args.workload != "icosphere_100k":
@@ -514,6 +548,10 @@ def main():
         ctx.set_learned(None)
         net.close()
     ctx.close()
+    if head is None:
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "Newton iters/s", "n_gpus": 1,
+                          "secondary": secondary}), flush=True)
+        return
     chosen = head["forms"]["chosen"]
     out = {
         "metric": METRIC,

@@ -39,8 +39,8 @@ enum {
 
 /* Newton-system preconditioners: point Jacobi (the plain form), 2x2 block Jacobi, smoothed-aggregation
  * algebraic multigrid under FP64 PCG (needs the UV map for its rigid near-null space), and the learned sparse
- * factor of an oc_gnn network (oc_newton_set_learned). AMG and LEARNED are available to the device-resident
- * Newton driver, not to the drop-in oc_solver_*. None has a reference counterpart: the reference factorizes
+ * factor of an oc_gnn network (oc_newton_set_learned). LEARNED is available to the device-resident Newton driver
+ * only; the others to the drop-in oc_solver_* as well. None has a reference counterpart: the reference factorizes
  * every system with SimplicialLDLT (EigenLibSolver.cpp:71-108). */
 enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1, OC_PRECOND_AMG = 2, OC_PRECOND_LEARNED = 3 };
 
@@ -165,8 +165,9 @@ int oc_set_air_mesh(oc_ctx* ctx, int nVair, const double* airUV, int nFair, cons
  * the air triangles positively oriented). Every mode resets the air mesh's rest shape to its current UV (z = 0,
  * areaThres_AM clamp as set), as a freshly built air mesh has, and evaluates *E = a0 E_SD + scaffold term.
  *   OC_SCAFFOLD_REFRESH: the triangulation is kept; the Newton system's pattern is unchanged.
- *   OC_SCAFFOLD_REMESH_WHEN_NEEDED: when the last step's injectivity bound was set by an air triangle (the air mesh
- *     held the step back), when maxInterval > 0 calls have passed since the last re-mesh, or when minAngleDeg > 0
+ *   OC_SCAFFOLD_REMESH_WHEN_NEEDED: when the air mesh held the last step back (the air triangles' injectivity bound
+ *     was below half of what the mesh triangles alone allowed, capped at a full step), when maxInterval > 0 calls
+ *     have passed since the last re-mesh, or when minAngleDeg > 0
  *     and the smallest angle of the air triangles is below it, the air mesh is first re-meshed by Lawson edge flips
  *     (the Delaunay triangulation of the current air vertices constrained by the map's boundary and the outer
  *     box) and the system re-patterned (only air rows change).
@@ -178,6 +179,11 @@ int oc_scaffold_update(oc_ctx* ctx, int mode, double minAngleDeg, int maxInterva
 /* The current air mesh: airUV nVair x 2 (air-mesh numbering: the boundary vertices first), Fair nFair x 3 (air
  * ids, after any flips). Either pointer may be null. */
 int oc_air_mesh_get(oc_ctx* ctx, double* airUV, int* Fair);
+/* Host wall times (ms) of the last Newton-system set-up (oc_newton_init, oc_newton_rebuild or a re-meshing
+ * oc_scaffold_update): ms[0] vertex lists, [1] pattern and slot maps, [2] their upload, [3] multigrid structure,
+ * [4] learned network's graph, [5] the whole set-up; and of the last oc_scaffold_update: [6] air-vertex download
+ * and edge flips (0 for a refresh), [7] the whole update. No reference counterpart (diagnostics). */
+int oc_setup_times(oc_ctx* ctx, double* ms);
 
 /* ---- LinSysSolver interface (LinSysSolver.hpp:22-256) -------------------- */
 /* LinSysSolver::set_pattern(vNeighbor, fixedVert) (LinSysSolver.hpp:37-135) + EigenLibSolver::set_pattern
@@ -239,7 +245,9 @@ typedef struct {
                          matrix, as for one triangle free to rotate about its fixed vertex, or FP32 element
                          Hessians that leave it slightly indefinite) and the step uses its last iterate, or -g
                          when it had none (truncated Newton); 5: Jacobi-PCG was used directly because
-                         multigrid failed (code 2) within the last 3 steps (it is tried again after them) */
+                         multigrid failed (code 2) within the last 3 steps (it is tried again after them); 6:
+                         multigrid-PCG reached pcgMaxIter without a breakdown and the step uses its last iterate
+                         (a descent direction: it minimises the quadratic model over the Krylov space so far) */
     int pcg_iters_first; /* iterations of the first attempt when pcg_fallback != 0 (pcg_iters counts all) */
     int precond_setups;  /* multigrid hierarchies built on this step (1, or 2 after a safe-interval retry) */
 } oc_iter_stats;

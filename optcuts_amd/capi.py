@@ -83,6 +83,7 @@ _SIGS = {
     "oc_scaffold_update": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double),
                                      C.POINTER(C.c_int)]),
     "oc_air_mesh_get": (C.c_int, [_ctx, _vp, _vp]),
+    "oc_setup_times": (C.c_int, [_ctx, _dp]),
     "oc_solver_set_pattern": (C.c_int, [_ctx, C.c_int, _ip, _ip, C.c_int, _ip, C.POINTER(C.c_int)]),
     "oc_solver_get_pattern": (C.c_int, [_ctx, _ip, _ip]),
     "oc_solver_update_a": (C.c_int, [_ctx, C.c_longlong, _ip, _ip, _dp]),
@@ -351,7 +352,7 @@ class Context:
                                          float(area_thres), float(w_scaf)))
         self.nVair, self.nFair = airUV.shape[0], Fair.shape[0]
 
-    def scaffold_update(self, mode=SCAFFOLD_REMESH_WHEN_NEEDED, min_angle_deg=0.0, max_interval=2):
+    def scaffold_update(self, mode=SCAFFOLD_REMESH_WHEN_NEEDED, min_angle_deg=0.0, max_interval=8):
         """The air mesh between steps (rest shape reset to the current UV; Lawson re-mesh per mode: when needed =
         the last step's bound came from an air triangle, max_interval calls have passed since the last re-mesh
         (0: no count), or an air angle is below min_angle_deg (0: no angle test)). Returns (energy, edges
@@ -360,6 +361,15 @@ class Context:
         self._chk(self.L.oc_scaffold_update(self.h, int(mode), float(min_angle_deg), int(max_interval), C.byref(e),
                                             C.byref(n)))
         return e.value, n.value
+
+    def setup_times(self):
+        """Host ms of the last set-up phases (vertex lists, pattern+maps, upload, multigrid structure, learned
+        graph, whole set-up) and of the last air-mesh update (download+flips, whole update)."""
+        ms = np.zeros(8)
+        self._chk(self.L.oc_setup_times(self.h, ms))
+        keys = ("vertex_lists", "pattern_maps", "upload", "amg_structure", "learned_graph", "setup_total",
+                "air_flips", "air_update_total")
+        return dict(zip(keys, ms.tolist()))
 
     def air_mesh_get(self):
         """(airUV (nVair, 2), Fair (nFair, 3)) of the current air mesh."""

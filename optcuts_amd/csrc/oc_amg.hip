@@ -143,6 +143,31 @@ __global__ void __launch_bounds__(kBlock) k_blockinv(int n, const int* __restric
         for (int j = 0; j < BS; ++j) o[i * BS + j] = (T)inv[i][j];
 }
 
+// FP32-stored levels: adds to each diagonal entry `factor` times its scalar row's absolute sum. Rounding a
+// symmetric positive definite matrix to FP32 perturbs each entry by at most 2^-24 of itself, a perturbation E
+// with sum_j |E_ij| <= 2^-24 sum_j |A_ij|; with factor >= 2^-24 (plus the rounding of the shifted diagonal
+// itself) E + shift is diagonally dominant, so the stored matrix stays positive definite whatever the
+// condition number (a V-cycle on an indefinite level breaks PCG). The shift only changes the preconditioner:
+// PCG multiplies by the FP64 matrix.
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_shift_diag(int n, const int* __restrict__ rowPtr,
+                                                       const int* __restrict__ diagPos, T* vals, double factor)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    double rs[BS] = {};
+    for (int k = rowPtr[u]; k < rowPtr[u + 1]; ++k) {
+        const T* b = vals + (size_t)k * BS * BS;
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int j = 0; j < BS; ++j) rs[i] += fabs((double)b[i * BS + j]);
+    }
+    T* d = vals + (size_t)diagPos[u] * BS * BS;
+#pragma unroll
+    for (int i = 0; i < BS; ++i) d[i * BS + i] = (T)((double)d[i * BS + i] + factor * rs[i]);
+}
+
 __device__ __forceinline__ double hash_unit(unsigned int i)
 {
     i ^= i >> 16;
@@ -1367,7 +1392,8 @@ long Amg<T>::level_blocks(int l) const
 
 template <typename T>
 int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
-                  const std::vector<unsigned char>& fixedMask, std::string& err)
+                  const std::vector<unsigned char>& fixedMask, std::string& err,
+                  const std::vector<unsigned char>* late)
 {
     ++generation_;
     // The previous hierarchy's rows are reused where they come out the same
@@ -1376,7 +1402,8 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
     spare_.clear();
     AmgPrevious prev;
     for (Level* l : L) prev.levels.push_back(l);
-    const int st = amg_build_structure(nV, rowPtr, colIdx, fixedMask, kCoarseMaxDofs, kMaxLevels, H, err, &prev);
+    const int st =
+        amg_build_structure(nV, rowPtr, colIdx, fixedMask, kCoarseMaxDofs, kMaxLevels, H, err, &prev, late);
     for (Level* l : L) spare_.push_back(std::move(static_cast<AmgHostLevel&>(*l)));
     release();
     if (st != 0) return -1;
@@ -1464,6 +1491,15 @@ void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double
             else
                 hipLaunchKernelGGL((k_cvt<double, T>), dim3(grid_for(nvals)), dim3(kBlock), 0, s, nvals,
                                    static_cast<const double*>(A0vals), A.vals);
+        }
+        if (std::is_same<T, float>::value) {  // keep the FP32-stored level positive definite
+            constexpr double kShift = 4.0 / 16777216.0;
+            if (bs == 2)
+                hipLaunchKernelGGL((k_shift_diag<T, 2>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, A.n, A.dRowPtr,
+                                   A.dDiagPos, A.vals, kShift);
+            else
+                hipLaunchKernelGGL((k_shift_diag<T, 3>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, A.n, A.dRowPtr,
+                                   A.dDiagPos, A.vals, kShift);
         }
         if (bs == 2)
             hipLaunchKernelGGL((k_blockinv<T, 2>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, A.n, A.dDiagPos, A.vals,

@@ -37,8 +37,11 @@ namespace oc {
 class AmgBase : public Preconditioner {
 public:
     ~AmgBase() override = default;
+    // late (may be null): nodes aggregated after all others (see aggregate_nodes), e.g. the rows an air-mesh
+    // change can touch, so that the rest of the hierarchy keeps its numbering across such changes
     virtual int setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
-                      const std::vector<unsigned char>& fixedMask, std::string& err) = 0;
+                      const std::vector<unsigned char>& fixedMask, std::string& err,
+                      const std::vector<unsigned char>* late = nullptr) = 0;
     // safe != 0: smoother interval and prolongator damping from the
     // Gershgorin bound of D^-1 A (guaranteed, slower convergence) instead of
     // the Lanczos estimate.
@@ -68,7 +71,8 @@ public:
     // part in aggregation. Allocates all device storage. Returns 0 or -1
     // (message in err).
     int setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
-              const std::vector<unsigned char>& fixedMask, std::string& err) override;
+              const std::vector<unsigned char>& fixedMask, std::string& err,
+              const std::vector<unsigned char>* late = nullptr) override;
 
     // Numeric hierarchy from the fine-level block values (same layout as the
     // pattern; double, or float when vals32) and the current UV (for the

@@ -285,6 +285,7 @@ struct AmgHostLevel {
     int n = 0, bs = 0;                       // nodes and block size of A at this level
     std::vector<int> rowPtr, colIdx, diagPos;
     std::vector<unsigned char> excluded;     // nodes outside every aggregate (fixed vertices)
+    std::vector<unsigned char> late;         // nodes that become aggregate roots only after all others (may be empty)
     // transfer to the next level
     int nc = 0;                              // aggregates (= nodes of the next level)
     std::vector<int> aggPtr, aggMem;         // members of each aggregate
@@ -300,27 +301,36 @@ struct AmgHostLevel {
 
 // Greedy aggregation (phase 1: roots whose neighbours are all free take
 // them; phase 2: leftovers join a neighbouring aggregate; phase 3: isolated
-// nodes become singletons). Excluded nodes get -1.
+// nodes become singletons). Excluded nodes get -1. Phase 1 visits nodes in
+// order, the `late` ones (if given) after all others: when only the rows of
+// late nodes change between two builds (the air mesh's rows of a scaffolded
+// system), every root among the other nodes takes the same neighbours and its
+// aggregate the same number, so the later levels' rows away from the change
+// come out the same and are copied.
 inline int aggregate_nodes(int n, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
-                           const std::vector<unsigned char>& excluded, std::vector<int>& agg)
+                           const std::vector<unsigned char>& excluded, std::vector<int>& agg,
+                           const std::vector<unsigned char>* late = nullptr)
 {
     agg.assign(n, -1);
     int na = 0;
-    for (int i = 0; i < n; ++i) {
-        if (excluded[i] || agg[i] >= 0) continue;
-        bool free = true;
-        for (int k = rowPtr[i]; k < rowPtr[i + 1] && free; ++k) {
-            const int w = colIdx[k];
-            if (w != i && !excluded[w] && agg[w] >= 0) free = false;
+    const bool twoPass = late && !late->empty();
+    for (int pass = 0; pass < (twoPass ? 2 : 1); ++pass)
+        for (int i = 0; i < n; ++i) {
+            if (excluded[i] || agg[i] >= 0) continue;
+            if (twoPass && (*late)[i] != pass) continue;
+            bool free = true;
+            for (int k = rowPtr[i]; k < rowPtr[i + 1] && free; ++k) {
+                const int w = colIdx[k];
+                if (w != i && !excluded[w] && agg[w] >= 0) free = false;
+            }
+            if (!free) continue;
+            agg[i] = na;
+            for (int k = rowPtr[i]; k < rowPtr[i + 1]; ++k) {
+                const int w = colIdx[k];
+                if (!excluded[w]) agg[w] = na;
+            }
+            ++na;
         }
-        if (!free) continue;
-        agg[i] = na;
-        for (int k = rowPtr[i]; k < rowPtr[i + 1]; ++k) {
-            const int w = colIdx[k];
-            if (!excluded[w]) agg[w] = na;
-        }
-        ++na;
-    }
     std::vector<int> agg1 = agg;
     for (int i = 0; i < n; ++i) {
         if (excluded[i] || agg1[i] >= 0) continue;
@@ -350,7 +360,8 @@ struct AmgPrevious {
 // levels, or until coarsening stalls). Returns 0, or -1 with a message.
 inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std::vector<int>& colIdx,
                                const std::vector<unsigned char>& fixedMask, long coarseMaxDofs, int maxLevels,
-                               std::vector<AmgHostLevel>& L, std::string& err, const AmgPrevious* prev = nullptr)
+                               std::vector<AmgHostLevel>& L, std::string& err, const AmgPrevious* prev = nullptr,
+                               const std::vector<unsigned char>* late0 = nullptr)
 {
     // Levels reuse the storage of L's elements on entry (a hierarchy two
     // set-ups old); every array is rewritten, and nc marks the levels that
@@ -368,6 +379,8 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
     L[0].rowPtr = rowPtr;
     L[0].colIdx = colIdx;
     L[0].excluded = fixedMask;
+    if (late0) L[0].late = *late0;
+    else L[0].late.clear();
     for (;;) {
         AmgHostLevel& A = L.back();
         A.diagPos.assign(A.n, -1);
@@ -395,7 +408,7 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
         }
         if (nIso == nIn) break;
         std::vector<int>& agg = A.agg;
-        const int nc = aggregate_nodes(A.n, A.rowPtr, A.colIdx, A.excluded, agg);
+        const int nc = aggregate_nodes(A.n, A.rowPtr, A.colIdx, A.excluded, agg, &A.late);
         if (nc == 0 || (long)(nc - nIso) * 10 > (long)(nIn - nIso) * 9) break;  // coupled part stopped coarsening
         A.nc = nc;
         // Rows of the earlier build's level that come out the same: matrix row
@@ -531,6 +544,13 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
         nx->n = nc;
         nx->bs = kAmgNullDim;
         nx->excluded.assign(nc, 0);
+        // an aggregate holding a late node is late on the next level
+        nx->late.clear();
+        if (!A.late.empty()) {
+            nx->late.assign(nc, 0);
+            for (int u = 0; u < A.n; ++u)
+                if (agg[u] >= 0 && A.late[u]) nx->late[agg[u]] = 1;
+        }
         {
             const int nt = setup_threads((long)A.tCol.size() * 4);
             std::vector<unsigned char> cleanC;

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -95,6 +96,10 @@ struct oc_ctx {
     // and neighbour counts (built on first use for the mesh of meshGen topoGen)
     DevBuf dDivGrad, dCohE, dSeam;
     DevBuf dMrgA, dMrgB, dMrgOff, dMrgTri, dMrgCorner, dMrgStatus, dMrgPos;  // merge-query screen
+    // avgEdgeLen summed in igl::avg_edge_length's order (the device reduction's order differs in the last bits;
+    // the merge screen's eps must be the reference's to the bit), for the mesh of meshGen refAelGen
+    double refAvgEdgeLen = 0.0;
+    unsigned long refAelGen = 0;
     unsigned long topoGen = 0;
     std::vector<unsigned char> isBnd;
     std::vector<int> nbrCount;
@@ -151,6 +156,9 @@ struct oc_ctx {
     int pcgMaxIter = 10000;
     int pcgHint = 0;  // iterations of the previous Newton step's solve (first-batch size)
     int amgSkip = 0;  // Newton steps left that go to Jacobi-PCG directly after multigrid failed
+    int amgFailStreak = 0;  // consecutive multigrid steps whose whole fallback chain ended in Jacobi-PCG
+    // host wall times (ms) of the last set-up and air-mesh update, for oc_setup_times
+    double tSetup[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool airLimited = false;  // the last step's injectivity bound came from an air triangle
     bool amgPending = false;  // multigrid selected, its structure not yet built for the current pattern
     int airSinceRemesh = 0;   // oc_scaffold_update calls since the air mesh was last re-meshed
@@ -250,6 +258,10 @@ constexpr int kMaxPartialBlocks = 1 << 16;
 constexpr int kRzRecords = 2048;  // r.z partial records of an external preconditioner
 constexpr size_t kMailBytes = 4096;  // pinned read-back words of a context
 constexpr int kAmgSkip = 3;          // steps without multigrid after it failed on a step
+#ifndef OC_AIR_LIMIT_RATIO
+#define OC_AIR_LIMIT_RATIO 0.5
+#endif
+constexpr double kAirLimitRatio = OC_AIR_LIMIT_RATIO;  // air bound below this share of the mesh's: re-mesh
 
 int ensure_scratch(oc_ctx* c, long nItems)
 {
@@ -593,8 +605,14 @@ void amg_numeric(oc_ctx* c, int safe)
 
 int amg_structure(oc_ctx* c);
 
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 int build_system(oc_ctx* c)
 {
+    const auto tb0 = std::chrono::steady_clock::now();
     const int nVt = c->nV + (c->nFair ? c->nVair - c->nBnd : 0);
     const int nFt = c->nF + c->nFair;
     c->nVt = nVt;
@@ -614,6 +632,8 @@ int build_system(oc_ctx* c)
     // vertex -> (triangle*3 + corner) lists in triangle order (the gradient
     // gathers use them, and so does everything below)
     build_vertex_lists(nVt, nFt, Ft.data(), c->vcPtr, c->vcSrc);
+    c->tSetup[0] = ms_since(tb0);
+    auto tp = std::chrono::steady_clock::now();
     // Pattern and slot maps. After an air-mesh change on the same mesh (the
     // scaffold is rebuilt every Newton iteration, Optimizer.cpp:251-257, and
     // re-patterned, :522-530), the rows of mesh vertices outside the old and
@@ -638,6 +658,8 @@ int build_system(oc_ctx* c)
     c->sysBuilt = true;
     c->sysMeshGen = c->meshGen;
     c->sysAirTouched = std::move(touched);
+    c->tSetup[1] = ms_since(tp);
+    tp = std::chrono::steady_clock::now();
     if (upload_pattern(c, c->sys)) return OC_ERR_HIP;
     // The PCG matrix stays FP64 in both precisions: rounding the assembled
     // Hessian to FP32 can make it indefinite along its near-null directions
@@ -656,9 +678,13 @@ int build_system(oc_ctx* c)
     OC_HIP(c, c->dTmpVec.ensure(sizeof(double) * 2 * (size_t)std::max(1, nVt)));
     if (ensure_pcg(c, nVt)) return OC_ERR_HIP;
     if (ensure_scratch(c, std::max(std::max(nFt, nVt), (int)c->sm.ubRow.size()))) return OC_ERR_HIP;
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    c->tSetup[2] = ms_since(tp);
+    tp = std::chrono::steady_clock::now();
     c->amgValid = false;
     c->amg = nullptr;
     c->ext = nullptr;
+    c->tSetup[3] = c->tSetup[4] = 0.0;
     if (c->nPrecond == OC_PRECOND_LEARNED) {
         if (!c->learned) {
             c->err = "OC_PRECOND_LEARNED needs a network (oc_newton_set_learned)";
@@ -675,6 +701,7 @@ int build_system(oc_ctx* c)
             return OC_ERR_STATE;
         }
         c->ext = &c->learned->net;
+        c->tSetup[4] = ms_since(tp);
     }
     c->amgPending = false;
     if (c->nPrecond == OC_PRECOND_AMG) {
@@ -685,6 +712,8 @@ int build_system(oc_ctx* c)
         else if (int st = amg_structure(c)) return st;
     }
     OC_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->nPrecond == OC_PRECOND_AMG) c->tSetup[3] = ms_since(tp);
+    c->tSetup[5] = ms_since(tb0);
     c->sysReady = true;
     return OC_OK;
 }
@@ -693,7 +722,15 @@ int build_system(oc_ctx* c)
 int amg_structure(oc_ctx* c)
 {
     std::string e;
-    if (c->amg->setup(c->stream, c->nVt, c->sys.rowPtr, c->sys.colIdx, c->fixedMaskSys, e) != 0) {
+    // the rows an air mesh can change (air vertices and the mesh vertices in air triangles) are aggregated
+    // last, so the hierarchy away from the air mesh keeps its numbering and is copied after a re-mesh
+    std::vector<unsigned char> late;
+    if (c->nFair) {
+        late.assign(c->nVt, 0);
+        for (int v : c->FairGlobal) late[v] = 1;
+    }
+    if (c->amg->setup(c->stream, c->nVt, c->sys.rowPtr, c->sys.colIdx, c->fixedMaskSys, e,
+                      late.empty() ? nullptr : &late) != 0) {
         c->err = e;
         return OC_ERR_HIP;
     }
@@ -1505,6 +1542,18 @@ int oc_merge_candidates(oc_ctx* c, int nCoh, const int* cohE, int* status, doubl
     const int n = (int)rowOf.size();
     *nCand = 0;
     if (n == 0) return OC_OK;
+    if (c->refAelGen != c->meshGen) {  // igl::avg_edge_length (avg_edge_length.cpp:17-30): half-edges in face order
+        double el = 0.0;
+        for (int t = 0; t < c->nF; ++t)
+            for (int j = 0; j < 3; ++j) {
+                const double* p = &c->Vh[3 * (size_t)c->Fh[3 * (size_t)t + j]];
+                const double* q = &c->Vh[3 * (size_t)c->Fh[3 * (size_t)t + (j + 1) % 3]];
+                const double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+                el += std::sqrt(dx * dx + dy * dy + dz * dz);
+            }
+        c->refAvgEdgeLen = el / (double)(3L * c->nF);
+        c->refAelGen = c->meshGen;
+    }
     if (upload(c, c->dMrgA, rA) || upload(c, c->dMrgB, rB) || upload(c, c->dMrgOff, off) ||
         upload(c, c->dMrgTri, tl) || upload(c, c->dMrgCorner, cl))
         return OC_ERR_HIP;
@@ -1512,7 +1561,7 @@ int oc_merge_candidates(oc_ctx* c, int nCoh, const int* cohE, int* status, doubl
     OC_HIP(c, c->dMrgPos.ensure(sizeof(double) * 2 * (size_t)n));
     oc::launch_merge_screen(c->stream, n, c->dMrgA.as<int>(), c->dMrgB.as<int>(), c->dMrgOff.as<int>(),
                             c->dMrgTri.as<int>(), c->dMrgCorner.as<int>(), c->dF0.as<int>(), c->dF1.as<int>(),
-                            c->dF2.as<int>(), c->dUV.as<double>(), 1.0e-6 * c->avgEdgeLen, c->dMrgStatus.as<int>(),
+                            c->dF2.as<int>(), c->dUV.as<double>(), 1.0e-6 * c->refAvgEdgeLen, c->dMrgStatus.as<int>(),
                             c->dMrgPos.as<double>());
     OC_HIP(c, hipGetLastError());
     std::vector<int> st(n);
@@ -2309,6 +2358,7 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     c->pcgMaxIter = pcgMaxIter;
     c->pcgHint = 0;
     c->amgSkip = 0;
+    c->amgFailStreak = 0;
     c->elemFp64 = false;
     c->sysPrecision = c->precision;
     // Optimizer constructor: exit(-1) on an inverted initial map (Optimizer.cpp:65-67)
@@ -2409,6 +2459,8 @@ int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, int maxInterval,
     hipStream_t s = c->stream;
     long nFlips = 0;
     ++c->airSinceRemesh;
+    const auto tu0 = std::chrono::steady_clock::now();
+    c->tSetup[6] = 0.0;
     if (mode != OC_SCAFFOLD_REFRESH) {
         std::vector<double> P(2 * (size_t)c->nVair);
         oc::launch_uv_gather(s, c->nVair, c->dAirL2G.as<int>(), c->dUV.as<double>(), nullptr,
@@ -2421,6 +2473,7 @@ int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, int maxInterval,
             c->airLimited = false;
             c->airSinceRemesh = 0;
             nFlips = oc::lawson_flips(P, c->FairLocal, 64L * c->nFair + 1024);
+            c->tSetup[6] = ms_since(tu0);
             if (nFlips > 0) {
                 for (size_t k = 0; k < c->FairLocal.size(); ++k) c->FairGlobal[k] = c->airL2G[c->FairLocal[k]];
                 if (int st = upload_tri_ids(c, c->nF, c->nFair, c->FairGlobal.data())) return st;
@@ -2450,6 +2503,14 @@ int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, int maxInterval,
     if (int st = system_energy(c, &c->lastE, &c->lastESD, &c->lastEScaf)) return st;
     if (E) *E = c->lastE;
     if (flips) *flips = (int)nFlips;
+    c->tSetup[7] = ms_since(tu0);
+    return OC_OK;
+}
+
+int oc_setup_times(oc_ctx* c, double* ms)
+{
+    if (!c || !ms) return OC_ERR_ARG;
+    std::copy(c->tSetup, c->tSetup + 8, ms);
     return OC_OK;
 }
 
@@ -2554,6 +2615,14 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
         S->pcg_fallback = 3;
         st = OC_OK;
     }
+    if (c->ext && st == OC_ERR_NOT_CONVERGED && it > 0 && c->nPrecond == OC_PRECOND_AMG) {
+        // Multigrid-PCG at its iteration cap without a breakdown: inexact (truncated) Newton with the iterate so
+        // far, which minimises the quadratic model over the Krylov space and so is a descent direction. Retrying
+        // with a safer hierarchy or Jacobi-PCG only spends time on systems the cap already judged hard.
+        S->pcg_iters_first = it;
+        S->pcg_fallback = 6;
+        st = OC_OK;
+    }
     if (c->ext && st != OC_OK) {
         // The preconditioned solve broke down or hit its cap. With the
         // learned factor in the mixed form the likely cause is the matrix
@@ -2570,14 +2639,20 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
         // back to Jacobi-PCG at the same tolerance.
         S->pcg_iters_first = it;
         S->pcg_fallback = 2;
-        if (system_f32(c) && c->nPrecond == OC_PRECOND_LEARNED) {
+        if (system_f32(c)) {
             c->elemFp64 = true;
             system_element_pass(c, system_set(c));
             system_assemble(c);
-            c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
+            if (c->nPrecond == OC_PRECOND_LEARNED) {
+                c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
+            } else if (c->amgValid) {
+                amg_numeric(c, 0);
+                ++S->precond_setups;
+            }
             int it2 = 0;
             st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, c->ext);
             it += it2;
+            if (st == OC_ERR_NOT_CONVERGED && it2 > 0 && c->nPrecond == OC_PRECOND_AMG) st = OC_OK;
             if (st == OC_OK) S->pcg_fallback = 4;
         }
         if (st != OC_OK && c->amgValid) {
@@ -2590,12 +2665,14 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
         }
         if (st != OC_OK) {
             st = jacobi_direction(c, S, it, rr);
-            // multigrid that failed twice on this matrix is skipped for the next kAmgSkip steps (retried
-            // afterwards): on maps whose Newton matrices it cannot precondition (the Tutte map of a cut genus-3
-            // surface squeezes triangles to 1e-4 of the mean area) its two attempts would be paid on every step
-            if (c->amgValid) c->amgSkip = kAmgSkip;
+            // multigrid whose attempts ended in Jacobi-PCG on two consecutive steps is skipped for the next
+            // kAmgSkip steps (retried afterwards): on maps whose Newton matrices it cannot precondition (the Tutte
+            // map of a cut genus-3 surface squeezes triangles to 1e-4 of the mean area) its attempts would be paid
+            // on every step; a single failure (one hard step of an otherwise easy solve) skips nothing
+            if (c->amgValid && ++c->amgFailStreak >= 2) c->amgSkip = kAmgSkip;
         }
     }
+    if (c->nPrecond == OC_PRECOND_AMG && S->pcg_fallback != 2) c->amgFailStreak = 0;
     S->pcg_iters = it;
     S->pcg_rel_res = rr;
     if (st != OC_OK && st != OC_ERR_NOT_CONVERGED) return st;
@@ -2818,9 +2895,12 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     // one read-back; more halvings than that continue on the host below
     constexpr int K = oc::kLSCand;
     unsigned long long* om = c->dOrdMin.as<unsigned long long>();
-    unsigned long long* omAir = c->nFair ? om + 1 : nullptr;  // the air triangles' own bound
+    unsigned long long* omAir = c->nFair ? om + 1 : nullptr;  // the air and the mesh triangles' own bounds
     oc::launch_set_u64(s, om, oc::dbl_to_ordered_host(1.0));
-    if (omAir) oc::launch_set_u64(s, omAir, oc::dbl_to_ordered_host(1.0));
+    if (omAir) {
+        oc::launch_set_u64(s, omAir, oc::dbl_to_ordered_host(1.0));
+        oc::launch_set_u64(s, omAir + 1, oc::dbl_to_ordered_host(1.0));
+    }
     oc::launch_step_bound(s, T, c->dUV.as<double>(), dir, om, 1.0, omAir);
     const double* esum = nullptr;
     if (c->nFair) {  // the scaffold changed: the energy is recomputed (Optimizer.cpp:588-592)
@@ -2846,8 +2926,9 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
         lastEScafStart = c->lastEScaf = D->lastEScaf;
     }
     const double stepSize = D->bound * 0.99;
-    // an air triangle set the injectivity bound (below 1): the air mesh held the step back
-    c->airLimited = c->nFair > 0 && D->bound < 1.0 && D->airBound <= D->bound;
+    // the air mesh held the step back: its triangles' bound is below kAirLimitRatio of what the mesh alone
+    // allows (capped at a full step)
+    c->airLimited = c->nFair > 0 && D->airBound < kAirLimitRatio * std::min(1.0, D->meshBound);
     int k0 = 0;  // first index of the candidates held in cand
     auto fetch = [&](int kstart) -> int {
         oc::launch_linesearch(s, T, c->dUV.as<double>(), dir, stepSize, kstart, c->dPartials.as<double>(),

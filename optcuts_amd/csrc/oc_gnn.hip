@@ -197,12 +197,10 @@ __global__ void __launch_bounds__(kBlock) k_gnn_edge_prep(int E, const int* __re
     const double h00 = l00 * pw, h01 = l00 * qw + l01 * rw;
     const double h10 = l10 * pw, h11 = l10 * qw + l11 * rw;
     Ahat[e] = make_float4((float)h00, (float)h01, (float)h10, (float)h11);
-    float4* o = reinterpret_cast<float4*>(ef + (size_t)e * kGnnNE);
+    float4* o = reinterpret_cast<float4*>(ef + (size_t)e * kGnnNEs);
     o[0] = make_float4((float)h00, (float)h01, (float)h10, (float)h11);
     o[1] = make_float4((float)sqrt(h00 * h00 + h01 * h01 + h10 * h10 + h11 * h11), w < u ? 1.0f : 0.0f,
                        w > u ? 1.0f : 0.0f, 0.0f);
-    o[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    o[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------------------- forward layers (MFMA)
@@ -228,7 +226,7 @@ __global__ void __launch_bounds__(kBlock) k_gnn_message(int E, const int* __rest
             a[1] = pack8(hu + 16 + 8 * half);
             a[2] = pack8(hw + 8 * half);
             a[3] = pack8(hw + 16 + 8 * half);
-            a[4] = pack8(ef + (size_t)r * kGnnNE + 8 * half);
+            a[4] = half ? zero8() : pack8(ef + (size_t)r * kGnnNEs);  // inputs 72..79 are zero
         } else {
 #pragma unroll
             for (int s = 0; s < 5; ++s) a[s] = zero8();
@@ -323,13 +321,13 @@ __global__ void __launch_bounds__(kBlock) k_gnn_heads_edge(int E, const int* __r
     for (int i = 0; i < 4; ++i) o[i] = theta[kGnnOffBo + i];
     const float* hu = h + (size_t)u * kGnnH;
     const float* hw = h + (size_t)w * kGnnH;
-    const float* er = ef + (size_t)e * kGnnNE;
+    const float* er = ef + (size_t)e * kGnnNEs;
     for (int k = 0; k < kGnnH; ++k) {
         const float xu = hu[k], xw = hw[k];
 #pragma unroll
         for (int i = 0; i < 4; ++i) o[i] += Wo[i * kGnnKM + k] * xu + Wo[i * kGnnKM + kGnnH + k] * xw;
     }
-    for (int k = 0; k < kGnnNE; ++k) {
+    for (int k = 0; k < kGnnNEs; ++k) {
         const float x = er[k];
 #pragma unroll
         for (int i = 0; i < 4; ++i) o[i] += Wo[i * kGnnKM + 2 * kGnnH + k] * x;
@@ -573,7 +571,9 @@ __global__ void __launch_bounds__(512) k_gnn_heads_wgrad(int E, int nV, const in
             float x = 1.0f;
             if (k >= 0)
                 x = k < kGnnH ? h[(size_t)u * kGnnH + k]
-                              : (k < 2 * kGnnH ? h[(size_t)w * kGnnH + k - kGnnH] : ef[(size_t)e * kGnnNE + k - 2 * kGnnH]);
+                              : (k < 2 * kGnnH ? h[(size_t)w * kGnnH + k - kGnnH]
+                                               : (k < 2 * kGnnH + kGnnNEs ? ef[(size_t)e * kGnnNEs + k - 2 * kGnnH]
+                                                                          : 0.0f));
             acc += di * x;
         }
     } else {
@@ -751,7 +751,7 @@ __global__ void __launch_bounds__(kBlock) k_gnn_message_bwd(int E, const int* __
                 if (e < E) {
                     x0 = h[(size_t)eu[e] * kGnnH + n];
                     x1 = h[(size_t)ew[e] * kGnnH + n];
-                    x2 = n < kGnnNE ? ef[e * kGnnNE + n] : 0.0f;
+                    x2 = n < kGnnNEs ? ef[e * kGnnNEs + n] : 0.0f;
                 }
                 b0[j] = to_bf16(x0);
                 b1[j] = to_bf16(x1);
@@ -967,7 +967,7 @@ int Gnn::set_pattern(hipStream_t s, int nVin, const std::vector<int>& rowPtr, co
     if (!bad) bad |= hipMemcpyAsync(dFixed, fixedMask.data(), n, hipMemcpyHostToDevice, s) != hipSuccess;
     const size_t nh = (size_t)n * kGnnH, mh = (size_t)std::max(nE, 1) * kGnnH;
     bad |= dalloc(dCinv, 4 * (size_t)n) | dalloc(dLogTr, n) | dalloc(dX, 4 * (size_t)n) |
-           dalloc(dAhat, 4 * (size_t)std::max(nE, 1)) | dalloc(dEf, (size_t)std::max(nE, 1) * kGnnNE);
+           dalloc(dAhat, 4 * (size_t)std::max(nE, 1)) | dalloc(dEf, (size_t)std::max(nE, 1) * kGnnNEs);
     for (int l = 0; l <= kGnnLayers; ++l) bad |= dalloc(dH[l], nh);
     for (int l = 0; l < kGnnLayers; ++l) bad |= dalloc(dA[l], nh) | dalloc(dR[l], nh) | dalloc(dM[l], mh);
     bad |= dalloc(dGo, 4 * (size_t)std::max(nE, 1)) | dalloc(dGd, 4 * (size_t)n);

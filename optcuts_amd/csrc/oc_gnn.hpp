@@ -31,7 +31,8 @@ namespace oc {
 
 constexpr int kGnnH = 32;
 constexpr int kGnnNX = 4;
-constexpr int kGnnNE = 16;
+constexpr int kGnnNE = 16;   // edge-feature inputs of the message and head layers (columns of W_m, W_o)
+constexpr int kGnnNEs = 8;   // edge features stored per edge: the 7 defined + 1 zero; inputs kGnnNEs.. are zero
 constexpr int kGnnKM = 2 * kGnnH + kGnnNE;  // 80
 constexpr int kGnnKH = 2 * kGnnH;           // 64
 constexpr int kGnnLayers = 3;

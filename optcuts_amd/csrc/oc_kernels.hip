@@ -591,9 +591,11 @@ __global__ void __launch_bounds__(kBlock) k_step_bound(TriSet T, const double2* 
     // NaN bounds never lower the step (the reference's 'bound < stepSize' is false)
     bound = bound != bound ? cap : bound;
     if (ordMinAir) {
-        // the air triangles' own minimum as well: whether the air mesh set the step
+        // the air triangles' and the mesh triangles' own minima as well: how far the air mesh held the step back
         const double b = wave_min((t < T.nF && t >= T.nFmesh) ? bound : cap);
         if ((threadIdx.x % kWave) == 0 && b < cap) atomicMin(ordMinAir, dbl_to_ordered(b));
+        const double m = wave_min(t < T.nFmesh ? bound : cap);
+        if ((threadIdx.x % kWave) == 0 && m < cap) atomicMin(ordMinAir + 1, dbl_to_ordered(m));
     }
     bound = wave_min(bound);
     if ((threadIdx.x % kWave) == 0 && bound < cap) atomicMin(ordMin, dbl_to_ordered(bound));
@@ -725,7 +727,8 @@ __global__ void k_ls_decide(const double* __restrict__ cand, const unsigned long
         r.lastE = lastEHost;
     }
     r.bound = ordered_to_dbl(*boundOrd);
-    r.airBound = airOrd ? ordered_to_dbl(*airOrd) : r.bound;
+    r.airBound = airOrd ? ordered_to_dbl(airOrd[0]) : r.bound;
+    r.meshBound = airOrd ? ordered_to_dbl(airOrd[1]) : r.bound;
     double alpha = r.bound * 0.99, alphaCommit = alpha;
     int k = 0, halv = 0, status = 0;
     double testE = cand[0];

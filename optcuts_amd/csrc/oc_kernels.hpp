@@ -103,13 +103,15 @@ struct LsDecision {
     double lastE, lastESD, lastEScaf;  // the energy the candidates were compared with
     double bound;        // the injectivity bound (before the 0.99 factor)
     double airBound;     // its minimum over the air triangles alone (cap when none)
+    double meshBound;    // and over the mesh triangles alone
     int halvings;
     int status;  // 0 accepted, 1 more than kLSCand halvings needed (nothing committed), 2 alpha underflowed to 0
     int pad[2];
 };
 // Applies Optimizer::lineSearch's rule (:597-629) to the candidates `cand` of a launch_linesearch from boundOrd
 // and, unless more candidates are needed, commits UV += alphaCommit d (n doubles). lastE: lastEHost, or with
-// esum non-null sMesh esum[0] + sAir esum[1]. airOrd (may be null): the air triangles' bound, reported.
+// esum non-null sMesh esum[0] + sAir esum[1]. airOrd (may be null): the air triangles' and the mesh triangles'
+// bounds (two words), reported.
 void launch_ls_decide_commit(hipStream_t s, const double* cand, const unsigned long long* boundOrd,
                              const unsigned long long* airOrd, const double* esum, double lastEHost, double sMesh,
                              double sAir, LsDecision* dec, long n, const double* d, double* UV);

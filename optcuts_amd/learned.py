@@ -71,6 +71,32 @@ def collect_systems(meshes, iters, precond=capi.PRECOND_AMG, pcg_tol=1e-4, devic
     return out
 
 
+def collect_closed_systems(names, iters, device=0, pcg_tol=1e-3, pcg_max_iter=100):
+    """Newton systems (mesh + air rows) of `iters` scaffolded Newton iterations of each closed workload
+    (optcuts_amd.workloads: cut to a disc, Tutte map, joint-mode a0, first air mesh), the chosen form's solve,
+    the air mesh updated between steps as the bench does."""
+    from . import workloads as W
+    out = []
+    ctx = capi.Context(device)
+    try:
+        for name in names:
+            w = W.prepare_closed(ctx, name)
+            ctx.newton_init(a0=w["a0"], rel_gl2_tol=1e-12, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                            pcg_rel_tol=pcg_tol, pcg_max_iter=pcg_max_iter)
+            for _ in range(iters):
+                s, done = ctx.newton_step()
+                if s["converged"]:
+                    break
+                rp, ci, va, b, fm = ctx.get_system()
+                out.append((rp, ci, va.copy(), fm))
+                if done:
+                    break
+                ctx.scaffold_update(capi.SCAFFOLD_REMESH_WHEN_NEEDED, 0.0, 2)
+    finally:
+        ctx.close()
+    return out
+
+
 def disc_meshes(grids, seeds):
     return [height_field_disc(m, seed=s) for m in grids for s in seeds]
 
@@ -146,14 +172,33 @@ def main():
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--probes", type=int, default=4)
+    ap.add_argument("--closed", default="",
+                    help="closed workloads (scaffolded joint-mode systems) to add to the disc systems, e.g. "
+                         "icosphere_nu30,tori_h0.06 (measured: lower loss, more PCG iterations on held-out closed "
+                         "systems, so the committed weights use discs only)")
+    ap.add_argument("--closed-iters", type=int, default=16)
+    ap.add_argument("--held-out", default="icosphere_nu38",
+                    help="closed workloads whose systems only evaluate the result")
     args = ap.parse_args()
     t0 = time.time()
     meshes = disc_meshes([int(x) for x in args.grids.split(",")], [int(x) for x in args.seeds.split(",")])
     systems = collect_systems(meshes, args.iters)
-    print("collected %d systems in %.1f s" % (len(systems), time.time() - t0), flush=True)
+    n_disc = len(systems)
+    if args.closed:
+        systems += collect_closed_systems(args.closed.split(","), args.closed_iters)
+    print("collected %d systems (%d disc, %d closed) in %.1f s" % (len(systems), n_disc, len(systems) - n_disc,
+                                                                   time.time() - t0), flush=True)
+    held = collect_closed_systems(args.held_out.split(","), 8) if args.held_out else []
     net = capi.Gnn(0)
+    rng = np.random.default_rng(0)
+    rhs = [rng.normal(size=2 * (len(s[0]) - 1)) for s in held]
     net.set_params(init_params(0))
+    base = [pcg_iterations(net, s, b, tol=1e-6, maxit=5000) for s, b in zip(held, rhs)]
     train(net, systems, args.epochs, lr=args.lr, probes=args.probes, log=lambda m: print(m, flush=True))
+    if held:
+        got = [pcg_iterations(net, s, b, tol=1e-6, maxit=5000) for s, b in zip(held, rhs)]
+        print("held-out closed systems (%s), PCG iterations to 1e-6: untrained (block Jacobi) %s, trained %s; "
+              "sum %d -> %d" % (args.held_out, base, got, sum(base), sum(got)), flush=True)
     save(net.get_params(), args.out)
     print("saved %s (%.1f s)" % (args.out, time.time() - t0), flush=True)
 

@@ -45,7 +45,14 @@ void build(int nV, int nF, const int* F, int nVt, int nFa, const int* Fa, int nF
     if (prev)
         for (const auto& l : prev->L) ap.levels.push_back(&l);
     if (storage) B.L = std::move(*storage);
-    if (oc::amg_build_structure(nVt, B.P.rowPtr, B.P.colIdx, fixed, 64, 10, B.L, err, prev ? &ap : nullptr) != 0)
+    // as the library does: the vertices of air triangles are aggregated last
+    std::vector<unsigned char> late;
+    if (nFa) {
+        late.assign(nVt, 0);
+        for (long k = 0; k < 3L * nFa; ++k) late[Fa[k]] = 1;
+    }
+    if (oc::amg_build_structure(nVt, B.P.rowPtr, B.P.colIdx, fixed, 64, 10, B.L, err, prev ? &ap : nullptr,
+                                late.empty() ? nullptr : &late) != 0)
         B.L.clear();
 }
 

@@ -165,6 +165,32 @@ def test_training_beats_untrained_network():
         net.close()
 
 
+def test_training_on_closed_surface_systems():
+    """Training on the scaffolded joint-mode systems of cut closed surfaces (mesh + air rows, genus 0 and 3) and
+    evaluating on systems of other resolutions. Measured: the loss falls by three orders of magnitude, yet the
+    trained factor needs more PCG iterations than block Jacobi on the held-out icosphere systems (482 -> 1051 on the
+    first), and on the genus-3 systems of the first Tutte steps neither reaches 1e-6 in 5,000 iterations: the
+    Frobenius objective does not track CG convergence on these spectra. The test keeps what must hold: the loss
+    decreases and the trained factor is still a usable (positive definite) preconditioner where block Jacobi
+    converges."""
+    train_sys = learned.collect_closed_systems(["icosphere_nu12", "tori_h0.12"], iters=8)
+    test_sys = learned.collect_closed_systems(["icosphere_nu16"], iters=4)
+    net = capi.Gnn(0)
+    try:
+        net.set_params(learned.init_params(0))
+        rng = np.random.default_rng(1)
+        rhs = [rng.normal(size=2 * (len(s[0]) - 1)) for s in test_sys]
+        base = [learned.pcg_iterations(net, s, b, tol=1e-6, maxit=5000) for s, b in zip(test_sys, rhs)]
+        hist = learned.train(net, train_sys, epochs=15, lr=2e-3, probes=4)
+        trained = [learned.pcg_iterations(net, s, b, tol=1e-6, maxit=5000) for s, b in zip(test_sys, rhs)]
+        print("closed: train loss first/last", hist[0], hist[-1], flush=True)
+        print("closed: PCG iterations untrained", base, "trained", trained, flush=True)
+        assert hist[-1] < 0.1 * hist[0]
+        assert max(base) < 5000 and max(trained) < 5000
+    finally:
+        net.close()
+
+
 def test_newton_learned_matches_reference_trace():
     """The learned preconditioner changes only how PCG converges: to a tight
     tolerance the Newton trajectory matches the oracle's LDL^T run to 1e-9."""
