@@ -44,12 +44,46 @@ def lanczos0_bytes(nnzb, nV):
     return 36 * nnzb + 4 * (nV + 1) + 100 * nV
 
 
+def cheb_bytes(n, blocks, bs, word=8):
+    """Algorithmic bytes of one V-cycle Chebyshev step (k_cheb_step) on a level of n rows and `blocks` stored
+    bs x bs blocks: the blocks and their column indices, row pointers, per row the inverse diagonal block, the
+    gathered direction (read once), the residual and the iterate read and written, the new direction written."""
+    return blocks * (bs * bs * word + 4) + 4 * (n + 1) + n * (bs * bs * word + 6 * bs * word)
+
+
+def cheb_levels(ctx, reps=50):
+    """Per multigrid level with a transfer: rows, blocks, block size, live HIP-event time of one Chebyshev step,
+    algorithmic bytes and the rate; and the aggregate over the coarse (3x3-block) levels, whose launches the
+    V-cycle makes equally often (two per level per cycle)."""
+    levels = ctx.amg_levels()
+    out = []
+    for l, (dofs, b, bs) in enumerate(levels[:-1]):
+        n = dofs // bs
+        ms = ctx.time_kernel(10 + l, reps=reps)
+        nb = cheb_bytes(n, b, bs)
+        out.append({"level": l, "nodes": n, "blocks": b, "block_size": bs, "avg_launch_ms": ms,
+                    "algorithmic_bytes": nb, "GB_per_s": nb / (ms * 1e-3) / 1e9,
+                    "frac": nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    coarse = [x for x in out if x["block_size"] == 3]
+    agg = None
+    if coarse:
+        t = sum(x["avg_launch_ms"] for x in coarse)
+        nb = sum(x["algorithmic_bytes"] for x in coarse)
+        agg = {"avg_launch_ms": t / len(coarse), "algorithmic_bytes_per_launch": nb / len(coarse),
+               "achieved": nb / (t * 1e-3) / 1e9}
+    return out, agg
+
+
 # HBM traffic per launch of the roofline kernels, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per
 # dispatch, mean over full dispatches), FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0: the factors that make the
-# same counters read the known size of a plain 512 MiB device copy. Summaries with commands: profiles/.
+# same counters read the known size of a plain 512 MiB device copy. Summaries with commands:
+# profiles/pmc_traffic_200k.md (disc) and profiles/pmc_traffic_tori_200k.md (fused tori).
 TRAFFIC = {
     ("disc_200k", "k_lanczos_fine"): (2.0 * 17072.8 + 3188.8) * 1024,
     ("disc_200k", "k_pcg_spmv"): (2.0 * 14334.2 + 3209.3) * 1024,
+    # three fused tori, chosen form: mean over multigrid levels 1-3 of the Chebyshev step; the PCG product
+    ("tori_200k", "k_cheb_step3"): ((2.0 * 6745.6 + 840.5) + (2.0 * 654.8 + 50.4) + (2.0 * 60.7 + 2.3)) / 3.0 * 1024,
+    ("tori_200k", "k_pcg_spmv"): (2.0 * 21640.6 + 3458.0) * 1024,
 }
 
 FALLBACK_NAMES = {1: "retry_safe_amg", 2: "jacobi", 3: "truncated", 4: "retry_fp64_elements", 5: "jacobi_direct",
@@ -124,6 +158,10 @@ def closed_run(ctx, w, form, args, steps, warmup, e_star=None, budget=None):
         s, done, ts, tu, nflip = closed_step(ctx, args)
         k += 1
         stats.append(s)
+        if getattr(args, "progress", False):
+            print("bench: step %d E_SD %.9g pcg %d fallback %d flips %d" % (k, s["E_SD"], s["pcg_iters"],
+                                                                          s["pcg_fallback"], nflip),
+                  file=sys.stderr, flush=True)
         rows.append((ts, tu, s["E_SD"], s["pcg_iters"], nflip))
         el = time.perf_counter() - t0
         if e_star is not None and hit is None and s["E_SD"] <= (1.0 + 1e-6) * e_star:
@@ -293,6 +331,8 @@ def disc_bench(ctx, args, targets):
             r["kernel_ms"]["precond_setup"] = ctx.time_kernel(8, reps=5)
         if precond == capi.PRECOND_AMG:
             r["kernel_ms"]["lanczos_fine_step"] = ctx.time_kernel(9, reps=50)
+            if precision == capi.PRECISION_FP64:
+                r["cheb_step_per_level"] = cheb_levels(ctx)[0]
         if e_star is not None and not args.no_tte:
             r["time_to_E_SD_1e-6"] = time_to_target(ctx, V, F, UV, tol, precond, precision, e_star, args.tte_max_iter)
         res[name] = r
@@ -425,10 +465,19 @@ def headline(ctx, args, targets, fast):
     spmv_ms = ctx.time_kernel(5, reps=50)
     nbytes = spmv_bytes(nnzb, nVt)
     achieved = nbytes / (spmv_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_pcg_spmv (PCG product on the 2x2-block Newton matrix, mesh + air)",
-                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": TRAFFIC.get((args.workload, "k_pcg_spmv")), "algorithmic_bytes_per_launch": nbytes,
-                "avg_launch_ms": spmv_ms, "system": {"triangles": nF, "vertices": nVt, "blocks": nnzb}}
+    spmv = {"bound": "hbm", "kernel": "k_pcg_spmv (PCG product on the 2x2-block Newton matrix, mesh + air)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": TRAFFIC.get((args.workload, "k_pcg_spmv")), "algorithmic_bytes_per_launch": nbytes,
+            "avg_launch_ms": spmv_ms, "system": {"triangles": nF, "vertices": nVt, "blocks": nnzb}}
+    # the dominant kernel of the chosen form on this workload: the V-cycle's Chebyshev step on the coarse
+    # (3x3-block) levels; per level and over those levels (each launched twice per V-cycle)
+    per_level, agg = cheb_levels(ctx)
+    roofline = {"bound": "hbm", "kernel": "k_cheb_step<double,3,false> (V-cycle Chebyshev step, coarse levels; "
+                                          "aggregate over the levels, which the V-cycle launches equally often)",
+                "achieved": agg["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": agg["achieved"] / HBM_PEAK_GBS, "traffic": TRAFFIC.get((args.workload, "k_cheb_step3")),
+                "algorithmic_bytes_per_launch": agg["algorithmic_bytes_per_launch"],
+                "avg_launch_ms": agg["avg_launch_ms"], "per_level": per_level, "pcg_spmv": spmv}
     return head, w, roofline
 
 
@@ -462,6 +511,7 @@ def main():
     ap.add_argument("--no-prescreen", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch PCG iterations kernel by kernel")
+    ap.add_argument("--progress", action="store_true", help="one stderr line per closed-surface Newton step")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--stream-probe", action="store_true",

@@ -362,9 +362,14 @@ int oc_charts_state(oc_ctx* ctx, double* E_SD, double* gradSq, double* step, int
  * projected H), 1 block assembly, 2 one PCG iteration (both kernels), 3 line-search candidates, 4 step bound,
  * 5 PCG SpMV kernel, 6 PCG update kernel, 7 one application of the multigrid or learned preconditioner, 8 its
  * set-up (multigrid hierarchy or network forward pass), 9 one fine-level Lanczos step of the multigrid set-up
- * (the lambda_max estimate). Needs oc_newton_init (7-9 with that preconditioner and one oc_newton_step); the
- * current search direction is kept. */
+ * (the lambda_max estimate), 10 + l one Chebyshev smoothing step of the V-cycle on multigrid level l (a level
+ * with a transfer, l < levels - 1). Needs oc_newton_init (7-9 and 10+ with that preconditioner and one
+ * oc_newton_step); the current search direction is kept. */
 int oc_time_kernel(oc_ctx* ctx, int which, int reps, double* ms);
+/* The multigrid hierarchy of the last Newton system: *nLevels, and per level (up to capacity) its unknowns (nodes
+ * times block size), stored blocks and block size (2 on the finest level, 3 below). *nLevels = 0 without multigrid. No reference
+ * counterpart (diagnostics for the per-level roofline). */
+int oc_amg_levels(oc_ctx* ctx, int capacity, int* nLevels, long long* rows, long long* blocks, int* blockSize);
 /* Sizes used for algorithmic-byte accounting: triangles, vertices (combined), stored blocks, upper blocks. */
 int oc_system_sizes(oc_ctx* ctx, long long* nF, long long* nV, long long* nnzBlocks, long long* nUpperBlocks);
 /* Copies the Newton system of the last oc_newton_step to the host (any pointer may be null): the full

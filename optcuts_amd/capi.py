@@ -84,6 +84,7 @@ _SIGS = {
                                      C.POINTER(C.c_int)]),
     "oc_air_mesh_get": (C.c_int, [_ctx, _vp, _vp]),
     "oc_setup_times": (C.c_int, [_ctx, _dp]),
+    "oc_amg_levels": (C.c_int, [_ctx, C.c_int, C.POINTER(C.c_int), _vp, _vp, _vp]),
     "oc_solver_set_pattern": (C.c_int, [_ctx, C.c_int, _ip, _ip, C.c_int, _ip, C.POINTER(C.c_int)]),
     "oc_solver_get_pattern": (C.c_int, [_ctx, _ip, _ip]),
     "oc_solver_update_a": (C.c_int, [_ctx, C.c_longlong, _ip, _ip, _dp]),
@@ -361,6 +362,17 @@ class Context:
         self._chk(self.L.oc_scaffold_update(self.h, int(mode), float(min_angle_deg), int(max_interval), C.byref(e),
                                             C.byref(n)))
         return e.value, n.value
+
+    def amg_levels(self):
+        """[(unknowns, blocks, block size)] per multigrid level of the last Newton system ([] without multigrid)."""
+        cap = 32
+        n = C.c_int()
+        rows = np.zeros(cap, np.int64)
+        blocks = np.zeros(cap, np.int64)
+        bs = np.zeros(cap, np.int32)
+        self._chk(self.L.oc_amg_levels(self.h, cap, C.byref(n), rows.ctypes.data, blocks.ctypes.data,
+                                       bs.ctypes.data))
+        return [(int(rows[l]), int(blocks[l]), int(bs[l])) for l in range(min(n.value, cap))]
 
     def setup_times(self):
         """Host ms of the last set-up phases (vertex lists, pattern+maps, upload, multigrid structure, learned

@@ -232,7 +232,13 @@ int row_grid(int n, int bs)
 // meshes and needs all kLanczosSteps; coarse levels settle sooner. At 200k
 // faces 8 coarse steps lose nothing and 6 under-estimate often enough to force
 // a safe-mode retry; 10 keeps a margin.
-int lanczos_steps(int level) { return level == 0 ? kLanczosSteps : kLanczosStepsCoarse; }
+#ifndef OC_LANCZOS_FINE
+#define OC_LANCZOS_FINE kLanczosSteps
+#endif
+#ifndef OC_LANCZOS_COARSE
+#define OC_LANCZOS_COARSE kLanczosStepsCoarse
+#endif
+int lanczos_steps(int level) { return level == 0 ? OC_LANCZOS_FINE : OC_LANCZOS_COARSE; }
 
 // Workgroup b is dispatched to XCD b % 8; chunk c of the rows goes to
 // workgroup (c % (G/8)) * 8 + c / (G/8), so each XCD owns one contiguous
@@ -1130,12 +1136,13 @@ __global__ void __launch_bounds__(kBlock) k_cheb_post1(const int* __restrict__ s
     }
 }
 
-// bc_c = sum over P blocks (u,c) of P^T res_u, kRestrictLanes lanes per coarse node.
+// bc_c = sum over P blocks (u,c) of P^T res_u, LN lanes per coarse node (the launcher picks LN from the mean
+// length of the restriction lists: on the small coarse levels a node gathers ~100 P blocks, and with few nodes
+// the kernel is one latency chain per lane, so the lanes per node grow with the list).
 // PRE1: the next level's first pre-smoothing step (k_cheb_pre1 from x = 0:
 // r = D^-1 b, d = r/theta, x = d) is row-local, so it runs here on each
 // restricted row instead of in a launch of its own.
-constexpr int kRestrictLanes = 16;
-template <typename T, int BS, bool PRE1>
+template <typename T, int BS, bool PRE1, int LN>
 __global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ stop, int nc, const int* __restrict__ rPtr,
                                                      const int* __restrict__ rP, const int* __restrict__ pRow,
                                                      const T* __restrict__ P, const T* __restrict__ res, T* bc,
@@ -1145,7 +1152,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ sto
     if (stop && *stop) return;
     int c0, c1;
     chunk_rows(nc, c0, c1);
-    constexpr int LN = kRestrictLanes, RPP = kBlock / kRestrictLanes;
+    constexpr int RPP = kBlock / LN;
     const int sub = threadIdx.x / LN, lane = threadIdx.x % LN;
     for (int c = c0 + sub; c < c1; c += RPP) {
         double s[NS] = {};
@@ -1188,25 +1195,36 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ sto
     }
 }
 
-// x_u += sum over P blocks (u,c) of P xc_c
+// x_u += sum over P blocks (u,c) of P xc_c, kProlongLanes lanes per fine row (one P block each per pass, the
+// partial sums combined by shuffles): the few blocks of a row are loaded together instead of one after another
+constexpr int kProlongLanes = 4;
 template <typename T, int BS>
 __global__ void __launch_bounds__(kBlock) k_prolong(const int* __restrict__ stop, int n, const int* __restrict__ pPtr, const int* __restrict__ pCol,
                                                     const T* __restrict__ P, const T* __restrict__ xc, T* x)
 {
     if (stop && *stop) return;
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n) return;
+    constexpr int LN = kProlongLanes;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int u = t / LN, lane = t % LN;
     double s[BS] = {};
-    for (int j = pPtr[u]; j < pPtr[u + 1]; ++j) {
-        const T* p = P + (size_t)j * BS * NS;
-        const T* xcv = xc + (size_t)pCol[j] * NS;
+    if (u < n) {
+        for (int j = pPtr[u] + lane; j < pPtr[u + 1]; j += LN) {
+            const T* p = P + (size_t)j * BS * NS;
+            const T* xcv = xc + (size_t)pCol[j] * NS;
 #pragma unroll
-        for (int i = 0; i < BS; ++i)
+            for (int i = 0; i < BS; ++i)
 #pragma unroll
-            for (int c = 0; c < NS; ++c) s[i] += (double)p[i * NS + c] * (double)xcv[c];
+                for (int c = 0; c < NS; ++c) s[i] += (double)p[i * NS + c] * (double)xcv[c];
+        }
     }
 #pragma unroll
-    for (int i = 0; i < BS; ++i) x[(size_t)u * BS + i] = (T)((double)x[(size_t)u * BS + i] + s[i]);
+    for (int off = LN / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < BS; ++i) s[i] += __shfl_xor(s[i], off, LN);
+    if (u < n && lane == 0) {
+#pragma unroll
+        for (int i = 0; i < BS; ++i) x[(size_t)u * BS + i] = (T)((double)x[(size_t)u * BS + i] + s[i]);
+    }
 }
 
 template <typename T>
@@ -1382,6 +1400,24 @@ void Amg<T>::lanczos0_step(hipStream_t s)
     LanczosBufs lb{{A.lz, A.lz + nv, A.lz + 2 * nv}, {A.lz + 3 * nv, A.lz + 4 * nv, A.lz + 5 * nv}};
     hipLaunchKernelGGL((k_lanczos<T, 2>), dim3(g), dim3(kBlock), 0, s, 2, kLanczosSteps, A.n, A.dRowPtr, A.dColIdx,
                        A.vals, A.dDiagPos, lb, dPartials + nPartials, g, dPartials, A.lz + 6 * nv);
+}
+
+template <typename T>
+void Amg<T>::cheb_step_level(hipStream_t s, int l)
+{
+    Level& A = *L[l];
+    if (A.bs == 2)
+        hipLaunchKernelGGL((k_cheb_step<T, 2, false>), dim3(row_grid(A.n, 2)), dim3(kBlock), 0, s, nullptr, A.n,
+                           A.dRowPtr, A.dColIdx, A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x, nullptr, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((k_cheb_step<T, 3, false>), dim3(row_grid(A.n, 3)), dim3(kBlock), 0, s, nullptr, A.n,
+                           A.dRowPtr, A.dColIdx, A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x, nullptr, nullptr, nullptr);
+}
+
+template <typename T>
+int Amg<T>::level_block_size(int l) const
+{
+    return L[l]->bs;
 }
 
 template <typename T>
@@ -1602,22 +1638,29 @@ void Amg<T>::restrict_to(hipStream_t s, int l, const int* stop)
     Level& A = *L[l];
     Level& N = *L[l + 1];
     const bool fuse = l + 2 < (int)L.size();
-    const dim3 g(rows_grid(A.nc, kBlock / kRestrictLanes));
+    // lanes per coarse node: the power of two nearest the mean restriction-list length, 16 to 64
+    const long meanLen = A.nc > 0 ? (long)A.rP.size() / A.nc : 0;
+    const int ln = meanLen >= 48 ? 64 : (meanLen >= 24 ? 32 : 16);
+    const dim3 g(rows_grid(A.nc, kBlock / ln));
+    const auto launch = [&](auto kern) {
+        if (fuse)
+            hipLaunchKernelGGL(kern, g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P, A.r, N.b, N.Dinv,
+                               N.lam, N.x, N.r, N.d);
+        else
+            hipLaunchKernelGGL(kern, g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P, A.r, N.b,
+                               nullptr, nullptr, nullptr, nullptr, nullptr);
+    };
+#define OC_RESTRICT(BS_, F_)                                                      \
+    (ln == 64 ? launch(k_restrict<T, BS_, F_, 64>)                                \
+              : (ln == 32 ? launch(k_restrict<T, BS_, F_, 32>) : launch(k_restrict<T, BS_, F_, 16>)))
     if (A.bs == 2) {
-        if (fuse)
-            hipLaunchKernelGGL((k_restrict<T, 2, true>), g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P,
-                               A.r, N.b, N.Dinv, N.lam, N.x, N.r, N.d);
-        else
-            hipLaunchKernelGGL((k_restrict<T, 2, false>), g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P,
-                               A.r, N.b, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (fuse) OC_RESTRICT(2, true);
+        else OC_RESTRICT(2, false);
     } else {
-        if (fuse)
-            hipLaunchKernelGGL((k_restrict<T, 3, true>), g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P,
-                               A.r, N.b, N.Dinv, N.lam, N.x, N.r, N.d);
-        else
-            hipLaunchKernelGGL((k_restrict<T, 3, false>), g, dim3(kBlock), 0, s, stop, A.nc, A.dRPtr, A.dRP, A.dPRow, A.P,
-                               A.r, N.b, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (fuse) OC_RESTRICT(3, true);
+        else OC_RESTRICT(3, false);
     }
+#undef OC_RESTRICT
 }
 
 template <typename T>
@@ -1639,7 +1682,7 @@ void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
                            A.b, A.x, A.r);
         restrict_to(s, l, stop);
         vcycle(s, l + 1, stop);
-        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
+        hipLaunchKernelGGL((k_prolong<T, 2>), dim3(grid_for((long)A.n * kProlongLanes)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
                            N.x, A.x);
         hipLaunchKernelGGL((k_cheb_post1<T, 2>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
                            A.Dinv, A.lam, A.b, A.x, A.x2, A.r, A.d);
@@ -1654,7 +1697,7 @@ void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
                            A.b, A.x, A.r);
         restrict_to(s, l, stop);
         vcycle(s, l + 1, stop);
-        hipLaunchKernelGGL((k_prolong<T, 3>), dim3(grid_for(A.n)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
+        hipLaunchKernelGGL((k_prolong<T, 3>), dim3(grid_for((long)A.n * kProlongLanes)), dim3(kBlock), 0, s, stop, A.n, A.dPPtr, A.dPCol, A.P,
                            N.x, A.x);
         hipLaunchKernelGGL((k_cheb_post1<T, 3>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,
                            A.Dinv, A.lam, A.b, A.x, A.x2, A.r, A.d);

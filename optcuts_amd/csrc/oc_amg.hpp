@@ -55,6 +55,10 @@ public:
     // timing). Call after numeric().
     virtual void lanczos0_prepare(hipStream_t s) = 0;
     virtual void lanczos0_step(hipStream_t s) = 0;
+    // One V-cycle Chebyshev step on level l (a level with a transfer), on that level's own vectors, as the
+    // V-cycle launches it; repeatable (for timing). Call after numeric().
+    virtual void cheb_step_level(hipStream_t s, int l) = 0;
+    virtual int level_block_size(int l) const = 0;
     virtual bool ready() const = 0;
 };
 
@@ -90,6 +94,8 @@ public:
     long level_blocks(int l) const override;
     void lanczos0_prepare(hipStream_t s) override;
     void lanczos0_step(hipStream_t s) override;
+    void cheb_step_level(hipStream_t s, int l) override;
+    int level_block_size(int l) const override;
     bool ready() const override { return ready_; }
 
 private:

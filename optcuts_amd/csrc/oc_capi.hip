@@ -258,6 +258,7 @@ constexpr int kMaxPartialBlocks = 1 << 16;
 constexpr int kRzRecords = 2048;  // r.z partial records of an external preconditioner
 constexpr size_t kMailBytes = 4096;  // pinned read-back words of a context
 constexpr int kAmgSkip = 3;          // steps without multigrid after it failed on a step
+constexpr int kMaxTimedLevels = 16;  // oc_time_kernel 10 + l: Chebyshev step on multigrid level l
 #ifndef OC_AIR_LIMIT_RATIO
 #define OC_AIR_LIMIT_RATIO 0.5
 #endif
@@ -2507,6 +2508,22 @@ int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, int maxInterval,
     return OC_OK;
 }
 
+int oc_amg_levels(oc_ctx* c, int capacity, int* nLevels, long long* rows, long long* blocks, int* blockSize)
+{
+    if (!c || !nLevels || capacity < 0) return OC_ERR_ARG;
+    if (!c->amgValid) {
+        *nLevels = 0;
+        return OC_OK;
+    }
+    *nLevels = c->amg->levels();
+    for (int l = 0; l < *nLevels && l < capacity; ++l) {
+        if (rows) rows[l] = c->amg->level_rows(l);
+        if (blocks) blocks[l] = c->amg->level_blocks(l);
+        if (blockSize) blockSize[l] = c->amg->level_block_size(l);
+    }
+    return OC_OK;
+}
+
 int oc_setup_times(oc_ctx* c, double* ms)
 {
     if (!c || !ms) return OC_ERR_ARG;
@@ -3020,7 +3037,11 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
 int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
 {
     if (!c) return OC_ERR_ARG;
-    OC_ARG(c, ms && reps > 0 && which >= 0 && which <= 9, "oc_time_kernel: bad args");
+    OC_ARG(c, ms && reps > 0 && which >= 0 && which <= 9 + kMaxTimedLevels, "oc_time_kernel: bad args");
+    if (which >= 10 && (!c->amgValid || which - 10 + 1 >= c->amg->levels())) {
+        c->err = "oc_time_kernel: no multigrid level with a transfer at that depth";
+        return OC_ERR_STATE;
+    }
     if ((which == 7 || which == 8) && !c->ext) {
         c->err = "oc_time_kernel: no multigrid or learned preconditioner is active";
         return OC_ERR_STATE;
@@ -3094,6 +3115,9 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
         case 9:
             c->amg->lanczos0_step(s);
             break;
+        default:  // 10 + l: one Chebyshev step of the V-cycle on multigrid level l
+            c->amg->cheb_step_level(s, which - 10);
+            break;
         }
     };
     if (which == 7) OC_HIP(c, c->dTmpVec.ensure(sizeof(double) * 2 * (size_t)c->nVt));
@@ -3130,11 +3154,33 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
     launch();  // warm
     OC_HIP(c, hipStreamSynchronize(s));
     // back-to-back launches between one pair of events: the per-launch
-    // average then excludes the event round trips
+    // average then excludes the event round trips. Single-kernel timings
+    // replay the launches from a captured graph, as the PCG iterations run:
+    // launched one by one from the host, a small coarse-level kernel would
+    // measure the host's launch rate instead of its own duration.
+    hipGraphExec_t gx = nullptr;
+    if (which == 5 || which == 6 || which >= 9) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            for (int r = 0; r < reps; ++r) launch();
+            if (hipStreamEndCapture(s, &g) == hipSuccess && g) {
+                if (hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess) gx = nullptr;
+                (void)hipGraphDestroy(g);
+            }
+        }
+        (void)hipGetLastError();
+        if (gx) {  // one untimed replay
+            OC_HIP(c, hipGraphLaunch(gx, s));
+            OC_HIP(c, hipStreamSynchronize(s));
+        }
+    }
     OC_HIP(c, hipEventRecord(e0, s));
-    for (int r = 0; r < reps; ++r) launch();
+    if (gx) OC_HIP(c, hipGraphLaunch(gx, s));
+    else
+        for (int r = 0; r < reps; ++r) launch();
     OC_HIP(c, hipEventRecord(e1, s));
     OC_HIP(c, hipEventSynchronize(e1));
+    if (gx) (void)hipGraphExecDestroy(gx);
     float t = 0.f;
     OC_HIP(c, hipEventElapsedTime(&t, e0, e1));
     *ms = (double)t / reps;
