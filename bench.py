@@ -185,6 +185,8 @@ def closed_run(ctx, w, form, args, steps, warmup, e_star=None, budget=None):
            "step_ms_max": float(a[:, 0].max() * 1e3), "pcg_iters_max": int(a[:, 3].max()),
            "remesh_phase_ms_median": ({k: float(np.median([t[k] for t in closed_step.remesh_times]))
                                        for k in closed_step.remesh_times[0]} if closed_step.remesh_times else None),
+           "remeshes_keeping_amg_structure": sum(1 for t in closed_step.remesh_times
+                                                 if t.get("amg_kept_remeshes", 0) > 0),
            "pcg_iters_median": float(np.median(a[:, 3])), "pcg_fallbacks": count_fallbacks(stats),
            "E_SD_first": float(a[0, 2]), "E_SD_last": float(a[-1, 2])}
     if e_star is not None:

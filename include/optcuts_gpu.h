@@ -180,9 +180,12 @@ int oc_scaffold_update(oc_ctx* ctx, int mode, double minAngleDeg, int maxInterva
  * ids, after any flips). Either pointer may be null. */
 int oc_air_mesh_get(oc_ctx* ctx, double* airUV, int* Fair);
 /* Host wall times (ms) of the last Newton-system set-up (oc_newton_init, oc_newton_rebuild or a re-meshing
- * oc_scaffold_update): ms[0] vertex lists, [1] pattern and slot maps, [2] their upload, [3] multigrid structure,
- * [4] learned network's graph, [5] the whole set-up; and of the last oc_scaffold_update: [6] air-vertex download
- * and edge flips (0 for a refresh), [7] the whole update. No reference counterpart (diagnostics). */
+ * oc_scaffold_update): ms[0] vertex lists, [1] pattern and slot maps, [2] their upload, [3] multigrid structure
+ * (or, after a re-mesh that kept it, the map of the Newton pattern onto it), [4] learned network's graph, [5] the
+ * whole set-up; and of the last oc_scaffold_update: [6] air-vertex download and edge flips (0 for a refresh), [7]
+ * the whole update; ms[8] = air re-meshes the current multigrid structure has been kept across (0: built for the
+ * current Newton pattern; at most 4, then it is rebuilt). ms holds 9 doubles. No reference counterpart
+ * (diagnostics). */
 int oc_setup_times(oc_ctx* ctx, double* ms);
 
 /* ---- LinSysSolver interface (LinSysSolver.hpp:22-256) -------------------- */

@@ -375,12 +375,13 @@ class Context:
         return [(int(rows[l]), int(blocks[l]), int(bs[l])) for l in range(min(n.value, cap))]
 
     def setup_times(self):
-        """Host ms of the last set-up phases (vertex lists, pattern+maps, upload, multigrid structure, learned
-        graph, whole set-up) and of the last air-mesh update (download+flips, whole update)."""
-        ms = np.zeros(8)
+        """Host ms of the last set-up phases (vertex lists, pattern+maps, upload, multigrid structure or its
+        remap, learned graph, whole set-up) and of the last air-mesh update (download+flips, whole update), and
+        amg_kept_remeshes: air re-meshes the current multigrid structure has been kept across."""
+        ms = np.zeros(9)
         self._chk(self.L.oc_setup_times(self.h, ms))
         keys = ("vertex_lists", "pattern_maps", "upload", "amg_structure", "learned_graph", "setup_total",
-                "air_flips", "air_update_total")
+                "air_flips", "air_update_total", "amg_kept_remeshes")
         return dict(zip(keys, ms.tolist()))
 
     def air_mesh_get(self):

@@ -173,6 +173,14 @@ struct oc_ctx {
     oc::Amg<float> amg32;   // FP32 hierarchy (mixed form)
     oc::AmgBase* amg = nullptr;
     bool amgValid = false;
+    // After an air re-mesh the multigrid structure of the previous Newton pattern is kept (kAmgRemapMax times
+    // in a row, then rebuilt): the Newton matrix is mapped onto that pattern (amgPat) before each numeric set-up
+    oc::HostPattern amgPat;                   // the pattern the current multigrid structure was built for
+    const oc::AmgBase* amgPatOwner = nullptr;  // the hierarchy holding that structure
+    bool amgOnOld = false;                    // amgPat differs from the Newton pattern: values go through the map
+    int amgRemaps = 0;                        // air re-meshes since the structure was last built
+    bool amgRebuild = false;                  // a solve on a kept structure needed a fallback: rebuild it
+    DevBuf dAmgOldRowPtr, dAmgMap, dAmgDropPtr, dAmgDrop, dAmgOldDiag, dAmgVals;
     oc_gnn* learned = nullptr;             // bound by oc_newton_set_learned (not owned)
     oc::Preconditioner* ext = nullptr;     // the external preconditioner of the Newton PCG, if any
 
@@ -258,6 +266,15 @@ constexpr int kMaxPartialBlocks = 1 << 16;
 constexpr int kRzRecords = 2048;  // r.z partial records of an external preconditioner
 constexpr size_t kMailBytes = 4096;  // pinned read-back words of a context
 constexpr int kAmgSkip = 3;          // steps without multigrid after it failed on a step
+#ifndef OC_AMG_REMAP_MAX
+#define OC_AMG_REMAP_MAX 4
+#endif
+constexpr int kAmgRemapMax = OC_AMG_REMAP_MAX;  // air re-meshes on one multigrid structure before it is rebuilt
+#ifndef OC_AMG_KEEP_DROP
+#define OC_AMG_KEEP_DROP 0.003
+#endif
+// a kept multigrid structure is rebuilt once the Newton pattern has this share of its blocks outside it
+constexpr double kAmgKeepDrop = OC_AMG_KEEP_DROP;
 constexpr int kMaxTimedLevels = 16;  // oc_time_kernel 10 + l: Chebyshev step on multigrid level l
 #ifndef OC_AIR_LIMIT_RATIO
 #define OC_AIR_LIMIT_RATIO 0.5
@@ -601,20 +618,36 @@ int step_bound(oc_ctx* c, const TriSet& T, const double* dDir, double cap, doubl
 // guaranteed smoother interval).
 void amg_numeric(oc_ctx* c, int safe)
 {
-    c->amg->numeric(c->stream, c->sys.dVals.p, c->sys.vals32, c->dUV.as<double>(), safe);
+    const void* vals = c->sys.dVals.p;
+    if (c->amgOnOld) {
+        oc::launch_remap_blocks(c->stream, c->amgPat.nV, c->dAmgOldRowPtr.as<int>(), c->dAmgMap.as<int>(),
+                                c->dAmgDropPtr.as<int>(), c->dAmgDrop.as<int>(), c->dAmgOldDiag.as<int>(),
+                                c->sys.dVals.as<double>(), c->dAmgVals.as<double>());
+        vals = c->dAmgVals.p;
+    }
+    c->amg->numeric(c->stream, vals, c->sys.vals32, c->dUV.as<double>(), safe);
 }
 
 int amg_structure(oc_ctx* c);
+int amg_remap(oc_ctx* c);
 
 double ms_since(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-int build_system(oc_ctx* c)
+// airOnly: only the air triangles changed since the last build (a re-mesh of oc_scaffold_update), so the
+// multigrid structure may be kept.
+int build_system(oc_ctx* c, bool airOnly = false)
 {
     const auto tb0 = std::chrono::steady_clock::now();
     const int nVt = c->nV + (c->nFair ? c->nVair - c->nBnd : 0);
+    oc::AmgBase* const amgTarget =
+        c->nPrecond != OC_PRECOND_AMG ? nullptr
+        : c->sysPrecision == OC_PRECISION_MIXED ? static_cast<oc::AmgBase*>(&c->amg32)
+                                                : static_cast<oc::AmgBase*>(&c->amg64);
+    const bool keepAmg = airOnly && amgTarget && c->amgValid && c->amg == amgTarget && !c->amgRebuild &&
+                         c->amgPatOwner == amgTarget && c->amgPat.nV == nVt && c->amgRemaps < kAmgRemapMax;
     const int nFt = c->nF + c->nFair;
     c->nVt = nVt;
     c->nFt = nFt;
@@ -710,7 +743,12 @@ int build_system(oc_ctx* c)
                                                       : static_cast<oc::AmgBase*>(&c->amg64);
         // while multigrid is being skipped after a failure, its structure is built when it is next used
         if (c->amgSkip > 0) c->amgPending = true;
-        else if (int st = amg_structure(c)) return st;
+        else if (keepAmg) {
+            const int st = amg_remap(c);
+            if (st > 0) return st;
+            if (st < 0)  // too far from the kept pattern
+                if (int st2 = amg_structure(c)) return st2;
+        } else if (int st = amg_structure(c)) return st;
     }
     OC_HIP(c, hipStreamSynchronize(c->stream));
     if (c->nPrecond == OC_PRECOND_AMG) c->tSetup[3] = ms_since(tp);
@@ -730,6 +768,9 @@ int amg_structure(oc_ctx* c)
         late.assign(c->nVt, 0);
         for (int v : c->FairGlobal) late[v] = 1;
     }
+    c->amgPatOwner = nullptr;
+    c->amgOnOld = false;
+    c->amgRebuild = false;
     if (c->amg->setup(c->stream, c->nVt, c->sys.rowPtr, c->sys.colIdx, c->fixedMaskSys, e,
                       late.empty() ? nullptr : &late) != 0) {
         c->err = e;
@@ -740,6 +781,48 @@ int amg_structure(oc_ctx* c)
         c->err = "AMG: too many partial records";
         return OC_ERR_STATE;
     }
+    c->amgPat = static_cast<const oc::HostPattern&>(c->sys);
+    c->amgPatOwner = c->amg;
+    c->amgRemaps = 0;
+    c->amgValid = true;
+    c->amgPending = false;
+    return OC_OK;
+}
+
+// Keeps the multigrid structure built for amgPat (same rows as the current Newton pattern, which differs from
+// it only in rows an air re-mesh touched): per block of amgPat the block of the current pattern with the same
+// (row, column) or -1, and per row the current blocks that amgPat lacks. amg_numeric maps the Newton matrix
+// through them (launch_remap_blocks: the lacking blocks go onto the diagonal by their Frobenius norms, which
+// keeps the mapped matrix symmetric positive definite), so the preconditioner is that of a nearby matrix; the
+// PCG products stay on the Newton matrix itself. Returns -1, keeping nothing, when more than kAmgKeepDrop of
+// amgPat's block count lies outside it (the caller then builds the structure anew).
+int amg_remap(oc_ctx* c)
+{
+    const oc::HostPattern& O = c->amgPat;
+    const oc::HostPattern& N = c->sys;
+    const int n = O.nV;
+    std::vector<int> map(O.colIdx.size()), dropPtr(n + 1, 0), drop;
+    for (int u = 0; u < n; ++u) {
+        int i = O.rowPtr[u], j = N.rowPtr[u];
+        const int ie = O.rowPtr[u + 1], je = N.rowPtr[u + 1];
+        while (i < ie || j < je) {
+            if (j == je || (i < ie && O.colIdx[i] < N.colIdx[j])) map[i++] = -1;
+            else if (i == ie || N.colIdx[j] < O.colIdx[i]) drop.push_back(j++);
+            else map[i++] = j++;
+        }
+        dropPtr[u + 1] = (int)drop.size();
+        if (O.diagPos[u] < 0 || map[O.diagPos[u]] < 0) return -1;
+    }
+    if ((double)drop.size() > kAmgKeepDrop * (double)O.nnzb()) return -1;
+    if (drop.empty()) drop.push_back(0);  // never read (empty rows); keeps the upload non-empty
+    if (upload(c, c->dAmgOldRowPtr, O.rowPtr) || upload(c, c->dAmgMap, map) || upload(c, c->dAmgDropPtr, dropPtr) ||
+        upload(c, c->dAmgDrop, drop) || upload(c, c->dAmgOldDiag, O.diagPos))
+        return OC_ERR_HIP;
+    OC_HIP(c, c->dAmgVals.ensure(sizeof(double) * 4 * (size_t)std::max<long>(1, O.nnzb())));
+    OC_HIP(c, hipStreamSynchronize(c->stream));  // the host maps end with this call
+    c->ext = c->amg;
+    c->amgOnOld = true;
+    ++c->amgRemaps;
     c->amgValid = true;
     c->amgPending = false;
     return OC_OK;
@@ -2479,7 +2562,7 @@ int oc_scaffold_update(oc_ctx* c, int mode, double minAngleDeg, int maxInterval,
                 for (size_t k = 0; k < c->FairLocal.size(); ++k) c->FairGlobal[k] = c->airL2G[c->FairLocal[k]];
                 if (int st = upload_tri_ids(c, c->nF, c->nFair, c->FairGlobal.data())) return st;
                 c->sysReady = false;
-                if (int st = build_system(c)) return st;
+                if (int st = build_system(c, true)) return st;
             }
         }
     }
@@ -2528,6 +2611,7 @@ int oc_setup_times(oc_ctx* c, double* ms)
 {
     if (!c || !ms) return OC_ERR_ARG;
     std::copy(c->tSetup, c->tSetup + 8, ms);
+    ms[8] = c->amgValid && c->amgOnOld ? (double)c->amgRemaps : 0.0;
     return OC_OK;
 }
 
@@ -2566,7 +2650,7 @@ void system_assemble(oc_ctx* c);
 // The preconditioner's numeric set-up for the assembled Newton matrix (no host synchronisation).
 int launch_precond_setup(oc_ctx* c, oc_iter_stats* S)
 {
-    if (c->amgPending && c->amgSkip == 0)
+    if ((c->amgPending || (c->amgRebuild && c->amgValid)) && c->amgSkip == 0)
         if (int st = amg_structure(c)) return st;
     // The multigrid hierarchy is rebuilt for every Newton matrix: reusing the
     // previous one stays a valid (SPD) preconditioner but, measured at 200k
@@ -2640,6 +2724,8 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
         S->pcg_fallback = 6;
         st = OC_OK;
     }
+    // a multigrid structure kept across air re-meshes is rebuilt for the next solve once one needed more
+    c->amgRebuild = c->amgRebuild || (c->amgOnOld && (st != OC_OK || S->pcg_fallback != 0));
     if (c->ext && st != OC_OK) {
         // The preconditioned solve broke down or hit its cap. With the
         // learned factor in the mixed form the likely cause is the matrix

@@ -1238,6 +1238,44 @@ __global__ void __launch_bounds__(kBlock) k_uv_gather(int n, const int* __restri
     if (out2) out2[k] = p;
 }
 
+// The Newton matrix mapped onto an earlier pattern (the one a multigrid structure was built for), per row u of
+// that pattern: each block takes the new matrix's block of the same (u, w) (map >= 0) or zero; the blocks of
+// row u that the earlier pattern lacks (drop list) are left out and their Frobenius norms added to both diagonal
+// entries of u's diagonal block. Dropping a symmetric pair of off-diagonal blocks B, B^T and adding |B| I to both
+// diagonal blocks adds [[|B| I, -B], [-B^T, |B| I]], which is positive semidefinite: the result stays symmetric
+// positive definite when the new matrix is.
+__global__ void __launch_bounds__(kBlock) k_remap_blocks(int n, const int* __restrict__ oldRowPtr,
+                                                         const int* __restrict__ map, const int* __restrict__ dropPtr,
+                                                         const int* __restrict__ drop, const int* __restrict__ oldDiag,
+                                                         const double4* __restrict__ newVals, double4* oldVals)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    for (int k = oldRowPtr[u]; k < oldRowPtr[u + 1]; ++k) {
+        const int m = map[k];
+        oldVals[k] = m >= 0 ? newVals[m] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    double comp = 0.0;
+    for (int q = dropPtr[u]; q < dropPtr[u + 1]; ++q) {
+        const double4 b = newVals[drop[q]];
+        comp += sqrt(b.x * b.x + b.y * b.y + b.z * b.z + b.w * b.w);
+    }
+    if (comp > 0.0) {
+        double4 d = oldVals[oldDiag[u]];
+        d.x += comp;
+        d.w += comp;
+        oldVals[oldDiag[u]] = d;
+    }
+}
+
+void launch_remap_blocks(hipStream_t s, int n, const int* oldRowPtr, const int* map, const int* dropPtr,
+                         const int* drop, const int* oldDiag, const double* newVals, double* oldVals)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_remap_blocks, dim3(grid_for(n)), dim3(kBlock), 0, s, n, oldRowPtr, map, dropPtr, drop,
+                       oldDiag, reinterpret_cast<const double4*>(newVals), reinterpret_cast<double4*>(oldVals));
+}
+
 void launch_uv_gather(hipStream_t s, int n, const int* ids, const double* UV, double* out3, double* out2)
 {
     hipLaunchKernelGGL(k_uv_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, n, ids,

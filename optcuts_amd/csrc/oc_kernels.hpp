@@ -193,6 +193,11 @@ namespace oc {
 void launch_count_inverted(hipStream_t s, const TriSet& T, const double* UV, unsigned int* count);
 // *p = v (one thread; keeps device scalars out of hipMemset so streams stay capturable).
 void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v);
+// The 2x2-block matrix newVals (current pattern) mapped onto an earlier pattern of the same n rows (oldRowPtr;
+// map: per earlier block the current block of the same (u, w) or -1; dropPtr/drop: per row the current blocks the
+// earlier pattern lacks, whose Frobenius norms go onto both diagonal entries of the row's diagonal block, oldDiag).
+void launch_remap_blocks(hipStream_t s, int n, const int* oldRowPtr, const int* map, const int* dropPtr,
+                         const int* drop, const int* oldDiag, const double* newVals, double* oldVals);
 // b = sign * g for n doubles.
 void launch_scale_copy(hipStream_t s, long n, double sign, const double* g, double* b);
 // For k < n, p = UV row ids[k] (row k when ids is null): out3[k] = (p.x, p.y, 0) and out2[k] = p (either may be

@@ -41,6 +41,32 @@ def test_closed_surface_scaffolded_steps_match_oracle(ctx, name, warm):
     have condition numbers beyond what an iterative solve resolves in FP64 (after 8 oracle steps, E_SD ~ 50, a
     PCG solve to 1e-13 still leaves the next energy 4e-8 from the LDL^T step's), so the device follows the oracle
     from the state after 16 steps (E_SD ~ 4.4)."""
+    follow_oracle(ctx, name, warm, 6, capi.SCAFFOLD_REMESH_WHEN_NEEDED)
+
+
+def test_remesh_keeps_multigrid_structure(ctx):
+    """An air re-mesh keeps the multigrid structure of the previous Newton pattern and maps the new Newton matrix
+    onto it, at most 4 re-meshes in a row and while at most 0.3% of the kept pattern's block count lies outside it
+    (then the structure is rebuilt). With a re-mesh forced after every step, each step still matches the oracle's
+    LDL^T step from the same state and air mesh (PCG to 1e-13: the preconditioner changes only the iteration
+    count) with no fallback; the kept-structure count goes up by one or back to 0 (rebuilt) at each flipping
+    re-mesh, stays put at one without flips and reaches 1 at least once."""
+    rec = follow_oracle(ctx, "icosphere_nu10", 0, 8, capi.SCAFFOLD_REMESH)
+    prev = 0
+    n_flipping = 0
+    for flips, kept in rec:
+        if flips > 0:
+            n_flipping += 1
+            assert kept in (prev + 1, 0) and kept <= 4
+        else:
+            assert kept == prev
+        prev = kept
+    assert n_flipping >= 2 and max(k for _, k in rec) >= 1
+
+
+def follow_oracle(ctx, name, warm, steps, mode):
+    """Device scaffolded Newton steps against the oracle's from the same state and air mesh; returns per step
+    (edges flipped by the air update, re-meshes the multigrid structure has been kept across)."""
     Vc, Fc, cohE, meta = S.workload(name)
     print(name, meta, flush=True)
     assert meta["genus"] == (0 if name.startswith("ico") else 3)
@@ -70,7 +96,8 @@ def test_closed_surface_scaffolded_steps_match_oracle(ctx, name, warm):
     so = ScaffoldOracle(Vc, Fc, airUV, Fair, bnd, thr, a0, w)
     assert E0 == pytest.approx(so.energy(X)[0], rel=1e-12)
     total_flips = 0
-    for k in range(6):
+    rec = []
+    for k in range(steps):
         Xo, Eo, step, halv, gsq = so.newton_step(X, None)
         s, done = ctx.newton_step()
         print(name, k, s["E_total"], Eo, s["step"], step, s["pcg_iters"], flush=True)
@@ -82,8 +109,9 @@ def test_closed_surface_scaffolded_steps_match_oracle(ctx, name, warm):
         uv = ctx.get_uv()
         auv, _ = ctx.air_mesh_get()
         assert np.abs(uv - Xo[: Vc.shape[0]]).max() <= 1e-7
-        E_upd, flips = ctx.scaffold_update(capi.SCAFFOLD_REMESH_WHEN_NEEDED, 20.0)
+        E_upd, flips = ctx.scaffold_update(mode, 20.0)
         total_flips += flips
+        rec.append((flips, int(ctx.setup_times()["amg_kept_remeshes"])))
         auv2, Fair2 = ctx.air_mesh_get()
         np.testing.assert_array_equal(auv2, auv)
         assert (signed_areas(auv2, Fair2) > 0).all()
@@ -94,7 +122,8 @@ def test_closed_surface_scaffolded_steps_match_oracle(ctx, name, warm):
         # a refreshed air mesh is at rest: its energy is 4 per triangle unless clamped by areaThres
         Ea = so.energy(X)[2]
         assert Ea <= 4.0 * w * (1 + 1e-9) or (signed_areas(auv2, Fair2) < thr).any()
-    print(name, "air flips", total_flips, flush=True)
+    print(name, "air flips", total_flips, "(flips, kept multigrid structure) per step", rec, flush=True)
+    return rec
 
 
 def test_scaffold_remesh_restores_quality(ctx):
