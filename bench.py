@@ -87,7 +87,7 @@ TRAFFIC = {
 }
 
 FALLBACK_NAMES = {1: "retry_safe_amg", 2: "jacobi", 3: "truncated", 4: "retry_fp64_elements", 5: "jacobi_direct",
-                  6: "truncated_at_cap"}
+                  6: "truncated_at_cap", 7: "ic0"}
 
 
 def count_fallbacks(stats):
@@ -116,6 +116,7 @@ def closed_forms(args):
             "learned": (capi.PRECOND_LEARNED, args.learned_tol, 2000, fp64),
             "learned_mixed": (capi.PRECOND_LEARNED, args.learned_tol, 2000, mixed),
             "block_jacobi_at_learned_tol": (capi.PRECOND_BLOCK_JACOBI, args.learned_tol, 2000, fp64),
+            "ic0_at_learned_tol": (capi.PRECOND_IC0, args.learned_tol, 2000, fp64),
             "plain": (capi.PRECOND_JACOBI, args.pcg_tol, 200000, fp64)}
 
 
@@ -225,7 +226,7 @@ def closed_bench(ctx, name, args, targets, forms=("chosen", "plain"), timed_step
         n = min(plain_steps, timed_steps) if form == "plain" else timed_steps
         r = closed_run(ctx, w, form, args, n, args.warmup)
         r.update({"pcg_rel_tol": tol, "pcg_max_iter": mx, "precision": {0: "fp64", 1: "mixed"}[prec],
-                  "precond": {0: "jacobi", 1: "block-jacobi", 2: "sa-amg", 3: "learned-factor"}[pc]})
+                  "precond": {0: "jacobi", 1: "block-jacobi", 2: "sa-amg", 3: "learned-factor", 4: "ic0"}[pc]})
         if e_star is not None and not args.no_tte:
             budget = None if form == "chosen" else (plain_budget if form == "plain" else form_budget)
             r["time_to_E_SD_1e-6"] = closed_run(ctx, w, form, args, tte_max, 0, e_star=e_star, budget=budget)
@@ -302,7 +303,9 @@ def disc_bench(ctx, args, targets):
              "mixed": (capi.PRECOND_AMG, args.mixed_tol, capi.PRECISION_MIXED),
              # controls at the loose tolerances of the faster forms: what inexact Newton alone gives
              "plain_at_chosen_tol": (capi.PRECOND_JACOBI, args.chosen_tol, capi.PRECISION_FP64),
-             "block_jacobi_at_learned_tol": (capi.PRECOND_BLOCK_JACOBI, args.learned_tol, capi.PRECISION_FP64)}
+             "block_jacobi_at_learned_tol": (capi.PRECOND_BLOCK_JACOBI, args.learned_tol, capi.PRECISION_FP64),
+             # the classic incomplete factor on the same pattern, at the learned factor's tolerance
+             "ic0_at_learned_tol": (capi.PRECOND_IC0, args.learned_tol, capi.PRECISION_FP64)}
     if have_weights:
         forms["learned"] = (capi.PRECOND_LEARNED, args.learned_tol, capi.PRECISION_FP64)
         forms["learned_mixed"] = (capi.PRECOND_LEARNED, args.learned_tol, capi.PRECISION_MIXED)
@@ -322,13 +325,13 @@ def disc_bench(ctx, args, targets):
         r = {"newton_iters_per_s": steps / el, "ms_per_step": 1e3 * el / steps,
              "pcg_iters_per_newton": float(np.mean([s["pcg_iters"] for s in timed])),
              "pcg_fallbacks": count_fallbacks(timed), "precond": {0: "jacobi", 1: "block-jacobi", 2: "sa-amg",
-                                                                   3: "learned-factor"}[precond],
+                                                                   3: "learned-factor", 4: "ic0"}[precond],
              "precision": {0: "fp64", 1: "mixed"}[precision], "pcg_rel_tol": tol}
         r["kernel_ms"] = {"element_pass": ctx.time_kernel(0, reps=10), "assembly": ctx.time_kernel(1, reps=10),
                           "pcg_spmv": ctx.time_kernel(5, reps=50), "pcg_update": ctx.time_kernel(6, reps=50),
                           "linesearch_candidates": ctx.time_kernel(3, reps=10),
                           "step_bound": ctx.time_kernel(4, reps=10)}
-        if precond in (capi.PRECOND_AMG, capi.PRECOND_LEARNED):
+        if precond in (capi.PRECOND_AMG, capi.PRECOND_LEARNED, capi.PRECOND_IC0):
             r["kernel_ms"]["precond_apply"] = ctx.time_kernel(7, reps=10)
             r["kernel_ms"]["precond_setup"] = ctx.time_kernel(8, reps=5)
         if precond == capi.PRECOND_AMG:
@@ -570,7 +573,7 @@ def main():
         net = capi.Gnn(0)
         net.set_params(learned.load_weights())
         ctx.set_learned(net)
-        fast = fast + ("learned", "learned_mixed", "block_jacobi_at_learned_tol")
+        fast = fast + ("learned", "learned_mixed", "block_jacobi_at_learned_tol", "ic0_at_learned_tol")
     from optcuts_amd import workloads as W
     head, w, roofline = None, None, None
     if not args.no_headline:

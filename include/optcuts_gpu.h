@@ -38,11 +38,14 @@ enum {
 };
 
 /* Newton-system preconditioners: point Jacobi (the plain form), 2x2 block Jacobi, smoothed-aggregation
- * algebraic multigrid under FP64 PCG (needs the UV map for its rigid near-null space), and the learned sparse
- * factor of an oc_gnn network (oc_newton_set_learned). LEARNED is available to the device-resident Newton driver
- * only; the others to the drop-in oc_solver_* as well. None has a reference counterpart: the reference factorizes
- * every system with SimplicialLDLT (EigenLibSolver.cpp:71-108). */
-enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1, OC_PRECOND_AMG = 2, OC_PRECOND_LEARNED = 3 };
+ * algebraic multigrid under FP64 PCG (needs the UV map for its rigid near-null space), the learned sparse
+ * factor of an oc_gnn network (oc_newton_set_learned), and block incomplete Cholesky IC(0) on the 2x2-block
+ * pattern (factored by fixed-point sweeps, applied by Jacobi-swept triangular solves; oc_ic0_set_sweeps). LEARNED
+ * is available to the device-resident Newton driver only; the others to the drop-in oc_solver_* as well. None has
+ * a reference counterpart: the reference factorizes every system with SimplicialLDLT (EigenLibSolver.cpp:71-108).
+ * IC(0)-PCG is also the learned form's fallback before Jacobi-PCG (pcg_fallback 7). */
+enum { OC_PRECOND_JACOBI = 0, OC_PRECOND_BLOCK_JACOBI = 1, OC_PRECOND_AMG = 2, OC_PRECOND_LEARNED = 3,
+       OC_PRECOND_IC0 = 4 };
 
 /* Arithmetic of the device-resident Newton driver.
  * FP64: every element term, the Hessian values and the PCG in FP64 (the plain form; matches the reference's
@@ -212,10 +215,20 @@ int oc_solver_solve(oc_ctx* ctx, const double* rhs, double* x);
 /* LinSysSolver::multiply (LinSysSolver.hpp:172-187): Ax = A x with the full symmetric matrix. */
 int oc_solver_multiply(oc_ctx* ctx, const double* x, double* Ax);
 /* The solve's parameters (EigenLibSolver has none: its factorization is exact): OC_PRECOND_JACOBI,
- * OC_PRECOND_BLOCK_JACOBI or OC_PRECOND_AMG (smoothed-aggregation multigrid built by factorize; a breakdown
- * rebuilds it with the Gershgorin interval, then falls back to Jacobi-PCG), relative residual relTol, at most
- * maxIter PCG iterations. */
+ * OC_PRECOND_BLOCK_JACOBI, OC_PRECOND_AMG (smoothed-aggregation multigrid built by factorize; a breakdown
+ * rebuilds it with the Gershgorin interval, then falls back to Jacobi-PCG) or OC_PRECOND_IC0 (block IC(0) built
+ * by factorize; a breakdown falls back to Jacobi-PCG), relative residual relTol, at most maxIter PCG
+ * iterations. */
 int oc_solver_set_params(oc_ctx* ctx, int precond, double relTol, int maxIter);
+/* z = M^-1 r (2*nV doubles each) with the preconditioner of the last factorize (OC_PRECOND_AMG or
+ * OC_PRECOND_IC0). No reference counterpart (diagnostics and tests). */
+int oc_solver_apply_precond(oc_ctx* ctx, const double* r, double* z);
+/* Sweep counts of OC_PRECOND_IC0 (the Newton driver's and the drop-in's): factorSweeps fixed-point sweeps after
+ * the starting factor (default 3; the exact IC(0) factor once they reach the dependency depth) and applySweeps
+ * Jacobi sweeps per triangular solve after the diagonal one (default 2; exact triangular solves at the depth).
+ * Takes effect at the next factorization (oc_solver_factorize, or the next Newton step). No reference
+ * counterpart. */
+int oc_ic0_set_sweeps(oc_ctx* ctx, int factorSweeps, int applySweeps);
 /* Coordinates (2*nV, [u0 v0 u1 v1 ...], e.g. the current UV of TriMesh::V) for OC_PRECOND_AMG's near-null
  * space (translations and the rotation of the plane per aggregate); without them the multigrid uses
  * translations only. Takes effect at the next factorize. No reference counterpart: LinSysSolver passes no
@@ -250,7 +263,8 @@ typedef struct {
                          when it had none (truncated Newton); 5: Jacobi-PCG was used directly because
                          multigrid failed (code 2) within the last 3 steps (it is tried again after them); 6:
                          multigrid-PCG reached pcgMaxIter without a breakdown and the step uses its last iterate
-                         (a descent direction: it minimises the quadratic model over the Krylov space so far) */
+                         (a descent direction: it minimises the quadratic model over the Krylov space so far);
+                         7: OC_PRECOND_LEARNED failed and block IC(0)-PCG at the same tolerance solved it */
     int pcg_iters_first; /* iterations of the first attempt when pcg_fallback != 0 (pcg_iters counts all) */
     int precond_setups;  /* multigrid hierarchies built on this step (1, or 2 after a safe-interval retry) */
 } oc_iter_stats;

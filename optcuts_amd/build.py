@@ -11,9 +11,9 @@ ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIB_DIR, "liboptcuts_gpu.so")
 SOURCES = [os.path.join(HERE, "csrc", f)
-           for f in ("oc_kernels.hip", "oc_amg.hip", "oc_gnn.hip", "oc_capi.hip", "oc_io.cpp")]
+           for f in ("oc_kernels.hip", "oc_amg.hip", "oc_gnn.hip", "oc_ic0.hip", "oc_capi.hip", "oc_io.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("oc_common.hpp", "oc_kernels.hpp", "oc_sd_element.hpp",
-                                                   "oc_amg.hpp", "oc_gnn.hpp", "oc_precond.hpp",
+                                                   "oc_amg.hpp", "oc_gnn.hpp", "oc_ic0.hpp", "oc_precond.hpp",
                                                    "oc_io.hpp")] + [
     os.path.join(ROOT, "include", "optcuts_gpu.h")]
 

@@ -24,6 +24,7 @@ PRECOND_JACOBI = 0
 PRECOND_BLOCK_JACOBI = 1
 PRECOND_AMG = 2
 PRECOND_LEARNED = 3
+PRECOND_IC0 = 4
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 TUTTE_COMPONENTS = 0
@@ -95,6 +96,8 @@ _SIGS = {
     "oc_solver_solve": (C.c_int, [_ctx, _dp, _dp]),
     "oc_solver_multiply": (C.c_int, [_ctx, _dp, _dp]),
     "oc_solver_set_params": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int]),
+    "oc_solver_apply_precond": (C.c_int, [_ctx, _dp, _dp]),
+    "oc_ic0_set_sweeps": (C.c_int, [_ctx, C.c_int, C.c_int]),
     "oc_solver_last_stats": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "oc_solver_set_coords": (C.c_int, [_ctx, _dp]),
     "oc_solver_last_fallback": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
@@ -445,6 +448,18 @@ class Context:
         UV = _f64(UV)
         assert UV.size == self._sol_n
         self._chk(self.L.oc_solver_set_coords(self.h, UV))
+
+    def solver_apply_precond(self, r):
+        """z = M^-1 r with the drop-in's factorized preconditioner (PRECOND_AMG or PRECOND_IC0)."""
+        r = _f64(r)
+        assert r.size == self._sol_n
+        z = np.zeros(self._sol_n)
+        self._chk(self.L.oc_solver_apply_precond(self.h, r, z))
+        return z
+
+    def ic0_set_sweeps(self, factor_sweeps=3, apply_sweeps=2):
+        """PRECOND_IC0's fixed-point factor sweeps and Jacobi sweeps per triangular solve."""
+        self._chk(self.L.oc_ic0_set_sweeps(self.h, int(factor_sweeps), int(apply_sweeps)))
 
     def solver_last_fallback(self):
         """0 none, 1 multigrid rebuilt with the Gershgorin interval, 2 Jacobi-PCG."""

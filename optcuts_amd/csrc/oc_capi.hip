@@ -7,6 +7,7 @@
 #include "oc_amg_structure.hpp"
 #include "oc_common.hpp"
 #include "oc_gnn.hpp"
+#include "oc_ic0.hpp"
 #include "oc_io.hpp"
 #include "oc_kernels.hpp"
 #include "oc_system_pattern.hpp"
@@ -182,6 +183,10 @@ struct oc_ctx {
     bool amgRebuild = false;                  // a solve on a kept structure needed a fallback: rebuild it
     DevBuf dAmgOldRowPtr, dAmgMap, dAmgDropPtr, dAmgDrop, dAmgOldDiag, dAmgVals;
     oc_gnn* learned = nullptr;             // bound by oc_newton_set_learned (not owned)
+    // block IC(0) of the Newton matrix: the preconditioner of OC_PRECOND_IC0, and the learned form's fallback
+    // (its structure then built when first needed for the current pattern)
+    oc::Ic0 ic0;
+    bool ic0Struct = false;
     oc::Preconditioner* ext = nullptr;     // the external preconditioner of the Newton PCG, if any
 
     // ---- PCG workspace ---------------------------------------------------------
@@ -235,6 +240,8 @@ struct oc_ctx {
     DevBuf dSolUV;
     bool solHaveUV = false;
     int solFallback = 0;  // what the last solve fell back to (0 none, 1 safe-interval multigrid, 2 Jacobi)
+    oc::Ic0 ic0Sol;         // IC(0) of the drop-in (structure per pattern)
+    bool solIc0Struct = false;
 
     const char* fail(const std::string& m)
     {
@@ -631,6 +638,22 @@ void amg_numeric(oc_ctx* c, int safe)
 int amg_structure(oc_ctx* c);
 int amg_remap(oc_ctx* c);
 
+// Block IC(0) structure of the current Newton pattern.
+int ic0_structure(oc_ctx* c)
+{
+    std::string e;
+    if (c->ic0.setup(c->stream, c->nVt, c->sys.rowPtr, c->sys.colIdx, e) != 0) {
+        c->err = e;
+        return OC_ERR_HIP;
+    }
+    if (c->ic0.rz_records() > kRzRecords) {
+        c->err = "IC(0): too many partial records";
+        return OC_ERR_STATE;
+    }
+    c->ic0Struct = true;
+    return OC_OK;
+}
+
 double ms_since(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -736,6 +759,11 @@ int build_system(oc_ctx* c, bool airOnly = false)
         }
         c->ext = &c->learned->net;
         c->tSetup[4] = ms_since(tp);
+    }
+    c->ic0Struct = false;
+    if (c->nPrecond == OC_PRECOND_IC0) {
+        if (int st = ic0_structure(c)) return st;
+        c->ext = &c->ic0;
     }
     c->amgPending = false;
     if (c->nPrecond == OC_PRECOND_AMG) {
@@ -1813,6 +1841,7 @@ int oc_solver_set_pattern(oc_ctx* c, int nV, const int* nbrPtr, const int* nbr, 
     c->solA.assign(nnzU, 0.0);
     c->solFixed = isFixed;
     c->solAmgStruct = false;
+    c->solIc0Struct = false;
     c->solHaveUV = false;
     if (upload_pattern(c, c->sol)) return OC_ERR_HIP;
     if (upload(c, c->dSolDst, c->solDst) || upload(c, c->dSolMir, c->solMir)) return OC_ERR_HIP;
@@ -1925,6 +1954,18 @@ int oc_solver_factorize(oc_ctx* c)
         c->amgSol.numeric(c->stream, c->sol.dVals.p, 0, c->dSolUV.as<double>());
         OC_HIP(c, hipGetLastError());
     }
+    if (c->solPrecond == OC_PRECOND_IC0) {
+        if (!c->solIc0Struct) {
+            std::string e;
+            if (c->ic0Sol.setup(c->stream, c->sol.nV, c->sol.rowPtr, c->sol.colIdx, e) != 0) {
+                c->err = "oc_solver_factorize: " + e;
+                return OC_ERR_HIP;
+            }
+            c->solIc0Struct = true;
+        }
+        c->ic0Sol.numeric(c->stream, c->sol.dVals.as<double>());
+        OC_HIP(c, hipGetLastError());
+    }
     OC_HIP(c, hipStreamSynchronize(c->stream));
     c->solFactorized = true;
     return OC_OK;
@@ -1974,6 +2015,15 @@ int oc_solver_solve(oc_ctx* c, const double* rhs, double* x)
             st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it2, rr);
             it += it2;
         }
+    } else if (c->solPrecond == OC_PRECOND_IC0) {
+        // block IC(0); on a breakdown Jacobi-PCG takes over
+        st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it, rr, &c->ic0Sol);
+        if (st == OC_ERR_BREAKDOWN) {
+            int it2;
+            c->solFallback = 2;
+            st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it2, rr);
+            it += it2;
+        }
     } else {
         st = run_pcg(c, c->sol, c->dSolB.as<double>(), c->solTol, c->solMaxIter, it, rr);
     }
@@ -1999,10 +2049,47 @@ int oc_solver_multiply(oc_ctx* c, const double* x, double* Ax)
     return OC_OK;
 }
 
+int oc_solver_apply_precond(oc_ctx* c, const double* r, double* z)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, c->solN > 0 && r && z, "oc_solver_apply_precond: no pattern");
+    if (!c->solFactorized) {
+        c->err = "oc_solver_apply_precond: factorize first";
+        return OC_ERR_STATE;
+    }
+    (void)hipSetDevice(c->device);
+    OC_HIP(c, c->dPcgPart.ensure(sizeof(double) * (size_t)kRzRecords));
+    OC_HIP(c, hipMemcpyAsync(c->dSolB.p, r, sizeof(double) * c->solN, hipMemcpyHostToDevice, c->stream));
+    oc::Preconditioner* M = c->solPrecond == OC_PRECOND_AMG ? static_cast<oc::Preconditioner*>(&c->amgSol)
+                            : c->solPrecond == OC_PRECOND_IC0 ? static_cast<oc::Preconditioner*>(&c->ic0Sol)
+                                                               : nullptr;
+    if (!M) {
+        c->err = "oc_solver_apply_precond: OC_PRECOND_AMG or OC_PRECOND_IC0 only";
+        return OC_ERR_STATE;
+    }
+    M->apply(c->stream, c->dSolB.as<double>(), c->dSolX.as<double>(), c->dPcgPart.as<double>(), nullptr);
+    OC_HIP(c, hipGetLastError());
+    OC_HIP(c, hipMemcpyAsync(z, c->dSolX.p, sizeof(double) * c->solN, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    return OC_OK;
+}
+
+int oc_ic0_set_sweeps(oc_ctx* c, int factorSweeps, int applySweeps)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, factorSweeps >= 0 && factorSweeps <= 100000 && applySweeps >= 0 && applySweeps <= 100000,
+           "oc_ic0_set_sweeps: bad sweep counts");
+    c->ic0.set_sweeps(factorSweeps, applySweeps);
+    c->ic0Sol.set_sweeps(factorSweeps, applySweeps);
+    c->solFactorized = false;
+    return OC_OK;
+}
+
 int oc_solver_set_params(oc_ctx* c, int precond, double relTol, int maxIter)
 {
     if (!c) return OC_ERR_ARG;
-    OC_ARG(c, (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI || precond == OC_PRECOND_AMG) &&
+    OC_ARG(c, (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI || precond == OC_PRECOND_AMG ||
+               precond == OC_PRECOND_IC0) &&
                   relTol > 0.0 && maxIter > 0,
            "oc_solver_set_params: bad parameters");
     c->solPrecond = precond;
@@ -2431,7 +2518,7 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     OC_ARG(c, c->nV > 0, "oc_newton_init: no mesh");
     OC_ARG(c, a0 > 0.0 && relGL2Tol > 0.0 && pcgRelTol > 0.0 && pcgMaxIter > 0 &&
                   (precond == OC_PRECOND_JACOBI || precond == OC_PRECOND_BLOCK_JACOBI || precond == OC_PRECOND_AMG ||
-                   precond == OC_PRECOND_LEARNED),
+                   precond == OC_PRECOND_LEARNED || precond == OC_PRECOND_IC0),
            "oc_newton_init: bad parameters");
     (void)hipSetDevice(c->device);
     c->a0 = a0;
@@ -2661,6 +2748,7 @@ int launch_precond_setup(oc_ctx* c, oc_iter_stats* S)
         ++S->precond_setups;
     }
     if (c->nPrecond == OC_PRECOND_LEARNED) c->learned->net.forward(c->stream, c->sys.dVals.p, c->sys.vals32);
+    if (c->nPrecond == OC_PRECOND_IC0) c->ic0.numeric(c->stream, c->sys.dVals.as<double>());
     return OC_OK;
 }
 
@@ -2748,6 +2836,8 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
             system_assemble(c);
             if (c->nPrecond == OC_PRECOND_LEARNED) {
                 c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
+            } else if (c->nPrecond == OC_PRECOND_IC0) {
+                c->ic0.numeric(s, c->sys.dVals.as<double>());
             } else if (c->amgValid) {
                 amg_numeric(c, 0);
                 ++S->precond_setups;
@@ -2765,6 +2855,16 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
             st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, c->ext);
             it += it2;
             if (st == OC_OK) S->pcg_fallback = 1;
+        }
+        if (st != OC_OK && c->nPrecond == OC_PRECOND_LEARNED) {
+            // the learned factor failed: block IC(0)-PCG at the same tolerance before Jacobi-PCG
+            if (c->ic0Struct || (st = ic0_structure(c)) == OC_OK) {
+                c->ic0.numeric(s, c->sys.dVals.as<double>());
+                int it2 = 0;
+                st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, &c->ic0);
+                it += it2;
+                if (st == OC_OK) S->pcg_fallback = 7;
+            }
         }
         if (st != OC_OK) {
             st = jacobi_direction(c, S, it, rr);
@@ -3129,7 +3229,7 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
         return OC_ERR_STATE;
     }
     if ((which == 7 || which == 8) && !c->ext) {
-        c->err = "oc_time_kernel: no multigrid or learned preconditioner is active";
+        c->err = "oc_time_kernel: no multigrid, learned or IC(0) preconditioner is active";
         return OC_ERR_STATE;
     }
     if (which == 9 && !c->amgValid) {
@@ -3195,6 +3295,7 @@ int oc_time_kernel(oc_ctx* c, int which, int reps, double* ms)
             break;
         case 8:
             if (c->amgValid) amg_numeric(c, 0);
+            else if (c->nPrecond == OC_PRECOND_IC0) c->ic0.numeric(s, c->sys.dVals.as<double>());
             else
                 c->learned->net.forward(s, c->sys.dVals.p, c->sys.vals32);
             break;

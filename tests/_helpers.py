@@ -181,3 +181,45 @@ def block_csr_to_sparse(rp, ci, vals):
     r = (2 * row[:, None, None] + np.arange(2)[None, :, None]).repeat(2, 2)
     c = (2 * ci[:, None, None] + np.arange(2)[None, None, :]).repeat(2, 1)
     return sp.csr_matrix((np.asarray(vals).ravel(), (r.ravel(), c.ravel())), shape=(2 * nV, 2 * nV))
+
+
+def ic0_dense(A, P):
+    """Sequential incomplete Cholesky IC(0) of the dense symmetric matrix A (2n x 2n) on the 2x2-block pattern P
+    (n x n boolean, symmetric, with its diagonal): the lower factor L, zero outside the lower half of the expanded
+    pattern, with (L L^T)_ij = A_ij on it (the defining property of IC(0)). Row by row, left to right:
+    L_ij = (A_ij - sum_{k<j} L_ik L_jk) / L_jj and L_ii = sqrt(A_ii - sum_{k<i} L_ik^2); L is zero off the
+    pattern, so the sums run over the columns both rows hold. Raises ValueError on a non-positive pivot."""
+    n = P.shape[0]
+    S = np.kron(np.asarray(P, bool), np.ones((2, 2), bool))
+    N = 2 * n
+    L = np.zeros((N, N))
+    for i in range(N):
+        for j in np.nonzero(S[i, : i + 1])[0]:
+            s = A[i, j] - L[i, :j] @ L[j, :j]
+            if i == j:
+                if not s > 0.0:
+                    raise ValueError("IC(0) breakdown at row %d" % i)
+                L[i, i] = np.sqrt(s)
+            else:
+                L[i, j] = s / L[j, j]
+    return L
+
+
+def upper_csr_to_dense(ia, ja, a):
+    """Full symmetric dense matrix of an upper-triangle CSR (the reference's LinSysSolver ia/ja/a)."""
+    N = len(ia) - 1
+    U = np.zeros((N, N))
+    for i in range(N):
+        for k in range(ia[i], ia[i + 1]):
+            U[i, ja[k]] = a[k]
+    return U + U.T - np.diag(np.diag(U))
+
+
+def block_pattern_of_upper(ia, ja):
+    """2x2-block pattern (n x n boolean, symmetric, with the diagonal) of an upper-triangle DOF CSR."""
+    N = len(ia) - 1
+    P = np.eye(N // 2, dtype=bool)
+    for i in range(N):
+        for k in range(ia[i], ia[i + 1]):
+            P[i // 2, ja[k] // 2] = P[ja[k] // 2, i // 2] = True
+    return P
