@@ -267,6 +267,7 @@ typedef struct {
                          7: OC_PRECOND_LEARNED failed and block IC(0)-PCG at the same tolerance solved it */
     int pcg_iters_first; /* iterations of the first attempt when pcg_fallback != 0 (pcg_iters counts all) */
     int precond_setups;  /* multigrid hierarchies built on this step (1, or 2 after a safe-interval retry) */
+    int ir_passes;       /* passes of the iterative refinement (oc_newton_set_refinement), 0 when it is off */
 } oc_iter_stats;
 
 /* Optimizer::precompute (Optimizer.cpp:154-201) for energyParams = {a0}: sets targetGRes =
@@ -289,6 +290,17 @@ int oc_pcg_graphs(oc_ctx* ctx, int enable);
 /* Selects OC_PRECISION_FP64 (default) or OC_PRECISION_MIXED for the next oc_newton_init. No reference
  * counterpart (the reference is FP64 throughout). */
 int oc_newton_set_precision(oc_ctx* ctx, int precision);
+/* Mixed-precision iterative refinement of the Newton solve (maxPasses > 0; 0, the default, turns it off): the PCG
+ * iterations multiply by the FP32 rounding of the FP64 Newton matrix (half the matrix traffic of each product)
+ * while the residual and the solution stay FP64: x = 0, r = b; each pass runs PCG on A32 d = r to
+ * min(0.5, pcgRelTol |b| / |r|), then x += d and r = b - A x with the FP64 matrix, until |r| <= pcgRelTol |b|, maxPasses
+ * passes, pcgMaxIter iterations in all, or a pass that does not halve |r| (the rounding can perturb the matrix's
+ * near-null directions, the rigid rotation about a single fixed vertex, beyond what refinement corrects; the solve
+ * then ends as one at its iteration cap). Breakdowns of a pass keep the iterate it reached; the fallbacks of
+ * pcg_fallback apply to the refined solve as to a plain one. pcg_rel_res then reports the FP64 residual and
+ * ir_passes the passes. Works with every precision and preconditioner; takes effect at the next step. No
+ * reference counterpart. */
+int oc_newton_set_refinement(oc_ctx* ctx, int maxPasses);
 /* Binds the network used by OC_PRECOND_LEARNED from the next oc_newton_init on (null unbinds). The context
  * does not own it; while bound, the network's pattern and activations belong to the context's Newton system,
  * and it must not be used from another context or through oc_gnn_set_system. No reference counterpart. */

@@ -44,7 +44,7 @@ class IterStats(C.Structure):
     _fields_ = [("E_total", C.c_double), ("E_SD", C.c_double), ("E_scaf", C.c_double), ("grad_sq", C.c_double),
                 ("step", C.c_double), ("halvings", C.c_int), ("stopped", C.c_int), ("converged", C.c_int),
                 ("pcg_iters", C.c_int), ("pcg_rel_res", C.c_double), ("pcg_fallback", C.c_int),
-                ("pcg_iters_first", C.c_int), ("precond_setups", C.c_int)]
+                ("pcg_iters_first", C.c_int), ("precond_setups", C.c_int), ("ir_passes", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -121,6 +121,7 @@ _SIGS = {
     "oc_gnn_adam_step": (C.c_int, [_vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]),
     "oc_gnn_precond_apply": (C.c_int, [_vp, _dp, _dp]),
     "oc_newton_set_precision": (C.c_int, [_ctx, C.c_int]),
+    "oc_newton_set_refinement": (C.c_int, [_ctx, C.c_int]),
     "oc_newton_init": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.POINTER(C.c_double)]),
     "oc_newton_step": (C.c_int, [_ctx, C.POINTER(IterStats), C.POINTER(C.c_int)]),
@@ -520,6 +521,11 @@ class Context:
         """Bind a Gnn for PRECOND_LEARNED (None unbinds); the context keeps a reference."""
         self._learned = net
         self._chk(self.L.oc_newton_set_learned(self.h, net.h if net is not None else None))
+
+    def set_refinement(self, max_passes):
+        """Mixed-precision iterative refinement of the Newton solve (PCG on the FP32-rounded matrix, FP64
+        residual), at most max_passes passes; 0 turns it off."""
+        self._chk(self.L.oc_newton_set_refinement(self.h, int(max_passes)))
 
     def newton_init(self, a0=1.0, rel_gl2_tol=1e-12, allow_edec_rel_tol=True, precond=PRECOND_JACOBI,
                     pcg_rel_tol=1e-10, pcg_max_iter=20000, precision=PRECISION_FP64):

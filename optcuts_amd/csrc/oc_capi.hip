@@ -183,6 +183,10 @@ struct oc_ctx {
     bool amgRebuild = false;                  // a solve on a kept structure needed a fallback: rebuild it
     DevBuf dAmgOldRowPtr, dAmgMap, dAmgDropPtr, dAmgDrop, dAmgOldDiag, dAmgVals;
     oc_gnn* learned = nullptr;             // bound by oc_newton_set_learned (not owned)
+    // iterative refinement (oc_newton_set_refinement): at most irMax passes of PCG on the FP32-rounded Newton
+    // matrix dVals32, each on the FP64 residual of the accumulated solution dXir
+    int irMax = 0;
+    DevBuf dVals32, dXir, dRir, dIrPart;
     // block IC(0) of the Newton matrix: the preconditioner of OC_PRECOND_IC0, and the learned form's fallback
     // (its structure then built when first needed for the current pattern)
     oc::Ic0 ic0;
@@ -413,10 +417,13 @@ int ensure_pcg(oc_ctx* c, int nV)
 
 // PCG on pattern P for right-hand side b (device), solution in c->dX.
 // Iterations are queued in batches; ctrl is read back between batches.
+// vals (null: P's own values) and vals32: the matrix the iterations multiply by, on P's pattern.
 int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxIter, int& iters, double& relRes,
-            oc::Preconditioner* amg = nullptr, int hint = 0)
+            oc::Preconditioner* amg = nullptr, int hint = 0, const void* vals = nullptr, int vals32 = 0)
 {
     const int nV = P.nV;
+    const void* V = vals ? vals : P.dVals.p;
+    const int V32 = vals ? vals32 : P.vals32;
     PcgCtrl* ctrl = c->dCtrl.as<PcgCtrl>();
     PcgCtrl h{};
     h.tol2 = relTol * relTol;
@@ -440,11 +447,11 @@ int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxI
         double* pNew = pbuf[par ^ 1];
         const double* pu = partU[par];
         if (amg)
-            oc::launch_pcg_spmv(c->stream, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.p, P.vals32,
+            oc::launch_pcg_spmv(c->stream, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), V, V32,
                                 c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), partRZ, amg->rz_records(), 1,
                                 pu + 1, GU, 2, partS, ctrl);
         else
-            oc::launch_pcg_spmv(c->stream, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.p, P.vals32,
+            oc::launch_pcg_spmv(c->stream, nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), V, V32,
                                 c->dZ.as<double>(), pOld, pNew, c->dQ.as<double>(), pu, GU, 2, pu + 1, GU, 2, partS,
                                 ctrl);
         oc::launch_pcg_update(c->stream, nV, pNew, c->dQ.as<double>(), Minv, c->dX.as<double>(), c->dR.as<double>(),
@@ -457,7 +464,7 @@ int run_pcg(oc_ctx* c, BlockPattern& P, const double* b, double relTol, int maxI
     constexpr int kChain = 8;
     if (c->pcgGraphs) {
         const uintptr_t sig[] = {(uintptr_t)nV,           (uintptr_t)P.dRowPtr.p, (uintptr_t)P.dColIdx.p,
-                                 (uintptr_t)P.dVals.p,    (uintptr_t)P.vals32,    (uintptr_t)Minv,
+                                 (uintptr_t)V,            (uintptr_t)V32,         (uintptr_t)Minv,
                                  (uintptr_t)c->dX.p,      (uintptr_t)c->dR.p,     (uintptr_t)c->dZ.p,
                                  (uintptr_t)c->dP0.p,     (uintptr_t)c->dP1.p,    (uintptr_t)c->dQ.p,
                                  (uintptr_t)c->dPcgPart.p, (uintptr_t)ctrl,       (uintptr_t)amg,
@@ -2720,6 +2727,14 @@ int oc_newton_set_precision(oc_ctx* c, int precision)
     return OC_OK;
 }
 
+int oc_newton_set_refinement(oc_ctx* c, int maxPasses)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, maxPasses >= 0 && maxPasses <= 100, "oc_newton_set_refinement: maxPasses must be in [0, 100]");
+    c->irMax = maxPasses;
+    return OC_OK;
+}
+
 namespace {
 // Element pass + Hessian assembly of the Newton system in the driver's precision.
 // FP32 element terms in the Newton system (mixed form, not escalated)
@@ -2776,6 +2791,89 @@ int jacobi_direction(oc_ctx* c, oc_iter_stats* S, int& it, double& rr)
 // Newton direction H d = b (b in c->dB, d in c->dX) with the selected
 // preconditioner and its fallbacks; fills the PCG fields of S. setupLaunched:
 // launch_precond_setup was already queued for this matrix.
+// Sum of the residual-norm records (fixed order, so the result does not depend on timing).
+int residual_sq(oc_ctx* c, long n, const double* b, const double* y, double* r, double& out)
+{
+    const int g = oc::launch_residual_norm(c->stream, n, b, y, r, c->dIrPart.as<double>());
+    OC_HIP(c, hipGetLastError());
+    std::vector<double> h(g);
+    OC_HIP(c, hipMemcpyAsync(h.data(), c->dIrPart.p, sizeof(double) * g, hipMemcpyDeviceToHost, c->stream));
+    OC_HIP(c, hipStreamSynchronize(c->stream));
+    double sum = 0.0;
+    for (double v : h) sum += v;
+    out = sum;
+    return OC_OK;
+}
+
+// Mixed-precision iterative refinement of the Newton solve A x = b (A FP64): the PCG iterations multiply by the
+// FP32 rounding of A (half the matrix traffic) and the residual and the solution are kept in FP64. (A diagonal
+// guard on the rounding, 4 * 2^-24 times each row's absolute sum, keeps A32 definite but was measured to stall
+// the refinement on the near-null rigid rotation; it is left off.)
+//   x = 0, r = b;  pass k: PCG on A32 d = r to tol_k = min(0.5, pcgTol |b| / |r|) (a breakdown keeps the
+//   iterate it reached), x += d, r = b - A x in FP64;  until |r| <= pcgTol |b|, irMax passes, the iteration
+//   cap or a pass that does not halve |r|. The result is in dX; relRes is the FP64 residual. Returns OC_OK,
+//   OC_ERR_NOT_CONVERGED (cap, passes spent or stagnation) or OC_ERR_BREAKDOWN (a first pass with no iteration).
+int refined_pcg(oc_ctx* c, int& iters, double& relRes, int& passes)
+{
+    hipStream_t s = c->stream;
+    BlockPattern& P = c->sys;
+    const long n = 2L * P.nV;
+    OC_HIP(c, c->dVals32.ensure(sizeof(float) * 4 * (size_t)std::max<long>(1, P.nnzb())));
+    OC_HIP(c, c->dXir.ensure(sizeof(double) * (size_t)n));
+    OC_HIP(c, c->dRir.ensure(sizeof(double) * (size_t)n));
+    OC_HIP(c, c->dIrPart.ensure(sizeof(double) * (size_t)oc::grid_for(n)));
+    oc::launch_round_blocks(s, P.nV, P.dRowPtr.as<int>(), P.dDiagPos.as<int>(), P.dVals.as<double>(),
+                            c->dVals32.as<float>(), 0.0);
+    double bb = 0.0, rrTrue = 0.0;
+    if (int st = residual_sq(c, n, c->dB.as<double>(), nullptr, c->dRir.as<double>(), bb)) return st;
+    iters = 0;
+    passes = 0;
+    relRes = 0.0;
+    if (bb == 0.0) {
+        oc::launch_scale_copy(s, n, 0.0, c->dB.as<double>(), c->dX.as<double>());
+        return OC_OK;
+    }
+    rrTrue = bb;
+    double rrPrev = bb;
+    int st = OC_ERR_NOT_CONVERGED;
+    while (passes < c->irMax && iters < c->pcgMaxIter) {
+        const double tolK = std::min(0.5, c->pcgTol * std::sqrt(bb / rrTrue));
+        int it = 0;
+        double rr = 0.0;
+        const int sp = run_pcg(c, P, c->dRir.as<double>(), tolK, c->pcgMaxIter - iters, it, rr, c->ext,
+                               passes == 0 ? c->pcgHint : 0, c->dVals32.p, 1);
+        if (sp != OC_OK && sp != OC_ERR_NOT_CONVERGED && sp != OC_ERR_BREAKDOWN) return sp;
+        iters += it;
+        ++passes;
+        if (it == 0) {
+            if (passes == 1) return OC_ERR_BREAKDOWN;
+            break;
+        }
+        // x += d (the first pass starts x from d); r = b - A x
+        if (passes == 1)
+            OC_HIP(c, hipMemcpyAsync(c->dXir.p, c->dX.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        else
+            oc::launch_axpy_commit(s, n, 1.0, c->dX.as<double>(), c->dXir.as<double>());
+        oc::launch_spmv(s, P.nV, P.dRowPtr.as<int>(), P.dColIdx.as<int>(), P.dVals.p, 0, c->dXir.as<double>(),
+                        c->dTmpVec.as<double>());
+        if (int e = residual_sq(c, n, c->dB.as<double>(), c->dTmpVec.as<double>(), c->dRir.as<double>(), rrTrue))
+            return e;
+        if (rrTrue <= c->pcgTol * c->pcgTol * bb) {
+            st = OC_OK;
+            break;
+        }
+        if (sp == OC_ERR_NOT_CONVERGED) break;  // the iteration cap
+        // stagnation: the FP32 rounding perturbs the Newton matrix's near-null directions (the rigid rotation
+        // about the fixed vertex) by more than the refinement can correct; the iterate so far is kept
+        if (rrTrue > 0.25 * rrPrev) break;
+        rrPrev = rrTrue;
+    }
+    if (passes > 0)
+        OC_HIP(c, hipMemcpyAsync(c->dX.p, c->dXir.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    relRes = std::sqrt(rrTrue / bb);
+    return st;
+}
+
 int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
 {
     hipStream_t s = c->stream;
@@ -2792,7 +2890,14 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
         S->pcg_rel_res = rr;
         return st != OC_OK && st != OC_ERR_NOT_CONVERGED ? st : OC_OK;
     }
-    int st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext, c->pcgHint);
+    int st;
+    if (c->irMax > 0) {
+        int passes = 0;
+        st = refined_pcg(c, it, rr, passes);
+        S->ir_passes = passes;
+    } else {
+        st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext, c->pcgHint);
+    }
     c->pcgHint = st == OC_OK ? it : 0;
     if (!c->ext && st == OC_ERR_BREAKDOWN) {
         // Jacobi-PCG met p.Ap <= 0: the Newton matrix is singular along p (one triangle whose only fixed vertex

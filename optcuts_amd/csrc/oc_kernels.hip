@@ -1276,6 +1276,63 @@ void launch_remap_blocks(hipStream_t s, int n, const int* oldRowPtr, const int* 
                        oldDiag, reinterpret_cast<const double4*>(newVals), reinterpret_cast<double4*>(oldVals));
 }
 
+// ---- iterative refinement ----------------------------------------------------------------------------------
+// Row u of the FP32 rounding of a 2x2-block matrix, with factor * (row abs sum) added to the diagonal entries of
+// u's diagonal block before rounding (a Gershgorin guard: with factor 4 * 2^-24 it exceeds what the rounding can
+// take from any eigenvalue, so a positive semidefinite FP64 matrix rounds to a positive definite FP32 one).
+__global__ void __launch_bounds__(kBlock) k_round_blocks(int n, const int* __restrict__ rowPtr,
+                                                         const int* __restrict__ diagPos,
+                                                         const double4* __restrict__ in, float4* out, double factor)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    double rs0 = 0.0, rs1 = 0.0;
+    for (int k = rowPtr[u]; k < rowPtr[u + 1]; ++k) {
+        const double4 v = in[k];
+        rs0 += fabs(v.x) + fabs(v.y);
+        rs1 += fabs(v.z) + fabs(v.w);
+    }
+    const int d = diagPos[u];
+    for (int k = rowPtr[u]; k < rowPtr[u + 1]; ++k) {
+        double4 v = in[k];
+        if (k == d) {
+            v.x += factor * rs0;
+            v.w += factor * rs1;
+        }
+        out[k] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+    }
+}
+
+void launch_round_blocks(hipStream_t s, int nV, const int* rowPtr, const int* diagPos, const double* in, float* out,
+                         double factor)
+{
+    if (nV <= 0) return;
+    hipLaunchKernelGGL(k_round_blocks, dim3(grid_for(nV)), dim3(kBlock), 0, s, nV, rowPtr, diagPos,
+                       reinterpret_cast<const double4*>(in), reinterpret_cast<float4*>(out), factor);
+}
+
+// r = b - y (y null: r = b) over n doubles and per-workgroup sums of r.r in part (grid_for(n) records).
+__global__ void __launch_bounds__(kBlock) k_residual_norm(long n, const double* __restrict__ b,
+                                                          const double* __restrict__ y, double* r, double* part)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[1] = {0.0};
+    if (k < n) {
+        const double v = y ? b[k] - y[k] : b[k];
+        if (r) r[k] = v;
+        acc[0] = v * v;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+int launch_residual_norm(hipStream_t s, long n, const double* b, const double* y, double* r, double* part)
+{
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(k_residual_norm, dim3(g), dim3(kBlock), 0, s, n, b, y, r, part);
+    return g;
+}
+
 void launch_uv_gather(hipStream_t s, int n, const int* ids, const double* UV, double* out3, double* out2)
 {
     hipLaunchKernelGGL(k_uv_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, n, ids,

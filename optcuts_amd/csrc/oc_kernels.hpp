@@ -198,6 +198,12 @@ void launch_set_u64(hipStream_t s, unsigned long long* p, unsigned long long v);
 // earlier pattern lacks, whose Frobenius norms go onto both diagonal entries of the row's diagonal block, oldDiag).
 void launch_remap_blocks(hipStream_t s, int n, const int* oldRowPtr, const int* map, const int* dropPtr,
                          const int* drop, const int* oldDiag, const double* newVals, double* oldVals);
+// Iterative refinement: out = the FP32 rounding of the 2x2-block matrix in (nV block rows) with factor times each
+// row's absolute sum added to its diagonal entries first; r = b - y (y null: r = b) over n doubles with
+// per-workgroup sums of r.r in part (returns the record count, grid_for(n); r may be null).
+void launch_round_blocks(hipStream_t s, int nV, const int* rowPtr, const int* diagPos, const double* in, float* out,
+                         double factor);
+int launch_residual_norm(hipStream_t s, long n, const double* b, const double* y, double* r, double* part);
 // b = sign * g for n doubles.
 void launch_scale_copy(hipStream_t s, long n, double sign, const double* g, double* b);
 // For k < n, p = UV row ids[k] (row k when ids is null): out3[k] = (p.x, p.y, 0) and out2[k] = p (either may be
