@@ -139,6 +139,9 @@ struct oc_ctx {
     int nVt = 0, nFt = 0;
     BlockPattern sys;
     oc::SystemMaps sm;  // upper blocks of sys and their assembly map
+    oc::HostPattern sysSpare;  // the previous build's pattern and maps: storage for the next build
+    oc::SystemMaps smSpare;
+    std::vector<int> FtBuf;  // combined triangle list (mesh, then air) of the last build
     std::vector<int> vcPtr, vcSrc;
     // what sys and sm were built from: the mesh (meshGen of oc_set_mesh) and
     // the mesh vertices in air triangles; rows of the other mesh vertices are
@@ -681,7 +684,8 @@ int build_system(oc_ctx* c, bool airOnly = false)
     const int nFt = c->nF + c->nFair;
     c->nVt = nVt;
     c->nFt = nFt;
-    std::vector<int> Ft(3 * (size_t)nFt);
+    std::vector<int>& Ft = c->FtBuf;
+    Ft.resize(3 * (size_t)nFt);
     std::copy(c->Fh.begin(), c->Fh.end(), Ft.begin());
     std::copy(c->FairGlobal.begin(), c->FairGlobal.end(), Ft.begin() + 3 * (size_t)c->nF);
 
@@ -710,15 +714,12 @@ int build_system(oc_ctx* c, bool airOnly = false)
         keep.resize(c->nV);
         for (int v = 0; v < c->nV; ++v) keep[v] = !touched[v] && !c->sysAirTouched[v];
     }
-    {
-        oc::HostPattern P;
-        oc::SystemMaps M;
-        oc::build_system_maps(nVt, Ft, c->fixedMaskSys.data(), c->vcPtr, c->vcSrc, P, M,
-                              keep.empty() ? nullptr : &c->sys, keep.empty() ? nullptr : &c->sm,
-                              keep.empty() ? nullptr : keep.data(), (int)keep.size());
-        static_cast<oc::HostPattern&>(c->sys) = std::move(P);
-        c->sm = std::move(M);
-    }
+    // built into the spare pattern and maps (the build before last's, whose memory is reused), then swapped in
+    oc::build_system_maps(nVt, Ft, c->fixedMaskSys.data(), c->vcPtr, c->vcSrc, c->sysSpare, c->smSpare,
+                          keep.empty() ? nullptr : &c->sys, keep.empty() ? nullptr : &c->sm,
+                          keep.empty() ? nullptr : keep.data(), (int)keep.size());
+    std::swap(static_cast<oc::HostPattern&>(c->sys), c->sysSpare);
+    std::swap(c->sm, c->smSpare);
     c->sysBuilt = true;
     c->sysMeshGen = c->meshGen;
     c->sysAirTouched = std::move(touched);

@@ -35,7 +35,8 @@ inline void build_vertex_lists(int nV, int nF, const int* F, std::vector<int>& p
         for (int k = 0; k < 3; ++k) ptr[F[3 * t + k] + 1]++;
     for (int v = 0; v < nV; ++v) ptr[v + 1] += ptr[v];
     src.resize(ptr[nV]);
-    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    static thread_local std::vector<int> fill;  // kept per host thread for its memory
+    fill.assign(ptr.begin(), ptr.end() - 1);
     for (int t = 0; t < nF; ++t)
         for (int k = 0; k < 3; ++k) src[fill[F[3 * t + k]]++] = 3 * t + k;
 }
@@ -60,6 +61,19 @@ struct SystemMaps {
 // copied from the earlier build instead of regenerated; the result is
 // identical to a build without them. Rows run in chunks on host threads and are written at their final
 // offsets, so the output does not depend on the thread count.
+// Work arrays of build_system_maps, kept per host thread so that repeated builds (one per air re-mesh) reuse their
+// memory instead of allocating and first-touching megabytes each time.
+struct PatternScratch {
+    std::vector<std::vector<int>> part;
+    std::vector<int> len, ubStart, ubStartO, posToUB, cnt;
+};
+inline PatternScratch& pattern_scratch()
+{
+    static thread_local PatternScratch s;
+    return s;
+}
+
+// P and M may hold an earlier build's arrays (their memory is reused; every entry is rewritten).
 inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigned char* fixed,
                               const std::vector<int>& vcPtr, const std::vector<int>& vcSrc, HostPattern& P,
                               SystemMaps& M, const HostPattern* prevP = nullptr, const SystemMaps* prevM = nullptr,
@@ -69,12 +83,16 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
     const int nth = setup_threads(3L * nFt);
     const bool reuse = prevP && prevM && keep;
     const auto kept = [&](int u) { return reuse && u < nKeep && u < prevP->nV && keep[u]; };
+    PatternScratch& S = pattern_scratch();
 
     // ---- block pattern
     P.nV = nVt;
     {
-        std::vector<std::vector<int>> part(nth);
-        std::vector<int> len(nVt);
+        std::vector<std::vector<int>>& part = S.part;
+        part.resize(nth);
+        for (auto& q : part) q.clear();
+        std::vector<int>& len = S.len;
+        len.resize(nVt);
         parallel_row_chunks(nVt, nth, [&](int ch, int r0, int r1) {
             std::vector<int> loc;
             auto& out = part[ch];
@@ -127,10 +145,12 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
 
     // ---- upper blocks (w >= u) in row order: rows are sorted and hold their
     // diagonal, so row u's upper blocks are its entries from the diagonal on
-    std::vector<int> ubStart(nVt + 1, 0);
+    std::vector<int>& ubStart = S.ubStart;
+    ubStart.assign(nVt + 1, 0);
     for (int u = 0; u < nVt; ++u) ubStart[u + 1] = ubStart[u] + (P.rowPtr[u + 1] - P.diagPos[u]);
     const int nUB = ubStart[nVt];
-    std::vector<int> posToUB(P.nnzb(), -1);
+    std::vector<int>& posToUB = S.posToUB;
+    posToUB.assign(P.nnzb(), -1);
     M.ubRow.resize(nUB);
     M.ubCol.resize(nUB);
     M.ubPosU.resize(nUB);
@@ -153,7 +173,7 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
     // u <= w, receives the element block (a, b) of every triangle t with
     // f[a] = u and f[b] = w (for u == w only a == b); walking u's triangles in
     // order gives each of its blocks its contributions in triangle order.
-    std::vector<int> ubStartO;
+    std::vector<int>& ubStartO = S.ubStartO;
     if (reuse) {
         ubStartO.assign(prevP->nV + 1, 0);
         for (int u = 0; u < prevP->nV; ++u)
@@ -174,7 +194,8 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
             }
         }
     };
-    std::vector<int> cnt(nUB, 0);
+    std::vector<int>& cnt = S.cnt;
+    cnt.assign(nUB, 0);
     parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
         for (int u = r0; u < r1; ++u) {
             if (kept(u)) {

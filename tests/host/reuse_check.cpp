@@ -101,16 +101,19 @@ int compare(const Build& a, const Build& b, char* what, int whatLen)
 
 extern "C" {
 
-// Air mesh 1 then air mesh 2 (nFa2 = 0: no air mesh); stats[0..3] = levels,
+// Air mesh 1 then air mesh 2 (nFa2 = 0: no air mesh), the second from scratch and incrementally (reusing the
+// first build's rows, written into storage that held an earlier build); stats[0..3] = levels,
 // kept mesh rows, level-0 aggregates, level-0 nodes of the second build.
 int reuse_check(int nV, int nF, const int* F, int nFixed, const int* fixedV, int nVt1, int nFa1, const int* Fa1,
                 int nVt2, int nFa2, const int* Fa2, int* stats, char* what, int whatLen)
 {
     std::string err;
-    Build b1, full, inc, donor;
+    Build b1, full, donor;
     build(nV, nF, F, nVt1, nFa1, Fa1, nFixed, fixedV, nullptr, b1, err);
     build(nV, nF, F, nVt2, nFa2, Fa2, nFixed, fixedV, nullptr, full, err);
     build(nV, nF, F, nVt1, nFa1, Fa1, nFixed, fixedV, nullptr, donor, err);
+    // as the library does, the incremental build writes into storage that holds an earlier build's arrays
+    Build inc = donor;
     build(nV, nF, F, nVt2, nFa2, Fa2, nFixed, fixedV, &b1, inc, err, &donor.L);
     int kept = 0;
     for (int v = 0; v < nV; ++v) kept += !b1.touched[v] && !full.touched[v];
