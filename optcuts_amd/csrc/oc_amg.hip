@@ -14,7 +14,14 @@ namespace {
 
 constexpr int NS = 3;              // near-null-space vectors: two translations and a rotation
 constexpr int kAmgMaxGrid = 1024;  // workgroups of a row kernel (bounds partial records)
-constexpr int kCoarseMaxDofs = 64;  // the coarsest inverse lives in LDS (64 x 64 doubles)
+#ifndef OC_COARSE_MAX_DOFS
+#define OC_COARSE_MAX_DOFS 64
+#endif
+// Coarsening stops at the first level with at most this many unknowns; its dense inverse is formed in LDS (up to
+// 128 x 128 doubles = 128 KiB of gfx950's 160 KiB would fit). Measured at 200k faces: 128 (a 38-node coarsest
+// level instead of 4) saves 13 us per V-cycle but its Gauss-Jordan inverse, one workgroup with a barrier pair per
+// pivot, takes 0.58 ms per set-up, so the disc step slows by 9%; 64 stays.
+constexpr int kCoarseMaxDofs = OC_COARSE_MAX_DOFS;
 constexpr int kLanczosSteps = 15;       // Lanczos steps of the lambda_max estimate (fine level; array size)
 constexpr int kLanczosStepsCoarse = 10;  // on coarse levels
 constexpr int kMaxLevels = 10;
@@ -924,20 +931,50 @@ __global__ void __launch_bounds__(256) k_dense_inverse(int nn, int bs, const int
                 for (int j = 0; j < bs; ++j) W[(u * bs + i) * N + w * bs + j] = (double)vals[(size_t)k * bs * bs + i * bs + j];
         }
     __syncthreads();
+    // Gauss-Jordan without pivoting (the matrix is symmetric positive definite), two barriers per pivot. For
+    // pivot p, colp[p & 1] holds column p with its own entry zeroed (so that the elimination leaves row p alone)
+    // and pv[p & 1] its reciprocal pivot. Thread pair (j, half) eliminates column j over one half of the rows,
+    // 8 rows per batch with all loads before the stores so that the LDS accesses overlap; then row p is scaled,
+    // column p set, and column p + 1 and its pivot taken for the next step (final once p is eliminated).
+    __shared__ double colp[2][kCoarseMaxDofs];
+    __shared__ double pv[2];
+    const int j = threadIdx.x >> 1, half = threadIdx.x & 1;
+    const int hN = (N + 1) / 2, i0 = half * hN, i1 = min(N, i0 + hN);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) colp[0][i] = i == 0 ? 0.0 : W[i * N];
+    if (threadIdx.x == 0) pv[0] = W[0] != 0.0 ? 1.0 / W[0] : 0.0;
+    __syncthreads();
     for (int p = 0; p < N; ++p) {
-        const double app = W[p * N + p];
-        const double piv = app != 0.0 ? 1.0 / app : 0.0;
-        for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
-            const int i = e / N, j = e % N;
-            if (i != p && j != p) W[e] -= W[i * N + p] * W[p * N + j] * piv;
+        const double* cp = colp[p & 1];
+        const double piv = pv[p & 1];
+        if (j < N && j != p) {
+            const double wpj = W[p * N + j] * piv;
+            for (int ib = i0; ib < i1; ib += 8) {
+                double w[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) w[q] = ib + q < i1 ? W[(ib + q) * N + j] : 0.0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) w[q] -= (ib + q < i1 ? cp[ib + q] : 0.0) * wpj;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (ib + q < i1 && ib + q != p) W[(ib + q) * N + j] = w[q];
+            }
         }
         __syncthreads();
-        for (int j = threadIdx.x; j < N; j += blockDim.x)
-            if (j != p) {
-                W[p * N + j] *= piv;
-                W[j * N + p] *= -piv;
+        const int pn = p + 1;
+        // row p scaled (its entry in column p + 1 by the column pass below, which reads it)
+        if (j < N && j != p && j != pn && half == 0) W[p * N + j] *= piv;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            W[i * N + p] = i == p ? piv : -cp[i] * piv;
+            if (pn < N) {
+                double v = W[i * N + pn];
+                if (i == p) {
+                    v *= piv;
+                    W[i * N + pn] = v;
+                }
+                colp[pn & 1][i] = i == pn ? 0.0 : v;
+                if (i == pn) pv[pn & 1] = v != 0.0 ? 1.0 / v : 0.0;
             }
-        if (threadIdx.x == 0) W[p * N + p] = piv;
+        }
         __syncthreads();
     }
     for (int i = threadIdx.x; i < N * N; i += blockDim.x) Inv[i] = W[i];
