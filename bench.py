@@ -111,8 +111,8 @@ def closed_forms(args):
     solve captures the exact step and long solves only cost time); the plain form runs Jacobi-PCG to 1e-10."""
     from optcuts_amd import capi
     fp64, mixed = capi.PRECISION_FP64, capi.PRECISION_MIXED
-    return {"chosen": (capi.PRECOND_AMG, args.chosen_tol, 100, fp64),
-            "mixed": (capi.PRECOND_AMG, args.chosen_tol, 100, mixed),
+    return {"chosen": (capi.PRECOND_AMG, args.chosen_tol, args.chosen_cap, fp64),
+            "mixed": (capi.PRECOND_AMG, args.chosen_tol, args.chosen_cap, mixed),
             "learned": (capi.PRECOND_LEARNED, args.learned_tol, 2000, fp64),
             "learned_mixed": (capi.PRECOND_LEARNED, args.learned_tol, 2000, mixed),
             "block_jacobi_at_learned_tol": (capi.PRECOND_BLOCK_JACOBI, args.learned_tol, 2000, fp64),
@@ -494,6 +494,8 @@ def main():
     ap.add_argument("--workload", default="tori_200k", choices=["tori_200k", "icosphere_100k", "icosphere_1m"])
     ap.add_argument("--pcg-tol", type=float, default=1e-10, help="plain form PCG relative tolerance")
     ap.add_argument("--chosen-tol", type=float, default=1e-3, help="chosen form (FP64 multigrid) PCG tolerance")
+    ap.add_argument("--chosen-cap", type=int, default=100,
+                    help="PCG iteration cap of the chosen and mixed forms on the closed surfaces")
     ap.add_argument("--mixed-tol", type=float, default=1e-4, help="mixed form PCG tolerance (disc section)")
     ap.add_argument("--learned-tol", type=float, default=5e-3, help="learned forms' PCG tolerance (disc section)")
     ap.add_argument("--air-interval", type=int, default=8,
