@@ -7,6 +7,7 @@
 #include <condition_variable>
 #include <functional>
 #include <mutex>
+#include <atomic>
 #include <thread>
 #include <numeric>
 #include <string>
@@ -131,8 +132,14 @@ inline void parallel_row_chunks(int n, int nChunks, Fn&& fn)
     }
     const std::function<void(int)> task(range);
     if (RowPool::instance().run(nChunks, task)) return;
+    // pool busy (another context is building): own threads, at most 16, claiming chunks in turn
+    const int nT = std::min(nChunks, 16);
+    std::atomic<int> next{0};
     std::vector<std::thread> th;
-    for (int c = 0; c < nChunks; ++c) th.emplace_back([&task, c] { task(c); });
+    for (int t = 0; t < nT; ++t)
+        th.emplace_back([&task, &next, nChunks] {
+            for (int c = next++; c < nChunks; c = next++) task(c);
+        });
     for (auto& t : th) t.join();
 }
 

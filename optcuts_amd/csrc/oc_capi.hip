@@ -13,6 +13,7 @@
 #include "oc_system_pattern.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <chrono>
 #include <cfloat>
@@ -185,6 +186,8 @@ struct oc_ctx {
     int amgRemaps = 0;                        // air re-meshes since the structure was last built
     bool amgRebuild = false;                  // a solve on a kept structure needed a fallback: rebuild it
     DevBuf dAmgOldRowPtr, dAmgMap, dAmgDropPtr, dAmgDrop, dAmgOldDiag, dAmgVals;
+    std::vector<int> amgMapH, amgDropPtrH, amgDropH;  // host side of the map (kept for its memory)
+    bool amgPatUploaded = false;                      // dAmgOldRowPtr / dAmgOldDiag hold amgPat
     oc_gnn* learned = nullptr;             // bound by oc_newton_set_learned (not owned)
     // iterative refinement (oc_newton_set_refinement): at most irMax passes of PCG on the FP32-rounded Newton
     // matrix dVals32, each on the FP64 residual of the accumulated solution dXir
@@ -819,6 +822,7 @@ int amg_structure(oc_ctx* c)
     }
     c->amgPat = static_cast<const oc::HostPattern&>(c->sys);
     c->amgPatOwner = c->amg;
+    c->amgPatUploaded = false;
     c->amgRemaps = 0;
     c->amgValid = true;
     c->amgPending = false;
@@ -837,22 +841,50 @@ int amg_remap(oc_ctx* c)
     const oc::HostPattern& O = c->amgPat;
     const oc::HostPattern& N = c->sys;
     const int n = O.nV;
-    std::vector<int> map(O.colIdx.size()), dropPtr(n + 1, 0), drop;
-    for (int u = 0; u < n; ++u) {
-        int i = O.rowPtr[u], j = N.rowPtr[u];
-        const int ie = O.rowPtr[u + 1], je = N.rowPtr[u + 1];
-        while (i < ie || j < je) {
-            if (j == je || (i < ie && O.colIdx[i] < N.colIdx[j])) map[i++] = -1;
-            else if (i == ie || N.colIdx[j] < O.colIdx[i]) drop.push_back(j++);
-            else map[i++] = j++;
+    std::vector<int>& map = c->amgMapH;
+    std::vector<int>& dropPtr = c->amgDropPtrH;
+    std::vector<int>& drop = c->amgDropH;
+    map.resize(O.colIdx.size());
+    dropPtr.assign(n + 1, 0);
+    // pass 1 (rows in parallel): the map, and per row the count of current blocks the kept pattern lacks
+    std::atomic<bool> bad{false};
+    const int nt = oc::setup_threads((long)O.colIdx.size());
+    const int nch = nt > 1 ? 8 * nt : 1;
+    oc::parallel_row_chunks(n, nch, [&](int, int r0, int r1) {
+        for (int u = r0; u < r1; ++u) {
+            int i = O.rowPtr[u], j = N.rowPtr[u], nd = 0;
+            const int ie = O.rowPtr[u + 1], je = N.rowPtr[u + 1];
+            while (i < ie || j < je) {
+                if (j == je || (i < ie && O.colIdx[i] < N.colIdx[j])) map[i++] = -1;
+                else if (i == ie || N.colIdx[j] < O.colIdx[i]) { ++nd; ++j; }
+                else map[i++] = j++;
+            }
+            dropPtr[u + 1] = nd;
+            if (O.diagPos[u] < 0 || map[O.diagPos[u]] < 0) bad = true;
         }
-        dropPtr[u + 1] = (int)drop.size();
-        if (O.diagPos[u] < 0 || map[O.diagPos[u]] < 0) return -1;
+    });
+    if (bad) return -1;
+    for (int u = 0; u < n; ++u) dropPtr[u + 1] += dropPtr[u];
+    if ((double)dropPtr[n] > kAmgKeepDrop * (double)O.nnzb()) return -1;
+    drop.resize(std::max(1, dropPtr[n]));  // one unread entry when empty keeps the upload non-empty
+    // pass 2: the lacking blocks of each row, in column order
+    oc::parallel_row_chunks(n, nch, [&](int, int r0, int r1) {
+        for (int u = r0; u < r1; ++u) {
+            int i = O.rowPtr[u], j = N.rowPtr[u], q = dropPtr[u];
+            const int ie = O.rowPtr[u + 1], je = N.rowPtr[u + 1];
+            while (j < je) {
+                if (i < ie && O.colIdx[i] < N.colIdx[j]) ++i;
+                else if (i == ie || N.colIdx[j] < O.colIdx[i]) drop[q++] = j++;
+                else { ++i; ++j; }
+            }
+        }
+    });
+    // the kept pattern's rows and diagonal positions go up once per structure
+    if (!c->amgPatUploaded) {
+        if (upload(c, c->dAmgOldRowPtr, O.rowPtr) || upload(c, c->dAmgOldDiag, O.diagPos)) return OC_ERR_HIP;
+        c->amgPatUploaded = true;
     }
-    if ((double)drop.size() > kAmgKeepDrop * (double)O.nnzb()) return -1;
-    if (drop.empty()) drop.push_back(0);  // never read (empty rows); keeps the upload non-empty
-    if (upload(c, c->dAmgOldRowPtr, O.rowPtr) || upload(c, c->dAmgMap, map) || upload(c, c->dAmgDropPtr, dropPtr) ||
-        upload(c, c->dAmgDrop, drop) || upload(c, c->dAmgOldDiag, O.diagPos))
+    if (upload(c, c->dAmgMap, map) || upload(c, c->dAmgDropPtr, dropPtr) || upload(c, c->dAmgDrop, drop))
         return OC_ERR_HIP;
     OC_HIP(c, c->dAmgVals.ensure(sizeof(double) * 4 * (size_t)std::max<long>(1, O.nnzb())));
     OC_HIP(c, hipStreamSynchronize(c->stream));  // the host maps end with this call

@@ -80,7 +80,11 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
                               const unsigned char* keep = nullptr, int nKeep = 0)
 {
     const long nFt = (long)Ft.size() / 3;
+    // rows in 8 chunks per host thread, claimed dynamically: the rows an air mesh changes (which are regenerated,
+    // not copied) sit together at the end of the numbering, so one chunk per thread would leave one thread with
+    // most of the work
     const int nth = setup_threads(3L * nFt);
+    const int nch = nth > 1 ? 8 * nth : 1;
     const bool reuse = prevP && prevM && keep;
     const auto kept = [&](int u) { return reuse && u < nKeep && u < prevP->nV && keep[u]; };
     PatternScratch& S = pattern_scratch();
@@ -89,11 +93,11 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
     P.nV = nVt;
     {
         std::vector<std::vector<int>>& part = S.part;
-        part.resize(nth);
+        part.resize(nch);
         for (auto& q : part) q.clear();
         std::vector<int>& len = S.len;
         len.resize(nVt);
-        parallel_row_chunks(nVt, nth, [&](int ch, int r0, int r1) {
+        parallel_row_chunks(nVt, nch, [&](int ch, int r0, int r1) {
             std::vector<int> loc;
             auto& out = part[ch];
             for (int v = r0; v < r1; ++v) {
@@ -122,7 +126,7 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
         for (int v = 0; v < nVt; ++v) P.rowPtr[v + 1] = P.rowPtr[v] + len[v];
         P.colIdx.resize(P.rowPtr[nVt]);
         P.diagPos.resize(nVt);
-        parallel_row_chunks(nVt, nth, [&](int ch, int r0, int r1) {
+        parallel_row_chunks(nVt, nch, [&](int ch, int r0, int r1) {
             size_t o = 0;
             const auto& in = part[ch];
             for (int v = r0; v < r1; ++v) {
@@ -155,7 +159,7 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
     M.ubCol.resize(nUB);
     M.ubPosU.resize(nUB);
     M.ubPosL.resize(nUB);
-    parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
+    parallel_row_chunks(nVt, nch, [&](int, int r0, int r1) {
         for (int u = r0; u < r1; ++u) {
             int b = ubStart[u];
             for (int k = P.diagPos[u]; k < P.rowPtr[u + 1]; ++k, ++b) {
@@ -196,7 +200,7 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
     };
     std::vector<int>& cnt = S.cnt;
     cnt.assign(nUB, 0);
-    parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
+    parallel_row_chunks(nVt, nch, [&](int, int r0, int r1) {
         for (int u = r0; u < r1; ++u) {
             if (kept(u)) {
                 const int bo = ubStartO[u];
@@ -211,7 +215,7 @@ inline void build_system_maps(int nVt, const std::vector<int>& Ft, const unsigne
     M.ubCPtr[0] = 0;
     for (int b = 0; b < nUB; ++b) M.ubCPtr[b + 1] = M.ubCPtr[b] + cnt[b];
     M.ubCSrc.resize(M.ubCPtr[nUB]);
-    parallel_row_chunks(nVt, nth, [&](int, int r0, int r1) {
+    parallel_row_chunks(nVt, nch, [&](int, int r0, int r1) {
         std::vector<int> fill;
         for (int u = r0; u < r1; ++u) {
             const int b0 = ubStart[u], b1 = ubStart[u + 1];
