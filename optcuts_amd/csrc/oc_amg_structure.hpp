@@ -150,6 +150,14 @@ inline int setup_threads(long work)
     return work < 20000 ? 1 : t;
 }
 
+// Row chunks for a parallel region: 8 per host thread, claimed dynamically, so that rows of uneven cost (the
+// rows an air mesh touches are numbered, and aggregated, last) spread over the threads.
+inline int setup_chunks(long work)
+{
+    const int t = setup_threads(work);
+    return t > 1 ? 8 * t : 1;
+}
+
 // Symbolic product rows: for each output row, the distinct columns (sorted)
 // and the contributing (left, right) pairs grouped by column in generation
 // order. gen(row, emit) calls emit(col, left, right) for every pair of the row
@@ -391,7 +399,7 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
     for (;;) {
         AmgHostLevel& A = L.back();
         A.diagPos.assign(A.n, -1);
-        parallel_row_chunks(A.n, setup_threads((long)A.colIdx.size()), [&](int, int r0, int r1) {
+        parallel_row_chunks(A.n, setup_chunks((long)A.colIdx.size()), [&](int, int r0, int r1) {
             for (int u = r0; u < r1; ++u)
                 for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k)
                     if (A.colIdx[k] == u) A.diagPos[u] = k;
@@ -429,7 +437,7 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
         const AmgHostLevel* Onext = O ? prev->levels[lv + 1] : nullptr;
         std::vector<unsigned char> cleanA, cleanP, cleanT;
         std::vector<int> dA, dP, dT;
-        const int nt0 = setup_threads((long)A.colIdx.size());
+        const int nt0 = setup_chunks((long)A.colIdx.size());
         if (O) {
             cleanA.assign(A.n, 0);
             cleanP.assign(A.n, 0);
@@ -522,7 +530,7 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
         // T = A P: pattern and (A block, P block) pairs, per row grouped by
         // coarse column in generation order (k ascending, then j).
         {
-            const int nt = setup_threads((long)A.pCol.size() * 8);
+            const int nt = setup_chunks((long)A.pCol.size() * 8);
             const auto gen = [&](int u, auto&& emit) {
                 for (int k = A.rowPtr[u]; k < A.rowPtr[u + 1]; ++k) {
                     const int w = A.colIdx[k];
@@ -559,7 +567,7 @@ inline int amg_build_structure(int nV, const std::vector<int>& rowPtr, const std
                 if (agg[u] >= 0 && A.late[u]) nx->late[agg[u]] = 1;
         }
         {
-            const int nt = setup_threads((long)A.tCol.size() * 4);
+            const int nt = setup_chunks((long)A.tCol.size() * 4);
             std::vector<unsigned char> cleanC;
             if (O) {
                 cleanC.assign(nc, 0);
