@@ -6,9 +6,12 @@
  * SymDirichletEnergy), an optional bijectivity air mesh ("block 1", the
  * Scaffold's airMesh), the assembled Hessian and the solver workspace.
  * All pointers in this header are HOST pointers to plain arrays; the library
- * copies to and from HBM itself. No call allocates after the set-up calls
- * (oc_set_mesh, oc_set_air_mesh, oc_solver_set_pattern) have sized the
- * context. Every function returns an OC_* status; nothing throws.
+ * copies to and from HBM itself. The set-up calls (oc_set_mesh, oc_set_air_mesh,
+ * oc_newton_init, oc_solver_set_pattern, oc_solver_factorize) size the device
+ * buffers; later calls reuse them and allocate only when a size grows (an air
+ * re-mesh with more blocks, a multigrid rebuild with larger levels, the first
+ * refinement pass or IC(0) fallback). oc_energy_noalloc never allocates.
+ * Every function returns an OC_* status; nothing throws.
  *
  * Array conventions (row-major, C order):
  *   Vrest  nV x 3 doubles       F      nF x 3 int32 (counter-clockwise in UV)
