@@ -44,6 +44,21 @@ def lanczos0_bytes(nnzb, nV):
     return 36 * nnzb + 4 * (nV + 1) + 100 * nV
 
 
+def element_bytes(nF, nV):
+    """Algorithmic bytes of one FP64 element pass (k_element with gradient and projected Hessian) over nF
+    triangles on nV vertices: per triangle 3 vertex indices (12 B) and 5 rest features (40 B) read, the gradient
+    (3 x 16 B) and the six 2x2 Hessian blocks (6 x 32 B) written; each vertex's UV (16 B) read once."""
+    return nF * (12 + 40 + 48 + 192) + 16 * nV
+
+
+def assembly_bytes(nF, nV, nnzb, nub):
+    """Algorithmic bytes of one assembly (k_assemble): per upper block its row, column, two slot positions and
+    contribution pointer (20 B); per contribution (6 per triangle: 3 diagonal and 3 off-diagonal element blocks)
+    its code (4 B) and its 2x2 block (32 B); every stored block written once (32 B) and per vertex the inverse
+    diagonal block (32 B)."""
+    return 20 * nub + 36 * 6 * nF + 32 * nnzb + 32 * nV
+
+
 def cheb_bytes(n, blocks, bs, word=8):
     """Algorithmic bytes of one V-cycle Chebyshev step (k_cheb_step) on a level of n rows and `blocks` stored
     bs x bs blocks: the blocks and their column indices, row pointers, per row the inverse diagonal block, the
@@ -474,6 +489,15 @@ def headline(ctx, args, targets, fast):
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": TRAFFIC.get((args.workload, "k_pcg_spmv")), "algorithmic_bytes_per_launch": nbytes,
             "avg_launch_ms": spmv_ms, "system": {"triangles": nF, "vertices": nVt, "blocks": nnzb}}
+    # element pass and assembly of the same Newton system (FP64)
+    def rate(kernel, ms, nbytes):
+        a = nbytes / (ms * 1e-3) / 1e9
+        return {"kernel": kernel, "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+                "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": ms}
+    element = rate("k_element<double,true,true> (energy, gradient, projected Hessian; FP64 VALU-bound)",
+                   ctx.time_kernel(0, reps=20), element_bytes(nF, nVt))
+    assembly = rate("k_assemble (gather of the element blocks into the block CSR)", ctx.time_kernel(1, reps=20),
+                    assembly_bytes(nF, nVt, nnzb, nub))
     # the dominant kernel of the chosen form on this workload: the V-cycle's Chebyshev step on the coarse
     # (3x3-block) levels; per level and over those levels (each launched twice per V-cycle)
     per_level, agg = cheb_levels(ctx)
@@ -482,7 +506,8 @@ def headline(ctx, args, targets, fast):
                 "achieved": agg["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": agg["achieved"] / HBM_PEAK_GBS, "traffic": TRAFFIC.get((args.workload, "k_cheb_step3")),
                 "algorithmic_bytes_per_launch": agg["algorithmic_bytes_per_launch"],
-                "avg_launch_ms": agg["avg_launch_ms"], "per_level": per_level, "pcg_spmv": spmv}
+                "avg_launch_ms": agg["avg_launch_ms"], "per_level": per_level, "pcg_spmv": spmv,
+                "element_pass": element, "assembly": assembly}
     return head, w, roofline
 
 
