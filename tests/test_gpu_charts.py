@@ -91,3 +91,42 @@ def test_charts_64_workload_converges_like_single_runs():
             assert abs(E_batched[k] - e1) <= 1e-6 * e1
     finally:
         ctx.close()
+
+
+def test_charts_64_workload_mixed_precision():
+    """The batched charts with FP32 element terms and the FP32-stored multigrid (OC_PRECISION_MIXED): no
+    inversions, energies fall monotonically per chart, and after as many batched iterations as the FP64 run
+    needed to stop every chart, each chart's E_SD is within 1e-6 of the FP64 run's final one (the line search
+    evaluates chart energies in FP64 either way). The mixed run's charts do not stop on their own at this
+    gradient tolerance (1e-16 relative: below the FP32 gradient's rounding), so it runs that many iterations."""
+    charts = batched_disc_charts(64, 500, 20000, base_seed=1000)
+    V, F, UV, vs, fs, fixed = union_of_charts(charts)
+    ctx = capi.Context(0)
+    try:
+        out = {}
+        for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+            ctx.set_mesh(V, F, fixed=fixed)
+            ctx.set_uv(UV)
+            ctx.set_charts(vs, fs)
+            ctx.newton_init(a0=1.0, rel_gl2_tol=1e-16, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                            pcg_rel_tol=1e-4, pcg_max_iter=1000, precision=prec)
+            prev = ctx.charts_state()["E_SD"].copy()
+            n_it = 60 if prec == capi.PRECISION_FP64 else out[capi.PRECISION_FP64][1]
+            for it in range(n_it):
+                s, done = ctx.newton_step()
+                st = ctx.charts_state()
+                assert np.all(st["E_SD"] <= prev * (1 + 1e-12)), (prec, it)
+                prev = st["E_SD"].copy()
+                if done:
+                    break
+            assert ctx.check_inversion()
+            if prec == capi.PRECISION_FP64:
+                assert np.all(st["state"] != 0)
+            out[prec] = (st["E_SD"].copy(), it + 1)
+            print("precision", prec, "iterations", it + 1, flush=True)
+        e64, emx = out[capi.PRECISION_FP64][0], out[capi.PRECISION_MIXED][0]
+        rel = np.abs(emx - e64) / e64
+        print("max rel diff mixed vs FP64", rel.max(), flush=True)
+        assert rel.max() <= 1e-6
+    finally:
+        ctx.close()
