@@ -126,6 +126,7 @@ def closed_forms(args):
     solve captures the exact step and long solves only cost time); the plain form runs Jacobi-PCG to 1e-10."""
     from optcuts_amd import capi
     fp64, mixed = capi.PRECISION_FP64, capi.PRECISION_MIXED
+    # (the chosen and mixed forms also loosen the solve to --loose-tol after a step shorter than --loose-below)
     return {"chosen": (capi.PRECOND_AMG, args.chosen_tol, args.chosen_cap, fp64),
             "mixed": (capi.PRECOND_AMG, args.chosen_tol, args.chosen_cap, mixed),
             "learned": (capi.PRECOND_LEARNED, args.learned_tol, 2000, fp64),
@@ -160,6 +161,9 @@ def closed_run(ctx, w, form, args, steps, warmup, e_star=None, budget=None):
     pc, tol, mx, prec = closed_forms(args)[form]
     W.restart(ctx, w)
     t0 = time.perf_counter()
+    # the fast multigrid forms loosen the solve after steps the injectivity bound cut short
+    fast_mg = form in ("chosen", "mixed")
+    ctx.set_forcing(args.loose_tol if fast_mg else 0.0, args.loose_below if fast_mg else 0.0)
     ctx.newton_init(a0=w["a0"], rel_gl2_tol=1e-12, allow_edec_rel_tol=False, precond=pc, pcg_rel_tol=tol,
                     pcg_max_iter=mx, precision=prec)
     t_init = time.perf_counter() - t0
@@ -532,6 +536,10 @@ def main():
     ap.add_argument("--workload", default="tori_200k", choices=["tori_200k", "icosphere_100k", "icosphere_1m"])
     ap.add_argument("--pcg-tol", type=float, default=1e-10, help="plain form PCG relative tolerance")
     ap.add_argument("--chosen-tol", type=float, default=1e-3, help="chosen form (FP64 multigrid) PCG tolerance")
+    ap.add_argument("--loose-tol", type=float, default=0.1,
+                    help="closed surfaces, chosen and mixed forms: PCG tolerance after a step shorter than "
+                         "--loose-below (0: off)")
+    ap.add_argument("--loose-below", type=float, default=0.2)
     ap.add_argument("--chosen-cap", type=int, default=100,
                     help="PCG iteration cap of the chosen and mixed forms on the closed surfaces")
     ap.add_argument("--mixed-tol", type=float, default=1e-4, help="mixed form PCG tolerance (disc section)")

@@ -304,6 +304,11 @@ int oc_newton_set_precision(oc_ctx* ctx, int precision);
  * ir_passes the passes. Works with every precision and preconditioner; takes effect at the next step. No
  * reference counterpart. */
 int oc_newton_set_refinement(oc_ctx* ctx, int maxPasses);
+/* Step-dependent PCG tolerance (looseTol > 0; 0, the default, turns it off): after a Newton step shorter than
+ * stepBelow (the injectivity bound cut it short, so the direction's accuracy goes mostly unused), and for the
+ * first step after oc_newton_init, the next solve runs to max(pcgRelTol, looseTol); otherwise to pcgRelTol.
+ * Takes effect at the next step. No reference counterpart (the reference solves exactly). */
+int oc_newton_set_forcing(oc_ctx* ctx, double looseTol, double stepBelow);
 /* Binds the network used by OC_PRECOND_LEARNED from the next oc_newton_init on (null unbinds). The context
  * does not own it; while bound, the network's pattern and activations belong to the context's Newton system,
  * and it must not be used from another context or through oc_gnn_set_system. No reference counterpart. */

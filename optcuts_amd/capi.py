@@ -122,6 +122,7 @@ _SIGS = {
     "oc_gnn_precond_apply": (C.c_int, [_vp, _dp, _dp]),
     "oc_newton_set_precision": (C.c_int, [_ctx, C.c_int]),
     "oc_newton_set_refinement": (C.c_int, [_ctx, C.c_int]),
+    "oc_newton_set_forcing": (C.c_int, [_ctx, C.c_double, C.c_double]),
     "oc_newton_init": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.POINTER(C.c_double)]),
     "oc_newton_step": (C.c_int, [_ctx, C.POINTER(IterStats), C.POINTER(C.c_int)]),
@@ -521,6 +522,11 @@ class Context:
         """Bind a Gnn for PRECOND_LEARNED (None unbinds); the context keeps a reference."""
         self._learned = net
         self._chk(self.L.oc_newton_set_learned(self.h, net.h if net is not None else None))
+
+    def set_forcing(self, loose_tol=0.0, step_below=0.0):
+        """After a Newton step shorter than step_below (and for the first step), solve to max(pcg_rel_tol,
+        loose_tol); loose_tol 0 turns it off."""
+        self._chk(self.L.oc_newton_set_forcing(self.h, float(loose_tol), float(step_below)))
 
     def set_refinement(self, max_passes):
         """Mixed-precision iterative refinement of the Newton solve (PCG on the FP32-rounded matrix, FP64

@@ -156,6 +156,9 @@ struct oc_ctx {
     double a0 = 1.0, relGL2Tol = 1e-12, targetGRes = 0.0;
     int allowEDecRelTol = 1;
     double lastE = 0.0, lastESD = 0.0, lastEScaf = 0.0;
+    // oc_newton_set_forcing: after a step shorter than looseBelow (the injectivity bound held it back, so the
+    // direction's accuracy is mostly unused) the next solve runs to max(pcgTol, looseTol); 0 turns it off
+    double looseTol = 0.0, looseBelow = 0.0, lastStep = 1.0;
     int nPrecond = OC_PRECOND_JACOBI;
     double pcgTol = 1e-10;
     int pcgMaxIter = 10000;
@@ -2570,6 +2573,7 @@ int oc_newton_init(oc_ctx* c, double a0, double relGL2Tol, int allowEDecRelTol, 
     c->pcgHint = 0;
     c->amgSkip = 0;
     c->amgFailStreak = 0;
+    c->lastStep = 0.0;  // the first solve counts as following a short step
     c->elemFp64 = false;
     c->sysPrecision = c->precision;
     // Optimizer constructor: exit(-1) on an inverted initial map (Optimizer.cpp:65-67)
@@ -2760,6 +2764,15 @@ int oc_newton_set_precision(oc_ctx* c, int precision)
     return OC_OK;
 }
 
+int oc_newton_set_forcing(oc_ctx* c, double looseTol, double stepBelow)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, looseTol >= 0.0 && looseTol < 1.0 && stepBelow >= 0.0, "oc_newton_set_forcing: bad parameters");
+    c->looseTol = looseTol;
+    c->looseBelow = stepBelow;
+    return OC_OK;
+}
+
 int oc_newton_set_refinement(oc_ctx* c, int maxPasses)
 {
     if (!c) return OC_ERR_ARG;
@@ -2923,13 +2936,16 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
         S->pcg_rel_res = rr;
         return st != OC_OK && st != OC_ERR_NOT_CONVERGED ? st : OC_OK;
     }
+    // the PCG tolerance of this step (looser after a step the injectivity bound cut short)
+    const double pcgTol = c->looseTol > 0.0 && c->lastStep < c->looseBelow ? std::max(c->pcgTol, c->looseTol)
+                                                                          : c->pcgTol;
     int st;
     if (c->irMax > 0) {
         int passes = 0;
         st = refined_pcg(c, it, rr, passes);
         S->ir_passes = passes;
     } else {
-        st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it, rr, c->ext, c->pcgHint);
+        st = run_pcg(c, c->sys, c->dB.as<double>(), pcgTol, c->pcgMaxIter, it, rr, c->ext, c->pcgHint);
     }
     c->pcgHint = st == OC_OK ? it : 0;
     if (!c->ext && st == OC_ERR_BREAKDOWN) {
@@ -2981,7 +2997,7 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
                 ++S->precond_setups;
             }
             int it2 = 0;
-            st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, c->ext);
+            st = run_pcg(c, c->sys, c->dB.as<double>(), pcgTol, c->pcgMaxIter, it2, rr, c->ext);
             it += it2;
             if (st == OC_ERR_NOT_CONVERGED && it2 > 0 && c->nPrecond == OC_PRECOND_AMG) st = OC_OK;
             if (st == OC_OK) S->pcg_fallback = 4;
@@ -2990,7 +3006,7 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
             amg_numeric(c, 1);
             ++S->precond_setups;
             int it2 = 0;
-            st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, c->ext);
+            st = run_pcg(c, c->sys, c->dB.as<double>(), pcgTol, c->pcgMaxIter, it2, rr, c->ext);
             it += it2;
             if (st == OC_OK) S->pcg_fallback = 1;
         }
@@ -2999,7 +3015,7 @@ int solve_direction(oc_ctx* c, oc_iter_stats* S, bool setupLaunched = false)
             if (c->ic0Struct || (st = ic0_structure(c)) == OC_OK) {
                 c->ic0.numeric(s, c->sys.dVals.as<double>());
                 int it2 = 0;
-                st = run_pcg(c, c->sys, c->dB.as<double>(), c->pcgTol, c->pcgMaxIter, it2, rr, &c->ic0);
+                st = run_pcg(c, c->sys, c->dB.as<double>(), pcgTol, c->pcgMaxIter, it2, rr, &c->ic0);
                 it += it2;
                 if (st == OC_OK) S->pcg_fallback = 7;
             }
@@ -3149,6 +3165,7 @@ int newton_step_charts(oc_ctx* c, oc_iter_stats* S, int* done)
     S->E_total = esumNew;
     S->E_SD = esumNew / c->a0;
     S->step = amax;
+    c->lastStep = amax;
     S->halvings = halvings;
     S->stopped = nLeft == 0;
     *done = nLeft == 0;
@@ -3350,6 +3367,7 @@ int oc_newton_step(oc_ctx* c, oc_iter_stats* S, int* done)
     S->E_SD = testESD;
     S->E_scaf = testES;
     S->step = alpha;
+    c->lastStep = alpha;
     S->halvings = halvings;
     S->stopped = stopped ? 1 : 0;
     *done = stopped ? 1 : 0;

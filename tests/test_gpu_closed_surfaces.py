@@ -195,3 +195,37 @@ def test_closed_surfaces_full_size_properties(ctx, name):
     auv, Fa = ctx.air_mesh_get()
     assert (signed_areas(auv, Fa) > 0).all()
     assert O.energy(w["F"], UVg, O.RestFeatures(w["V"], w["F"])) == pytest.approx(ctx.energy(), rel=1e-12)
+
+
+def test_step_dependent_tolerance(ctx):
+    """oc_newton_set_forcing: from the Tutte map of the genus-3 surface the injectivity bound cuts the first steps
+    short, and after each step shorter than 0.2 (and for the first) the solve stops at relative residual 0.1
+    instead of 1e-13; after longer steps it reaches 1e-13. Energies decrease throughout."""
+    w = W.prepare_closed(ctx, "tori_h0.12")
+    ctx.set_forcing(0.1, 0.2)
+    try:
+        E_prev = ctx.newton_init(a0=w["a0"], rel_gl2_tol=1e-12, allow_edec_rel_tol=False, precond=capi.PRECOND_AMG,
+                                 pcg_rel_tol=1e-13, pcg_max_iter=5000)
+        prev_step = 0.0
+        n_loose = n_tight = 0
+        for k in range(30):
+            s, done = ctx.newton_step()
+            E_upd, flips = ctx.scaffold_update(capi.SCAFFOLD_REMESH_WHEN_NEEDED, 20.0)
+            print(k, "step", s["step"], "pcg", s["pcg_iters"], "res", s["pcg_rel_res"], "fallback", s["pcg_fallback"],
+                  flush=True)
+            assert s["E_total"] < E_prev
+            E_prev = E_upd
+            if s["pcg_fallback"] == 0:
+                if prev_step < 0.2:
+                    assert s["pcg_rel_res"] <= 0.1
+                    n_loose += s["pcg_rel_res"] > 1e-10
+                else:
+                    assert s["pcg_rel_res"] <= 1e-13
+                    n_tight += 1
+            prev_step = s["step"]
+            if done:
+                break
+        assert n_loose >= 1 and n_tight >= 1
+    finally:
+        ctx.set_forcing(0.0, 0.0)
+
