@@ -129,10 +129,10 @@ def closed_forms(args):
     # (the chosen and mixed forms also loosen the solve to --loose-tol after a step shorter than --loose-below)
     return {"chosen": (capi.PRECOND_AMG, args.chosen_tol, args.chosen_cap, fp64),
             "mixed": (capi.PRECOND_AMG, args.chosen_tol, args.chosen_cap, mixed),
-            "learned": (capi.PRECOND_LEARNED, args.learned_tol, 2000, fp64),
-            "learned_mixed": (capi.PRECOND_LEARNED, args.learned_tol, 2000, mixed),
-            "block_jacobi_at_learned_tol": (capi.PRECOND_BLOCK_JACOBI, args.learned_tol, 2000, fp64),
-            "ic0_at_learned_tol": (capi.PRECOND_IC0, args.learned_tol, 2000, fp64),
+            "learned": (capi.PRECOND_LEARNED, args.closed_learned_tol, 2000, fp64),
+            "learned_mixed": (capi.PRECOND_LEARNED, args.closed_learned_tol, 2000, mixed),
+            "block_jacobi_at_learned_tol": (capi.PRECOND_BLOCK_JACOBI, args.closed_learned_tol, 2000, fp64),
+            "ic0_at_learned_tol": (capi.PRECOND_IC0, args.closed_learned_tol, 2000, fp64),
             "plain": (capi.PRECOND_JACOBI, args.pcg_tol, 200000, fp64)}
 
 
@@ -161,8 +161,8 @@ def closed_run(ctx, w, form, args, steps, warmup, e_star=None, budget=None):
     pc, tol, mx, prec = closed_forms(args)[form]
     W.restart(ctx, w)
     t0 = time.perf_counter()
-    # the fast multigrid forms loosen the solve after steps the injectivity bound cut short
-    fast_mg = form in ("chosen", "mixed")
+    # the forms in --forcing-forms loosen the solve after steps the injectivity bound cut short
+    fast_mg = form in args.forcing_forms.split(",")
     ctx.set_forcing(args.loose_tol if fast_mg else 0.0, args.loose_below if fast_mg else 0.0)
     ctx.newton_init(a0=w["a0"], rel_gl2_tol=1e-12, allow_edec_rel_tol=False, precond=pc, pcg_rel_tol=tol,
                     pcg_max_iter=mx, precision=prec)
@@ -540,6 +540,12 @@ def main():
                     help="closed surfaces, chosen and mixed forms: PCG tolerance after a step shorter than "
                          "--loose-below (0: off)")
     ap.add_argument("--loose-below", type=float, default=0.2)
+    ap.add_argument("--forcing-forms",
+                    default="chosen,mixed,learned,learned_mixed,block_jacobi_at_learned_tol,ic0_at_learned_tol",
+                    help="closed-surface forms that use the step-dependent tolerance")
+    ap.add_argument("--closed-learned-tol", type=float, default=1e-4,
+                    help="closed surfaces: PCG tolerance of the learned forms and their controls (the disc "
+                         "section uses --learned-tol)")
     ap.add_argument("--chosen-cap", type=int, default=100,
                     help="PCG iteration cap of the chosen and mixed forms on the closed surfaces")
     ap.add_argument("--mixed-tol", type=float, default=1e-4, help="mixed form PCG tolerance (disc section)")
@@ -651,9 +657,9 @@ def main():
         ctx.set_learned(None)
         net.close()
     ctx.close()
-    if head is None:
+    if head is None or "chosen" not in head["forms"]:
         print(json.dumps({"metric": METRIC, "value": None, "unit": "Newton iters/s", "n_gpus": 1,
-                          "secondary": secondary}), flush=True)
+                          "headline": head, "secondary": secondary}), flush=True)
         return
     chosen = head["forms"]["chosen"]
     out = {
