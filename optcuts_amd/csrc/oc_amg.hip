@@ -372,14 +372,15 @@ struct RowPass {
 struct RowChunk {
     int r0, r1, passes;
 };
+// Row chunk of workgroup b of G (the V-cycle kernels' bodies also run as one
+// workgroup covering every row: b = 0, G = 1).
 template <int BS>
-__device__ __forceinline__ RowChunk row_chunk(int n)
+__device__ __forceinline__ RowChunk row_chunk(int n, int b, int G)
 {
     constexpr int RPP = kBlock / RowShape<BS>::lanes, UNROLL = RowShape<BS>::unroll;
-    const int G = gridDim.x;
     const int c = (n + G - 1) / G;
     RowChunk rc;
-    rc.r0 = xcd_chunk(blockIdx.x, G) * c;
+    rc.r0 = xcd_chunk(b, G) * c;
     rc.r1 = min(n, rc.r0 + c);
     rc.passes = max(1, (c + RPP * UNROLL - 1) / (RPP * UNROLL));
     return rc;
@@ -1025,20 +1026,18 @@ __global__ void __launch_bounds__(kBlock) k_cheb_pre1(const int* __restrict__ st
 // so the new d goes to dout. FINAL: z = x (FP64) and partial records of
 // rFinal.z per workgroup. Launch with row_grid workgroups.
 template <typename T, int BS, bool FINAL>
-__global__ void __launch_bounds__(kBlock) k_cheb_step(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
-                                                      const int* __restrict__ colIdx, const T* __restrict__ vals,
-                                                      const T* __restrict__ Dinv, const double* __restrict__ lam,
-                                                      const T* __restrict__ din, T* dout, T* r, T* x,
-                                                      const double* __restrict__ rFinal, double* zFinal,
-                                                      double* rzPart)
+__device__ __forceinline__ void cheb_step_rows(int blk, int G, int n, const int* __restrict__ rowPtr,
+                                               const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                               const T* __restrict__ Dinv, const double* __restrict__ lam,
+                                               const T* __restrict__ din, T* dout, T* r, T* x,
+                                               const double* __restrict__ rFinal, double* zFinal, double* rzPart)
 {
-    if (stop && *stop) return;
     constexpr int LANES = RowShape<BS>::lanes, UNROLL = RowShape<BS>::unroll, RPP = kBlock / LANES;
     const ChebP cp = cheb_params(*lam);
     const double rho0 = 1.0 / cp.sigma;
     const double rho1 = 1.0 / (2.0 * cp.sigma - rho0);
     const double c1 = rho1 * rho0, c2 = 2.0 * rho1 / cp.delta;
-    const RowChunk rc = row_chunk<BS>(n);
+    const RowChunk rc = row_chunk<BS>(n, blk, G);
     const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     double acc[1] = {0.0};
     for (int p = 0; p < rc.passes; ++p) {
@@ -1086,19 +1085,31 @@ __global__ void __launch_bounds__(kBlock) k_cheb_step(const int* __restrict__ st
     }
     if (FINAL) {
         block_sum<1>(acc);
-        if (threadIdx.x == 0) rzPart[blockIdx.x] = acc[0];
+        if (threadIdx.x == 0) rzPart[blk] = acc[0];
     }
+}
+
+template <typename T, int BS, bool FINAL>
+__global__ void __launch_bounds__(kBlock) k_cheb_step(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
+                                                      const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                      const T* __restrict__ Dinv, const double* __restrict__ lam,
+                                                      const T* __restrict__ din, T* dout, T* r, T* x,
+                                                      const double* __restrict__ rFinal, double* zFinal,
+                                                      double* rzPart)
+{
+    if (stop && *stop) return;
+    cheb_step_rows<T, BS, FINAL>(blockIdx.x, gridDim.x, n, rowPtr, colIdx, vals, Dinv, lam, din, dout, r, x, rFinal,
+                                 zFinal, rzPart);
 }
 
 // residual res = b - A x (row_grid workgroups)
 template <typename T, int BS>
-__global__ void __launch_bounds__(kBlock) k_residual(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
-                                                     const int* __restrict__ colIdx, const T* __restrict__ vals,
-                                                     const T* __restrict__ b, const T* __restrict__ x, T* res)
+__device__ __forceinline__ void residual_rows(int blk, int G, int n, const int* __restrict__ rowPtr,
+                                              const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                              const T* __restrict__ b, const T* __restrict__ x, T* res)
 {
-    if (stop && *stop) return;
     constexpr int LANES = RowShape<BS>::lanes, UNROLL = RowShape<BS>::unroll, RPP = kBlock / LANES;
-    const RowChunk rc = row_chunk<BS>(n);
+    const RowChunk rc = row_chunk<BS>(n, blk, G);
     const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     for (int p = 0; p < rc.passes; ++p) {
         RowPass<T, BS, UNROLL> rp;
@@ -1121,19 +1132,27 @@ __global__ void __launch_bounds__(kBlock) k_residual(const int* __restrict__ sto
     }
 }
 
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_residual(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
+                                                     const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                     const T* __restrict__ b, const T* __restrict__ x, T* res)
+{
+    if (stop && *stop) return;
+    residual_rows<T, BS>(blockIdx.x, gridDim.x, n, rowPtr, colIdx, vals, b, x, res);
+}
+
 // post-smoothing first step from the prolongated x: r = D^-1 (b - A x); d = r/theta;
 // xout = x + d (row_grid workgroups)
 template <typename T, int BS>
-__global__ void __launch_bounds__(kBlock) k_cheb_post1(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
-                                                       const int* __restrict__ colIdx, const T* __restrict__ vals,
-                                                       const T* __restrict__ Dinv, const double* __restrict__ lam,
-                                                       const T* __restrict__ b, const T* __restrict__ xin,
-                                                       T* xout, T* r, T* d)
+__device__ __forceinline__ void cheb_post1_rows(int blk, int G, int n, const int* __restrict__ rowPtr,
+                                                const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                const T* __restrict__ Dinv, const double* __restrict__ lam,
+                                                const T* __restrict__ b, const T* __restrict__ xin, T* xout, T* r,
+                                                T* d)
 {
-    if (stop && *stop) return;
     constexpr int LANES = RowShape<BS>::lanes, UNROLL = RowShape<BS>::unroll, RPP = kBlock / LANES;
     const ChebP cp = cheb_params(*lam);
-    const RowChunk rc = row_chunk<BS>(n);
+    const RowChunk rc = row_chunk<BS>(n, blk, G);
     const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     for (int p = 0; p < rc.passes; ++p) {
         RowPass<T, BS, UNROLL> rp;
@@ -1173,6 +1192,17 @@ __global__ void __launch_bounds__(kBlock) k_cheb_post1(const int* __restrict__ s
     }
 }
 
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_cheb_post1(const int* __restrict__ stop, int n, const int* __restrict__ rowPtr,
+                                                       const int* __restrict__ colIdx, const T* __restrict__ vals,
+                                                       const T* __restrict__ Dinv, const double* __restrict__ lam,
+                                                       const T* __restrict__ b, const T* __restrict__ xin,
+                                                       T* xout, T* r, T* d)
+{
+    if (stop && *stop) return;
+    cheb_post1_rows<T, BS>(blockIdx.x, gridDim.x, n, rowPtr, colIdx, vals, Dinv, lam, b, xin, xout, r, d);
+}
+
 // bc_c = sum over P blocks (u,c) of P^T res_u, LN lanes per coarse node (the launcher picks LN from the mean
 // length of the restriction lists: on the small coarse levels a node gathers ~100 P blocks, and with few nodes
 // the kernel is one latency chain per lane, so the lanes per node grow with the list).
@@ -1180,15 +1210,14 @@ __global__ void __launch_bounds__(kBlock) k_cheb_post1(const int* __restrict__ s
 // r = D^-1 b, d = r/theta, x = d) is row-local, so it runs here on each
 // restricted row instead of in a launch of its own.
 template <typename T, int BS, bool PRE1, int LN>
-__global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ stop, int nc, const int* __restrict__ rPtr,
-                                                     const int* __restrict__ rP, const int* __restrict__ pRow,
-                                                     const T* __restrict__ P, const T* __restrict__ res, T* bc,
-                                                     const T* __restrict__ DinvN, const double* __restrict__ lamN,
-                                                     T* xN, T* rN, T* dN)
+__device__ __forceinline__ void restrict_rows(int blk, int G, int nc, const int* __restrict__ rPtr,
+                                              const int* __restrict__ rP, const int* __restrict__ pRow,
+                                              const T* __restrict__ P, const T* __restrict__ res, T* bc,
+                                              const T* __restrict__ DinvN, const double* __restrict__ lamN, T* xN,
+                                              T* rN, T* dN)
 {
-    if (stop && *stop) return;
-    int c0, c1;
-    chunk_rows(nc, c0, c1);
+    const int cc = (nc + G - 1) / G;
+    const int c0 = blk * cc, c1 = min(nc, c0 + cc);
     constexpr int RPP = kBlock / LN;
     const int sub = threadIdx.x / LN, lane = threadIdx.x % LN;
     for (int c = c0 + sub; c < c1; c += RPP) {
@@ -1232,16 +1261,27 @@ __global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ sto
     }
 }
 
+template <typename T, int BS, bool PRE1, int LN>
+__global__ void __launch_bounds__(kBlock) k_restrict(const int* __restrict__ stop, int nc, const int* __restrict__ rPtr,
+                                                     const int* __restrict__ rP, const int* __restrict__ pRow,
+                                                     const T* __restrict__ P, const T* __restrict__ res, T* bc,
+                                                     const T* __restrict__ DinvN, const double* __restrict__ lamN,
+                                                     T* xN, T* rN, T* dN)
+{
+    if (stop && *stop) return;
+    restrict_rows<T, BS, PRE1, LN>(blockIdx.x, gridDim.x, nc, rPtr, rP, pRow, P, res, bc, DinvN, lamN, xN, rN, dN);
+}
+
 // x_u += sum over P blocks (u,c) of P xc_c, kProlongLanes lanes per fine row (one P block each per pass, the
 // partial sums combined by shuffles): the few blocks of a row are loaded together instead of one after another
 constexpr int kProlongLanes = 4;
 template <typename T, int BS>
-__global__ void __launch_bounds__(kBlock) k_prolong(const int* __restrict__ stop, int n, const int* __restrict__ pPtr, const int* __restrict__ pCol,
-                                                    const T* __restrict__ P, const T* __restrict__ xc, T* x)
+__device__ __forceinline__ void prolong_rows(int blk, int n, const int* __restrict__ pPtr,
+                                             const int* __restrict__ pCol, const T* __restrict__ P,
+                                             const T* __restrict__ xc, T* x)
 {
-    if (stop && *stop) return;
     constexpr int LN = kProlongLanes;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * kBlock + threadIdx.x;
     const int u = t / LN, lane = t % LN;
     double s[BS] = {};
     if (u < n) {
@@ -1264,11 +1304,17 @@ __global__ void __launch_bounds__(kBlock) k_prolong(const int* __restrict__ stop
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_dense_apply(const int* __restrict__ stop, int N,
-                                                     const double* __restrict__ Inv, const T* __restrict__ b, T* x)
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_prolong(const int* __restrict__ stop, int n, const int* __restrict__ pPtr, const int* __restrict__ pCol,
+                                                    const T* __restrict__ P, const T* __restrict__ xc, T* x)
 {
     if (stop && *stop) return;
+    prolong_rows<T, BS>(blockIdx.x, n, pPtr, pCol, P, xc, x);
+}
+
+template <typename T>
+__device__ __forceinline__ void dense_apply_rows(int N, const double* __restrict__ Inv, const T* __restrict__ b, T* x)
+{
     __shared__ double bs[kCoarseMaxDofs];
     for (int j = threadIdx.x; j < N; j += blockDim.x) bs[j] = (double)b[j];
     __syncthreads();
@@ -1277,6 +1323,14 @@ __global__ void __launch_bounds__(256) k_dense_apply(const int* __restrict__ sto
         for (int j = 0; j < N; ++j) s += Inv[(size_t)j * N + i] * bs[j];  // symmetric: column i = row i
         x[i] = (T)s;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_apply(const int* __restrict__ stop, int N,
+                                                     const double* __restrict__ Inv, const T* __restrict__ b, T* x)
+{
+    if (stop && *stop) return;
+    dense_apply_rows<T>(N, Inv, b, x);
 }
 
 // z = x (FP64) and partial records of r.z over the same row chunks as the level-0 step kernel
@@ -1638,11 +1692,10 @@ void Amg<T>::numeric(hipStream_t s, const void* A0vals, int vals32, const double
 
 // x = D^-1 b on a block-diagonal coarsest level (exact)
 template <typename T, int BS>
-__global__ void __launch_bounds__(kBlock) k_coarse_diag(const int* __restrict__ stop, int n, const T* __restrict__ Dinv,
-                                                        const T* __restrict__ b, T* x)
+__device__ __forceinline__ void coarse_diag_rows(int blk, int n, const T* __restrict__ Dinv, const T* __restrict__ b,
+                                                 T* x)
 {
-    if (stop && *stop) return;
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int u = blk * kBlock + threadIdx.x;
     if (u >= n) return;
     const T* D = Dinv + (size_t)u * BS * BS;
 #pragma unroll
@@ -1652,6 +1705,78 @@ __global__ void __launch_bounds__(kBlock) k_coarse_diag(const int* __restrict__ 
         for (int j = 0; j < BS; ++j) acc += (double)D[i * BS + j] * (double)b[(size_t)u * BS + j];
         x[(size_t)u * BS + i] = (T)acc;
     }
+}
+
+template <typename T, int BS>
+__global__ void __launch_bounds__(kBlock) k_coarse_diag(const int* __restrict__ stop, int n, const T* __restrict__ Dinv,
+                                                        const T* __restrict__ b, T* x)
+{
+    if (stop && *stop) return;
+    coarse_diag_rows<T, BS>(blockIdx.x, n, Dinv, b, x);
+}
+
+// The last smoothed level's part of the V-cycle, with 3x3 blocks, in one
+// workgroup: the level's remaining pre-smoothing step, its residual, the
+// restriction, the coarsest solve (dense inverse, or block diagonal with
+// 2x2 or 3x3 blocks: cbs), the prolongation and both post-smoothing steps.
+// The phases are the bodies of the kernels the V-cycle launches on larger
+// levels, run over every row by this one workgroup (each row's arithmetic,
+// lane groups and shuffle order included, is the same as in those launches,
+// so the result is bit-identical), with barriers where the kernel boundaries
+// were. On a level of a few dozen nodes each of those launches is one chain of
+// dependent loads; here the chains follow each other inside one launch.
+#ifndef OC_TAIL_MAX_NODES
+#define OC_TAIL_MAX_NODES 128
+#endif
+constexpr int kTailMaxNodes = OC_TAIL_MAX_NODES;  // largest last smoothed level run as one workgroup (0: never)
+struct TailArgs {
+    // smoothed level (3x3 blocks)
+    int n;
+    const int *rowPtr, *colIdx, *rPtr, *rP, *pRow, *pPtr, *pCol;
+    const void *vals, *Dinv, *P;
+    const double* lam;
+    void *b, *x, *x2, *r, *d, *d2;
+    // coarsest level
+    int nc, cbs, dense;
+    const void* DinvC;
+    const double* Inv;
+    void *bC, *xC;
+};
+
+template <typename T, int LN>
+__global__ void __launch_bounds__(kBlock) k_vcycle_tail(const int* __restrict__ stop, TailArgs a)
+{
+    if (stop && *stop) return;
+    const T* vals = static_cast<const T*>(a.vals);
+    const T* Dinv = static_cast<const T*>(a.Dinv);
+    const T* P = static_cast<const T*>(a.P);
+    T *b = static_cast<T*>(a.b), *x = static_cast<T*>(a.x), *x2 = static_cast<T*>(a.x2);
+    T *r = static_cast<T*>(a.r), *d = static_cast<T*>(a.d), *d2 = static_cast<T*>(a.d2);
+    T *bC = static_cast<T*>(a.bC), *xC = static_cast<T*>(a.xC);
+    cheb_step_rows<T, 3, false>(0, 1, a.n, a.rowPtr, a.colIdx, vals, Dinv, a.lam, d, d2, r, x, nullptr, nullptr,
+                                nullptr);
+    __syncthreads();
+    residual_rows<T, 3>(0, 1, a.n, a.rowPtr, a.colIdx, vals, b, x, r);
+    __syncthreads();
+    restrict_rows<T, 3, false, LN>(0, 1, a.nc, a.rPtr, a.rP, a.pRow, P, r, bC, nullptr, nullptr, nullptr, nullptr,
+                                   nullptr);
+    __syncthreads();
+    if (a.dense) {
+        dense_apply_rows<T>(a.nc * a.cbs, a.Inv, bC, xC);
+    } else {
+        const T* DinvC = static_cast<const T*>(a.DinvC);
+        for (int blk = 0; blk * kBlock < a.nc; ++blk) {
+            if (a.cbs == 2) coarse_diag_rows<T, 2>(blk, a.nc, DinvC, bC, xC);
+            else coarse_diag_rows<T, 3>(blk, a.nc, DinvC, bC, xC);
+        }
+    }
+    __syncthreads();
+    for (int blk = 0; blk * kBlock < a.n * kProlongLanes; ++blk) prolong_rows<T, 3>(blk, a.n, a.pPtr, a.pCol, P, xC, x);
+    __syncthreads();
+    cheb_post1_rows<T, 3>(0, 1, a.n, a.rowPtr, a.colIdx, vals, Dinv, a.lam, b, x, x2, r, d);
+    __syncthreads();
+    cheb_step_rows<T, 3, false>(0, 1, a.n, a.rowPtr, a.colIdx, vals, Dinv, a.lam, d, d2, r, x2, nullptr, nullptr,
+                                nullptr);
 }
 
 template <typename T>
@@ -1701,6 +1826,45 @@ void Amg<T>::restrict_to(hipStream_t s, int l, const int* stop)
 }
 
 template <typename T>
+void Amg<T>::vcycle_tail(hipStream_t s, int l, const int* stop)
+{
+    Level& A = *L[l];
+    Level& C = *L[l + 1];
+    TailArgs a;
+    a.n = A.n;
+    a.rowPtr = A.dRowPtr;
+    a.colIdx = A.dColIdx;
+    a.rPtr = A.dRPtr;
+    a.rP = A.dRP;
+    a.pRow = A.dPRow;
+    a.pPtr = A.dPPtr;
+    a.pCol = A.dPCol;
+    a.vals = A.vals;
+    a.Dinv = A.Dinv;
+    a.P = A.P;
+    a.lam = A.lam;
+    a.b = A.b;
+    a.x = A.x;
+    a.x2 = A.x2;
+    a.r = A.r;
+    a.d = A.d;
+    a.d2 = A.d2;
+    a.nc = C.n;
+    a.cbs = C.bs;
+    a.dense = coarseDiag ? 0 : 1;
+    a.DinvC = C.Dinv;
+    a.Inv = dCoarseInv;
+    a.bC = C.b;
+    a.xC = C.x;
+    // lanes per coarse node as restrict_to picks them
+    const long meanLen = A.nc > 0 ? (long)A.rP.size() / A.nc : 0;
+    const int ln = meanLen >= 48 ? 64 : (meanLen >= 24 ? 32 : 16);
+    if (ln == 64) hipLaunchKernelGGL((k_vcycle_tail<T, 64>), dim3(1), dim3(kBlock), 0, s, stop, a);
+    else if (ln == 32) hipLaunchKernelGGL((k_vcycle_tail<T, 32>), dim3(1), dim3(kBlock), 0, s, stop, a);
+    else hipLaunchKernelGGL((k_vcycle_tail<T, 16>), dim3(1), dim3(kBlock), 0, s, stop, a);
+}
+
+template <typename T>
 void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
 {
     Level& A = *L[l];
@@ -1728,6 +1892,11 @@ void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
         if (l == 0)  // levels below the finest start from the restriction's fused first step
             hipLaunchKernelGGL((k_cheb_pre1<T, 3, T>), dim3(gUp), dim3(kBlock), 0, s, stop, A.n, A.b, A.b, A.Dinv, A.lam,
                                A.x, A.r, A.d);
+        if (l + 2 == (int)L.size() && A.n <= kTailMaxNodes) {
+            vcycle_tail(s, l, stop);
+            std::swap(A.x, A.x2);
+            return;
+        }
         hipLaunchKernelGGL((k_cheb_step<T, 3, false>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx,
                            A.vals, A.Dinv, A.lam, A.d, A.d2, A.r, A.x, nullptr, nullptr, nullptr);
         hipLaunchKernelGGL((k_residual<T, 3>), dim3(g), dim3(kBlock), 0, s, stop, A.n, A.dRowPtr, A.dColIdx, A.vals,

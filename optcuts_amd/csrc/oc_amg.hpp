@@ -123,6 +123,7 @@ private:
     bool ready_ = false;
     void release();
     void vcycle(hipStream_t s, int l, const int* stop);
+    void vcycle_tail(hipStream_t s, int l, const int* stop);
     void restrict_to(hipStream_t s, int l, const int* stop);
     void coarse_solve(hipStream_t s, const int* stop);
 };
