@@ -232,6 +232,12 @@ int oc_solver_apply_precond(oc_ctx* ctx, const double* r, double* z);
  * Takes effect at the next factorization (oc_solver_factorize, or the next Newton step). No reference
  * counterpart. */
 int oc_ic0_set_sweeps(oc_ctx* ctx, int factorSweeps, int applySweeps);
+/* Largest last smoothed multigrid level (nodes) whose part of the V-cycle (its smoothing, residual, transfers and
+ * the coarsest solve) runs as one single-workgroup launch instead of seven launches; the result is bit-identical
+ * either way. 0 turns it off, -1 restores the built-in limit (128). Takes effect when a multigrid structure is
+ * next built (oc_newton_init, a re-mesh that rebuilds it, oc_solver_set_pattern + oc_solver_factorize). No
+ * reference counterpart (diagnostics and tests). */
+int oc_amg_set_tail(oc_ctx* ctx, int maxNodes);
 /* Coordinates (2*nV, [u0 v0 u1 v1 ...], e.g. the current UV of TriMesh::V) for OC_PRECOND_AMG's near-null
  * space (translations and the rotation of the plane per aggregate); without them the multigrid uses
  * translations only. Takes effect at the next factorize. No reference counterpart: LinSysSolver passes no

@@ -98,6 +98,7 @@ _SIGS = {
     "oc_solver_set_params": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_int]),
     "oc_solver_apply_precond": (C.c_int, [_ctx, _dp, _dp]),
     "oc_ic0_set_sweeps": (C.c_int, [_ctx, C.c_int, C.c_int]),
+    "oc_amg_set_tail": (C.c_int, [_ctx, C.c_int]),
     "oc_solver_last_stats": (C.c_int, [_ctx, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "oc_solver_set_coords": (C.c_int, [_ctx, _dp]),
     "oc_solver_last_fallback": (C.c_int, [_ctx, C.POINTER(C.c_int)]),
@@ -458,6 +459,11 @@ class Context:
         z = np.zeros(self._sol_n)
         self._chk(self.L.oc_solver_apply_precond(self.h, r, z))
         return z
+
+    def amg_set_tail(self, max_nodes=-1):
+        """Largest last smoothed multigrid level run as one single-workgroup launch (0: off, -1: built-in limit);
+        takes effect at the next multigrid structure set-up."""
+        self._chk(self.L.oc_amg_set_tail(self.h, int(max_nodes)))
 
     def ic0_set_sweeps(self, factor_sweeps=3, apply_sweeps=2):
         """PRECOND_IC0's fixed-point factor sweeps and Jacobi sweeps per triangular solve."""

@@ -1523,6 +1523,7 @@ int Amg<T>::setup(hipStream_t s, int nV, const std::vector<int>& rowPtr, const s
                   const std::vector<unsigned char>* late)
 {
     ++generation_;
+    tailMax_ = tailPending_;
     // The previous hierarchy's rows are reused where they come out the same
     // (an air-mesh change leaves most of the mesh rows as they were).
     std::vector<AmgHostLevel> H = std::move(spare_);
@@ -1892,7 +1893,7 @@ void Amg<T>::vcycle(hipStream_t s, int l, const int* stop)
         if (l == 0)  // levels below the finest start from the restriction's fused first step
             hipLaunchKernelGGL((k_cheb_pre1<T, 3, T>), dim3(gUp), dim3(kBlock), 0, s, stop, A.n, A.b, A.b, A.Dinv, A.lam,
                                A.x, A.r, A.d);
-        if (l + 2 == (int)L.size() && A.n <= kTailMaxNodes) {
+        if (l + 2 == (int)L.size() && A.n <= (tailMax_ < 0 ? kTailMaxNodes : tailMax_)) {
             vcycle_tail(s, l, stop);
             std::swap(A.x, A.x2);
             return;

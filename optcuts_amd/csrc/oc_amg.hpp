@@ -60,6 +60,9 @@ public:
     virtual void cheb_step_level(hipStream_t s, int l) = 0;
     virtual int level_block_size(int l) const = 0;
     virtual bool ready() const = 0;
+    // Largest last smoothed level (nodes) whose part of the V-cycle runs as one workgroup launch (0: never);
+    // takes effect at the next setup().
+    virtual void set_tail_max(int nodes) = 0;
 };
 
 template <typename T>
@@ -97,6 +100,7 @@ public:
     void cheb_step_level(hipStream_t s, int l) override;
     int level_block_size(int l) const override;
     bool ready() const override { return ready_; }
+    void set_tail_max(int nodes) override { tailPending_ = nodes; }
 
 private:
     struct Level;
@@ -121,6 +125,7 @@ private:
     // structure build writes into it instead of fresh allocations.
     std::vector<AmgHostLevel> spare_;
     bool ready_ = false;
+    int tailMax_ = -1, tailPending_ = -1;  // set_tail_max (-1: the built-in limit), active since the last setup()
     void release();
     void vcycle(hipStream_t s, int l, const int* stop);
     void vcycle_tail(hipStream_t s, int l, const int* stop);

@@ -2128,6 +2128,17 @@ int oc_ic0_set_sweeps(oc_ctx* c, int factorSweeps, int applySweeps)
     return OC_OK;
 }
 
+int oc_amg_set_tail(oc_ctx* c, int maxNodes)
+{
+    if (!c) return OC_ERR_ARG;
+    OC_ARG(c, maxNodes >= -1 && maxNodes <= 100000, "oc_amg_set_tail: bad node count");
+    for (oc::AmgBase* a : {static_cast<oc::AmgBase*>(&c->amg64), static_cast<oc::AmgBase*>(&c->amg32),
+                           static_cast<oc::AmgBase*>(&c->amgTut), static_cast<oc::AmgBase*>(&c->amgSol)})
+        a->set_tail_max(maxNodes);
+    c->solFactorized = false;
+    return OC_OK;
+}
+
 int oc_solver_set_params(oc_ctx* c, int precond, double relTol, int maxIter)
 {
     if (!c) return OC_ERR_ARG;
